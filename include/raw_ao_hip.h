@@ -1,0 +1,131 @@
+/* raw_ao_hip: the MI355X (gfx950) implementation of raw-native's ray-traced AO path,
+ * behind a plain C ABI. Everything is synchronous unless a function takes a stream;
+ * every function returns RAW_AO_OK (0) or a negative code and leaves the message in
+ * raw_ao_last_error(ctx). No C++ exception crosses this boundary. The caller owns
+ * host arrays; the context owns device memory. One context per host thread.
+ * Results never depend on launch, tile or queue parameters.
+ *
+ * Layouts are the reference's: row-major images, y down; a Vec3 is 3 floats (AoS, as
+ * raw::Buffer<Vec3> stores it); masks are one byte a pixel (0 off, nonzero covered);
+ * triangles are 9 floats (a, b, c) as raw::Tri (raw/math/primitives.hpp:7).
+ *
+ * Entry points ending in _dev take DEVICE pointers (e.g. from hipMalloc or a torch
+ * tensor's data_ptr) and a hipStream_t passed as void* (NULL: the context's stream);
+ * they enqueue work and return without waiting.
+ */
+#ifndef RAW_AO_HIP_H
+#define RAW_AO_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RAW_AO_ABI_VERSION 1u
+
+enum {
+    RAW_AO_OK = 0,
+    RAW_AO_ERR_ARG = -1,        /* a bad argument (sizes, null pointers, no scene) */
+    RAW_AO_ERR_DEVICE = -2,     /* a HIP call failed */
+    RAW_AO_ERR_BUILD = -3,      /* the PLOC build could not finish (a round without a merge) */
+    RAW_AO_ERR_NO_DEVICE = -4   /* no GPU: the reference's exit code 4, an "unverifiable" run */
+};
+
+typedef struct raw_ao_ctx raw_ao_ctx;
+
+/* The ABI version this library implements (RAW_AO_ABI_VERSION). */
+uint32_t raw_ao_abi_version(void);
+
+/* Open a context on device `device_ordinal`. Replaces the reference's process-wide
+ * rhi::device() (raw/rhi/rhi.hpp:284-286); RAW_AO_ERR_NO_DEVICE where it would report
+ * no adapter (src/tools/gpu_run.cpp:52-59). On failure *out is still a valid context
+ * that carries the message (close it). */
+int raw_ao_open(int device_ordinal, raw_ao_ctx** out);
+void raw_ao_close(raw_ao_ctx* ctx);
+/* The first error since the last successful call; "" when none. */
+const char* raw_ao_last_error(const raw_ao_ctx* ctx);
+
+/* Options (raw_ao_set_option). */
+enum {
+    /* 1 (default): sin/cos of the sample angle reproduce the HOST libm bit for bit
+     * (the reference calls std::cos/std::sin, src/renderer/ray_ao.cpp:36 and
+     * src/renderer/ssao.cpp:21-24); 0: correctly rounded. */
+    RAW_AO_OPT_HOST_LIBM_TRIG = 1,
+    /* The traversal kernel: 0 auto, 1 ray per lane. */
+    RAW_AO_OPT_TRACE_KERNEL = 2
+};
+int raw_ao_set_option(raw_ao_ctx* ctx, int option, int value);
+
+/* ---- the scene: PLOC BVH build on the device ---------------------------------------
+ * Replaces buildPlocGpu (raw/renderer/rt_gpu.hpp:19; src/renderer/gpu/rt_gpu.cpp:25-95)
+ * and, as the AO accelerator, LinearAccel::build (raw/renderer/accel.hpp:19,
+ * src/renderer/accel.cpp:13-25): triangles in, the context's BVH out. radius: the PLOC
+ * search radius (raw::rt::kPlocRadius = 16, raw/renderer/rt_bvh.hpp:30), 1..32. */
+int raw_ao_build_ploc(raw_ao_ctx* ctx, const float* tris9, uint32_t ntri, int radius);
+int raw_ao_build_ploc_dev(raw_ao_ctx* ctx, const float* d_tris9, uint32_t ntri, int radius, void* stream);
+
+/* The built binary tree as raw::rt::Tree (raw/renderer/rt_bvh.hpp:23-28): node count
+ * (2n-1), root id and rounds; then the nodes in the reference's order (leaves 0..n-1 in
+ * sorted order, internal nodes in creation order): boxes6 {lo xyz, hi xyz} and links3
+ * {left, right, tri} per node. For the node-for-node identity check
+ * (src/renderer/gpu/rt_bvh_parity.cpp:80-123). */
+int raw_ao_tree_info(raw_ao_ctx* ctx, uint32_t* nnodes, int32_t* root, uint32_t* rounds);
+int raw_ao_read_tree(raw_ao_ctx* ctx, float* boxes6, int32_t* links3);
+/* The build's Morton keys (keys[i] of input triangle i) and sorted order (order[p], the
+ * triangle at sorted position p) (src/renderer/rt_bvh.cpp:56-73). */
+int raw_ao_read_keys(raw_ao_ctx* ctx, uint32_t* keys, uint32_t* order);
+
+/* ---- ray-traced AO (the raw-rt-ao-v1 oracle) ---------------------------------------
+ * Replaces computeRTAO (raw/renderer/ray_ao.hpp:6-8, src/renderer/ray_ao.cpp:20-58)
+ * over the context's BVH: ao[y*w+x] = open/spp for covered pixels, 1.0 elsewhere.
+ * position, normal: w*h Vec3 each; mask: w*h bytes; spp 1..4096; radius > 0. */
+int raw_ao_trace_ao(raw_ao_ctx* ctx, const float* position, const float* normal, const uint8_t* mask,
+                    int w, int h, int spp, float radius, float* ao);
+int raw_ao_trace_ao_dev(raw_ao_ctx* ctx, const float* d_position, const float* d_normal, const uint8_t* d_mask,
+                        int w, int h, int spp, float radius, float* d_ao, void* stream);
+
+/* ---- screen-space AO ---------------------------------------------------------------
+ * Replaces computeSSAO (raw/renderer/ssao.hpp:5-6, src/renderer/ssao.cpp:11-39). */
+int raw_ao_ssao(raw_ao_ctx* ctx, const float* position, const float* normal, const uint8_t* mask,
+                int w, int h, int samples, float radius, float* ao);
+int raw_ao_ssao_dev(raw_ao_ctx* ctx, const float* d_position, const float* d_normal, const uint8_t* d_mask,
+                    int w, int h, int samples, float radius, float* d_ao, void* stream);
+
+/* ---- the reconcile's per-pixel part ------------------------------------------------
+ * The error map and maximum of reconcile (raw/cert/reconcile.hpp:12-13,
+ * src/cert/reconcile.cpp:11-16): err[i] = |approx - truth| where covered, else 0;
+ * *max_error = the largest. The float64 row-major sum of squares stays with the caller
+ * (raw_ao_host reconcile()), so the RMSE bits match the reference's. */
+int raw_ao_error_map(raw_ao_ctx* ctx, const float* approx, const float* truth, const uint8_t* mask,
+                     int w, int h, float* err, float* max_error);
+int raw_ao_error_map_dev(raw_ao_ctx* ctx, const float* d_approx, const float* d_truth, const uint8_t* d_mask,
+                         int w, int h, float* d_err, float* d_max_error, void* stream);
+
+/* ---- general rays through the context's BVH ----------------------------------------
+ * Replaces traceGpu (raw/renderer/rt_gpu.hpp:27-28, src/renderer/gpu/rt_gpu.cpp:97-133):
+ * rays8 {origin xyz, direction xyz, tMax, pad}; any != 0 stops at the first hit with
+ * t < tMax; otherwise the nearest (ties to the lower triangle index). out4 per ray:
+ * {t or -1, triangle as a float-exact integer (-1 none), nodes visited, triangles tested}.
+ * far_scale widens each box's exit (1 + 2 gamma(3) normally; a value below 1 is the
+ * reference's short-far control). */
+int raw_ao_trace_rays(raw_ao_ctx* ctx, const float* rays8, uint32_t nrays, int any, float far_scale, float* out4);
+
+/* ---- measurement -------------------------------------------------------------------
+ * Node visits and triangle tests of the last trace_ao over its rays (from a separate
+ * counting launch when enabled with raw_ao_count_next_trace), and the milliseconds of the
+ * last launches of each kernel, timed with HIP events on the launch stream. */
+int raw_ao_count_next_trace(raw_ao_ctx* ctx, int enable);
+int raw_ao_trace_counts(raw_ao_ctx* ctx, uint64_t* node_visits, uint64_t* tri_tests, uint64_t* rays);
+int raw_ao_last_ms(raw_ao_ctx* ctx, float* build_ms, float* trace_ms, float* ssao_ms, float* error_ms);
+/* Verification entry point (no reference counterpart): the sin and cos of every sample
+ * angle phi = 6.2831853f * k / 2^24, k < 2^24, exactly as the AO and SSAO kernels use
+ * them, so a test can compare all of them with the host libm (sin_out, cos_out: 2^24
+ * floats each). */
+int raw_ao_angle_table(raw_ao_ctx* ctx, float* sin_out, float* cos_out);
+/* Bytes of the context's device-resident BVH (nodes + triangles) in its traversal form. */
+int raw_ao_bvh_bytes(raw_ao_ctx* ctx, uint64_t* node_bytes, uint64_t* tri_bytes, uint32_t* node_record_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,164 @@
+// A C entry layer over the reference's OWN sources, compiled from where they lie under
+// the reference tree (see oracle/Makefile). TEST INFRASTRUCTURE ONLY: it lets the tests
+// cross-check the oracle restatement against the reference itself on inputs the golden
+// hashes do not cover, and gives bench.py a "reference" kind of CPU baseline. The
+// product never links or loads it.
+//
+// Signatures mirror oracle_api.h with a ref_ prefix.
+#include "raw/cert/reconcile.hpp"
+#include "raw/renderer/accel.hpp"
+#include "raw/renderer/raster.hpp"
+#include "raw/renderer/ray_ao.hpp"
+#include "raw/renderer/rt_bvh.hpp"
+#include "raw/renderer/rt_scenes.hpp"
+#include "raw/renderer/ssao.hpp"
+#include "raw/renderer/swr_scenes.hpp"
+#include "raw/scene/scene.hpp"
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+raw::GBuffer toG(const float* position, const float* normal, const uint8_t* mask, int w, int h) {
+    raw::GBuffer g;
+    g.resize(w, h);
+    for (size_t i = 0; i < size_t(w) * size_t(h); ++i) {
+        g.position.px[i] = {position[3 * i], position[3 * i + 1], position[3 * i + 2]};
+        g.normal.px[i] = {normal[3 * i], normal[3 * i + 1], normal[3 * i + 2]};
+        g.mask.px[i] = mask[i];
+    }
+    return g;
+}
+raw::Scene triScene(const float* tris9, size_t ntri) {
+    raw::Scene s;
+    raw::Mesh m;
+    for (size_t i = 0; i < ntri; ++i)
+        for (int k = 0; k < 3; ++k) {
+            m.positions.push_back({tris9[9 * i + 3 * size_t(k)], tris9[9 * i + 3 * size_t(k) + 1], tris9[9 * i + 3 * size_t(k) + 2]});
+            m.normals.push_back({0, 1, 0});
+            m.indices.push_back(int(3 * i + size_t(k)));
+        }
+    s.meshes.push_back(std::move(m));
+    return s;
+}
+}  // namespace
+
+extern "C" {
+
+void ref_rasterize(const float* vpos, const float* vnrm, const int32_t* idx, size_t ntri,
+                   const float eye[3], const float center[3], const float up[3],
+                   float fovy, float aspect, float nearZ, float farZ, int w, int h,
+                   float* depth, float* position, float* normal, uint8_t* mask) {
+    raw::Scene s;
+    raw::Mesh m;
+    int32_t maxv = -1;
+    for (size_t i = 0; i < 3 * ntri; ++i) { m.indices.push_back(idx[i]); if (idx[i] > maxv) maxv = idx[i]; }
+    for (int32_t v = 0; v <= maxv; ++v) {
+        m.positions.push_back({vpos[3 * v], vpos[3 * v + 1], vpos[3 * v + 2]});
+        m.normals.push_back({vnrm[3 * v], vnrm[3 * v + 1], vnrm[3 * v + 2]});
+    }
+    s.meshes.push_back(std::move(m));
+    s.camera.eye = {eye[0], eye[1], eye[2]};
+    s.camera.center = {center[0], center[1], center[2]};
+    s.camera.up = {up[0], up[1], up[2]};
+    s.camera.fovy = fovy; s.camera.aspect = aspect; s.camera.nearZ = nearZ; s.camera.farZ = farZ;
+    raw::GBuffer g = raw::rasterize(s, w, h);
+    for (size_t i = 0; i < size_t(w) * size_t(h); ++i) {
+        depth[i] = g.depth.px[i];
+        position[3 * i] = g.position.px[i].x; position[3 * i + 1] = g.position.px[i].y; position[3 * i + 2] = g.position.px[i].z;
+        normal[3 * i] = g.normal.px[i].x; normal[3 * i + 1] = g.normal.px[i].y; normal[3 * i + 2] = g.normal.px[i].z;
+        mask[i] = g.mask.px[i];
+    }
+}
+
+// The test scene (two meshes) rasterized as the reference's CLI does.
+void ref_render_test_scene(int w, int h, const float eye[3], const float center[3], float fovy,
+                           float* position, float* normal, uint8_t* mask) {
+    raw::Scene s = raw::buildTestScene(w, h);
+    s.camera.eye = {eye[0], eye[1], eye[2]};
+    s.camera.center = {center[0], center[1], center[2]};
+    s.camera.fovy = fovy;
+    raw::GBuffer g = raw::rasterize(s, w, h);
+    for (size_t i = 0; i < size_t(w) * size_t(h); ++i) {
+        position[3 * i] = g.position.px[i].x; position[3 * i + 1] = g.position.px[i].y; position[3 * i + 2] = g.position.px[i].z;
+        normal[3 * i] = g.normal.px[i].x; normal[3 * i + 1] = g.normal.px[i].y; normal[3 * i + 2] = g.normal.px[i].z;
+        mask[i] = g.mask.px[i];
+    }
+}
+
+// computeRTAO through LinearAccel; mode 0 BVH, 1 scan, 2 skew (set as the reference reads it).
+void ref_rtao(const float* tris9, size_t ntri, int mode, const float* position, const float* normal,
+              const uint8_t* mask, int w, int h, int samples, float radius, int threads, float* ao) {
+    if (mode == 1) setenv("RAW_NATIVE_AO_ORACLE", "scan", 1);
+    else if (mode == 2) setenv("RAW_NATIVE_AO_ORACLE", "skew", 1);
+    else unsetenv("RAW_NATIVE_AO_ORACLE");
+    raw::LinearAccel accel;
+    accel.build(triScene(tris9, ntri));
+    unsetenv("RAW_NATIVE_AO_ORACLE");
+    const raw::GBuffer g = toG(position, normal, mask, w, h);
+    const raw::Buffer<float> out = raw::computeRTAO(g, accel, samples, radius, nullptr, threads);
+    std::memcpy(ao, out.px.data(), sizeof(float) * size_t(w) * size_t(h));
+}
+
+void ref_ssao(const float* position, const float* normal, const uint8_t* mask, int w, int h, int samples,
+              float radius, int threads, float* ao) {
+    const raw::GBuffer g = toG(position, normal, mask, w, h);
+    const raw::Buffer<float> out = raw::computeSSAO(g, samples, radius, nullptr, threads);
+    std::memcpy(ao, out.px.data(), sizeof(float) * size_t(w) * size_t(h));
+}
+
+void ref_reconcile(const float* approx, const float* truth, const uint8_t* mask, int w, int h, float tolerance,
+                   float* error_map, float* rmse, float* max_error, int32_t* pixels, int32_t* within) {
+    raw::Buffer<float> a, t;
+    raw::Buffer<uint8_t> m;
+    a.resize(w, h); t.resize(w, h); m.resize(w, h);
+    std::memcpy(a.px.data(), approx, sizeof(float) * size_t(w) * size_t(h));
+    std::memcpy(t.px.data(), truth, sizeof(float) * size_t(w) * size_t(h));
+    std::memcpy(m.px.data(), mask, size_t(w) * size_t(h));
+    const raw::ReconcileResult r = raw::reconcile(a, t, m, tolerance);
+    std::memcpy(error_map, r.errorMap.px.data(), sizeof(float) * size_t(w) * size_t(h));
+    *rmse = r.rmse; *max_error = r.maxError; *pixels = r.pixels; *within = r.withinTolerance ? 1 : 0;
+}
+
+int32_t ref_ploc_build(const float* tris9, size_t ntri, int radius, float* boxes6, int32_t* links3, uint32_t* rounds) {
+    std::vector<raw::Tri> t(ntri);
+    for (size_t i = 0; i < ntri; ++i)
+        t[i] = {{tris9[9 * i], tris9[9 * i + 1], tris9[9 * i + 2]}, {tris9[9 * i + 3], tris9[9 * i + 4], tris9[9 * i + 5]},
+                {tris9[9 * i + 6], tris9[9 * i + 7], tris9[9 * i + 8]}};
+    const raw::rt::Tree tree = raw::rt::buildPloc(t, radius);
+    for (size_t i = 0; i < tree.nodes.size(); ++i) {
+        for (int k = 0; k < 3; ++k) { boxes6[6 * i + size_t(k)] = tree.nodes[i].lo[k]; boxes6[6 * i + 3 + size_t(k)] = tree.nodes[i].hi[k]; }
+        links3[3 * i] = tree.nodes[i].left; links3[3 * i + 1] = tree.nodes[i].right; links3[3 * i + 2] = tree.nodes[i].tri;
+    }
+    if (rounds) *rounds = tree.rounds;
+    return tree.root;
+}
+
+size_t ref_scene_tris(const char* name, size_t count, uint64_t seed, float* tris9) {
+    const std::string s(name);
+    std::vector<raw::Tri> t;
+    if (s == "dense_block") t = raw::rt::trianglesOf(raw::rt::denseBlock().geo);
+    else if (s == "retro_room") t = raw::rt::trianglesOf(raw::swr::retroRoom().geo);
+    else if (s == "iso_street") t = raw::rt::trianglesOf(raw::swr::isoStreet().geo);
+    else if (s == "clip_stress") t = raw::rt::trianglesOf(raw::swr::clipStress().geo);
+    else if (s == "soup") t = raw::rt::soup(count, seed);
+    else return 0;
+    if (tris9)
+        for (size_t i = 0; i < t.size(); ++i) {
+            const raw::Vec3 v[3] = {t[i].a, t[i].b, t[i].c};
+            for (int k = 0; k < 3; ++k) { tris9[9 * i + 3 * size_t(k)] = v[k].x; tris9[9 * i + 3 * size_t(k) + 1] = v[k].y; tris9[9 * i + 3 * size_t(k) + 2] = v[k].z; }
+        }
+    return t.size();
+}
+
+double ref_binned_sah_cost(const float* tris9, size_t ntri, size_t* node_count) {
+    std::vector<raw::Tri> t(ntri);
+    for (size_t i = 0; i < ntri; ++i)
+        t[i] = {{tris9[9 * i], tris9[9 * i + 1], tris9[9 * i + 2]}, {tris9[9 * i + 3], tris9[9 * i + 4], tris9[9 * i + 5]},
+                {tris9[9 * i + 6], tris9[9 * i + 7], tris9[9 * i + 8]}};
+    return raw::rt::binnedSahCost(t, node_count);
+}
+
+}  // extern "C"

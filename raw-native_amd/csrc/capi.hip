@@ -1,0 +1,406 @@
+// The C ABI of include/raw_ao_hip.h over the gfx950 kernels.
+#include "../../include/raw_ao_hip.h"
+#include "ctx.hpp"
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+struct raw_ao_ctx {
+    raw_ao::Ctx c;
+};
+
+namespace raw_ao {
+
+bool hip_ok(Ctx& c, hipError_t e, const char* what) {
+    if (e == hipSuccess) return true;
+    if (c.err.empty()) c.err = std::string(what) + ": " + hipGetErrorString(e);
+    return false;
+}
+bool ensure(Ctx& c, DevBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b.cap >= bytes) return true;
+    if (b.p) { (void)hipFree(b.p); b.p = nullptr; b.cap = 0; }
+    void* p = nullptr;
+    if (!hip_ok(c, hipMalloc(&p, bytes), "hipMalloc")) return false;
+    b.p = p;
+    b.cap = bytes;
+    return true;
+}
+int trig_table_upload(Ctx& c) {
+    if (c.trig_ready) return 0;
+    std::vector<uint8_t> t;
+    if (!build_trig_table(t, c.err)) return RAW_AO_ERR_DEVICE;
+    if (!hip_ok(c, hipMalloc(reinterpret_cast<void**>(&c.d_trig), t.size()), "hipMalloc trig") ||
+        !hip_ok(c, hipMemcpy(c.d_trig, t.data(), t.size(), hipMemcpyHostToDevice), "trig upload"))
+        return RAW_AO_ERR_DEVICE;
+    c.trig_ready = true;
+    return 0;
+}
+
+}  // namespace raw_ao
+
+using raw_ao::Ctx;
+using raw_ao::ensure;
+using raw_ao::hip_ok;
+
+namespace {
+// The first error stands until a call succeeds.
+int fail(Ctx& c, int code, const char* msg) {
+    if (c.err.empty()) c.err = msg;
+    return code;
+}
+int code_of(int internal) {
+    if (internal == 0) return RAW_AO_OK;
+    if (internal == -1) return RAW_AO_ERR_ARG;
+    if (internal == -3) return RAW_AO_ERR_BUILD;
+    return RAW_AO_ERR_DEVICE;
+}
+hipStream_t pick(Ctx& c, void* s) { return s ? static_cast<hipStream_t>(s) : c.stream; }
+bool begin(Ctx& c) {
+    c.err.clear();
+    return hip_ok(c, hipSetDevice(c.device), "hipSetDevice");
+}
+bool image_ok(int w, int h) { return w > 0 && h > 0 && int64_t(w) * int64_t(h) <= (int64_t(1) << 30); }
+float elapsed(Ctx& c, int a, int b) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, c.ev[a], c.ev[b]) != hipSuccess) return 0.0f;
+    return ms;
+}
+}  // namespace
+
+extern "C" {
+
+uint32_t raw_ao_abi_version(void) { return RAW_AO_ABI_VERSION; }
+
+int raw_ao_open(int device_ordinal, raw_ao_ctx** out) {
+    if (!out) return RAW_AO_ERR_ARG;
+    raw_ao_ctx* x = new (std::nothrow) raw_ao_ctx;
+    if (!x) return RAW_AO_ERR_ARG;
+    *out = x;
+    Ctx& c = x->c;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(c, RAW_AO_ERR_NO_DEVICE, "no GPU device");
+    if (device_ordinal < 0 || device_ordinal >= count) return fail(c, RAW_AO_ERR_ARG, "device ordinal out of range");
+    c.device = device_ordinal;
+    if (!hip_ok(c, hipSetDevice(c.device), "hipSetDevice") || !hip_ok(c, hipStreamCreate(&c.stream), "hipStreamCreate"))
+        return RAW_AO_ERR_DEVICE;
+    for (auto& e : c.ev)
+        if (!hip_ok(c, hipEventCreate(&e), "hipEventCreate")) return RAW_AO_ERR_DEVICE;
+    return RAW_AO_OK;
+}
+
+void raw_ao_close(raw_ao_ctx* x) {
+    if (!x) return;
+    Ctx& c = x->c;
+    if (c.stream) {
+        (void)hipSetDevice(c.device);
+        (void)hipStreamSynchronize(c.stream);
+        raw_ao::DevBuf* bufs[] = {&c.tris_in, &c.keys, &c.vals, &c.keys2, &c.vals2, &c.hist, &c.boxes, &c.links,
+                                  &c.clus_a, &c.clus_b, &c.nn, &c.flags, &c.blk, &c.state, &c.g4, &c.bounds_part,
+                                  &c.nodes, &c.tris_sorted, &c.st_pos, &c.st_nrm, &c.st_mask, &c.st_out, &c.st_out2,
+                                  &c.st_cnt, &c.st_rays, &c.st_max, &c.counters};
+        for (auto* b : bufs)
+            if (b->p) (void)hipFree(b->p);
+        if (c.d_trig) (void)hipFree(c.d_trig);
+        if (c.h_state) (void)hipHostFree(c.h_state);
+        for (auto& e : c.ev)
+            if (e) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(c.stream);
+    }
+    delete x;
+}
+
+const char* raw_ao_last_error(const raw_ao_ctx* x) { return x ? x->c.err.c_str() : "null context"; }
+
+int raw_ao_set_option(raw_ao_ctx* x, int option, int value) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    c.err.clear();
+    if (option == RAW_AO_OPT_HOST_LIBM_TRIG) { c.host_trig = value != 0; return RAW_AO_OK; }
+    if (option == RAW_AO_OPT_TRACE_KERNEL) { c.trace_kernel = value; return RAW_AO_OK; }
+    return fail(c, RAW_AO_ERR_ARG, "unknown option");
+}
+
+int raw_ao_build_ploc_dev(raw_ao_ctx* x, const float* d_tris9, uint32_t ntri, int radius, void* stream) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    if (ntri && !d_tris9) return fail(c, RAW_AO_ERR_ARG, "null triangles");
+    if (ntri > (1u << 30)) return fail(c, RAW_AO_ERR_ARG, "too many triangles");
+    hipStream_t s = pick(c, stream);
+    if (ntri) {
+        if (!ensure(c, c.tris_in, 36 * size_t(ntri))) return RAW_AO_ERR_DEVICE;
+        if (d_tris9 != c.tris_in.p &&
+            !hip_ok(c, hipMemcpyAsync(c.tris_in.p, d_tris9, 36 * size_t(ntri), hipMemcpyDeviceToDevice, s), "tri copy"))
+            return RAW_AO_ERR_DEVICE;
+    }
+    (void)hipEventRecord(c.ev[0], s);
+    int r = raw_ao::ploc_build(c, c.tris_in.as<float>(), ntri, radius, s);
+    if (r == 0) r = raw_ao::bvh_layout(c, s);
+    (void)hipEventRecord(c.ev[1], s);
+    if (r != 0) return code_of(r);
+    if (!hip_ok(c, hipStreamSynchronize(s), "build")) return RAW_AO_ERR_DEVICE;
+    c.ms_build = elapsed(c, 0, 1);
+    return RAW_AO_OK;
+}
+
+int raw_ao_build_ploc(raw_ao_ctx* x, const float* tris9, uint32_t ntri, int radius) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    if (ntri && !tris9) return fail(c, RAW_AO_ERR_ARG, "null triangles");
+    if (ntri > (1u << 30)) return fail(c, RAW_AO_ERR_ARG, "too many triangles");
+    if (ntri) {
+        if (!ensure(c, c.tris_in, 36 * size_t(ntri)) ||
+            !hip_ok(c, hipMemcpyAsync(c.tris_in.p, tris9, 36 * size_t(ntri), hipMemcpyHostToDevice, c.stream), "tri upload"))
+            return RAW_AO_ERR_DEVICE;
+    }
+    return raw_ao_build_ploc_dev(x, ntri ? c.tris_in.as<float>() : nullptr, ntri, radius, nullptr);
+}
+
+int raw_ao_tree_info(raw_ao_ctx* x, uint32_t* nnodes, int32_t* root, uint32_t* rounds) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    c.err.clear();
+    if (nnodes) *nnodes = c.ntri ? 2 * c.ntri - 1 : 0;
+    if (root) *root = c.root;
+    if (rounds) *rounds = c.rounds;
+    return RAW_AO_OK;
+}
+
+int raw_ao_read_tree(raw_ao_ctx* x, float* boxes6, int32_t* links3) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    if (!c.ntri) return RAW_AO_OK;
+    if (!boxes6 || !links3) return fail(c, RAW_AO_ERR_ARG, "null output");
+    const size_t nn = 2 * size_t(c.ntri) - 1;
+    if (!hip_ok(c, hipMemcpy(boxes6, c.boxes.p, 24 * nn, hipMemcpyDeviceToHost), "tree read") ||
+        !hip_ok(c, hipMemcpy(links3, c.links.p, 12 * nn, hipMemcpyDeviceToHost), "tree read"))
+        return RAW_AO_ERR_DEVICE;
+    return RAW_AO_OK;
+}
+
+int raw_ao_read_keys(raw_ao_ctx* x, uint32_t* keys, uint32_t* order) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    if (!c.ntri) return RAW_AO_OK;
+    if (!keys || !order) return fail(c, RAW_AO_ERR_ARG, "null output");
+    // keys are kept sorted; recompute the per-triangle key order on the host side of the copy
+    std::vector<uint32_t> sk(c.ntri);
+    if (!hip_ok(c, hipMemcpy(sk.data(), c.keys.p, 4 * size_t(c.ntri), hipMemcpyDeviceToHost), "keys read") ||
+        !hip_ok(c, hipMemcpy(order, c.vals.p, 4 * size_t(c.ntri), hipMemcpyDeviceToHost), "order read"))
+        return RAW_AO_ERR_DEVICE;
+    for (uint32_t p = 0; p < c.ntri; ++p) {
+        if (order[p] >= c.ntri) return fail(c, RAW_AO_ERR_DEVICE, "sorted order out of range");
+        keys[order[p]] = sk[p];
+    }
+    return RAW_AO_OK;
+}
+
+int raw_ao_trace_ao_dev(raw_ao_ctx* x, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int spp,
+                        float radius, float* d_ao, void* stream) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    if (!image_ok(w, h) || !d_pos || !d_nrm || !d_mask || !d_ao) return fail(c, RAW_AO_ERR_ARG, "bad image arguments");
+    if (spp < 1 || spp > 4096) return fail(c, RAW_AO_ERR_ARG, "spp must be in 1..4096");
+    if (!(radius > 0.0f)) return fail(c, RAW_AO_ERR_ARG, "radius must be > 0");
+    if (c.ntri && c.root < 0) return fail(c, RAW_AO_ERR_ARG, "no BVH: call raw_ao_build_ploc first");
+    if (c.host_trig) {
+        const int r = raw_ao::trig_table_upload(c);
+        if (r) return r;
+    }
+    return code_of(raw_ao::trace_ao(c, d_pos, d_nrm, d_mask, w, h, spp, radius, d_ao, pick(c, stream)));
+}
+
+int raw_ao_trace_ao(raw_ao_ctx* x, const float* pos, const float* nrm, const uint8_t* mask, int w, int h, int spp,
+                    float radius, float* ao) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    if (!image_ok(w, h) || !pos || !nrm || !mask || !ao) return fail(c, RAW_AO_ERR_ARG, "bad image arguments");
+    const size_t np = size_t(w) * size_t(h);
+    if (!ensure(c, c.st_pos, 12 * np) || !ensure(c, c.st_nrm, 12 * np) || !ensure(c, c.st_mask, np) ||
+        !ensure(c, c.st_out, 4 * np))
+        return RAW_AO_ERR_DEVICE;
+    hipStream_t s = c.stream;
+    if (!hip_ok(c, hipMemcpyAsync(c.st_pos.p, pos, 12 * np, hipMemcpyHostToDevice, s), "upload") ||
+        !hip_ok(c, hipMemcpyAsync(c.st_nrm.p, nrm, 12 * np, hipMemcpyHostToDevice, s), "upload") ||
+        !hip_ok(c, hipMemcpyAsync(c.st_mask.p, mask, np, hipMemcpyHostToDevice, s), "upload"))
+        return RAW_AO_ERR_DEVICE;
+    const int r = raw_ao_trace_ao_dev(x, c.st_pos.as<float>(), c.st_nrm.as<float>(), c.st_mask.as<uint8_t>(), w, h, spp,
+                                      radius, c.st_out.as<float>(), nullptr);
+    if (r) return r;
+    if (!hip_ok(c, hipMemcpyAsync(ao, c.st_out.p, 4 * np, hipMemcpyDeviceToHost, s), "download") ||
+        !hip_ok(c, hipStreamSynchronize(s), "trace"))
+        return RAW_AO_ERR_DEVICE;
+    c.ms_trace = elapsed(c, 2, 3);
+    if (c.have_counts) {
+        if (!hip_ok(c, hipMemcpy(&c.last_counts, c.counters.p, sizeof c.last_counts, hipMemcpyDeviceToHost), "counters"))
+            return RAW_AO_ERR_DEVICE;
+    }
+    raw_ao::TraceCounters tc{};
+    if (!hip_ok(c, hipMemcpy(&tc, c.counters.p, sizeof tc, hipMemcpyDeviceToHost), "counters")) return RAW_AO_ERR_DEVICE;
+    c.last_counts.overflows = tc.overflows;
+    return RAW_AO_OK;
+}
+
+int raw_ao_ssao_dev(raw_ao_ctx* x, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int samples,
+                    float radius, float* d_ao, void* stream) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    if (!image_ok(w, h) || !d_pos || !d_nrm || !d_mask || !d_ao) return fail(c, RAW_AO_ERR_ARG, "bad image arguments");
+    if (samples < 0 || samples > 4096) return fail(c, RAW_AO_ERR_ARG, "samples must be in 0..4096");
+    if (c.host_trig) {
+        const int r = raw_ao::trig_table_upload(c);
+        if (r) return r;
+    }
+    return code_of(raw_ao::ssao(c, d_pos, d_nrm, d_mask, w, h, samples, radius, d_ao, pick(c, stream)));
+}
+
+int raw_ao_ssao(raw_ao_ctx* x, const float* pos, const float* nrm, const uint8_t* mask, int w, int h, int samples, float radius,
+                float* ao) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    if (!image_ok(w, h) || !pos || !nrm || !mask || !ao) return fail(c, RAW_AO_ERR_ARG, "bad image arguments");
+    const size_t np = size_t(w) * size_t(h);
+    if (!ensure(c, c.st_pos, 12 * np) || !ensure(c, c.st_nrm, 12 * np) || !ensure(c, c.st_mask, np) ||
+        !ensure(c, c.st_out, 4 * np))
+        return RAW_AO_ERR_DEVICE;
+    hipStream_t s = c.stream;
+    if (!hip_ok(c, hipMemcpyAsync(c.st_pos.p, pos, 12 * np, hipMemcpyHostToDevice, s), "upload") ||
+        !hip_ok(c, hipMemcpyAsync(c.st_nrm.p, nrm, 12 * np, hipMemcpyHostToDevice, s), "upload") ||
+        !hip_ok(c, hipMemcpyAsync(c.st_mask.p, mask, np, hipMemcpyHostToDevice, s), "upload"))
+        return RAW_AO_ERR_DEVICE;
+    const int r = raw_ao_ssao_dev(x, c.st_pos.as<float>(), c.st_nrm.as<float>(), c.st_mask.as<uint8_t>(), w, h, samples, radius,
+                                  c.st_out.as<float>(), nullptr);
+    if (r) return r;
+    if (!hip_ok(c, hipMemcpyAsync(ao, c.st_out.p, 4 * np, hipMemcpyDeviceToHost, s), "download") ||
+        !hip_ok(c, hipStreamSynchronize(s), "ssao"))
+        return RAW_AO_ERR_DEVICE;
+    c.ms_ssao = elapsed(c, 4, 5);
+    return RAW_AO_OK;
+}
+
+int raw_ao_error_map_dev(raw_ao_ctx* x, const float* d_a, const float* d_t, const uint8_t* d_mask, int w, int h, float* d_err,
+                         float* d_max, void* stream) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    if (!image_ok(w, h) || !d_a || !d_t || !d_mask || !d_err || !d_max) return fail(c, RAW_AO_ERR_ARG, "bad image arguments");
+    return code_of(raw_ao::error_map(c, d_a, d_t, d_mask, w, h, d_err, d_max, pick(c, stream)));
+}
+
+int raw_ao_error_map(raw_ao_ctx* x, const float* a, const float* t, const uint8_t* mask, int w, int h, float* err,
+                     float* max_error) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    if (!image_ok(w, h) || !a || !t || !mask || !err || !max_error) return fail(c, RAW_AO_ERR_ARG, "bad image arguments");
+    const size_t np = size_t(w) * size_t(h);
+    if (!ensure(c, c.st_pos, 4 * np) || !ensure(c, c.st_nrm, 4 * np) || !ensure(c, c.st_mask, np) ||
+        !ensure(c, c.st_out, 4 * np) || !ensure(c, c.st_max, 16))
+        return RAW_AO_ERR_DEVICE;
+    hipStream_t s = c.stream;
+    if (!hip_ok(c, hipMemcpyAsync(c.st_pos.p, a, 4 * np, hipMemcpyHostToDevice, s), "upload") ||
+        !hip_ok(c, hipMemcpyAsync(c.st_nrm.p, t, 4 * np, hipMemcpyHostToDevice, s), "upload") ||
+        !hip_ok(c, hipMemcpyAsync(c.st_mask.p, mask, np, hipMemcpyHostToDevice, s), "upload"))
+        return RAW_AO_ERR_DEVICE;
+    const int r = raw_ao_error_map_dev(x, c.st_pos.as<float>(), c.st_nrm.as<float>(), c.st_mask.as<uint8_t>(), w, h,
+                                       c.st_out.as<float>(), c.st_max.as<float>(), nullptr);
+    if (r) return r;
+    if (!hip_ok(c, hipMemcpyAsync(err, c.st_out.p, 4 * np, hipMemcpyDeviceToHost, s), "download") ||
+        !hip_ok(c, hipMemcpyAsync(max_error, c.st_max.p, 4, hipMemcpyDeviceToHost, s), "download") ||
+        !hip_ok(c, hipStreamSynchronize(s), "error map"))
+        return RAW_AO_ERR_DEVICE;
+    c.ms_error = elapsed(c, 6, 7);
+    return RAW_AO_OK;
+}
+
+int raw_ao_trace_rays(raw_ao_ctx* x, const float* rays8, uint32_t nrays, int any, float far_scale, float* out4) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    if (nrays && (!rays8 || !out4)) return fail(c, RAW_AO_ERR_ARG, "null rays");
+    if (c.ntri && c.root < 0) return fail(c, RAW_AO_ERR_ARG, "no BVH: call raw_ao_build_ploc first");
+    if (!(far_scale > 0.0f)) return fail(c, RAW_AO_ERR_ARG, "far_scale must be > 0");
+    if (!nrays) return RAW_AO_OK;
+    if (!ensure(c, c.st_rays, 32 * size_t(nrays)) || !ensure(c, c.st_out2, 16 * size_t(nrays))) return RAW_AO_ERR_DEVICE;
+    hipStream_t s = c.stream;
+    if (!hip_ok(c, hipMemcpyAsync(c.st_rays.p, rays8, 32 * size_t(nrays), hipMemcpyHostToDevice, s), "upload"))
+        return RAW_AO_ERR_DEVICE;
+    const int r = raw_ao::trace_rays(c, c.st_rays.as<float>(), nrays, any, far_scale, c.st_out2.as<float>(), s);
+    if (r) return code_of(r);
+    if (!hip_ok(c, hipMemcpyAsync(out4, c.st_out2.p, 16 * size_t(nrays), hipMemcpyDeviceToHost, s), "download") ||
+        !hip_ok(c, hipStreamSynchronize(s), "trace_rays"))
+        return RAW_AO_ERR_DEVICE;
+    return RAW_AO_OK;
+}
+
+int raw_ao_count_next_trace(raw_ao_ctx* x, int enable) {
+    if (!x) return RAW_AO_ERR_ARG;
+    x->c.err.clear();
+    x->c.count_next = enable != 0;
+    return RAW_AO_OK;
+}
+
+int raw_ao_trace_counts(raw_ao_ctx* x, uint64_t* visits, uint64_t* tests, uint64_t* rays) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    c.err.clear();
+    if (!c.have_counts) return fail(c, RAW_AO_ERR_ARG, "no counting trace yet (raw_ao_count_next_trace)");
+    if (visits) *visits = c.last_counts.visits;
+    if (tests) *tests = c.last_counts.tests;
+    if (rays) *rays = c.last_counts.rays;
+    return RAW_AO_OK;
+}
+
+int raw_ao_last_ms(raw_ao_ctx* x, float* build_ms, float* trace_ms, float* ssao_ms, float* error_ms) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    if (!hip_ok(c, hipStreamSynchronize(c.stream), "sync")) return RAW_AO_ERR_DEVICE;
+    if (build_ms) *build_ms = c.ms_build;
+    if (trace_ms) *trace_ms = elapsed(c, 2, 3);
+    if (ssao_ms) *ssao_ms = elapsed(c, 4, 5);
+    if (error_ms) *error_ms = elapsed(c, 6, 7);
+    return RAW_AO_OK;
+}
+
+int raw_ao_angle_table(raw_ao_ctx* x, float* sin_out, float* cos_out) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    if (!sin_out || !cos_out) return fail(c, RAW_AO_ERR_ARG, "null output");
+    if (c.host_trig) {
+        const int r = raw_ao::trig_table_upload(c);
+        if (r) return r;
+    }
+    const size_t bytes = 4u << 24;
+    if (!ensure(c, c.st_out, bytes) || !ensure(c, c.st_out2, bytes)) return RAW_AO_ERR_DEVICE;
+    const int r = raw_ao::angle_table(c, c.st_out.as<float>(), c.st_out2.as<float>(), c.stream);
+    if (r) return code_of(r);
+    if (!hip_ok(c, hipMemcpyAsync(sin_out, c.st_out.p, bytes, hipMemcpyDeviceToHost, c.stream), "download") ||
+        !hip_ok(c, hipMemcpyAsync(cos_out, c.st_out2.p, bytes, hipMemcpyDeviceToHost, c.stream), "download") ||
+        !hip_ok(c, hipStreamSynchronize(c.stream), "angles"))
+        return RAW_AO_ERR_DEVICE;
+    return RAW_AO_OK;
+}
+
+int raw_ao_bvh_bytes(raw_ao_ctx* x, uint64_t* node_bytes, uint64_t* tri_bytes, uint32_t* node_record_bytes) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    c.err.clear();
+    const uint64_t n = c.ntri;
+    if (node_bytes) *node_bytes = n ? 64ull * (n > 1 ? n - 1 : 1) : 0;
+    if (tri_bytes) *tri_bytes = 48ull * n;
+    if (node_record_bytes) *node_record_bytes = 64;
+    return RAW_AO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+// The context behind the C ABI (include/raw_ao_hip.h) and the internal launch API
+// shared by the kernel translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace raw_ao {
+
+// A grow-only device allocation.
+struct DevBuf {
+    void* p{nullptr};
+    size_t cap{0};
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Device state of the PLOC rounds, double-buffered by round parity so a round's
+// kernels read one copy while the scan writes the next.
+struct PlocState {
+    uint32_t m;        // clusters this round
+    uint32_t next_id;  // id of the next internal node
+    uint32_t merges;   // merges made by the round that produced this state
+    uint32_t pad;
+};
+
+struct TraceCounters {
+    unsigned long long visits, tests, rays, overflows;
+};
+
+struct Ctx {
+    int device{0};
+    hipStream_t stream{nullptr};
+    std::string err;
+    bool host_trig{true};
+    int trace_kernel{0};
+    bool count_next{false};
+
+    // host-libm sin/cos correction table, one nibble per 24-bit angle index
+    uint8_t* d_trig{nullptr};
+    bool trig_ready{false};
+
+    // the scene
+    uint32_t ntri{0};
+    int radius{16};
+    int32_t root{-1};
+    uint32_t rounds{0};
+    DevBuf tris_in;              // n x 9 floats, input order
+    DevBuf keys, vals, keys2, vals2, hist;
+    DevBuf boxes, links;         // (2n-1) x 6 floats, (2n-1) x 3 ints: the reference's node order
+    DevBuf clus_a, clus_b, nn, flags, blk;
+    DevBuf state;                // PlocState[2]
+    PlocState* h_state{nullptr}; // pinned mirror of the state read back each round
+    DevBuf g4;                   // centroid box lo xyz and scale
+    DevBuf bounds_part;
+    // traversal form
+    DevBuf nodes;                // BVH2 records, 64 B each
+    DevBuf tris_sorted;          // 48 B each, sorted (leaf) order
+    uint32_t trav_root{0};
+
+    // staging for host-pointer entry points
+    DevBuf st_pos, st_nrm, st_mask, st_out, st_out2, st_cnt, st_rays, st_max;
+    DevBuf counters;             // TraceCounters
+    TraceCounters last_counts{};
+    bool have_counts{false};
+
+    hipEvent_t ev[8]{};
+    float ms_build{0}, ms_trace{0}, ms_ssao{0}, ms_error{0};
+};
+
+bool ensure(Ctx& c, DevBuf& b, size_t bytes);
+bool hip_ok(Ctx& c, hipError_t e, const char* what);
+
+// Kernel-side entry points (each in its own translation unit).
+int ploc_build(Ctx& c, const float* d_tris, uint32_t n, int radius, hipStream_t s);
+int bvh_layout(Ctx& c, hipStream_t s);
+int trig_table_upload(Ctx& c);
+int trace_ao(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int spp,
+             float radius, float* d_ao, hipStream_t s);
+int angle_table(Ctx& c, float* d_sin, float* d_cos, hipStream_t s);
+int trace_rays(Ctx& c, const float* d_rays, uint32_t n, int any, float far_scale, float* d_out, hipStream_t s);
+int ssao(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int samples,
+         float radius, float* d_ao, hipStream_t s);
+int error_map(Ctx& c, const float* d_a, const float* d_t, const uint8_t* d_mask, int w, int h, float* d_err,
+              float* d_max, hipStream_t s);
+
+// Host-libm correction table (trig_table.cpp): 2^23 bytes, one nibble per index.
+bool build_trig_table(std::vector<uint8_t>& out, std::string& err);
+
+}  // namespace raw_ao

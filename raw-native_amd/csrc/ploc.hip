@@ -1,0 +1,393 @@
+// PLOC BVH build on gfx950: the tree of raw::rt::buildPloc (src/renderer/rt_bvh.cpp:54-114),
+// node for node and bit for bit.
+//
+//   bounds   centroid box (two-level reduction) and the power-of-two scale
+//   morton   30-bit keys, one thread a triangle
+//   sort     stable LSD radix sort of (key, index), 8-bit digits, LDS-staged ranks from
+//            wave64 ballots; stability with ascending indices gives std::sort's order
+//            of (key, index) pairs (rt_bvh.cpp:73)
+//   leaves   one leaf a sorted position
+//   rounds   per round three launches: nearest neighbours within the radius from an
+//            LDS window with a halo (each block recomputes its halo's neighbours, so
+//            mutual pairs are decided without a second pass), a single-block scan of the
+//            blocks' merge and survivor counts, and the merge/compaction with in-block
+//            ranks from ballots. Ids and positions come from prefix sums: the tree does
+//            not depend on scheduling.
+#include "ao_math.hpp"
+#include "ctx.hpp"
+#include <cfloat>
+
+namespace raw_ao {
+namespace {
+
+constexpr int BT = 256;          // threads a block
+constexpr int RS_ITEMS = 8;      // keys a thread in the radix passes
+constexpr int RS_TILE = BT * RS_ITEMS;
+constexpr int MAX_R = 32;        // largest PLOC radius supported
+
+__device__ __forceinline__ unsigned long long lanemask_lt() {
+    const unsigned lane = __lane_id();
+    return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+__device__ __forceinline__ uint32_t spread(uint32_t v) {   // rt_bvh.cpp:25-31
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__device__ __forceinline__ V3 centroid(const float* __restrict__ t) {   // rt_bvh.cpp:34-37
+    const float third = 0.333333343f;
+    return v3((t[0] + t[3] + t[6]) * third, (t[1] + t[4] + t[7]) * third, (t[2] + t[5] + t[8]) * third);
+}
+__device__ __forceinline__ float area6(float lx, float ly, float lz, float hx, float hy, float hz) {   // rt_bvh.cpp:46-52
+    const float dx = hx - lx, dy = hy - ly, dz = hz - lz;
+    return 2.0f * (dy * (dx + dz) + dz * dx);
+}
+
+// ---- centroid bounds --------------------------------------------------------------
+// The reference folds min/max in index order; only the sign of a zero bound could
+// depend on the order, and neither the keys nor the scale read it, so a parallel
+// reduction gives the same keys.
+__global__ __launch_bounds__(BT) void k_bounds_part(const float* __restrict__ tris, uint32_t n, float* __restrict__ part) {
+    float l[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, h[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (uint32_t i = blockIdx.x * BT + threadIdx.x; i < n; i += gridDim.x * BT) {
+        const V3 c = centroid(tris + 9 * size_t(i));
+        const float x[3] = {c.x, c.y, c.z};
+        for (int k = 0; k < 3; ++k) { l[k] = stdmin(l[k], x[k]); h[k] = stdmax(h[k], x[k]); }
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int k = 0; k < 3; ++k) {
+            l[k] = stdmin(l[k], __shfl_xor(l[k], off));
+            h[k] = stdmax(h[k], __shfl_xor(h[k], off));
+        }
+    __shared__ float s[BT / 64][6];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 3; ++k) { s[w][k] = l[k]; s[w][3 + k] = h[k]; }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        float v = s[0][threadIdx.x];
+        for (int i = 1; i < BT / 64; ++i) v = threadIdx.x < 3 ? stdmin(v, s[i][threadIdx.x]) : stdmax(v, s[i][threadIdx.x]);
+        part[6 * blockIdx.x + threadIdx.x] = v;
+    }
+}
+__global__ void k_bounds_final(const float* __restrict__ part, int nparts, float* __restrict__ g4) {
+    if (threadIdx.x != 0) return;
+    float l[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, h[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int p = 0; p < nparts; ++p)
+        for (int k = 0; k < 3; ++k) { l[k] = stdmin(l[k], part[6 * p + k]); h[k] = stdmax(h[k], part[6 * p + 3 + k]); }
+    const float extent = stdmax(h[0] - l[0], stdmax(h[1] - l[1], h[2] - l[2]));
+    float s = 1.0f;   // pow2Scale (rt_bvh.cpp:38-44)
+    if (extent > 0.0f) {
+        while (extent * s > 1.0f) s *= 0.5f;
+        while (extent * s <= 0.5f) s *= 2.0f;
+    }
+    g4[0] = l[0]; g4[1] = l[1]; g4[2] = l[2]; g4[3] = s;
+}
+
+__global__ __launch_bounds__(BT) void k_morton(const float* __restrict__ tris, uint32_t n, const float* __restrict__ g4,
+                                               uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint32_t i = blockIdx.x * BT + threadIdx.x;
+    if (i >= n) return;
+    const V3 c = centroid(tris + 9 * size_t(i));
+    const float s = g4[3];
+    const uint32_t qx = min(1023u, uint32_t((c.x - g4[0]) * s * 1023.0f));
+    const uint32_t qy = min(1023u, uint32_t((c.y - g4[1]) * s * 1023.0f));
+    const uint32_t qz = min(1023u, uint32_t((c.z - g4[2]) * s * 1023.0f));
+    keys[i] = (spread(qx) << 2) | (spread(qy) << 1) | spread(qz);
+    vals[i] = i;
+}
+
+// ---- stable LSD radix sort ----------------------------------------------------------
+__global__ __launch_bounds__(BT) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n, int shift,
+                                                   uint32_t* __restrict__ hist, uint32_t nblk) {
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * RS_TILE;
+    for (int it = 0; it < RS_ITEMS; ++it) {
+        const uint32_t i = base + it * BT + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Exclusive scan of `count` words in place, one block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_scan_excl(uint32_t* __restrict__ data, uint32_t count, uint32_t* __restrict__ total) {
+    __shared__ uint32_t s[1024];
+    const uint32_t per = (count + 1023) / 1024;
+    const uint32_t a = threadIdx.x * per, b = min(count, a + per);
+    uint32_t sum = 0;
+    for (uint32_t i = a; i < b; ++i) sum += data[i];
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = threadIdx.x >= uint32_t(off) ? s[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = s[threadIdx.x] - sum;
+    for (uint32_t i = a; i < b; ++i) { const uint32_t v = data[i]; data[i] = run; run += v; }
+    if (total && threadIdx.x == 1023) *total = s[1023];
+}
+
+__global__ __launch_bounds__(BT) void k_radix_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                      uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n,
+                                                      int shift, const uint32_t* __restrict__ offs, uint32_t nblk) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[BT / 64][256];
+    const int w = threadIdx.x >> 6;
+    run[threadIdx.x] = offs[threadIdx.x * nblk + blockIdx.x];
+    const unsigned long long lt = lanemask_lt();
+    for (int it = 0; it < RS_ITEMS; ++it) {
+        for (int k = 0; k < BT / 64; ++k) wcnt[k][threadIdx.x] = 0;
+        __syncthreads();
+        const uint32_t i = blockIdx.x * RS_TILE + it * BT + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t key = valid ? kin[i] : 0u;
+        const uint32_t d = (key >> shift) & 255u;
+        unsigned long long same = __ballot(valid);
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const unsigned long long bm = __ballot(bit);
+            same &= bit ? bm : ~bm;
+        }
+        const uint32_t rank = __popcll(same & lt);
+        if (valid && rank == 0) wcnt[w][d] = __popcll(same);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[d] + rank;
+            for (int k = 0; k < w; ++k) pos += wcnt[k][d];
+            kout[pos] = key;
+            vout[pos] = vin[i];
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (int k = 0; k < BT / 64; ++k) add += wcnt[k][threadIdx.x];
+        run[threadIdx.x] += add;
+        __syncthreads();
+    }
+}
+
+// ---- leaves (rt_bvh.cpp:74-82) ----------------------------------------------------
+__global__ __launch_bounds__(BT) void k_leaves(const float* __restrict__ tris, const uint32_t* __restrict__ order, uint32_t n,
+                                               float* __restrict__ boxes, int32_t* __restrict__ links,
+                                               uint32_t* __restrict__ clus) {
+    const uint32_t p = blockIdx.x * BT + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t t = order[p];
+    const float* v = tris + 9 * size_t(t);
+    float* b = boxes + 6 * size_t(p);
+    for (int a = 0; a < 3; ++a) {
+        float l = v[a], h = v[a];
+        l = stdmin(l, v[3 + a]); h = stdmax(h, v[3 + a]);
+        l = stdmin(l, v[6 + a]); h = stdmax(h, v[6 + a]);
+        b[a] = l;
+        b[3 + a] = h;
+    }
+    links[3 * size_t(p)] = -1;
+    links[3 * size_t(p) + 1] = -1;
+    links[3 * size_t(p) + 2] = int32_t(t);
+    clus[p] = p;
+}
+
+// ---- PLOC rounds (rt_bvh.cpp:84-111) ------------------------------------------------
+// Nearest neighbour of every position of this block and of a halo of `radius` on each
+// side, then the merge/survive flags of this block's positions and its two counts.
+__global__ __launch_bounds__(BT) void k_ploc_nn(const PlocState* __restrict__ st, int radius, const float* __restrict__ boxes,
+                                                const uint32_t* __restrict__ clus, uint32_t* __restrict__ nn,
+                                                uint8_t* __restrict__ flags, uint32_t* __restrict__ counts, uint32_t nblk) {
+    __shared__ float sb[6][BT + 4 * MAX_R];
+    __shared__ int32_t snn[BT + 2 * MAX_R];
+    __shared__ uint32_t wsum[BT / 64][2];
+    const int32_t m = int32_t(st->m);
+    const int32_t b0 = int32_t(blockIdx.x) * BT;
+    const int32_t base = b0 - 2 * radius;          // position of sb[*][0]
+    const int span = BT + 4 * radius;
+    for (int k = threadIdx.x; k < span; k += BT) {
+        const int32_t pos = base + k;
+        if (pos >= 0 && pos < m) {
+            const float* b = boxes + 6 * size_t(clus[pos]);
+            for (int a = 0; a < 6; ++a) sb[a][k] = b[a];
+        }
+    }
+    __syncthreads();
+    const int nspan = BT + 2 * radius;             // neighbours of positions b0-radius ..
+    for (int q = threadIdx.x; q < nspan; q += BT) {
+        const int32_t pos = b0 - radius + q;
+        if (pos < 0 || pos >= m) continue;
+        const int ip = pos - base;
+        const float al0 = sb[0][ip], al1 = sb[1][ip], al2 = sb[2][ip], ah0 = sb[3][ip], ah1 = sb[4][ip], ah2 = sb[5][ip];
+        float best = FLT_MAX;
+        int32_t bj = pos;
+        const int32_t j0 = max(0, pos - radius), j1 = min(m - 1, pos + radius);
+        for (int32_t j = j0; j <= j1; ++j) {
+            if (j == pos) continue;
+            const int jp = j - base;
+            const float d = area6(stdmin(al0, sb[0][jp]), stdmin(al1, sb[1][jp]), stdmin(al2, sb[2][jp]),
+                                  stdmax(ah0, sb[3][jp]), stdmax(ah1, sb[4][jp]), stdmax(ah2, sb[5][jp]));
+            if (d < best) { best = d; bj = j; }
+        }
+        snn[q] = bj;
+    }
+    __syncthreads();
+    const int32_t pos = b0 + int32_t(threadIdx.x);
+    bool merge = false, survive = false;
+    if (pos < m) {
+        const int32_t j = snn[threadIdx.x + radius];
+        const bool mutual = j != pos && snn[j - (b0 - radius)] == pos;
+        merge = mutual && pos < j;
+        survive = !(mutual && pos > j);
+        nn[pos] = uint32_t(j);
+        flags[pos] = uint8_t((merge ? 1 : 0) | (survive ? 2 : 0));
+    }
+    const unsigned long long bm = __ballot(merge), bs = __ballot(survive);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { wsum[w][0] = __popcll(bm); wsum[w][1] = __popcll(bs); }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint32_t s = 0;
+        for (int k = 0; k < BT / 64; ++k) s += wsum[k][threadIdx.x];
+        counts[threadIdx.x * nblk + blockIdx.x] = s;   // merges then survivors, block-major
+    }
+}
+
+// Exclusive scans of the blocks' merge and survivor counts and the next round's state.
+__global__ __launch_bounds__(1024) void k_ploc_scan(const PlocState* __restrict__ in, PlocState* __restrict__ out,
+                                                    uint32_t* __restrict__ counts, uint32_t nblk) {
+    __shared__ uint32_t s[2][1024];
+    const uint32_t per = (nblk + 1023) / 1024;
+    const uint32_t a = threadIdx.x * per, b = min(nblk, a + per);
+    uint32_t sum[2] = {0, 0};
+    for (int k = 0; k < 2; ++k)
+        for (uint32_t i = a; i < b; ++i) sum[k] += counts[k * nblk + i];
+    s[0][threadIdx.x] = sum[0];
+    s[1][threadIdx.x] = sum[1];
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t v0 = threadIdx.x >= uint32_t(off) ? s[0][threadIdx.x - off] : 0u;
+        const uint32_t v1 = threadIdx.x >= uint32_t(off) ? s[1][threadIdx.x - off] : 0u;
+        __syncthreads();
+        s[0][threadIdx.x] += v0;
+        s[1][threadIdx.x] += v1;
+        __syncthreads();
+    }
+    for (int k = 0; k < 2; ++k) {
+        uint32_t run = s[k][threadIdx.x] - sum[k];
+        for (uint32_t i = a; i < b; ++i) { const uint32_t v = counts[k * nblk + i]; counts[k * nblk + i] = run; run += v; }
+    }
+    if (threadIdx.x == 1023) {
+        PlocState o;
+        o.m = s[1][1023];
+        o.merges = s[0][1023];
+        o.next_id = in->next_id + o.merges;
+        o.pad = 0;
+        *out = o;
+    }
+}
+
+// New nodes from mutual pairs (id = next_id + merge rank, in position order) and the
+// survivors compacted in position order.
+__global__ __launch_bounds__(BT) void k_ploc_merge(const PlocState* __restrict__ st, const uint32_t* __restrict__ nn,
+                                                   const uint8_t* __restrict__ flags, const uint32_t* __restrict__ offs,
+                                                   uint32_t nblk, const uint32_t* __restrict__ cin, uint32_t* __restrict__ cout,
+                                                   float* __restrict__ boxes, int32_t* __restrict__ links) {
+    __shared__ uint32_t wsum[BT / 64][2];
+    const uint32_t m = st->m;
+    const uint32_t pos = blockIdx.x * BT + threadIdx.x;
+    const uint8_t f = pos < m ? flags[pos] : uint8_t(0);
+    const bool merge = f & 1u, survive = (f >> 1) & 1u;
+    const unsigned long long lt = lanemask_lt();
+    const unsigned long long bm = __ballot(merge), bs = __ballot(survive);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { wsum[w][0] = __popcll(bm); wsum[w][1] = __popcll(bs); }
+    __syncthreads();
+    if (!survive) return;
+    uint32_t S = offs[blockIdx.x] + __popcll(bm & lt), SV = offs[nblk + blockIdx.x] + __popcll(bs & lt);
+    for (int k = 0; k < w; ++k) { S += wsum[k][0]; SV += wsum[k][1]; }
+    const uint32_t a = cin[pos];
+    if (!merge) { cout[SV] = a; return; }
+    const uint32_t id = st->next_id + S;
+    const uint32_t b = cin[nn[pos]];
+    const float* A = boxes + 6 * size_t(a);
+    const float* B = boxes + 6 * size_t(b);
+    float* N = boxes + 6 * size_t(id);
+    for (int k = 0; k < 3; ++k) { N[k] = stdmin(A[k], B[k]); N[3 + k] = stdmax(A[3 + k], B[3 + k]); }
+    links[3 * size_t(id)] = int32_t(a);
+    links[3 * size_t(id) + 1] = int32_t(b);
+    links[3 * size_t(id) + 2] = -1;
+    cout[SV] = id;
+}
+
+uint32_t cdiv(uint64_t a, uint32_t b) { return uint32_t((a + b - 1) / b); }
+
+}  // namespace
+
+int ploc_build(Ctx& c, const float* d_tris, uint32_t n, int radius, hipStream_t s) {
+    c.ntri = n;
+    c.radius = radius;
+    c.root = -1;
+    c.rounds = 0;
+    if (radius < 1 || radius > MAX_R) { c.err = "PLOC radius must be in 1..32"; return -1; }
+    if (n == 0) return 0;
+    const size_t nn = 2 * size_t(n) - 1;
+    const uint32_t nblk_rs = cdiv(n, RS_TILE);
+    const uint32_t nblk = cdiv(n, BT);
+    if (!ensure(c, c.keys, 4 * size_t(n)) || !ensure(c, c.vals, 4 * size_t(n)) || !ensure(c, c.keys2, 4 * size_t(n)) ||
+        !ensure(c, c.vals2, 4 * size_t(n)) || !ensure(c, c.hist, 4 * 256 * size_t(nblk_rs)) ||
+        !ensure(c, c.boxes, 24 * nn) || !ensure(c, c.links, 12 * nn) || !ensure(c, c.clus_a, 4 * size_t(n)) ||
+        !ensure(c, c.clus_b, 4 * size_t(n)) || !ensure(c, c.nn, 4 * size_t(n)) || !ensure(c, c.flags, size_t(n)) ||
+        !ensure(c, c.blk, 8 * size_t(nblk)) || !ensure(c, c.state, 2 * sizeof(PlocState)) || !ensure(c, c.g4, 16) ||
+        !ensure(c, c.bounds_part, 6 * 4 * 1024))
+        return -2;
+    if (!c.h_state && !hip_ok(c, hipHostMalloc(reinterpret_cast<void**>(&c.h_state), sizeof(PlocState)), "hipHostMalloc"))
+        return -2;
+    const uint32_t nparts = std::min<uint32_t>(1024, nblk);
+    k_bounds_part<<<nparts, BT, 0, s>>>(d_tris, n, c.bounds_part.as<float>());
+    k_bounds_final<<<1, 64, 0, s>>>(c.bounds_part.as<float>(), int(nparts), c.g4.as<float>());
+    k_morton<<<nblk, BT, 0, s>>>(d_tris, n, c.g4.as<float>(), c.keys.as<uint32_t>(), c.vals.as<uint32_t>());
+    uint32_t *ki = c.keys.as<uint32_t>(), *vi = c.vals.as<uint32_t>(), *ko = c.keys2.as<uint32_t>(), *vo = c.vals2.as<uint32_t>();
+    for (int shift = 0; shift < 30; shift += 8) {
+        k_radix_hist<<<nblk_rs, BT, 0, s>>>(ki, n, shift, c.hist.as<uint32_t>(), nblk_rs);
+        k_scan_excl<<<1, 1024, 0, s>>>(c.hist.as<uint32_t>(), 256 * nblk_rs, nullptr);
+        k_radix_scatter<<<nblk_rs, BT, 0, s>>>(ki, vi, ko, vo, n, shift, c.hist.as<uint32_t>(), nblk_rs);
+        std::swap(ki, ko);
+        std::swap(vi, vo);
+    }
+    // four passes: the sorted pairs are back in keys/vals
+    k_leaves<<<nblk, BT, 0, s>>>(d_tris, vi, n, c.boxes.as<float>(), c.links.as<int32_t>(), c.clus_a.as<uint32_t>());
+    PlocState init{n, n, 0, 0};
+    if (!hip_ok(c, hipMemcpyAsync(c.state.p, &init, sizeof init, hipMemcpyHostToDevice, s), "state upload")) return -2;
+    PlocState* d_st = c.state.as<PlocState>();
+    uint32_t m = n;
+    int parity = 0;
+    uint32_t *cin = c.clus_a.as<uint32_t>(), *cout = c.clus_b.as<uint32_t>();
+    while (m > 1) {
+        const uint32_t nb = cdiv(m, BT);
+        k_ploc_nn<<<nb, BT, 0, s>>>(d_st + parity, radius, c.boxes.as<float>(), cin, c.nn.as<uint32_t>(),
+                                    c.flags.as<uint8_t>(), c.blk.as<uint32_t>(), nb);
+        k_ploc_scan<<<1, 1024, 0, s>>>(d_st + parity, d_st + (parity ^ 1), c.blk.as<uint32_t>(), nb);
+        k_ploc_merge<<<nb, BT, 0, s>>>(d_st + parity, c.nn.as<uint32_t>(), c.flags.as<uint8_t>(), c.blk.as<uint32_t>(), nb,
+                                       cin, cout, c.boxes.as<float>(), c.links.as<int32_t>());
+        if (!hip_ok(c, hipMemcpyAsync(c.h_state, d_st + (parity ^ 1), sizeof(PlocState), hipMemcpyDeviceToHost, s), "state read") ||
+            !hip_ok(c, hipStreamSynchronize(s), "PLOC round"))
+            return -2;
+        if (c.h_state->merges == 0) { c.err = "PLOC round made no merge (degenerate input)"; return -3; }
+        m = c.h_state->m;
+        parity ^= 1;
+        std::swap(cin, cout);
+        ++c.rounds;
+    }
+    uint32_t root = 0;
+    if (!hip_ok(c, hipMemcpyAsync(&root, cin, 4, hipMemcpyDeviceToHost, s), "root read") ||
+        !hip_ok(c, hipStreamSynchronize(s), "PLOC build"))
+        return -2;
+    c.root = int32_t(root);
+    return 0;
+}
+
+}  // namespace raw_ao

@@ -1,0 +1,226 @@
+"""Python host side of raw_ao_hip: a ctypes binding of the C ABI (include/raw_ao_hip.h)
+that mirrors the reference's AO/reconcile interface:
+
+    computeRTAO  (raw/renderer/ray_ao.hpp:6-8)       -> Context.compute_rtao
+    computeSSAO  (raw/renderer/ssao.hpp:5-6)         -> Context.compute_ssao
+    reconcile    (raw/cert/reconcile.hpp:12-13)      -> Context.reconcile
+    buildPlocGpu (raw/renderer/rt_gpu.hpp:19)        -> Context.build_ploc / read_tree
+    traceGpu     (raw/renderer/rt_gpu.hpp:27-28)     -> Context.trace_rays
+
+Arrays are numpy (host) or torch tensors on the GPU (the *_dev calls). Every call goes to
+the HIP library; there is no CPU fallback: without the built library or a GPU this
+module raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import host  # noqa: F401  (float64 reconcile sum, certificate, scenes, rasterizer)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "build", "libraw_ao_hip.so")
+
+OK, ERR_ARG, ERR_DEVICE, ERR_BUILD, ERR_NO_DEVICE = 0, -1, -2, -3, -4
+OPT_HOST_LIBM_TRIG, OPT_TRACE_KERNEL = 1, 2
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+_lib = None
+
+# (name, restype, argtypes) of every symbol include/raw_ao_hip.h declares
+SIGNATURES = [
+    ("raw_ao_abi_version", C.c_uint32, []),
+    ("raw_ao_open", C.c_int, [C.c_int, C.POINTER(_vp)]),
+    ("raw_ao_close", None, [_vp]),
+    ("raw_ao_last_error", C.c_char_p, [_vp]),
+    ("raw_ao_set_option", C.c_int, [_vp, C.c_int, C.c_int]),
+    ("raw_ao_build_ploc", C.c_int, [_vp, _f32p, C.c_uint32, C.c_int]),
+    ("raw_ao_build_ploc_dev", C.c_int, [_vp, _vp, C.c_uint32, C.c_int, _vp]),
+    ("raw_ao_tree_info", C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_int32), C.POINTER(C.c_uint32)]),
+    ("raw_ao_read_tree", C.c_int, [_vp, _f32p, _i32p]),
+    ("raw_ao_read_keys", C.c_int, [_vp, _u32p, _u32p]),
+    ("raw_ao_trace_ao", C.c_int, [_vp, _f32p, _f32p, _u8p, C.c_int, C.c_int, C.c_int, C.c_float, _f32p]),
+    ("raw_ao_trace_ao_dev", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp]),
+    ("raw_ao_ssao", C.c_int, [_vp, _f32p, _f32p, _u8p, C.c_int, C.c_int, C.c_int, C.c_float, _f32p]),
+    ("raw_ao_ssao_dev", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp]),
+    ("raw_ao_error_map", C.c_int, [_vp, _f32p, _f32p, _u8p, C.c_int, C.c_int, _f32p, C.POINTER(C.c_float)]),
+    ("raw_ao_error_map_dev", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp]),
+    ("raw_ao_trace_rays", C.c_int, [_vp, _f32p, C.c_uint32, C.c_int, C.c_float, _f32p]),
+    ("raw_ao_count_next_trace", C.c_int, [_vp, C.c_int]),
+    ("raw_ao_trace_counts", C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    ("raw_ao_last_ms", C.c_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                 C.POINTER(C.c_float)]),
+    ("raw_ao_angle_table", C.c_int, [_vp, _f32p, _f32p]),
+    ("raw_ao_bvh_bytes", C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+]
+
+
+class RawAoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("raw_ao_hip error %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    """The loaded HIP library. Raises when it is not built: there is no fallback."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("raw_ao_hip is not built (%s): run __graft_entry__.build()" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (contiguous, on the GPU)."""
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return C.c_void_p(t.data_ptr())
+
+
+class Context:
+    """One raw_ao_hip context (device memory, a HIP stream, the built BVH)."""
+
+    def __init__(self, device=0):
+        self._h = _vp()
+        L = lib()
+        rc = L.raw_ao_open(device, C.byref(self._h))
+        if rc != OK:
+            msg = L.raw_ao_last_error(self._h).decode() if self._h else "open failed"
+            self.close()
+            raise RawAoError(rc, msg)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().raw_ao_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != OK:
+            raise RawAoError(rc, lib().raw_ao_last_error(self._h).decode())
+
+    def set_option(self, opt, value):
+        self._check(lib().raw_ao_set_option(self._h, opt, int(value)))
+
+    # ---- scene ---------------------------------------------------------------------
+    def build_ploc(self, tris9, radius=16):
+        t = np.ascontiguousarray(tris9, np.float32).reshape(-1, 9)
+        self._check(lib().raw_ao_build_ploc(self._h, t, t.shape[0], radius))
+
+    def build_ploc_dev(self, tris9_tensor, radius=16, stream=None):
+        n = tris9_tensor.numel() // 9
+        self._check(lib().raw_ao_build_ploc_dev(self._h, _ptr(tris9_tensor), n, radius, _vp(stream)))
+
+    def tree_info(self):
+        nn, root, rounds = C.c_uint32(), C.c_int32(), C.c_uint32()
+        self._check(lib().raw_ao_tree_info(self._h, C.byref(nn), C.byref(root), C.byref(rounds)))
+        return nn.value, root.value, rounds.value
+
+    def read_tree(self):
+        nn, root, rounds = self.tree_info()
+        boxes = np.zeros((max(nn, 1), 6), np.float32)
+        links = np.zeros((max(nn, 1), 3), np.int32)
+        self._check(lib().raw_ao_read_tree(self._h, boxes, links))
+        return boxes[:nn], links[:nn], root, rounds
+
+    def read_keys(self, ntri):
+        keys = np.zeros(max(ntri, 1), np.uint32)
+        order = np.zeros(max(ntri, 1), np.uint32)
+        self._check(lib().raw_ao_read_keys(self._h, keys, order))
+        return keys[:ntri], order[:ntri]
+
+    # ---- AO ------------------------------------------------------------------------
+    def compute_rtao(self, position, normal, mask, samples=64, radius=2.0):
+        mask = np.ascontiguousarray(mask, np.uint8)
+        h, w = mask.shape
+        ao = np.zeros((h, w), np.float32)
+        self._check(lib().raw_ao_trace_ao(self._h, np.ascontiguousarray(position, np.float32),
+                                          np.ascontiguousarray(normal, np.float32), mask, w, h, samples, radius, ao))
+        return ao
+
+    def compute_rtao_dev(self, position, normal, mask, w, h, samples, radius, out, stream=None):
+        self._check(lib().raw_ao_trace_ao_dev(self._h, _ptr(position), _ptr(normal), _ptr(mask), w, h, samples, radius,
+                                              _ptr(out), _vp(stream)))
+
+    def compute_ssao(self, position, normal, mask, samples=24, radius=2.0):
+        mask = np.ascontiguousarray(mask, np.uint8)
+        h, w = mask.shape
+        ao = np.zeros((h, w), np.float32)
+        self._check(lib().raw_ao_ssao(self._h, np.ascontiguousarray(position, np.float32),
+                                      np.ascontiguousarray(normal, np.float32), mask, w, h, samples, radius, ao))
+        return ao
+
+    def compute_ssao_dev(self, position, normal, mask, w, h, samples, radius, out, stream=None):
+        self._check(lib().raw_ao_ssao_dev(self._h, _ptr(position), _ptr(normal), _ptr(mask), w, h, samples, radius,
+                                          _ptr(out), _vp(stream)))
+
+    def error_map(self, approx, truth, mask):
+        mask = np.ascontiguousarray(mask, np.uint8)
+        h, w = mask.shape
+        err = np.zeros((h, w), np.float32)
+        mx = C.c_float()
+        self._check(lib().raw_ao_error_map(self._h, np.ascontiguousarray(approx, np.float32),
+                                           np.ascontiguousarray(truth, np.float32), mask, w, h, err, C.byref(mx)))
+        return err, mx.value
+
+    def error_map_dev(self, approx, truth, mask, w, h, err, maxbuf, stream=None):
+        self._check(lib().raw_ao_error_map_dev(self._h, _ptr(approx), _ptr(truth), _ptr(mask), w, h, _ptr(err),
+                                               _ptr(maxbuf), _vp(stream)))
+
+    def reconcile(self, approx, truth, mask, tolerance=0.12):
+        """reconcile (src/cert/reconcile.cpp:4-21): the error map and maximum on the GPU, the
+        float64 row-major sum of squares on the host, as the reference orders it."""
+        err, mx = self.error_map(approx, truth, mask)
+        rmse, pixels = host.rmse_of(err, mask)
+        return host.ReconcileResult(err, rmse, mx, pixels, rmse <= np.float32(tolerance))
+
+    def trace_rays(self, rays8, any_hit, far_scale=None):
+        r = np.ascontiguousarray(rays8, np.float32).reshape(-1, 8)
+        out = np.zeros((r.shape[0], 4), np.float32)
+        fs = host.ROBUST_FAR_SCALE if far_scale is None else far_scale
+        self._check(lib().raw_ao_trace_rays(self._h, r, r.shape[0], int(bool(any_hit)), fs, out))
+        return out
+
+    # ---- measurement ---------------------------------------------------------------
+    def count_next_trace(self, enable=True):
+        self._check(lib().raw_ao_count_next_trace(self._h, int(bool(enable))))
+
+    def trace_counts(self):
+        v, t, r = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self._check(lib().raw_ao_trace_counts(self._h, C.byref(v), C.byref(t), C.byref(r)))
+        return v.value, t.value, r.value
+
+    def last_ms(self):
+        b, t, s, e = C.c_float(), C.c_float(), C.c_float(), C.c_float()
+        self._check(lib().raw_ao_last_ms(self._h, C.byref(b), C.byref(t), C.byref(s), C.byref(e)))
+        return dict(build=b.value, trace=t.value, ssao=s.value, error=e.value)
+
+    def angle_table(self):
+        s = np.zeros(1 << 24, np.float32)
+        c = np.zeros(1 << 24, np.float32)
+        self._check(lib().raw_ao_angle_table(self._h, s, c))
+        return s, c
+
+    def bvh_bytes(self):
+        nb, tb, rec = C.c_uint64(), C.c_uint64(), C.c_uint32()
+        self._check(lib().raw_ao_bvh_bytes(self._h, C.byref(nb), C.byref(tb), C.byref(rec)))
+        return nb.value, tb.value, rec.value

@@ -1,0 +1,132 @@
+"""Host-side pieces of the AO path (libraw_ao_host.so): the float64 row-major reconcile
+sum, the raw-cert/2 writer, the reference's test scene, the seeded scene generator and
+the G-buffer rasterizer that produce the path's inputs.
+"""
+import collections
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HOST_LIB = os.path.join(os.path.dirname(HERE), "build", "libraw_ao_host.so")
+
+# kSlabExit = 1 + 2 gamma(3), gamma(n) = n eps / (1 - n eps), eps = 2^-24 (raw/renderer/rt_bvh.hpp:34)
+ROBUST_FAR_SCALE = float(np.float32(1.0) + np.float32(2.0) * (np.float32(3.0) * np.float32(5.96046448e-8))
+                         / (np.float32(1.0) - np.float32(3.0) * np.float32(5.96046448e-8)))
+
+ReconcileResult = collections.namedtuple("ReconcileResult", "error_map rmse max_error pixels within_tolerance")
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(HOST_LIB):
+            raise RuntimeError("libraw_ao_host is not built (%s): run __graft_entry__.build()" % HOST_LIB)
+        L = C.CDLL(HOST_LIB)
+        L.rah_rmse.restype = C.c_float
+        L.rah_rmse.argtypes = [_f32p, _u8p, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+        L.rah_certificate.restype = C.c_size_t
+        L.rah_certificate.argtypes = [C.c_float, C.c_float, C.c_int32, C.c_float, C.c_int, C.c_int, C.c_char_p, C.c_char_p,
+                                      C.POINTER(C.c_char_p), C.POINTER(C.c_char_p), C.c_int, C.c_int, C.c_double, C.c_int,
+                                      C.c_double, C.c_char_p, C.c_size_t]
+        L.rah_test_scene.restype = C.c_size_t
+        L.rah_test_scene.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.rah_scene_seeded.restype = C.c_size_t
+        L.rah_scene_seeded.argtypes = [C.c_uint64, C.c_int, C.c_float, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_int32)]
+        L.rah_orbit_camera.restype = None
+        L.rah_orbit_camera.argtypes = [C.c_uint64, C.c_float, C.c_float, _f32p, _f32p]
+        L.rah_rasterize.restype = None
+        L.rah_rasterize.argtypes = [_f32p, _f32p, _i32p, C.c_size_t, _f32p, _f32p, _f32p, C.c_float, C.c_float, C.c_float,
+                                    C.c_float, C.c_int, C.c_int, _f32p, _f32p, _f32p, _u8p]
+        _lib = L
+    return _lib
+
+
+def rmse_of(err, mask):
+    """float32 RMSE of the covered pixels: float64 row-major sum (src/cert/reconcile.cpp:9-18)."""
+    mask = np.ascontiguousarray(mask, np.uint8)
+    h, w = mask.shape
+    px = C.c_int32()
+    r = lib().rah_rmse(np.ascontiguousarray(err, np.float32), mask, w, h, None, C.byref(px))
+    return np.float32(r), px.value
+
+
+def certificate(rmse, max_error, pixels, tolerance=0.12, rt=64, ss=24, renderer="", params_json="{}", outputs=(),
+                motion_coherence=None, hdr_headroom=None):
+    """raw-cert/2 text (src/cert/certificate.cpp:38-85); absent channels are null."""
+    names = (C.c_char_p * max(1, len(outputs)))(*[n.encode() for n, _ in outputs])
+    hashes = (C.c_char_p * max(1, len(outputs)))(*[h.encode() for _, h in outputs])
+    args = [C.c_float(rmse), C.c_float(max_error), int(pixels), C.c_float(tolerance), rt, ss, renderer.encode(),
+            params_json.encode(), names, hashes, len(outputs), int(motion_coherence is not None),
+            float(motion_coherence or 0.0), int(hdr_headroom is not None), float(hdr_headroom or 0.0)]
+    n = lib().rah_certificate(*args, None, 0)
+    buf = C.create_string_buffer(n + 1)
+    lib().rah_certificate(*args, buf, n + 1)
+    return buf.value.decode()
+
+
+def test_scene_mesh():
+    """The reference's 14-triangle test scene (src/scene/scene.cpp:18-32): (vpos, vnrm, idx)."""
+    n = lib().rah_test_scene(None, None, None)
+    vpos = np.zeros((2 * n, 3), np.float32)
+    vnrm = np.zeros((2 * n, 3), np.float32)
+    idx = np.zeros((n, 3), np.int32)
+    lib().rah_test_scene(vpos.ctypes.data, vnrm.ctypes.data, idx.ctypes.data)
+    return vpos, vnrm, idx
+
+
+SceneSpec = collections.namedtuple("SceneSpec", "seed objects p_sphere subdiv smin smax ground height")
+# ~1M triangles: a 40 x 40 ground quad and 2,000 boxes and icospheres (subdivision 3,
+# 1,280 triangles) in a 40 x 40 x 10 volume, sizes log-uniform in [0.05, 2.0].
+SCENE_1M = SceneSpec(0x5EED, 2000, 0.385, 3, 0.05, 2.0, 40.0, 10.0)
+
+
+def scene_seeded(spec=SCENE_1M):
+    """The seeded scene as (vpos, vnrm, idx, nspheres)."""
+    L = lib()
+    nv, ns = C.c_size_t(), C.c_int32()
+    a = [spec.seed, spec.objects, spec.p_sphere, spec.subdiv, spec.smin, spec.smax, spec.ground, spec.height]
+    n = L.rah_scene_seeded(*a, None, None, None, C.byref(nv), C.byref(ns))
+    vpos = np.zeros((nv.value, 3), np.float32)
+    vnrm = np.zeros((nv.value, 3), np.float32)
+    idx = np.zeros((n, 3), np.int32)
+    L.rah_scene_seeded(*a, vpos.ctypes.data, vnrm.ctypes.data, idx.ctypes.data, C.byref(nv), C.byref(ns))
+    return vpos, vnrm, idx, ns.value
+
+
+def mesh_tris(vpos, idx):
+    return np.ascontiguousarray(vpos[idx].reshape(-1, 9), dtype=np.float32)
+
+
+def orbit_camera(seed, radius=28.0, height=14.0, target=(0.0, 2.0, 0.0)):
+    eye = np.zeros(3, np.float32)
+    lib().rah_orbit_camera(seed, radius, height, np.asarray(target, np.float32), eye)
+    return eye
+
+
+def rasterize(vpos, vnrm, idx, eye, center, up, fovy, aspect, near, far, w, h):
+    """raw::rasterize with default options: (depth, position, normal, mask)."""
+    depth = np.zeros((h, w), np.float32)
+    pos = np.zeros((h, w, 3), np.float32)
+    nrm = np.zeros((h, w, 3), np.float32)
+    mask = np.zeros((h, w), np.uint8)
+    lib().rah_rasterize(np.ascontiguousarray(vpos, np.float32), np.ascontiguousarray(vnrm, np.float32),
+                        np.ascontiguousarray(idx, np.int32), idx.shape[0], np.asarray(eye, np.float32),
+                        np.asarray(center, np.float32), np.asarray(up, np.float32), fovy, aspect, near, far, w, h,
+                        depth, pos, nrm, mask)
+    return depth, pos, nrm, mask
+
+
+def cli_gbuffer(w=256, h=256, eye=(4, 4, 6), target=(0, 1, 0), fovy=0.9):
+    """The G-buffer the reference CLI renders for its test scene (src/tools/run.cpp:12-26,
+    src/tools/cli_params.cpp:12-18): aspect w/h, near 0.1, far 100, up (0,1,0)."""
+    vpos, vnrm, idx = test_scene_mesh()
+    aspect = float(np.float32(w) / np.float32(h))
+    return rasterize(vpos, vnrm, idx, eye, target, (0, 1, 0), fovy, aspect, 0.1, 100.0, w, h)

@@ -1,0 +1,245 @@
+"""ctypes bindings of the CPU oracle (oracle/build/liboracle_ao.so) and of the reference
+built from its own sources (oracle/_ref/libref_ao.so). Test infrastructure only: the
+checker the HIP path is compared with. The product never imports this module.
+"""
+import ctypes as C
+import os
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "build", "liboracle_ao.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref_ao.so")
+
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+SZ = C.c_size_t
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = C.CDLL(ORACLE_SO)
+        L.orc_hash01.argtypes = [C.c_uint32] * 3
+        L.orc_hash01.restype = C.c_float
+        L.orc_ao_ray.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, f32p, f32p, f32p, f32p]
+        L.orc_intersect.argtypes = [f32p, f32p, f32p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.orc_intersect.restype = C.c_int
+        L.orc_rasterize.argtypes = [f32p, f32p, i32p, SZ, f32p, f32p, f32p, C.c_float, C.c_float, C.c_float,
+                                    C.c_float, C.c_int, C.c_int, f32p, f32p, f32p, u8p]
+        L.orc_test_scene.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_test_scene.restype = SZ
+        L.orc_bvh_build.argtypes = [f32p, SZ]
+        L.orc_bvh_build.restype = C.c_void_p
+        L.orc_bvh_free.argtypes = [C.c_void_p]
+        L.orc_bvh_node_count.argtypes = [C.c_void_p]
+        L.orc_bvh_node_count.restype = SZ
+        L.orc_bvh_occluded.argtypes = [C.c_void_p, f32p, f32p, C.c_float]
+        L.orc_bvh_occluded.restype = C.c_int
+        L.orc_rtao.argtypes = [C.c_void_p, f32p, SZ, C.c_int, f32p, f32p, u8p, C.c_int, C.c_int, C.c_int, C.c_int,
+                               C.c_int, C.c_float, C.c_int, f32p]
+        L.orc_ssao.argtypes = [f32p, f32p, u8p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, f32p]
+        L.orc_reconcile.argtypes = [f32p, f32p, u8p, C.c_int, C.c_int, C.c_float, f32p, C.POINTER(C.c_float),
+                                    C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.orc_ploc_build.argtypes = [f32p, SZ, C.c_int, f32p, i32p, C.POINTER(C.c_uint32)]
+        L.orc_ploc_build.restype = C.c_int32
+        L.orc_ploc_keys.argtypes = [f32p, SZ, u32p, u32p, f32p]
+        L.orc_tree_hash.argtypes = [f32p, i32p, SZ]
+        L.orc_tree_hash.restype = C.c_uint64
+        L.orc_sah_cost.argtypes = [f32p, SZ, C.c_int32]
+        L.orc_sah_cost.restype = C.c_double
+        L.orc_binned_sah_cost.argtypes = [f32p, SZ, C.POINTER(SZ)]
+        L.orc_binned_sah_cost.restype = C.c_double
+        L.orc_scan_rays.argtypes = [f32p, SZ, f32p, f32p, SZ, C.c_int, C.c_int, f32p, i32p]
+        L.orc_scene_tris.argtypes = [C.c_char_p, SZ, C.c_uint64, C.c_void_p]
+        L.orc_scene_tris.restype = SZ
+        L.orc_libm_angles.argtypes = [f32p, f32p]
+        _lib = L
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        L = C.CDLL(REF_SO)
+        L.ref_rasterize.argtypes = lib().orc_rasterize.argtypes
+        L.ref_render_test_scene.argtypes = [C.c_int, C.c_int, f32p, f32p, C.c_float, f32p, f32p, u8p]
+        L.ref_rtao.argtypes = [f32p, SZ, C.c_int, f32p, f32p, u8p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, f32p]
+        L.ref_ssao.argtypes = lib().orc_ssao.argtypes
+        L.ref_reconcile.argtypes = lib().orc_reconcile.argtypes
+        L.ref_ploc_build.argtypes = lib().orc_ploc_build.argtypes
+        L.ref_ploc_build.restype = C.c_int32
+        L.ref_scene_tris.argtypes = lib().orc_scene_tris.argtypes
+        L.ref_scene_tris.restype = SZ
+        L.ref_binned_sah_cost.argtypes = lib().orc_binned_sah_cost.argtypes
+        L.ref_binned_sah_cost.restype = C.c_double
+        _ref = L
+    return _ref
+
+
+# ---- convenience wrappers ---------------------------------------------------------
+
+def test_scene_mesh():
+    """The 14-triangle test scene (src/scene/scene.cpp:18-32): (vpos, vnrm, idx)."""
+    L = lib()
+    n = L.orc_test_scene(None, None, None)
+    vpos = np.zeros((4 * n // 2, 3), np.float32)
+    vnrm = np.zeros_like(vpos)
+    idx = np.zeros((n, 3), np.int32)
+    L.orc_test_scene(vpos.ctypes.data, vnrm.ctypes.data, idx.ctypes.data)
+    return vpos, vnrm, idx
+
+
+def mesh_tris(vpos, idx):
+    return np.ascontiguousarray(vpos[idx].reshape(-1, 9), dtype=np.float32)
+
+
+def rasterize(vpos, vnrm, idx, eye, center, up, fovy, aspect, near, far, w, h, impl=None):
+    L = impl or lib()
+    fn = L.ref_rasterize if impl is not None else L.orc_rasterize
+    depth = np.zeros((h, w), np.float32)
+    pos = np.zeros((h, w, 3), np.float32)
+    nrm = np.zeros((h, w, 3), np.float32)
+    mask = np.zeros((h, w), np.uint8)
+    fn(np.ascontiguousarray(vpos, np.float32), np.ascontiguousarray(vnrm, np.float32),
+       np.ascontiguousarray(idx, np.int32), idx.shape[0], np.asarray(eye, np.float32),
+       np.asarray(center, np.float32), np.asarray(up, np.float32), fovy, aspect, near, far, w, h,
+       depth, pos, nrm, mask)
+    return depth, pos, nrm, mask
+
+
+def cli_gbuffer(w=256, h=256, eye=(4, 4, 6), target=(0, 1, 0), fovy=0.9):
+    """The G-buffer the reference CLI renders for the test scene (src/tools/run.cpp:12-26,
+    src/tools/cli_params.cpp:12-18): aspect w/h, near 0.1, far 100, up (0,1,0)."""
+    vpos, vnrm, idx = test_scene_mesh()
+    aspect = np.float32(w) / np.float32(h)
+    return rasterize(vpos, vnrm, idx, eye, target, (0, 1, 0), fovy, float(aspect), 0.1, 100.0, w, h)
+
+
+class OracleBvh:
+    def __init__(self, tris9):
+        self.tris = np.ascontiguousarray(tris9, np.float32).reshape(-1, 9)
+        self.h = lib().orc_bvh_build(self.tris, self.tris.shape[0])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_bvh_free(self.h)
+            self.h = None
+
+    def rtao(self, pos, nrm, mask, samples, radius, mode=0, threads=None, rows=None):
+        h, w = mask.shape
+        y0, y1 = rows if rows is not None else (0, h)
+        ao = np.ones((h, w), np.float32)
+        lib().orc_rtao(self.h, self.tris, self.tris.shape[0], mode, np.ascontiguousarray(pos, np.float32),
+                       np.ascontiguousarray(nrm, np.float32), np.ascontiguousarray(mask, np.uint8), w, h, y0, y1,
+                       samples, radius, threads or os.cpu_count() or 1, ao)
+        return ao
+
+
+def rtao(tris9, pos, nrm, mask, samples=64, radius=2.0, mode=0, threads=None):
+    return OracleBvh(tris9).rtao(pos, nrm, mask, samples, radius, mode, threads)
+
+
+def ssao(pos, nrm, mask, samples=24, radius=2.0, threads=None):
+    h, w = mask.shape
+    ao = np.zeros((h, w), np.float32)
+    lib().orc_ssao(np.ascontiguousarray(pos, np.float32), np.ascontiguousarray(nrm, np.float32),
+                   np.ascontiguousarray(mask, np.uint8), w, h, samples, radius, threads or os.cpu_count() or 1, ao)
+    return ao
+
+
+def reconcile(approx, truth, mask, tol=0.12):
+    h, w = mask.shape
+    em = np.zeros((h, w), np.float32)
+    rmse, mx = C.c_float(), C.c_float()
+    px, within = C.c_int32(), C.c_int32()
+    lib().orc_reconcile(np.ascontiguousarray(approx, np.float32), np.ascontiguousarray(truth, np.float32),
+                        np.ascontiguousarray(mask, np.uint8), w, h, tol, em, C.byref(rmse), C.byref(mx),
+                        C.byref(px), C.byref(within))
+    return em, rmse.value, mx.value, px.value, bool(within.value)
+
+
+def scene_tris(name, count=0, seed=0, impl=None):
+    if impl is not None:
+        n = impl.ref_scene_tris(name.encode(), count, seed, None)
+        out = np.zeros((n, 9), np.float32)
+        impl.ref_scene_tris(name.encode(), count, seed, out.ctypes.data)
+        return out
+    n = lib().orc_scene_tris(name.encode(), count, seed, None)
+    out = np.zeros((n, 9), np.float32)
+    lib().orc_scene_tris(name.encode(), count, seed, out.ctypes.data)
+    return out
+
+
+def ploc(tris9, radius=16, impl=None):
+    tris9 = np.ascontiguousarray(tris9, np.float32).reshape(-1, 9)
+    n = tris9.shape[0]
+    nn = max(2 * n - 1, 1)
+    boxes = np.zeros((nn, 6), np.float32)
+    links = np.zeros((nn, 3), np.int32)
+    rounds = C.c_uint32()
+    fn = impl.ref_ploc_build if impl is not None else lib().orc_ploc_build
+    root = fn(tris9, n, radius, boxes, links, C.byref(rounds))
+    return boxes, links, int(root), rounds.value
+
+
+def tree_hash(boxes, links):
+    return lib().orc_tree_hash(np.ascontiguousarray(boxes, np.float32), np.ascontiguousarray(links, np.int32),
+                               boxes.shape[0])
+
+
+def ploc_keys(tris9):
+    tris9 = np.ascontiguousarray(tris9, np.float32).reshape(-1, 9)
+    n = tris9.shape[0]
+    keys = np.zeros(n, np.uint32)
+    order = np.zeros(n, np.uint32)
+    g4 = np.zeros(4, np.float32)
+    lib().orc_ploc_keys(tris9, n, keys, order, g4)
+    return keys, order, g4
+
+
+def scan_rays(tris9, rays6, tmax, any_hit, threads=None):
+    tris9 = np.ascontiguousarray(tris9, np.float32).reshape(-1, 9)
+    rays6 = np.ascontiguousarray(rays6, np.float32).reshape(-1, 6)
+    n = rays6.shape[0]
+    t = np.zeros(n, np.float32)
+    tri = np.zeros(n, np.int32)
+    lib().orc_scan_rays(tris9, tris9.shape[0], rays6, np.ascontiguousarray(tmax, np.float32), n, int(any_hit),
+                        threads or os.cpu_count() or 1, t, tri)
+    return t, tri
+
+
+def libm_angles():
+    s = np.zeros(1 << 24, np.float32)
+    c = np.zeros(1 << 24, np.float32)
+    lib().orc_libm_angles(s, c)
+    return s, c
+
+
+# ---- the reference's file formats (src/core/image.cpp:34-69), for hashing -----------
+
+def pfm1_bytes(img):
+    h, w = img.shape
+    return b"Pf\n%d %d\n-1.0\n" % (w, h) + np.ascontiguousarray(img[::-1], dtype="<f4").tobytes()
+
+
+def pgm_bytes(img):
+    h, w = img.shape
+    v = np.clip(img.astype(np.float32), np.float32(0), np.float32(1))
+    b = np.floor(v * np.float32(255.0) + np.float32(0.5)).astype(np.uint8)
+    return b"P5\n%d %d\n255\n" % (w, h) + b.tobytes()
+
+
+def mask_pgm_bytes(mask):
+    h, w = mask.shape
+    return b"P5\n%d %d\n255\n" % (w, h) + np.where(mask != 0, 255, 0).astype(np.uint8).tobytes()

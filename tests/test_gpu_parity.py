@@ -1,0 +1,222 @@
+"""The HIP path against the oracle and the reference's golden vectors, through the C ABI.
+Integer and byte results (Morton keys, sort order, trees, AO counts) must be bit-exact;
+AO images are compared bit for bit, with the north-star bound (within 1/64 on >= 99.9%
+of pixels and RMSE <= 1e-3) asserted as well where a different tree could matter."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import raw_ao
+from cases import CASES, KNOWN, KNOWN_CASE, TREES, params, sha
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def test_angle_table_matches_host_libm(ctx):
+    """Every one of the 2^24 sample angles: the device's sin/cos == the host libm's."""
+    s, c = ctx.angle_table()
+    ls, lc = O.libm_angles()
+    assert np.array_equal(bits(s), bits(ls))
+    assert np.array_equal(bits(c), bits(lc))
+
+
+def test_angle_table_correctly_rounded_mode(ctx):
+    """With the host-libm table off, the angles are the correctly rounded values."""
+    ctx.set_option(raw_ao.OPT_HOST_LIBM_TRIG, 0)
+    try:
+        s, c = ctx.angle_table()
+    finally:
+        ctx.set_option(raw_ao.OPT_HOST_LIBM_TRIG, 1)
+    k = np.arange(1 << 24, dtype=np.float32) / np.float32(1 << 24)
+    phi = (np.float32(6.2831853) * k).astype(np.float64)
+    assert np.array_equal(bits(c), bits(np.cos(phi).astype(np.float32)))
+    assert np.array_equal(bits(s), bits(np.sin(phi).astype(np.float32)))
+
+
+@pytest.mark.parametrize("scene", sorted(TREES))
+def test_ploc_tree_matches_oracle(ctx, scene):
+    """Morton keys, sorted order and the whole tree node for node; the golden hash."""
+    t = O.scene_tris(scene, 20000, 7)
+    ctx.build_ploc(t, 16)
+    keys, order = ctx.read_keys(t.shape[0])
+    okeys, oorder, _ = O.ploc_keys(t)
+    assert np.array_equal(keys, okeys)
+    assert np.array_equal(order, oorder)
+    boxes, links, root, rounds = ctx.read_tree()
+    ob, ol, oroot, orounds = O.ploc(t, 16)
+    assert (root, rounds) == (oroot, orounds)
+    assert np.array_equal(links, ol)
+    assert np.array_equal(bits(boxes), bits(ob))
+    assert "%016x" % O.tree_hash(boxes, links) == TREES[scene]["tree_hash"]
+
+
+def test_ploc_radius_8(ctx):
+    t = O.scene_tris("dense_block")
+    ctx.build_ploc(t, 8)
+    boxes, links, root, rounds = ctx.read_tree()
+    ob, ol, oroot, orounds = O.ploc(t, 8)
+    assert (root, rounds) == (oroot, orounds)
+    assert np.array_equal(links, ol) and np.array_equal(bits(boxes), bits(ob))
+
+
+def test_ploc_1m_scene_matches_oracle(ctx):
+    vpos, _, idx, _ = raw_ao.host.scene_seeded()
+    t = raw_ao.host.mesh_tris(vpos, idx)
+    ctx.build_ploc(t, 16)
+    boxes, links, root, rounds = ctx.read_tree()
+    ob, ol, oroot, orounds = O.ploc(t, 16)
+    assert (root, rounds) == (oroot, orounds)
+    assert np.array_equal(links, ol) and np.array_equal(bits(boxes), bits(ob))
+
+
+def test_ploc_edge_sizes(ctx):
+    """1, 2, 3 and 257 triangles (a partial block and a halo at both ends)."""
+    for n in (1, 2, 3, 257):
+        t = O.scene_tris("soup", n, 3)
+        ctx.build_ploc(t, 16)
+        boxes, links, root, rounds = ctx.read_tree()
+        ob, ol, oroot, orounds = O.ploc(t, 16)
+        assert (root, rounds) == (oroot, orounds), n
+        assert np.array_equal(links, ol) and np.array_equal(bits(boxes), bits(ob)), n
+
+
+def golden_frame(ctx, name):
+    p = params(name)
+    _, pos, nrm, mask = raw_ao.host.cli_gbuffer(p["w"], p["h"], p["eye"], p["target"], p["fovy"])
+    vpos, _, idx = raw_ao.host.test_scene_mesh()
+    ctx.build_ploc(raw_ao.host.mesh_tris(vpos, idx), 16)
+    ao = ctx.compute_rtao(pos, nrm, mask, 64, 2.0)
+    ss = ctx.compute_ssao(pos, nrm, mask, 24, 2.0)
+    rec = ctx.reconcile(ss, ao, mask, 0.12)
+    return mask, ao, ss, rec
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_golden_frame(ctx, name):
+    """ao_rt.pfm, ao_ss.pfm and the error map's bytes from the GPU equal the reference's."""
+    mask, ao, ss, rec = golden_frame(ctx, name)
+    f = CASES[name]["files"]
+    assert sha(O.pfm1_bytes(ao)) == f["ao_rt.pfm"]
+    assert sha(O.pfm1_bytes(ss)) == f["ao_ss.pfm"]
+    assert sha(O.pgm_bytes(rec.error_map)) == f["ao_error.pgm"]
+
+
+@pytest.mark.parametrize("short", sorted(KNOWN_CASE))
+def test_golden_reconcile_and_verdict(ctx, short):
+    mask, ao, ss, rec = golden_frame(ctx, KNOWN_CASE[short])
+    k = KNOWN[short]
+    assert rec.pixels == k["pixels"]
+    assert "%.9g" % rec.rmse == "%.9g" % k["rmse"] and "%.9g" % rec.max_error == "%.9g" % k["maxError"]
+    assert ("verified" if rec.within_tolerance else "refuted") == k["verdict"]
+    cert = raw_ao.host.certificate(rec.rmse, rec.max_error, rec.pixels, 0.12)
+    assert '"verdict":"%s"' % k["verdict"] in cert and '"oracle":"raw-rt-ao-v1"' in cert
+
+
+def big_frame(w, h, seed=1):
+    vpos, vnrm, idx, _ = raw_ao.host.scene_seeded()
+    eye = raw_ao.host.orbit_camera(seed)
+    _, pos, nrm, mask = raw_ao.host.rasterize(vpos, vnrm, idx, eye, (0, 2, 0), (0, 1, 0), 0.9,
+                                              float(np.float32(w) / np.float32(h)), 0.1, 200.0, w, h)
+    return raw_ao.host.mesh_tris(vpos, idx), pos, nrm, mask
+
+
+def test_1m_scene_ao_matches_oracle(ctx):
+    """The ~1M-triangle scene at radius 1.0, 64 spp: per pixel against the CPU reference
+    restatement (binned-SAH BVH). Bound: within 1/64 on >= 99.9% of pixels, RMSE <= 1e-3."""
+    tris, pos, nrm, mask = big_frame(240, 136)
+    ctx.build_ploc(tris, 16)
+    ao = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
+    ref = O.rtao(tris, pos, nrm, mask, 64, 1.0)
+    d = np.abs(ao.astype(np.float64) - ref.astype(np.float64))
+    assert mask.sum() > 10000
+    assert np.mean(d <= 1.0 / 64 + 1e-9) >= 0.999
+    assert np.sqrt(np.mean(d ** 2)) <= 1e-3
+    assert np.mean(bits(ao) == bits(ref)) >= 0.999
+
+
+@pytest.mark.parametrize("spp", [1, 16, 33, 128])
+def test_spp_other_than_64(ctx, spp):
+    p = params("default")
+    _, pos, nrm, mask = raw_ao.host.cli_gbuffer(96, 96)
+    vpos, _, idx = raw_ao.host.test_scene_mesh()
+    tris = raw_ao.host.mesh_tris(vpos, idx)
+    ctx.build_ploc(tris, 16)
+    ao = ctx.compute_rtao(pos, nrm, mask, spp, 2.0)
+    assert np.array_equal(bits(ao), bits(O.rtao(tris, pos, nrm, mask, spp, 2.0)))
+    assert p is not None
+
+
+def test_empty_scene_is_all_open(ctx):
+    _, pos, nrm, mask = raw_ao.host.cli_gbuffer(64, 64)
+    ctx.build_ploc(np.zeros((0, 9), np.float32), 16)
+    ao = ctx.compute_rtao(pos, nrm, mask, 64, 2.0)
+    assert np.all(ao == 1.0)
+
+
+def make_rays(n, seed):
+    """Random rays in the [-12, 12]^3 cube (src/renderer/gpu/rt_bvh_parity.cpp:35-48)."""
+    rng = np.random.default_rng(seed)
+    o = (rng.random((n, 3), dtype=np.float32) * 24 - 12).astype(np.float32)
+    d = (rng.random((n, 3), dtype=np.float32) - np.float32(0.5)).astype(np.float32)
+    d = (d * (np.float32(1) / np.sqrt((d * d).sum(1, keepdims=True)).astype(np.float32))).astype(np.float32)
+    return o, d
+
+
+@pytest.mark.parametrize("scene", ["soup", "dense_block"])
+def test_trace_rays_closest_and_any_equal_scan(ctx, scene):
+    """Closest hit: the same triangle and t as the scan; any hit: the same hit/miss."""
+    t = O.scene_tris(scene, 20000, 7)
+    ctx.build_ploc(t, 16)
+    n = 20000 if scene == "soup" else 4000
+    o, d = make_rays(n, 0xB2B2 + t.shape[0])
+    tmax = np.full(n, 1e30, np.float32)
+    rays = np.hstack([o, d, tmax[:, None], np.zeros((n, 1), np.float32)]).astype(np.float32)
+    out = ctx.trace_rays(rays, any_hit=False)
+    st, stri = O.scan_rays(t, np.hstack([o, d]), tmax, any_hit=False)
+    assert np.array_equal(out[:, 1].astype(np.int32), stri)
+    hit = stri >= 0
+    assert np.array_equal(bits(out[hit, 0]), bits(st[hit]))
+    any_max = (np.random.default_rng(77).random(n, dtype=np.float32) * 8).astype(np.float32)
+    rays[:, 6] = any_max
+    out = ctx.trace_rays(rays, any_hit=True)
+    _, atri = O.scan_rays(t, np.hstack([o, d]), any_max, any_hit=True)
+    assert np.array_equal(out[:, 1] >= 0, atri >= 0)
+
+
+def test_short_far_control_loses_hits(ctx):
+    """A slab exit shortened to 0.9 must lose hits (rt_bvh_parity.cpp:167-173)."""
+    t = O.scene_tris("retro_room")
+    ctx.build_ploc(t, 16)
+    o, d = make_rays(20000, 0xB2B2 + t.shape[0])
+    rays = np.hstack([o, d, np.full((20000, 1), 1e30, np.float32), np.zeros((20000, 1), np.float32)]).astype(np.float32)
+    full = ctx.trace_rays(rays, any_hit=False)
+    short = ctx.trace_rays(rays, any_hit=False, far_scale=0.9)
+    assert np.sum((full[:, 1] >= 0) & (short[:, 1] != full[:, 1])) > 0
+
+
+def test_error_map_matches_oracle(ctx):
+    rng = np.random.default_rng(9)
+    a = rng.random((123, 77), dtype=np.float32)
+    b = rng.random((123, 77), dtype=np.float32)
+    m = (rng.random((123, 77)) < 0.7).astype(np.uint8)
+    rec = ctx.reconcile(a, b, m, 0.3)
+    em, rmse, mx, px, within = O.reconcile(a, b, m, 0.3)
+    assert np.array_equal(bits(rec.error_map), bits(em))
+    assert (float(rec.rmse), rec.max_error, rec.pixels, bool(rec.within_tolerance)) == (rmse, mx, px, within)
+
+
+def test_repeatable(ctx):
+    """Two runs give the same bytes (no scheduling-dependent results)."""
+    tris, pos, nrm, mask = big_frame(128, 72, seed=3)
+    ctx.build_ploc(tris, 16)
+    b1 = ctx.read_tree()
+    a1 = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
+    ctx.build_ploc(tris, 16)
+    b2 = ctx.read_tree()
+    a2 = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
+    assert np.array_equal(b1[1], b2[1]) and np.array_equal(bits(b1[0]), bits(b2[0]))
+    assert np.array_equal(bits(a1), bits(a2))

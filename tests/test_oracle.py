@@ -1,0 +1,205 @@
+"""The CPU oracle pinned to the reference's own golden vectors, closed forms and
+invariants (no GPU). Everything the GPU path is later compared with is checked here."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from cases import CASES, KNOWN, KNOWN_CASE, TREES, params, sha
+
+
+def render(name, threads=None):
+    p = params(name)
+    _, pos, nrm, mask = O.cli_gbuffer(p["w"], p["h"], p["eye"], p["target"], p["fovy"])
+    vpos, _, idx = O.test_scene_mesh()
+    tris = O.mesh_tris(vpos, idx)
+    ao = O.rtao(tris, pos, nrm, mask, 64, 2.0, threads=threads or p["threads"])
+    ss = O.ssao(pos, nrm, mask, 24, 2.0, threads=threads or p["threads"])
+    return pos, nrm, mask, ao, ss
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_identity_golden_hashes(name):
+    """ao_rt.pfm, ao_ss.pfm, mask.pgm and the three previews byte for byte
+    (evidence/identity-golden.json; src/core/image.cpp:34-69 formats)."""
+    pos, nrm, mask, ao, ss = render(name)
+    em, rmse, mx, px, within = O.reconcile(ss, ao, mask, 0.12)
+    f = CASES[name]["files"]
+    assert sha(O.mask_pgm_bytes(mask)) == f["mask.pgm"]
+    assert sha(O.pfm1_bytes(ao)) == f["ao_rt.pfm"]
+    assert sha(O.pfm1_bytes(ss)) == f["ao_ss.pfm"]
+    assert sha(O.pgm_bytes(em)) == f["ao_error.pgm"]
+    assert sha(O.pgm_bytes(ao)) == f["ao_rt.pgm"]
+    assert sha(O.pgm_bytes(ss)) == f["ao_ss.pgm"]
+
+
+@pytest.mark.parametrize("short", sorted(KNOWN_CASE))
+def test_reconcile_known_answers(short):
+    """pixels, rmse, maxError and verdict (evidence/wasm-vs-native-512.json)."""
+    _, _, mask, ao, ss = render(KNOWN_CASE[short])
+    _, rmse, mx, px, within = O.reconcile(ss, ao, mask, 0.12)
+    k = KNOWN[short]
+    assert px == k["pixels"]
+    assert "%.9g" % rmse == "%.9g" % k["rmse"]
+    assert "%.9g" % mx == "%.9g" % k["maxError"]
+    assert ("verified" if within else "refuted") == k["verdict"]
+
+
+def test_reconcile_default_256():
+    """37,996 pixels, rmse 0.129436031 (docs/architecture/adr/0012-media-engine.md:96)."""
+    _, _, mask, ao, ss = render("default")
+    _, rmse, _, px, within = O.reconcile(ss, ao, mask, 0.12)
+    assert (px, "%.9g" % rmse, within) == (37996, "0.129436031", False)
+
+
+def test_reconcile_two_pixels():
+    """tests/test_reconcile.cpp:4-21: max 0.4, rmse sqrt(0.16/2) > 0.1 refuted; self 0."""
+    a = np.array([[0.5, 0.8]], np.float32)
+    b = np.array([[0.5, 0.4]], np.float32)
+    m = np.ones((1, 2), np.uint8)
+    em, rmse, mx, px, within = O.reconcile(a, b, m, 0.1)
+    assert px == 2 and not within
+    assert mx == np.float32(0.8) - np.float32(0.4)
+    e = np.float64(np.float32(0.8) - np.float32(0.4))
+    assert rmse == np.float32(np.sqrt(e * e / 2))
+    _, rmse2, _, _, within2 = O.reconcile(a, a, m, 0.1)
+    assert rmse2 == 0.0 and within2
+
+
+@pytest.mark.parametrize("scene", sorted(TREES))
+def test_ploc_tree_hash(scene):
+    """The whole PLOC tree by FNV-1a hash, and its rounds (evidence/rt-r2-bvh-*.json)."""
+    t = O.scene_tris(scene, 20000, 7)
+    g = TREES[scene]
+    assert t.shape[0] == g["triangles"]
+    boxes, links, root, rounds = O.ploc(t)
+    assert "%016x" % O.tree_hash(boxes, links) == g["tree_hash"]
+    assert rounds == g["rounds"]
+    assert root == 2 * t.shape[0] - 2
+    sah = O.lib().orc_sah_cost(boxes, boxes.shape[0], root)
+    assert round(sah, 4) == g["sah"]
+    nodes = O.SZ()
+    binned = O.lib().orc_binned_sah_cost(t, t.shape[0], O.C.byref(nodes))
+    assert round(binned, 4) == g["m3_binned_sah"] and nodes.value == g["binned_nodes"]
+
+
+def test_ploc_radius_8_control():
+    """Radius 8 must change the dense_block tree (rt_bvh_parity.cpp:161-166)."""
+    t = O.scene_tris("dense_block")
+    b16, l16, _, _ = O.ploc(t, 16)
+    b8, l8, _, _ = O.ploc(t, 8)
+    diff = np.count_nonzero(np.any(l16 != l8, axis=1) | np.any(b16.view(np.uint32) != b8.view(np.uint32), axis=1))
+    assert diff > 1000
+
+
+def test_ploc_sorted_order_is_stable_key_order():
+    """Sorting (code, index) pairs == a stable sort by code (the radix sort's contract)."""
+    t = O.scene_tris("soup", 20000, 7)
+    keys, order, g4 = O.ploc_keys(t)
+    assert np.array_equal(order, np.argsort(keys, kind="stable").astype(np.uint32))
+    assert keys.max() < (1 << 30)
+
+
+def test_intersect_known_answers():
+    """tests/test_primitives.cpp:73-85."""
+    tri = np.array([-1, -1, 0, 1, -1, 0, 0, 1, 0], np.float32)
+    t, u, v = O.C.c_float(), O.C.c_float(), O.C.c_float()
+
+    def hit(o, d):
+        return O.lib().orc_intersect(np.array(o, np.float32), np.array(d, np.float32), tri, O.C.byref(t), O.C.byref(u),
+                                     O.C.byref(v))
+    assert hit((0, 0, 5), (0, 0, -1)) and abs(t.value - 5.0) < 1e-4
+    assert not hit((5, 5, 5), (0, 0, -1))
+    assert hit((0.5, -0.5, 5), (0, 0, -1)) and abs(u.value - 0.625) < 1e-5 and abs(v.value - 0.25) < 1e-5
+    assert hit((0, 0, -2), (0, 0, 1))
+    assert not hit((0, 0, 5e-5), (0, 0, -1))
+
+
+def test_watertight_closed_mesh():
+    """No ray from inside a closed mesh escapes through a shared edge or vertex
+    (tests/test_primitives.cpp:86-97): an octahedron, rays at every vertex, every edge
+    midpoint and random edge points."""
+    V = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], np.float32) * np.float32(1.7)
+    F = [(0, 2, 4), (2, 1, 4), (1, 3, 4), (3, 0, 4), (2, 0, 5), (1, 2, 5), (3, 1, 5), (0, 3, 5)]
+    tris = np.array([np.concatenate([V[a], V[b], V[c]]) for a, b, c in F], np.float32)
+    rng = np.random.default_rng(5)
+    targets = list(V)
+    for a, b, c in F:
+        for e0, e1 in ((a, b), (b, c), (c, a)):
+            targets.append((V[e0] + V[e1]) * np.float32(0.5))
+            for s in rng.random(6, dtype=np.float32):
+                targets.append(V[e0] + (V[e1] - V[e0]) * s)
+    rays = []
+    for o in (rng.random((8, 3), dtype=np.float32) - np.float32(0.5)) * np.float32(0.3):
+        for tg in targets:
+            d = (np.asarray(tg, np.float32) - o).astype(np.float32)
+            rays.append(np.concatenate([o, d / np.float32(np.sqrt(np.float32(d @ d)))]))
+    rays = np.array(rays, np.float32)
+    t, tri = O.scan_rays(tris, rays, np.full(len(rays), 1e30, np.float32), any_hit=False)
+    assert np.all(tri >= 0), "%d rays leaked" % int(np.sum(tri < 0))
+
+
+def test_bvh_equals_scan_on_random_rays():
+    """Bvh::occluded equals the scan of every triangle (tests/test_bvh.cpp:28-64)."""
+    t = O.scene_tris("soup", 3000, 11)
+    bvh = O.OracleBvh(t)
+    rng = np.random.default_rng(3)
+    o = (rng.random((4000, 3), dtype=np.float32) * 24 - 12).astype(np.float32)
+    d = rng.standard_normal((4000, 3)).astype(np.float32)
+    d /= np.sqrt((d * d).sum(1, keepdims=True)).astype(np.float32)
+    tmax = (rng.random(4000, dtype=np.float32) * 8).astype(np.float32)
+    _, tri = O.scan_rays(t, np.hstack([o, d]), tmax, any_hit=True)
+    got = [O.lib().orc_bvh_occluded(bvh.h, np.ascontiguousarray(o[k]), np.ascontiguousarray(d[k]), float(tmax[k]))
+           for k in range(4000)]
+    assert np.array_equal(np.array(got, bool), tri >= 0)
+
+
+def test_ao_bvh_equals_scan_and_skew_breaks_it():
+    """computeRTAO on the BVH == the scan; the skew control (x0.99) must differ
+    (tests/test_ao_bvh.cpp:48-93, RAW_NATIVE_AO_ORACLE=scan|skew)."""
+    _, pos, nrm, mask = O.cli_gbuffer(128, 128)
+    vpos, _, idx = O.test_scene_mesh()
+    tris = O.mesh_tris(vpos, idx)
+    b = O.OracleBvh(tris)
+    a_bvh = b.rtao(pos, nrm, mask, 64, 2.0, mode=0)
+    a_scan = b.rtao(pos, nrm, mask, 64, 2.0, mode=1)
+    a_skew = b.rtao(pos, nrm, mask, 64, 2.0, mode=2)
+    assert np.array_equal(a_bvh, a_scan)
+    assert not np.array_equal(a_skew, a_scan)
+
+
+def test_open_plane_and_flat_ssao():
+    """Closed forms: nothing above a plane -> every ray open (AO 1.0); SSAO of a flat plane
+    is 1.0 (n . d = 0 for every tap)."""
+    tris = np.array([[-50, 0, -50, 50, 0, -50, 50, 0, 50], [-50, 0, -50, 50, 0, 50, -50, 0, 50]], np.float32)
+    h, w = 16, 16
+    ys, xs = np.mgrid[0:h, 0:w].astype(np.float32)
+    pos = np.stack([xs * np.float32(0.1), np.zeros_like(xs), ys * np.float32(0.1)], -1).astype(np.float32)
+    nrm = np.zeros_like(pos)
+    nrm[..., 1] = 1
+    mask = np.ones((h, w), np.uint8)
+    assert np.all(O.rtao(tris, pos, nrm, mask, 64, 2.0) == 1.0)
+    assert np.all(O.ssao(pos, nrm, mask, 24, 2.0) == 1.0)
+
+
+def test_sampler_matches_cosine_closed_form():
+    """A point under an infinite plane at height hgt < R is blocked iff cos(theta) > hgt/R;
+    under cosine sampling P(cos > c) = 1 - c^2, so E[AO] = (hgt/R)^2."""
+    hgt, R = 0.5, 1.0
+    tris = np.array([[-1e3, hgt, -1e3, 1e3, hgt, 1e3, 1e3, hgt, -1e3], [-1e3, hgt, -1e3, -1e3, hgt, 1e3, 1e3, hgt, 1e3]],
+                    np.float32)
+    h = w = 96
+    pos = np.zeros((h, w, 3), np.float32)
+    pos[..., 1] = np.float32(-1e-3)   # the origin offset p + n * 1e-3 lands on y = 0
+    nrm = np.zeros_like(pos)
+    nrm[..., 1] = 1
+    ao = O.rtao(tris, pos, nrm, np.ones((h, w), np.uint8), 64, R)
+    n = h * w * 64
+    expect = (hgt / R) ** 2
+    assert abs(ao.mean() - expect) < 5 * np.sqrt(expect * (1 - expect) / n)
+
+
+def test_threads_do_not_change_bytes():
+    """Output identical for any thread count (raw/core/parallel.hpp:2-4)."""
+    _, _, _, a1, s1 = render("default", threads=1)
+    _, _, _, a5, s5 = render("default", threads=5)
+    assert np.array_equal(a1, a5) and np.array_equal(s1, s5)
