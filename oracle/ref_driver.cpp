@@ -102,6 +102,22 @@ void ref_rtao(const float* tris9, size_t ntri, int mode, const float* position, 
     std::memcpy(ao, out.px.data(), sizeof(float) * size_t(w) * size_t(h));
 }
 
+// The same, with the accelerator built once (LinearAccel::build, accel.cpp:13-25) and kept,
+// so a caller can time the build and the trace apart (BVH mode only).
+void* ref_accel_build(const float* tris9, size_t ntri) {
+    unsetenv("RAW_NATIVE_AO_ORACLE");
+    auto* a = new raw::LinearAccel;
+    a->build(triScene(tris9, ntri));
+    return a;
+}
+void ref_accel_free(void* a) { delete static_cast<raw::LinearAccel*>(a); }
+void ref_rtao_accel(void* accel, const float* position, const float* normal, const uint8_t* mask, int w, int h,
+                    int samples, float radius, int threads, float* ao) {
+    const raw::GBuffer g = toG(position, normal, mask, w, h);
+    const raw::Buffer<float> out = raw::computeRTAO(g, *static_cast<raw::LinearAccel*>(accel), samples, radius, nullptr, threads);
+    std::memcpy(ao, out.px.data(), sizeof(float) * size_t(w) * size_t(h));
+}
+
 void ref_ssao(const float* position, const float* normal, const uint8_t* mask, int w, int h, int samples,
               float radius, int threads, float* ao) {
     const raw::GBuffer g = toG(position, normal, mask, w, h);

@@ -354,6 +354,9 @@ int raw_ao_trace_counts(raw_ao_ctx* x, uint64_t* visits, uint64_t* tests, uint64
     Ctx& c = x->c;
     c.err.clear();
     if (!c.have_counts) return fail(c, RAW_AO_ERR_ARG, "no counting trace yet (raw_ao_count_next_trace)");
+    if (!hip_ok(c, hipSetDevice(c.device), "hipSetDevice") ||
+        !hip_ok(c, hipMemcpy(&c.last_counts, c.counters.p, sizeof c.last_counts, hipMemcpyDeviceToHost), "counters"))
+        return RAW_AO_ERR_DEVICE;
     if (visits) *visits = c.last_counts.visits;
     if (tests) *tests = c.last_counts.tests;
     if (rays) *rays = c.last_counts.rays;

@@ -83,6 +83,10 @@ def ref():
         L.ref_scene_tris.restype = SZ
         L.ref_binned_sah_cost.argtypes = lib().orc_binned_sah_cost.argtypes
         L.ref_binned_sah_cost.restype = C.c_double
+        L.ref_accel_build.argtypes = [f32p, SZ]
+        L.ref_accel_build.restype = C.c_void_p
+        L.ref_accel_free.argtypes = [C.c_void_p]
+        L.ref_rtao_accel.argtypes = [C.c_void_p, f32p, f32p, u8p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, f32p]
         _ref = L
     return _ref
 
