@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--cpu-row-step", type=int, default=4)
     ap.add_argument("--calib-copy", type=int, default=0, help="also run a plain device copy of this many MiB "
                     "(the known-size kernel that scales the HBM counters)")
+    ap.add_argument("--kernel", choices=["wide", "binary"], default="wide", help="traversal form")
+    ap.add_argument("--tri-q", type=int, default=0, help="wide kernel's triangle-phase threshold (1..64; 0: default)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the AO path's multi-GPU form (replicated BVH, row-split frame) is "
@@ -113,6 +115,9 @@ def main():
     d_max = torch.zeros(4, dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
     ctx = raw_ao.Context(0)
+    ctx.set_option(raw_ao.OPT_TRACE_KERNEL, raw_ao.TRACE_BINARY if a.kernel == "binary" else raw_ao.TRACE_WIDE)
+    if a.tri_q:
+        ctx.set_option(raw_ao.OPT_TRI_PHASE, a.tri_q)
 
     host_ms = []
 

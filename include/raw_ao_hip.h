@@ -22,6 +22,8 @@ extern "C" {
 #endif
 
 #define RAW_AO_ABI_VERSION 1u
+/* The largest scene a context builds (2^25 triangles: 32-bit node offsets). */
+#define RAW_AO_MAX_TRIANGLES (1u << 25)
 
 enum {
     RAW_AO_OK = 0,
@@ -51,8 +53,17 @@ enum {
      * (the reference calls std::cos/std::sin, src/renderer/ray_ao.cpp:36 and
      * src/renderer/ssao.cpp:21-24); 0: correctly rounded. */
     RAW_AO_OPT_HOST_LIBM_TRIG = 1,
-    /* The traversal kernel: 0 auto, 1 ray per lane. */
-    RAW_AO_OPT_TRACE_KERNEL = 2
+    /* The traversal form, chosen before raw_ao_build_ploc: RAW_AO_TRACE_WIDE (default)
+     * or RAW_AO_TRACE_BINARY. Results are the same; only the speed differs. */
+    RAW_AO_OPT_TRACE_KERNEL = 2,
+    /* Wide kernel tuning, 1..64 (default 16): a wave runs its triangle phase once this
+     * many 64ths of its live lanes wait on leaf triangles (or none can take a node step).
+     * Results are the same for every value; only the speed differs. */
+    RAW_AO_OPT_TRI_PHASE = 3
+};
+enum {
+    RAW_AO_TRACE_WIDE = 0,     /* 4-wide nodes, LDS stack, triangle tests batched per wave */
+    RAW_AO_TRACE_BINARY = 1    /* the binary PLOC tree, both children's boxes in one 64 B record */
 };
 int raw_ao_set_option(raw_ao_ctx* ctx, int option, int value);
 

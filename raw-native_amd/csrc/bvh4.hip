@@ -1,0 +1,573 @@
+// The 4-wide traversal form of the PLOC tree and the AO / general-ray kernels over it.
+//
+// Collapse (top down, one level of the wide tree a step, deterministic): a wide node starts
+// from an inner node of the binary tree with its two children as slots, then twice opens
+// the inner slot of largest surface area into its two children (ties to the lower slot),
+// so it has 2 to 4 slots. Inner slots become the next level's wide nodes; ids come from
+// prefix sums over the level in slot order, so the layout never depends on scheduling.
+//
+// Record (128 B, 8 x 16 B): slot boxes as structure-of-arrays (lo x of the 4 slots, lo y,
+// lo z, hi x, hi y, hi z), padded outward as in the binary form; then the 4 slot refs
+// (>= 0 a wide node, < 0 ~ a leaf slot = sorted triangle position, EMPTY unused); then
+// 16 B spare. One lane reads 7 x 16 B and tests 4 boxes.
+//
+// Traversal (one ray a lane): the per-lane stack's top STACK_LDS entries live in LDS,
+// deeper entries in a private spill array. A node step records the leaf slots its ray
+// hits as pending; a lane with pending leaves does no node step until the wave tests
+// them together, in a triangle phase the wave enters when at least a quarter of its live
+// lanes wait on leaves or none can step, so node and triangle code are not interleaved
+// lane by lane.
+#include "ao_math.hpp"
+#include "ctx.hpp"
+#include <algorithm>
+#include <cfloat>
+
+namespace raw_ao {
+namespace {
+
+constexpr int BT = 256;
+constexpr int32_t EMPTY = int32_t(0x80000000);
+constexpr float kExit = 1.00001f;
+constexpr float kReach = 1.0001f;
+constexpr float kFar = 3.0e38f;           // box of an empty slot: a point no finite reach gets to
+constexpr int STACK_LDS = 16;
+constexpr int STACK_SPILL = 48;
+
+struct alignas(16) Node4 { float4 lx, ly, lz, hx, hy, hz; int4 ref; float4 spare; };
+
+struct WideState { uint32_t count, base, next_base, pad; };
+
+__device__ __forceinline__ float pad_of(float l, float h) {
+    return (h - l) * 6.103515625e-05f + (fabsf(l) + fabsf(h)) * 9.5367431640625e-07f + 1e-30f;   // 2^-14, 2^-20
+}
+__device__ __forceinline__ float box_area(const float* b) {
+    const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+    return 2.0f * (dy * (dx + dz) + dz * dx);
+}
+__device__ __forceinline__ unsigned long long lanemask_lt() {
+    const unsigned lane = __lane_id();
+    return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+// Up to 4 slots (binary node ids) of the wide node rooted at binary inner node f.
+__device__ __forceinline__ int expand(const float* __restrict__ boxes, const int32_t* __restrict__ links, uint32_t n,
+                                      int32_t f, int32_t slot[4]) {
+    slot[0] = links[3 * size_t(f)];
+    slot[1] = links[3 * size_t(f) + 1];
+    int cnt = 2;
+    for (int e = 0; e < 2; ++e) {
+        int best = -1;
+        float ba = -1.0f;
+        for (int k = 0; k < cnt; ++k) {
+            if (uint32_t(slot[k]) < n) continue;   // a leaf
+            const float a = box_area(boxes + 6 * size_t(slot[k]));
+            if (best < 0 || a > ba) { best = k; ba = a; }
+        }
+        if (best < 0) break;
+        const int32_t x = slot[best];
+        for (int k = cnt; k > best + 1; --k) slot[k] = slot[k - 1];
+        slot[best] = links[3 * size_t(x)];
+        slot[best + 1] = links[3 * size_t(x) + 1];
+        ++cnt;
+    }
+    return cnt;
+}
+
+// Pass 1 of a level: each wide node's inner-slot count, summed per block.
+__global__ __launch_bounds__(BT) void k_wide_count(const WideState* __restrict__ st, const uint32_t* __restrict__ frontier,
+                                                   const float* __restrict__ boxes, const int32_t* __restrict__ links,
+                                                   uint32_t n, uint32_t* __restrict__ blk) {
+    __shared__ uint32_t ws[BT / 64];
+    const uint32_t p = blockIdx.x * BT + threadIdx.x;
+    uint32_t c = 0;
+    if (p < st->count) {
+        int32_t slot[4];
+        const int cnt = expand(boxes, links, n, int32_t(frontier[p]), slot);
+        for (int k = 0; k < cnt; ++k) c += uint32_t(slot[k]) >= n;
+    }
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int k = 0; k < BT / 64; ++k) s += ws[k];
+        blk[blockIdx.x] = s;
+    }
+}
+
+// Pass 2: block offsets (exclusive scan, one block of 1024) and the next level's size.
+__global__ __launch_bounds__(1024) void k_wide_scan(const WideState* __restrict__ in, WideState* __restrict__ out,
+                                                    uint32_t* __restrict__ blk, uint32_t nblk) {
+    __shared__ uint32_t s[1024];
+    const uint32_t per = (nblk + 1023) / 1024;
+    const uint32_t a = threadIdx.x * per, b = min(nblk, a + per);
+    uint32_t sum = 0;
+    for (uint32_t i = a; i < b; ++i) sum += blk[i];
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = threadIdx.x >= uint32_t(off) ? s[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = s[threadIdx.x] - sum;
+    for (uint32_t i = a; i < b; ++i) { const uint32_t v = blk[i]; blk[i] = run; run += v; }
+    if (threadIdx.x == 1023) {
+        const WideState i = *in;
+        WideState o;
+        o.count = s[1023];
+        o.base = i.base + i.count;
+        o.next_base = o.base + o.count;
+        o.pad = 0;
+        *out = o;
+    }
+}
+
+// Pass 3: write the records of this level and the next level's frontier.
+__global__ __launch_bounds__(BT) void k_wide_write(const WideState* __restrict__ st, const uint32_t* __restrict__ frontier,
+                                                   uint32_t* __restrict__ next, const uint32_t* __restrict__ blkoff,
+                                                   const float* __restrict__ boxes, const int32_t* __restrict__ links,
+                                                   uint32_t n, Node4* __restrict__ nodes) {
+    __shared__ uint32_t ws[BT / 64];
+    const uint32_t p = blockIdx.x * BT + threadIdx.x;
+    const WideState s = *st;
+    int32_t slot[4] = {EMPTY, EMPTY, EMPTY, EMPTY};
+    int cnt = 0;
+    uint32_t c = 0;
+    if (p < s.count) {
+        cnt = expand(boxes, links, n, int32_t(frontier[p]), slot);
+        for (int k = 0; k < cnt; ++k) c += uint32_t(slot[k]) >= n;
+    }
+    uint32_t x = c;   // inclusive wave scan of the inner-slot counts
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) ws[w] = x;
+    __syncthreads();
+    if (p >= s.count) return;
+    uint32_t off = blkoff[blockIdx.x] + x - c;
+    for (int k = 0; k < w; ++k) off += ws[k];
+    float bl[3][4], bh[3][4];
+    int32_t ref[4];
+    uint32_t masks = 0;   // leaf slots in bits 0-3, inner slots in bits 4-7
+    for (int k = 0; k < 4; ++k) {
+        if (k >= cnt) {
+            for (int a = 0; a < 3; ++a) { bl[a][k] = kFar; bh[a][k] = kFar; }
+            ref[k] = EMPTY;
+            continue;
+        }
+        masks |= uint32_t(slot[k]) < n ? (1u << k) : (16u << k);
+        const float* b = boxes + 6 * size_t(slot[k]);
+        for (int a = 0; a < 3; ++a) {
+            const float pd = pad_of(b[a], b[3 + a]);
+            bl[a][k] = b[a] - pd;
+            bh[a][k] = b[3 + a] + pd;
+        }
+        if (uint32_t(slot[k]) < n) {
+            ref[k] = ~slot[k];
+        } else {
+            ref[k] = int32_t(s.next_base + off);
+            next[off] = uint32_t(slot[k]);
+            ++off;
+        }
+    }
+    Node4 r;
+    r.lx = make_float4(bl[0][0], bl[0][1], bl[0][2], bl[0][3]);
+    r.ly = make_float4(bl[1][0], bl[1][1], bl[1][2], bl[1][3]);
+    r.lz = make_float4(bl[2][0], bl[2][1], bl[2][2], bl[2][3]);
+    r.hx = make_float4(bh[0][0], bh[0][1], bh[0][2], bh[0][3]);
+    r.hy = make_float4(bh[1][0], bh[1][1], bh[1][2], bh[1][3]);
+    r.hz = make_float4(bh[2][0], bh[2][1], bh[2][2], bh[2][3]);
+    r.ref = make_int4(ref[0], ref[1], ref[2], ref[3]);
+    r.spare = make_float4(__uint_as_float(masks), 0, 0, 0);
+    nodes[s.base + p] = r;
+}
+
+// A single-triangle scene: one record, the leaf in slot 0.
+__global__ void k_wide_single(const float* __restrict__ boxes, Node4* __restrict__ nodes) {
+    if (threadIdx.x != 0) return;
+    Node4 r;
+    float l[3], h[3];
+    for (int a = 0; a < 3; ++a) {
+        const float pd = pad_of(boxes[a], boxes[3 + a]);
+        l[a] = boxes[a] - pd;
+        h[a] = boxes[3 + a] + pd;
+    }
+    r.lx = make_float4(l[0], kFar, kFar, kFar);
+    r.ly = make_float4(l[1], kFar, kFar, kFar);
+    r.lz = make_float4(l[2], kFar, kFar, kFar);
+    r.hx = make_float4(h[0], kFar, kFar, kFar);
+    r.hy = make_float4(h[1], kFar, kFar, kFar);
+    r.hz = make_float4(h[2], kFar, kFar, kFar);
+    r.ref = make_int4(~0, EMPTY, EMPTY, EMPTY);
+    r.spare = make_float4(__uint_as_float(1u), 0, 0, 0);
+    nodes[0] = r;
+}
+
+// ---- traversal -------------------------------------------------------------------
+struct Ray {
+    V3 o, d, inv, oinv;
+    RayFrame f;
+};
+__device__ __forceinline__ Ray make_ray(V3 o, V3 d) {
+    Ray r;
+    r.o = o;
+    r.d = d;
+    r.inv = v3(d.x != 0.0f ? 1.0f / d.x : copysignf(1e30f, d.x), d.y != 0.0f ? 1.0f / d.y : copysignf(1e30f, d.y),
+               d.z != 0.0f ? 1.0f / d.z : copysignf(1e30f, d.z));
+    r.oinv = v3(o.x * r.inv.x, o.y * r.inv.y, o.z * r.inv.z);
+    r.f = ray_frame(d);
+    return r;
+}
+__device__ __forceinline__ bool slab1(const Ray& r, float lx, float ly, float lz, float hx, float hy, float hz, float reach,
+                                      float exit_scale) {
+    const float x0 = fmaf(lx, r.inv.x, -r.oinv.x), x1 = fmaf(hx, r.inv.x, -r.oinv.x);
+    const float y0 = fmaf(ly, r.inv.y, -r.oinv.y), y1 = fmaf(hy, r.inv.y, -r.oinv.y);
+    const float z0 = fmaf(lz, r.inv.z, -r.oinv.z), z1 = fmaf(hz, r.inv.z, -r.oinv.z);
+    const float t0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+    const float t1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), reach));
+    return t0 <= t1 * exit_scale;
+}
+// The 4 slots' hit bits (bit k: slot k's box is entered within reach).
+__device__ __forceinline__ uint32_t slab4(const Ray& r, const Node4& nd, float reach, float exit_scale) {
+    uint32_t m = 0;
+    m |= slab1(r, nd.lx.x, nd.ly.x, nd.lz.x, nd.hx.x, nd.hy.x, nd.hz.x, reach, exit_scale) ? 1u : 0u;
+    m |= slab1(r, nd.lx.y, nd.ly.y, nd.lz.y, nd.hx.y, nd.hy.y, nd.hz.y, reach, exit_scale) ? 2u : 0u;
+    m |= slab1(r, nd.lx.z, nd.ly.z, nd.lz.z, nd.hx.z, nd.hy.z, nd.hz.z, reach, exit_scale) ? 4u : 0u;
+    m |= slab1(r, nd.lx.w, nd.ly.w, nd.lz.w, nd.hx.w, nd.hy.w, nd.hz.w, reach, exit_scale) ? 8u : 0u;
+    return m;
+}
+// The same test with each axis' near and far planes picked by the ray's direction signs
+// (near = lo where 1/d >= 0, else hi): for a box with lo <= hi these are exactly the min
+// and max of the two plane distances above, so the hit bits are the same, with five
+// fewer operations a box. `np` is the node as 8 float4 (lo x, lo y, lo z, hi x, hi y,
+// hi z, refs, masks); ox/oy/oz are 0 or 3 by the signs.
+__device__ __forceinline__ float4 fma4(float4 p, float s, float c) {
+    return make_float4(fmaf(p.x, s, c), fmaf(p.y, s, c), fmaf(p.z, s, c), fmaf(p.w, s, c));
+}
+__device__ __forceinline__ float4 ld4(const char* __restrict__ base, uint32_t off) {
+    return *reinterpret_cast<const float4*>(base + off);
+}
+// base: the node array; off: the node's byte offset; ox/oy/oz: 0 or 48 (byte offset of
+// the far planes' float4 from the near ones' when 1/d < 0).
+__device__ __forceinline__ uint32_t slab4s(const Ray& r, const char* __restrict__ base, uint32_t off, uint32_t ox, uint32_t oy,
+                                           uint32_t oz, float reach, float exit_scale) {
+    const float4 nx = fma4(ld4(base, off + ox), r.inv.x, -r.oinv.x), fx = fma4(ld4(base, off + 48 - ox), r.inv.x, -r.oinv.x);
+    const float4 ny = fma4(ld4(base, off + 16 + oy), r.inv.y, -r.oinv.y), fy = fma4(ld4(base, off + 64 - oy), r.inv.y, -r.oinv.y);
+    const float4 nz = fma4(ld4(base, off + 32 + oz), r.inv.z, -r.oinv.z), fz = fma4(ld4(base, off + 80 - oz), r.inv.z, -r.oinv.z);
+    const auto one = [&](float a, float b, float c, float d, float e, float f) {
+        const float t0 = fmaxf(fmaxf(a, b), fmaxf(c, 0.0f));
+        const float t1 = fminf(fminf(d, e), fminf(f, reach));
+        return t0 <= t1 * exit_scale;
+    };
+    return (one(nx.x, ny.x, nz.x, fx.x, fy.x, fz.x) ? 1u : 0u) | (one(nx.y, ny.y, nz.y, fx.y, fy.y, fz.y) ? 2u : 0u) |
+           (one(nx.z, ny.z, nz.z, fx.z, fy.z, fz.z) ? 4u : 0u) | (one(nx.w, ny.w, nz.w, fx.w, fy.w, fz.w) ? 8u : 0u);
+}
+__device__ __forceinline__ int32_t ref_of(const int4& r, int k) {
+    const int32_t lo = (k & 1) ? r.y : r.x, hi = (k & 1) ? r.w : r.z;
+    return (k & 2) ? hi : lo;
+}
+__device__ __forceinline__ bool tri_hit(const Ray& r, const float4* __restrict__ ts, uint32_t slot, float& t) {
+    const float4 p0 = ts[3 * size_t(slot)], p1 = ts[3 * size_t(slot) + 1], p2 = ts[3 * size_t(slot) + 2];
+    return intersect_t(r.o, r.d, r.f, v3(p0.x, p0.y, p0.z), v3(p0.w, p1.x, p1.y), v3(p1.z, p1.w, p2.x), t);
+}
+
+// The per-lane stack: the top STACK_LDS entries live in LDS (a column per lane, entry e
+// at lds_col[e * BT]), deeper ones in a private spill array; the stack pointer stays in a
+// register.
+// Any hit with t < tmax through the wide tree. Returns 1 occluded, 0 open, -1 the stack
+// would overflow (the caller answers that ray by the scan).
+template <bool COUNT>
+__device__ int occluded4(const Ray& r, float tmax, const Node4* __restrict__ nodes, const float4* __restrict__ ts,
+                         uint32_t ntri, uint32_t* lds_col, uint32_t tri_q, uint32_t& visits, uint32_t& tests) {
+    const float reach = tmax * kReach;
+    const uint32_t ox = r.inv.x >= 0.0f ? 0u : 48u, oy = r.inv.y >= 0.0f ? 0u : 48u, oz = r.inv.z >= 0.0f ? 0u : 48u;
+    const char* const base = reinterpret_cast<const char*>(nodes);
+    uint32_t spill[STACK_SPILL];
+    int sp = 0;
+    int32_t node = 0;                 // the wide root
+    uint32_t pend = 0;                // pending leaf slots of `leafref`
+    int4 leafref = make_int4(0, 0, 0, 0);
+    int result = 0;
+    bool live = true;
+    // A lane's node steps are bounded by the tree's record count; more means a malformed
+    // tree, and the lane stops with -1 (answered by the scan). Every iteration some live
+    // lane steps or tests, so the loop ends.
+    uint32_t work = 0;
+    while (true) {
+        // node step: lanes with a node and nothing pending
+        if (live && pend == 0 && node >= 0) {
+            const uint32_t off = uint32_t(node) * uint32_t(sizeof(Node4));
+            if (COUNT) ++visits;
+            const uint32_t hit = slab4s(r, base, off, ox, oy, oz, reach, kExit);
+            const int4 ref = *reinterpret_cast<const int4*>(base + off + 96);
+            const uint32_t masks = *reinterpret_cast<const uint32_t*>(base + off + 112);   // leaf slots bits 0-3, inner 4-7
+            pend = hit & masks & 15u;
+            leafref = ref;
+            uint32_t ih = hit & (masks >> 4);
+            if (ih) {
+                node = ref_of(ref, __builtin_ctz(ih));
+                ih &= ih - 1u;
+                while (ih) {
+                    const uint32_t v = uint32_t(ref_of(ref, __builtin_ctz(ih)));
+                    ih &= ih - 1u;
+                    if (sp < STACK_LDS) lds_col[sp * BT] = v;
+                    else if (sp < STACK_LDS + STACK_SPILL) spill[sp - STACK_LDS] = v;
+                    else { result = -1; live = false; }
+                    ++sp;
+                }
+            } else if (sp > 0) {
+                --sp;
+                node = int32_t(sp < STACK_LDS ? lds_col[sp * BT] : spill[sp - STACK_LDS]);
+            } else {
+                node = -1;
+            }
+            if (++work > ntri + 4) { result = -1; live = false; }
+        }
+        const bool waiting = live && pend != 0;
+        const bool can_step = live && pend == 0 && node >= 0;
+        const unsigned long long wb = __ballot(waiting), sb = __ballot(can_step), lb = __ballot(live);
+        if (lb == 0) break;
+        // triangle step: when tri_q/64 of the live lanes wait on leaves, or none can step
+        if (sb == 0 || 64u * uint32_t(__popcll(wb)) >= tri_q * uint32_t(__popcll(lb))) {
+            if (waiting) {
+                const int k = __builtin_ctz(pend);
+                pend &= pend - 1u;
+                const uint32_t slot = uint32_t(~ref_of(leafref, k));
+                float t;
+                if (COUNT) ++tests;
+                if (slot < ntri && tri_hit(r, ts, slot, t) && t < tmax) { result = 1; live = false; }
+            }
+        }
+        if (live && pend == 0 && node < 0) live = false;   // finished: open
+    }
+    return result;
+}
+
+__device__ bool scan_occluded(const Ray& r, float tmax, const float4* __restrict__ ts, uint32_t ntri) {
+    for (uint32_t i = 0; i < ntri; ++i) {
+        float t;
+        if (tri_hit(r, ts, i, t) && t < tmax) return true;
+    }
+    return false;
+}
+
+struct TraceArgs4 {
+    const float* pos;
+    const float* nrm;
+    const uint8_t* mask;
+    float* ao;
+    uint32_t* cnt;
+    const Node4* nodes;
+    const float4* tris;
+    const uint8_t* trig;
+    TraceCounters* counters;
+    uint32_t w, npix, spp, ntri, tri_q;
+    float radius;
+};
+
+template <bool SPP64, bool COUNT>
+__global__ __launch_bounds__(BT) void k_trace_ao4(TraceArgs4 a) {
+    __shared__ uint32_t stk[STACK_LDS * BT];
+    const uint64_t gid = uint64_t(blockIdx.x) * BT + threadIdx.x;
+    const uint32_t pix = SPP64 ? uint32_t(gid >> 6) : uint32_t(gid / a.spp);
+    if (pix >= a.npix) return;
+    const uint32_t s = uint32_t(gid - uint64_t(pix) * a.spp);
+    if (!a.mask[pix]) {
+        if (s == 0) a.ao[pix] = 1.0f;
+        return;
+    }
+    const V3 p = v3(a.pos[3 * size_t(pix)], a.pos[3 * size_t(pix) + 1], a.pos[3 * size_t(pix) + 2]);
+    const V3 n = v3(a.nrm[3 * size_t(pix)], a.nrm[3 * size_t(pix) + 1], a.nrm[3 * size_t(pix) + 2]);
+    const uint32_t y = pix / a.w, x = pix - y * a.w;
+    V3 t, b;
+    basis(n, t, b);
+    const V3 origin = add(p, scl(n, 1e-3f));
+    const uint32_t k1 = hash24(x, y, 2 * s), k2 = hash24(x, y, 2 * s + 1);
+    const float u1 = float(k1) / float(0x1000000), u2 = float(k2) / float(0x1000000);
+    float sphi, cphi;
+    sincos_base(6.2831853f * u2, sphi, cphi);
+    if (a.trig) trig_adjust((a.trig[k2 >> 1] >> ((k2 & 1u) * 4)) & 15u, sphi, cphi);
+    const V3 dir = ao_dir(u1, sphi, cphi, n, t, b);
+    bool occ = false;
+    uint32_t visits = 0, tests = 0;
+    if (a.ntri) {
+        const Ray r = make_ray(origin, dir);
+        const int res = occluded4<COUNT>(r, a.radius, a.nodes, a.tris, a.ntri, stk + threadIdx.x, a.tri_q, visits, tests);
+        if (res < 0) {
+            occ = scan_occluded(r, a.radius, a.tris, a.ntri);
+            atomicAdd(&a.counters->overflows, 1ull);
+        } else {
+            occ = res == 1;
+        }
+    }
+    if (COUNT) {
+        atomicAdd(&a.counters->visits, (unsigned long long)visits);
+        atomicAdd(&a.counters->tests, (unsigned long long)tests);
+        atomicAdd(&a.counters->rays, 1ull);
+    }
+    if (SPP64) {
+        const unsigned long long open = __ballot(!occ);
+        if (s == 0) a.ao[pix] = float(__popcll(open)) / 64.0f;
+    } else if (!occ) {
+        atomicAdd(&a.cnt[pix], 1u);
+    }
+}
+
+// Nearest (ties to the lower triangle index) or any hit of general rays.
+__global__ __launch_bounds__(BT) void k_trace_rays4(const float* __restrict__ rays8, uint32_t nrays, int any, float far_scale,
+                                                    const Node4* __restrict__ nodes, const float4* __restrict__ ts,
+                                                    uint32_t ntri, float* __restrict__ out4) {
+    __shared__ uint32_t stk[STACK_LDS * BT];
+    const uint32_t k = blockIdx.x * BT + threadIdx.x;
+    if (k >= nrays) return;
+    const float* q = rays8 + 8 * size_t(k);
+    const Ray r = make_ray(v3(q[0], q[1], q[2]), v3(q[3], q[4], q[5]));
+    const float tmax = q[6];
+    const float exit_scale = far_scale >= 1.0f ? fmaxf(far_scale, kExit) : far_scale;
+    float best = tmax;
+    int32_t bi = -1;
+    uint32_t visits = 0, tests = 0;
+    bool overflow = false;
+    if (ntri) {
+        uint32_t* const lds_col = stk + threadIdx.x;
+        uint32_t spill[STACK_SPILL];
+        int sp = 0;
+        int32_t node = 0;
+        bool done = false;
+        while (node >= 0 && !done) {
+            if (visits > 4 * ntri + 4) { overflow = true; break; }   // a malformed tree
+            const Node4 nd = nodes[node];
+            ++visits;
+            const uint32_t hit = slab4(r, nd, best * kReach, exit_scale);
+            int32_t next = -1;
+            for (int c = 0; c < 4 && !done; ++c) {
+                if (!((hit >> c) & 1u)) continue;
+                const int32_t ch = ref_of(nd.ref, c);
+                if (ch == EMPTY) continue;
+                if (ch < 0) {
+                    const uint32_t slot = uint32_t(~ch);
+                    if (slot >= ntri) continue;
+                    float t;
+                    ++tests;
+                    if (!tri_hit(r, ts, slot, t)) continue;
+                    const int32_t id = __float_as_int(ts[3 * size_t(slot) + 2].y);
+                    if (t < best || (bi >= 0 && t == best && id < bi)) {
+                        best = t;
+                        bi = id;
+                        if (any) done = true;
+                    }
+                } else if (next < 0) {
+                    next = ch;
+                } else if (sp < STACK_LDS) {
+                    lds_col[sp++ * BT] = uint32_t(ch);
+                } else if (sp < STACK_LDS + STACK_SPILL) {
+                    spill[sp++ - STACK_LDS] = uint32_t(ch);
+                } else {
+                    overflow = true;
+                    done = true;
+                }
+            }
+            if (done) break;
+            if (next >= 0) node = next;
+            else if (sp > 0) { --sp; node = int32_t(sp < STACK_LDS ? lds_col[sp * BT] : spill[sp - STACK_LDS]); }
+            else node = -1;
+        }
+    }
+    if (overflow) { best = -1.0f; bi = -2; }
+    out4[4 * size_t(k)] = bi >= 0 ? best : -1.0f;
+    out4[4 * size_t(k) + 1] = float(bi);
+    out4[4 * size_t(k) + 2] = float(visits);
+    out4[4 * size_t(k) + 3] = float(tests);
+}
+
+uint32_t cdiv(uint64_t a, uint32_t b) { return uint32_t((a + b - 1) / b); }
+
+}  // namespace
+
+int wide_build(Ctx& c, hipStream_t s) {
+    const uint32_t n = c.ntri;
+    if (n == 0) return 0;
+    if (!ensure(c, c.nodes4, sizeof(Node4) * std::max<size_t>(1, n - 1)) || !ensure(c, c.wfront_a, 4 * size_t(n)) ||
+        !ensure(c, c.wfront_b, 4 * size_t(n)) || !ensure(c, c.wblk, 4 * size_t(cdiv(n, BT) + 1)) ||
+        !ensure(c, c.wstate, 2 * sizeof(WideState)))
+        return -2;
+    if (n == 1) {
+        k_wide_single<<<1, 64, 0, s>>>(c.boxes.as<float>(), c.nodes4.as<Node4>());
+        c.wide_nodes = 1;
+        return hip_ok(c, hipGetLastError(), "wide launch") ? 0 : -2;
+    }
+    WideState init{1, 0, 1, 0};
+    const uint32_t root = uint32_t(c.root);
+    if (!hip_ok(c, hipMemcpyAsync(c.wfront_a.p, &root, 4, hipMemcpyHostToDevice, s), "frontier") ||
+        !hip_ok(c, hipMemcpyAsync(c.wstate.p, &init, sizeof init, hipMemcpyHostToDevice, s), "wide state") ||
+        !hip_ok(c, hipStreamSynchronize(s), "wide init"))
+        return -2;
+    WideState* d_st = c.wstate.as<WideState>();
+    uint32_t *fin = c.wfront_a.as<uint32_t>(), *fout = c.wfront_b.as<uint32_t>();
+    uint64_t ub = 1;   // upper bound of this level's size (grids are sized by it)
+    int parity = 0;
+    for (int level = 0;; ++level) {
+        const uint32_t nb = cdiv(ub, BT);
+        k_wide_count<<<nb, BT, 0, s>>>(d_st + parity, fin, c.boxes.as<float>(), c.links.as<int32_t>(), n, c.wblk.as<uint32_t>());
+        k_wide_scan<<<1, 1024, 0, s>>>(d_st + parity, d_st + (parity ^ 1), c.wblk.as<uint32_t>(), nb);
+        k_wide_write<<<nb, BT, 0, s>>>(d_st + parity, fin, fout, c.wblk.as<uint32_t>(), c.boxes.as<float>(),
+                                       c.links.as<int32_t>(), n, c.nodes4.as<Node4>());
+        parity ^= 1;
+        std::swap(fin, fout);
+        ub = std::min<uint64_t>(4 * ub, n);
+        if ((level & 3) == 3 || ub >= n) {   // read the real size back now and then
+            WideState h;
+            if (!hip_ok(c, hipMemcpyAsync(&h, d_st + parity, sizeof h, hipMemcpyDeviceToHost, s), "wide state") ||
+                !hip_ok(c, hipStreamSynchronize(s), "wide level"))
+                return -2;
+            if (h.count == 0) { c.wide_nodes = h.base; break; }
+            ub = h.count;
+        }
+        if (level > 4 * 64) { c.err = "wide collapse did not finish"; return -2; }
+    }
+    return hip_ok(c, hipGetLastError(), "wide build") ? 0 : -2;
+}
+
+int trace_ao4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int spp, float radius,
+              float* d_ao, hipStream_t s) {
+    const uint32_t npix = uint32_t(w) * uint32_t(h);
+    if (!ensure(c, c.counters, sizeof(TraceCounters))) return -2;
+    if (!hip_ok(c, hipMemsetAsync(c.counters.p, 0, sizeof(TraceCounters), s), "counter clear")) return -2;
+    TraceArgs4 a{d_pos, d_nrm, d_mask, d_ao, nullptr, c.nodes4.as<Node4>(), c.tris_sorted.as<float4>(),
+                 c.host_trig ? c.d_trig : nullptr, c.counters.as<TraceCounters>(), uint32_t(w), npix, uint32_t(spp),
+                 c.ntri, uint32_t(c.tri_q), radius};
+    const bool spp64 = spp == 64;
+    if (!spp64) {
+        if (!ensure(c, c.st_cnt, 4 * size_t(npix))) return -2;
+        a.cnt = c.st_cnt.as<uint32_t>();
+        if (!hip_ok(c, hipMemsetAsync(a.cnt, 0, 4 * size_t(npix), s), "count clear")) return -2;
+    }
+    const uint32_t blocks = cdiv(uint64_t(npix) * uint64_t(spp), BT);
+    const bool count = c.count_next;
+    (void)hipEventRecord(c.ev[2], s);
+    if (spp64) {
+        if (count) k_trace_ao4<true, true><<<blocks, BT, 0, s>>>(a);
+        else k_trace_ao4<true, false><<<blocks, BT, 0, s>>>(a);
+    } else {
+        if (count) k_trace_ao4<false, true><<<blocks, BT, 0, s>>>(a);
+        else k_trace_ao4<false, false><<<blocks, BT, 0, s>>>(a);
+    }
+    (void)hipEventRecord(c.ev[3], s);
+    if (!spp64) ao_finish(c, d_mask, a.cnt, npix, uint32_t(spp), d_ao, s);
+    if (!hip_ok(c, hipGetLastError(), "trace launch")) return -2;
+    c.have_counts = count;
+    return 0;
+}
+
+int trace_rays4(Ctx& c, const float* d_rays, uint32_t n, int any, float far_scale, float* d_out, hipStream_t s) {
+    if (n == 0) return 0;
+    k_trace_rays4<<<cdiv(n, BT), BT, 0, s>>>(d_rays, n, any, far_scale, c.nodes4.as<Node4>(), c.tris_sorted.as<float4>(),
+                                              c.ntri, d_out);
+    return hip_ok(c, hipGetLastError(), "trace_rays launch") ? 0 : -2;
+}
+
+}  // namespace raw_ao

@@ -96,9 +96,9 @@ void raw_ao_close(raw_ao_ctx* x) {
     if (c.stream) {
         (void)hipSetDevice(c.device);
         (void)hipStreamSynchronize(c.stream);
-        raw_ao::DevBuf* bufs[] = {&c.tris_in, &c.keys, &c.vals, &c.keys2, &c.vals2, &c.hist, &c.boxes, &c.links,
+        raw_ao::DevBuf* bufs[] = {&c.tris_in, &c.keys, &c.vals, &c.keys2, &c.vals2, &c.hist, &c.rs_tot, &c.boxes, &c.links,
                                   &c.clus_a, &c.clus_b, &c.nn, &c.flags, &c.blk, &c.state, &c.g4, &c.bounds_part,
-                                  &c.nodes, &c.tris_sorted, &c.st_pos, &c.st_nrm, &c.st_mask, &c.st_out, &c.st_out2,
+                                  &c.nodes, &c.tris_sorted, &c.nodes4, &c.wfront_a, &c.wfront_b, &c.wblk, &c.wstate, &c.st_pos, &c.st_nrm, &c.st_mask, &c.st_out, &c.st_out2,
                                   &c.st_cnt, &c.st_rays, &c.st_max, &c.counters};
         for (auto* b : bufs)
             if (b->p) (void)hipFree(b->p);
@@ -118,7 +118,16 @@ int raw_ao_set_option(raw_ao_ctx* x, int option, int value) {
     Ctx& c = x->c;
     c.err.clear();
     if (option == RAW_AO_OPT_HOST_LIBM_TRIG) { c.host_trig = value != 0; return RAW_AO_OK; }
-    if (option == RAW_AO_OPT_TRACE_KERNEL) { c.trace_kernel = value; return RAW_AO_OK; }
+    if (option == RAW_AO_OPT_TRACE_KERNEL) {
+        if (value != RAW_AO_TRACE_WIDE && value != RAW_AO_TRACE_BINARY) return fail(c, RAW_AO_ERR_ARG, "unknown trace kernel");
+        c.trace_kernel = value;
+        return RAW_AO_OK;
+    }
+    if (option == RAW_AO_OPT_TRI_PHASE) {
+        if (value < 1 || value > 64) return fail(c, RAW_AO_ERR_ARG, "triangle phase threshold must be in 1..64");
+        c.tri_q = value;
+        return RAW_AO_OK;
+    }
     return fail(c, RAW_AO_ERR_ARG, "unknown option");
 }
 
@@ -127,7 +136,7 @@ int raw_ao_build_ploc_dev(raw_ao_ctx* x, const float* d_tris9, uint32_t ntri, in
     Ctx& c = x->c;
     if (!begin(c)) return RAW_AO_ERR_DEVICE;
     if (ntri && !d_tris9) return fail(c, RAW_AO_ERR_ARG, "null triangles");
-    if (ntri > (1u << 30)) return fail(c, RAW_AO_ERR_ARG, "too many triangles");
+    if (ntri > RAW_AO_MAX_TRIANGLES) return fail(c, RAW_AO_ERR_ARG, "more than RAW_AO_MAX_TRIANGLES triangles");
     hipStream_t s = pick(c, stream);
     if (ntri) {
         if (!ensure(c, c.tris_in, 36 * size_t(ntri))) return RAW_AO_ERR_DEVICE;
@@ -138,6 +147,8 @@ int raw_ao_build_ploc_dev(raw_ao_ctx* x, const float* d_tris9, uint32_t ntri, in
     (void)hipEventRecord(c.ev[0], s);
     int r = raw_ao::ploc_build(c, c.tris_in.as<float>(), ntri, radius, s);
     if (r == 0) r = raw_ao::bvh_layout(c, s);
+    if (r == 0 && c.trace_kernel != RAW_AO_TRACE_BINARY) r = raw_ao::wide_build(c, s);
+    c.built_kernel = c.trace_kernel;
     (void)hipEventRecord(c.ev[1], s);
     if (r != 0) return code_of(r);
     if (!hip_ok(c, hipStreamSynchronize(s), "build")) return RAW_AO_ERR_DEVICE;
@@ -150,7 +161,7 @@ int raw_ao_build_ploc(raw_ao_ctx* x, const float* tris9, uint32_t ntri, int radi
     Ctx& c = x->c;
     if (!begin(c)) return RAW_AO_ERR_DEVICE;
     if (ntri && !tris9) return fail(c, RAW_AO_ERR_ARG, "null triangles");
-    if (ntri > (1u << 30)) return fail(c, RAW_AO_ERR_ARG, "too many triangles");
+    if (ntri > RAW_AO_MAX_TRIANGLES) return fail(c, RAW_AO_ERR_ARG, "more than RAW_AO_MAX_TRIANGLES triangles");
     if (ntri) {
         if (!ensure(c, c.tris_in, 36 * size_t(ntri)) ||
             !hip_ok(c, hipMemcpyAsync(c.tris_in.p, tris9, 36 * size_t(ntri), hipMemcpyHostToDevice, c.stream), "tri upload"))
@@ -213,7 +224,11 @@ int raw_ao_trace_ao_dev(raw_ao_ctx* x, const float* d_pos, const float* d_nrm, c
         const int r = raw_ao::trig_table_upload(c);
         if (r) return r;
     }
-    return code_of(raw_ao::trace_ao(c, d_pos, d_nrm, d_mask, w, h, spp, radius, d_ao, pick(c, stream)));
+    if (c.ntri && c.trace_kernel != c.built_kernel)
+        return fail(c, RAW_AO_ERR_ARG, "the trace kernel option changed after the build: build again");
+    if (c.trace_kernel == RAW_AO_TRACE_BINARY)
+        return code_of(raw_ao::trace_ao(c, d_pos, d_nrm, d_mask, w, h, spp, radius, d_ao, pick(c, stream)));
+    return code_of(raw_ao::trace_ao4(c, d_pos, d_nrm, d_mask, w, h, spp, radius, d_ao, pick(c, stream)));
 }
 
 int raw_ao_trace_ao(raw_ao_ctx* x, const float* pos, const float* nrm, const uint8_t* mask, int w, int h, int spp,
@@ -334,7 +349,11 @@ int raw_ao_trace_rays(raw_ao_ctx* x, const float* rays8, uint32_t nrays, int any
     hipStream_t s = c.stream;
     if (!hip_ok(c, hipMemcpyAsync(c.st_rays.p, rays8, 32 * size_t(nrays), hipMemcpyHostToDevice, s), "upload"))
         return RAW_AO_ERR_DEVICE;
-    const int r = raw_ao::trace_rays(c, c.st_rays.as<float>(), nrays, any, far_scale, c.st_out2.as<float>(), s);
+    if (c.ntri && c.trace_kernel != c.built_kernel)
+        return fail(c, RAW_AO_ERR_ARG, "the trace kernel option changed after the build: build again");
+    const int r = c.trace_kernel == RAW_AO_TRACE_BINARY
+                      ? raw_ao::trace_rays(c, c.st_rays.as<float>(), nrays, any, far_scale, c.st_out2.as<float>(), s)
+                      : raw_ao::trace_rays4(c, c.st_rays.as<float>(), nrays, any, far_scale, c.st_out2.as<float>(), s);
     if (r) return code_of(r);
     if (!hip_ok(c, hipMemcpyAsync(out4, c.st_out2.p, 16 * size_t(nrays), hipMemcpyDeviceToHost, s), "download") ||
         !hip_ok(c, hipStreamSynchronize(s), "trace_rays"))
@@ -400,9 +419,10 @@ int raw_ao_bvh_bytes(raw_ao_ctx* x, uint64_t* node_bytes, uint64_t* tri_bytes, u
     Ctx& c = x->c;
     c.err.clear();
     const uint64_t n = c.ntri;
-    if (node_bytes) *node_bytes = n ? 64ull * (n > 1 ? n - 1 : 1) : 0;
+    const bool wide = c.built_kernel == RAW_AO_TRACE_WIDE;
+    if (node_bytes) *node_bytes = !n ? 0 : (wide ? 128ull * c.wide_nodes : 64ull * (n > 1 ? n - 1 : 1));
     if (tri_bytes) *tri_bytes = 48ull * n;
-    if (node_record_bytes) *node_record_bytes = 64;
+    if (node_record_bytes) *node_record_bytes = wide ? 128 : 64;
     return RAW_AO_OK;
 }
 

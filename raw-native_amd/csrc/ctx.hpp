@@ -21,7 +21,10 @@ struct PlocState {
     uint32_t m;        // clusters this round
     uint32_t next_id;  // id of the next internal node
     uint32_t merges;   // merges made by the round that produced this state
-    uint32_t pad;
+    uint32_t stalled;  // sticky: some round with m > 1 made no merge
+    uint32_t rounds;   // rounds with m > 1 so far
+    uint32_t root;     // the last cluster (final state only)
+    uint32_t pad[2];
 };
 
 struct TraceCounters {
@@ -33,7 +36,9 @@ struct Ctx {
     hipStream_t stream{nullptr};
     std::string err;
     bool host_trig{true};
-    int trace_kernel{0};
+    int trace_kernel{0};         // RAW_AO_TRACE_WIDE or RAW_AO_TRACE_BINARY
+    int built_kernel{0};         // the form the last build made
+    int tri_q{16};               // wide kernel: triangle phase when tri_q/64 of live lanes wait
     bool count_next{false};
 
     // host-libm sin/cos correction table, one nibble per 24-bit angle index
@@ -46,17 +51,20 @@ struct Ctx {
     int32_t root{-1};
     uint32_t rounds{0};
     DevBuf tris_in;              // n x 9 floats, input order
-    DevBuf keys, vals, keys2, vals2, hist;
+    DevBuf keys, vals, keys2, vals2, hist, rs_tot;
     DevBuf boxes, links;         // (2n-1) x 6 floats, (2n-1) x 3 ints: the reference's node order
     DevBuf clus_a, clus_b, nn, flags, blk;
     DevBuf state;                // PlocState[2]
     PlocState* h_state{nullptr}; // pinned mirror of the state read back each round
     DevBuf g4;                   // centroid box lo xyz and scale
     DevBuf bounds_part;
-    // traversal form
-    DevBuf nodes;                // BVH2 records, 64 B each
+    // traversal forms
+    DevBuf nodes;                // binary records, 64 B each (the ray-per-lane binary kernel)
     DevBuf tris_sorted;          // 48 B each, sorted (leaf) order
     uint32_t trav_root{0};
+    DevBuf nodes4;               // 4-wide records, 128 B each, root at 0
+    DevBuf wfront_a, wfront_b, wblk, wstate;
+    uint32_t wide_nodes{0};
 
     // staging for host-pointer entry points
     DevBuf st_pos, st_nrm, st_mask, st_out, st_out2, st_cnt, st_rays, st_max;
@@ -78,6 +86,11 @@ int trig_table_upload(Ctx& c);
 int trace_ao(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int spp,
              float radius, float* d_ao, hipStream_t s);
 int angle_table(Ctx& c, float* d_sin, float* d_cos, hipStream_t s);
+int ao_finish(Ctx& c, const uint8_t* d_mask, const uint32_t* d_cnt, uint32_t npix, uint32_t spp, float* d_ao, hipStream_t s);
+int wide_build(Ctx& c, hipStream_t s);
+int trace_ao4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int spp,
+              float radius, float* d_ao, hipStream_t s);
+int trace_rays4(Ctx& c, const float* d_rays, uint32_t n, int any, float far_scale, float* d_out, hipStream_t s);
 int trace_rays(Ctx& c, const float* d_rays, uint32_t n, int any, float far_scale, float* d_out, hipStream_t s);
 int ssao(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int samples,
          float radius, float* d_ao, hipStream_t s);

@@ -73,11 +73,22 @@ __global__ __launch_bounds__(BT) void k_bounds_part(const float* __restrict__ tr
         part[6 * blockIdx.x + threadIdx.x] = v;
     }
 }
-__global__ void k_bounds_final(const float* __restrict__ part, int nparts, float* __restrict__ g4) {
-    if (threadIdx.x != 0) return;
+__global__ __launch_bounds__(BT) void k_bounds_final(const float* __restrict__ part, int nparts, float* __restrict__ g4) {
     float l[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, h[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-    for (int p = 0; p < nparts; ++p)
+    for (int p = threadIdx.x; p < nparts; p += BT)
         for (int k = 0; k < 3; ++k) { l[k] = stdmin(l[k], part[6 * p + k]); h[k] = stdmax(h[k], part[6 * p + 3 + k]); }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int k = 0; k < 3; ++k) {
+            l[k] = stdmin(l[k], __shfl_xor(l[k], off));
+            h[k] = stdmax(h[k], __shfl_xor(h[k], off));
+        }
+    __shared__ float s6[BT / 64][6];
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 3; ++k) { s6[threadIdx.x >> 6][k] = l[k]; s6[threadIdx.x >> 6][3 + k] = h[k]; }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int i = 1; i < BT / 64; ++i)
+        for (int k = 0; k < 3; ++k) { l[k] = stdmin(l[k], s6[i][k]); h[k] = stdmax(h[k], s6[i][3 + k]); }
     const float extent = stdmax(h[0] - l[0], stdmax(h[1] - l[1], h[2] - l[2]));
     float s = 1.0f;   // pow2Scale (rt_bvh.cpp:38-44)
     if (extent > 0.0f) {
@@ -115,33 +126,56 @@ __global__ __launch_bounds__(BT) void k_radix_hist(const uint32_t* __restrict__ 
     hist[threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Exclusive scan of `count` words in place, one block of 1024 threads.
-__global__ __launch_bounds__(1024) void k_scan_excl(uint32_t* __restrict__ data, uint32_t count, uint32_t* __restrict__ total) {
-    __shared__ uint32_t s[1024];
-    const uint32_t per = (count + 1023) / 1024;
-    const uint32_t a = threadIdx.x * per, b = min(count, a + per);
-    uint32_t sum = 0;
-    for (uint32_t i = a; i < b; ++i) sum += data[i];
-    s[threadIdx.x] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = threadIdx.x >= uint32_t(off) ? s[threadIdx.x - off] : 0u;
+// One block a digit: the exclusive scan of that digit's counts over the tiles (a contiguous
+// row of the [digit][tile] histogram), in place, and the digit's total.
+__global__ __launch_bounds__(BT) void k_radix_colscan(uint32_t* __restrict__ hist, uint32_t nblk, uint32_t* __restrict__ totals) {
+    __shared__ uint32_t wsum[BT / 64];
+    uint32_t* row = hist + size_t(blockIdx.x) * nblk;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nblk; base += BT) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < nblk ? row[i] : 0u;
+        uint32_t x = v;   // inclusive wave scan
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
         __syncthreads();
-        s[threadIdx.x] += v;
+        uint32_t before = carry;
+        for (int k = 0; k < w; ++k) before += wsum[k];
+        if (i < nblk) row[i] = before + x - v;
+        uint32_t tile = 0;
+        for (int k = 0; k < BT / 64; ++k) tile += wsum[k];
+        carry += tile;
         __syncthreads();
     }
-    uint32_t run = s[threadIdx.x] - sum;
-    for (uint32_t i = a; i < b; ++i) { const uint32_t v = data[i]; data[i] = run; run += v; }
-    if (total && threadIdx.x == 1023) *total = s[1023];
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
 __global__ __launch_bounds__(BT) void k_radix_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                       uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n,
-                                                      int shift, const uint32_t* __restrict__ offs, uint32_t nblk) {
+                                                      int shift, const uint32_t* __restrict__ offs, uint32_t nblk,
+                                                      const uint32_t* __restrict__ totals) {
     __shared__ uint32_t run[256];
     __shared__ uint32_t wcnt[BT / 64][256];
     const int w = threadIdx.x >> 6;
-    run[threadIdx.x] = offs[threadIdx.x * nblk + blockIdx.x];
+    {   // digit bases: exclusive scan of the 256 digit totals
+        const uint32_t v = totals[threadIdx.x];
+        uint32_t x = v;
+        const int lane = threadIdx.x & 63;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wcnt[0][w] = x;
+        __syncthreads();
+        uint32_t before = 0;
+        for (int k = 0; k < w; ++k) before += wcnt[0][k];
+        run[threadIdx.x] = before + x - v + offs[threadIdx.x * nblk + blockIdx.x];
+        __syncthreads();
+    }
     const unsigned long long lt = lanemask_lt();
     for (int it = 0; it < RS_ITEMS; ++it) {
         for (int k = 0; k < BT / 64; ++k) wcnt[k][threadIdx.x] = 0;
@@ -281,11 +315,13 @@ __global__ __launch_bounds__(1024) void k_ploc_scan(const PlocState* __restrict_
         for (uint32_t i = a; i < b; ++i) { const uint32_t v = counts[k * nblk + i]; counts[k * nblk + i] = run; run += v; }
     }
     if (threadIdx.x == 1023) {
-        PlocState o;
+        const PlocState i = *in;
+        PlocState o{};
         o.m = s[1][1023];
         o.merges = s[0][1023];
-        o.next_id = in->next_id + o.merges;
-        o.pad = 0;
+        o.next_id = i.next_id + o.merges;
+        o.stalled = i.stalled | ((i.m > 1 && o.merges == 0) ? 1u : 0u);
+        o.rounds = i.rounds + (i.m > 1 ? 1u : 0u);
         *out = o;
     }
 }
@@ -323,6 +359,115 @@ __global__ __launch_bounds__(BT) void k_ploc_merge(const PlocState* __restrict__
     cout[SV] = id;
 }
 
+// All remaining rounds once at most TAIL_M clusters are left, in one workgroup with the
+// clusters' ids and boxes in LDS: a thread owns TAIL_PER consecutive positions, so the
+// block-wide prefix sums run in position order. Same rounds, same order, same ids.
+constexpr int TAIL_T = 1024, TAIL_PER = 4, TAIL_M = TAIL_T * TAIL_PER;
+__global__ __launch_bounds__(TAIL_T) void k_ploc_tail(const PlocState* __restrict__ st_in, PlocState* __restrict__ st_out,
+                                                      int radius, const uint32_t* __restrict__ cin, float* __restrict__ boxes,
+                                                      int32_t* __restrict__ links) {
+    __shared__ float sb[6][TAIL_M];
+    __shared__ uint32_t sid[TAIL_M];
+    __shared__ int8_t snn[TAIL_M];
+    __shared__ uint32_t wsum[TAIL_T / 64][2];
+    PlocState s = *st_in;
+    uint32_t m = s.m;
+    if (m > uint32_t(TAIL_M)) return;   // the host launches this only below TAIL_M
+    for (uint32_t i = threadIdx.x; i < m; i += TAIL_T) {
+        const uint32_t c = cin[i];
+        sid[i] = c;
+        for (int a = 0; a < 6; ++a) sb[a][i] = boxes[6 * size_t(c) + a];
+    }
+    __syncthreads();
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i0 = threadIdx.x * TAIL_PER;
+    while (m > 1) {
+        for (int k = 0; k < TAIL_PER; ++k) {
+            const int i = i0 + k;
+            if (i >= int(m)) break;
+            const float al0 = sb[0][i], al1 = sb[1][i], al2 = sb[2][i], ah0 = sb[3][i], ah1 = sb[4][i], ah2 = sb[5][i];
+            float best = FLT_MAX;
+            int bj = i;
+            const int j0 = max(0, i - radius), j1 = min(int(m) - 1, i + radius);
+            for (int j = j0; j <= j1; ++j) {
+                if (j == i) continue;
+                const float d = area6(stdmin(al0, sb[0][j]), stdmin(al1, sb[1][j]), stdmin(al2, sb[2][j]),
+                                      stdmax(ah0, sb[3][j]), stdmax(ah1, sb[4][j]), stdmax(ah2, sb[5][j]));
+                if (d < best) { best = d; bj = j; }
+            }
+            snn[i] = int8_t(bj - i);
+        }
+        __syncthreads();
+        bool mg[TAIL_PER], sv[TAIL_PER];
+        uint32_t cm = 0, cs = 0;
+        for (int k = 0; k < TAIL_PER; ++k) {
+            const int i = i0 + k;
+            mg[k] = sv[k] = false;
+            if (i >= int(m)) continue;
+            const int j = i + snn[i];
+            const bool mutual = j != i && j + snn[j] == i;
+            mg[k] = mutual && i < j;
+            sv[k] = !(mutual && i > j);
+            cm += mg[k];
+            cs += sv[k];
+        }
+        uint32_t xm = cm, xs = cs;   // inclusive wave scans
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t ym = __shfl_up(xm, off), ys = __shfl_up(xs, off);
+            if (lane >= off) { xm += ym; xs += ys; }
+        }
+        if (lane == 63) { wsum[w][0] = xm; wsum[w][1] = xs; }
+        __syncthreads();
+        uint32_t S = xm - cm, SV = xs - cs, tm = 0, ts = 0;
+        for (int k = 0; k < TAIL_T / 64; ++k) {
+            if (k < w) { S += wsum[k][0]; SV += wsum[k][1]; }
+            tm += wsum[k][0];
+            ts += wsum[k][1];
+        }
+        uint32_t nid[TAIL_PER], npos[TAIL_PER];
+        float nb[TAIL_PER][6];
+        for (int k = 0; k < TAIL_PER; ++k) {
+            const int i = i0 + k;
+            npos[k] = 0xffffffffu;
+            if (!sv[k]) continue;
+            npos[k] = SV++;
+            if (!mg[k]) {
+                nid[k] = sid[i];
+                for (int a = 0; a < 6; ++a) nb[k][a] = sb[a][i];
+                continue;
+            }
+            const int j = i + snn[i];
+            const uint32_t id = s.next_id + S++;
+            for (int a = 0; a < 3; ++a) {
+                nb[k][a] = stdmin(sb[a][i], sb[a][j]);
+                nb[k][3 + a] = stdmax(sb[3 + a][i], sb[3 + a][j]);
+            }
+            for (int a = 0; a < 6; ++a) boxes[6 * size_t(id) + a] = nb[k][a];
+            links[3 * size_t(id)] = int32_t(sid[i]);
+            links[3 * size_t(id) + 1] = int32_t(sid[j]);
+            links[3 * size_t(id) + 2] = -1;
+            nid[k] = id;
+        }
+        __syncthreads();
+        for (int k = 0; k < TAIL_PER; ++k) {
+            if (npos[k] == 0xffffffffu) continue;
+            sid[npos[k]] = nid[k];
+            for (int a = 0; a < 6; ++a) sb[a][npos[k]] = nb[k][a];
+        }
+        __syncthreads();
+        s.next_id += tm;
+        s.merges = tm;
+        s.rounds += 1;
+        m = ts;
+        if (tm == 0) { s.stalled = 1; break; }
+    }
+    if (threadIdx.x == 0) {
+        s.m = m;
+        s.root = sid[0];
+        *st_out = s;
+    }
+}
+
 uint32_t cdiv(uint64_t a, uint32_t b) { return uint32_t((a + b - 1) / b); }
 
 }  // namespace
@@ -341,52 +486,64 @@ int ploc_build(Ctx& c, const float* d_tris, uint32_t n, int radius, hipStream_t 
         !ensure(c, c.vals2, 4 * size_t(n)) || !ensure(c, c.hist, 4 * 256 * size_t(nblk_rs)) ||
         !ensure(c, c.boxes, 24 * nn) || !ensure(c, c.links, 12 * nn) || !ensure(c, c.clus_a, 4 * size_t(n)) ||
         !ensure(c, c.clus_b, 4 * size_t(n)) || !ensure(c, c.nn, 4 * size_t(n)) || !ensure(c, c.flags, size_t(n)) ||
-        !ensure(c, c.blk, 8 * size_t(nblk)) || !ensure(c, c.state, 2 * sizeof(PlocState)) || !ensure(c, c.g4, 16) ||
-        !ensure(c, c.bounds_part, 6 * 4 * 1024))
+        !ensure(c, c.blk, 8 * size_t(nblk)) || !ensure(c, c.state, 3 * sizeof(PlocState)) || !ensure(c, c.g4, 16) ||
+        !ensure(c, c.bounds_part, 6 * 4 * 1024) || !ensure(c, c.rs_tot, 4 * 256))
         return -2;
-    if (!c.h_state && !hip_ok(c, hipHostMalloc(reinterpret_cast<void**>(&c.h_state), sizeof(PlocState)), "hipHostMalloc"))
+    if (!c.h_state && !hip_ok(c, hipHostMalloc(reinterpret_cast<void**>(&c.h_state), 2 * sizeof(PlocState)), "hipHostMalloc"))
         return -2;
     const uint32_t nparts = std::min<uint32_t>(1024, nblk);
     k_bounds_part<<<nparts, BT, 0, s>>>(d_tris, n, c.bounds_part.as<float>());
-    k_bounds_final<<<1, 64, 0, s>>>(c.bounds_part.as<float>(), int(nparts), c.g4.as<float>());
+    k_bounds_final<<<1, BT, 0, s>>>(c.bounds_part.as<float>(), int(nparts), c.g4.as<float>());
     k_morton<<<nblk, BT, 0, s>>>(d_tris, n, c.g4.as<float>(), c.keys.as<uint32_t>(), c.vals.as<uint32_t>());
     uint32_t *ki = c.keys.as<uint32_t>(), *vi = c.vals.as<uint32_t>(), *ko = c.keys2.as<uint32_t>(), *vo = c.vals2.as<uint32_t>();
     for (int shift = 0; shift < 30; shift += 8) {
         k_radix_hist<<<nblk_rs, BT, 0, s>>>(ki, n, shift, c.hist.as<uint32_t>(), nblk_rs);
-        k_scan_excl<<<1, 1024, 0, s>>>(c.hist.as<uint32_t>(), 256 * nblk_rs, nullptr);
-        k_radix_scatter<<<nblk_rs, BT, 0, s>>>(ki, vi, ko, vo, n, shift, c.hist.as<uint32_t>(), nblk_rs);
+        k_radix_colscan<<<256, BT, 0, s>>>(c.hist.as<uint32_t>(), nblk_rs, c.rs_tot.as<uint32_t>());
+        k_radix_scatter<<<nblk_rs, BT, 0, s>>>(ki, vi, ko, vo, n, shift, c.hist.as<uint32_t>(), nblk_rs, c.rs_tot.as<uint32_t>());
         std::swap(ki, ko);
         std::swap(vi, vo);
     }
     // four passes: the sorted pairs are back in keys/vals
     k_leaves<<<nblk, BT, 0, s>>>(d_tris, vi, n, c.boxes.as<float>(), c.links.as<int32_t>(), c.clus_a.as<uint32_t>());
-    PlocState init{n, n, 0, 0};
-    if (!hip_ok(c, hipMemcpyAsync(c.state.p, &init, sizeof init, hipMemcpyHostToDevice, s), "state upload")) return -2;
+    // Rounds: the device keeps the cluster count; the host reads it back only every
+    // kBatch rounds and sizes the grids by the last count it read (an upper bound: the
+    // surplus blocks find no positions and exit). Below TAIL_M clusters one workgroup
+    // runs every remaining round.
+    constexpr int kBatch = 6;
+    PlocState init{};
+    init.m = n;
+    init.next_id = n;
+    c.h_state[1] = init;
+    if (!hip_ok(c, hipMemcpyAsync(c.state.p, &c.h_state[1], sizeof init, hipMemcpyHostToDevice, s), "state upload")) return -2;
     PlocState* d_st = c.state.as<PlocState>();
     uint32_t m = n;
     int parity = 0;
     uint32_t *cin = c.clus_a.as<uint32_t>(), *cout = c.clus_b.as<uint32_t>();
-    while (m > 1) {
+    while (m > uint32_t(TAIL_M)) {
         const uint32_t nb = cdiv(m, BT);
-        k_ploc_nn<<<nb, BT, 0, s>>>(d_st + parity, radius, c.boxes.as<float>(), cin, c.nn.as<uint32_t>(),
-                                    c.flags.as<uint8_t>(), c.blk.as<uint32_t>(), nb);
-        k_ploc_scan<<<1, 1024, 0, s>>>(d_st + parity, d_st + (parity ^ 1), c.blk.as<uint32_t>(), nb);
-        k_ploc_merge<<<nb, BT, 0, s>>>(d_st + parity, c.nn.as<uint32_t>(), c.flags.as<uint8_t>(), c.blk.as<uint32_t>(), nb,
-                                       cin, cout, c.boxes.as<float>(), c.links.as<int32_t>());
-        if (!hip_ok(c, hipMemcpyAsync(c.h_state, d_st + (parity ^ 1), sizeof(PlocState), hipMemcpyDeviceToHost, s), "state read") ||
-            !hip_ok(c, hipStreamSynchronize(s), "PLOC round"))
+        for (int r = 0; r < kBatch; ++r) {
+            k_ploc_nn<<<nb, BT, 0, s>>>(d_st + parity, radius, c.boxes.as<float>(), cin, c.nn.as<uint32_t>(),
+                                        c.flags.as<uint8_t>(), c.blk.as<uint32_t>(), nb);
+            k_ploc_scan<<<1, 1024, 0, s>>>(d_st + parity, d_st + (parity ^ 1), c.blk.as<uint32_t>(), nb);
+            k_ploc_merge<<<nb, BT, 0, s>>>(d_st + parity, c.nn.as<uint32_t>(), c.flags.as<uint8_t>(), c.blk.as<uint32_t>(),
+                                           nb, cin, cout, c.boxes.as<float>(), c.links.as<int32_t>());
+            parity ^= 1;
+            std::swap(cin, cout);
+        }
+        if (!hip_ok(c, hipMemcpyAsync(c.h_state, d_st + parity, sizeof(PlocState), hipMemcpyDeviceToHost, s), "state read") ||
+            !hip_ok(c, hipStreamSynchronize(s), "PLOC rounds"))
             return -2;
-        if (c.h_state->merges == 0) { c.err = "PLOC round made no merge (degenerate input)"; return -3; }
+        if (c.h_state->stalled) { c.err = "PLOC round made no merge (degenerate input)"; return -3; }
         m = c.h_state->m;
-        parity ^= 1;
-        std::swap(cin, cout);
-        ++c.rounds;
     }
-    uint32_t root = 0;
-    if (!hip_ok(c, hipMemcpyAsync(&root, cin, 4, hipMemcpyDeviceToHost, s), "root read") ||
+    k_ploc_tail<<<1, TAIL_T, 0, s>>>(d_st + parity, d_st + 2, radius, cin, c.boxes.as<float>(), c.links.as<int32_t>());
+    if (!hip_ok(c, hipGetLastError(), "PLOC tail launch") ||
+        !hip_ok(c, hipMemcpyAsync(c.h_state, d_st + 2, sizeof(PlocState), hipMemcpyDeviceToHost, s), "state read") ||
         !hip_ok(c, hipStreamSynchronize(s), "PLOC build"))
         return -2;
-    c.root = int32_t(root);
+    if (c.h_state->stalled || c.h_state->m != 1) { c.err = "PLOC round made no merge (degenerate input)"; return -3; }
+    c.rounds = c.h_state->rounds;
+    c.root = int32_t(c.h_state->root);
     return 0;
 }
 

@@ -299,6 +299,11 @@ uint32_t cdiv(uint64_t a, uint32_t b) { return uint32_t((a + b - 1) / b); }
 
 }  // namespace
 
+int ao_finish(Ctx& c, const uint8_t* d_mask, const uint32_t* d_cnt, uint32_t npix, uint32_t spp, float* d_ao, hipStream_t s) {
+    k_ao_finish<<<cdiv(npix, BT), BT, 0, s>>>(d_mask, d_cnt, npix, spp, d_ao);
+    return hip_ok(c, hipGetLastError(), "finish launch") ? 0 : -2;
+}
+
 int angle_table(Ctx& c, float* d_sin, float* d_cos, hipStream_t s) {
     k_angles<<<(1u << 24) / BT, BT, 0, s>>>(c.host_trig ? c.d_trig : nullptr, d_sin, d_cos);
     return hip_ok(c, hipGetLastError(), "angles launch") ? 0 : -2;

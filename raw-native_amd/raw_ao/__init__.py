@@ -23,7 +23,8 @@ PKG = os.path.dirname(HERE)
 LIB_PATH = os.path.join(PKG, "build", "libraw_ao_hip.so")
 
 OK, ERR_ARG, ERR_DEVICE, ERR_BUILD, ERR_NO_DEVICE = 0, -1, -2, -3, -4
-OPT_HOST_LIBM_TRIG, OPT_TRACE_KERNEL = 1, 2
+OPT_HOST_LIBM_TRIG, OPT_TRACE_KERNEL, OPT_TRI_PHASE = 1, 2, 3
+TRACE_WIDE, TRACE_BINARY = 0, 1
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")

@@ -209,6 +209,41 @@ def test_error_map_matches_oracle(ctx):
     assert (float(rec.rmse), rec.max_error, rec.pixels, bool(rec.within_tolerance)) == (rmse, mx, px, within)
 
 
+@pytest.mark.parametrize("name", ["default-512", "close-512"])
+def test_binary_kernel_golden(ctx, name):
+    """The binary-tree kernel gives the same golden bytes as the default wide one."""
+    ctx.set_option(raw_ao.OPT_TRACE_KERNEL, raw_ao.TRACE_BINARY)
+    try:
+        mask, ao, ss, rec = golden_frame(ctx, name)
+    finally:
+        ctx.set_option(raw_ao.OPT_TRACE_KERNEL, raw_ao.TRACE_WIDE)
+    assert sha(O.pfm1_bytes(ao)) == CASES[name]["files"]["ao_rt.pfm"]
+
+
+def test_binary_and_wide_agree_on_1m_scene(ctx):
+    tris, pos, nrm, mask = big_frame(320, 180, seed=5)
+    ctx.set_option(raw_ao.OPT_TRACE_KERNEL, raw_ao.TRACE_BINARY)
+    try:
+        ctx.build_ploc(tris, 16)
+        a_bin = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
+    finally:
+        ctx.set_option(raw_ao.OPT_TRACE_KERNEL, raw_ao.TRACE_WIDE)
+    ctx.build_ploc(tris, 16)
+    a_wide = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
+    assert np.array_equal(bits(a_bin), bits(a_wide))
+
+
+def test_option_change_without_rebuild_is_an_error(ctx):
+    t = O.scene_tris("soup", 100, 1)
+    ctx.build_ploc(t, 16)
+    ctx.set_option(raw_ao.OPT_TRACE_KERNEL, raw_ao.TRACE_BINARY)
+    try:
+        with pytest.raises(raw_ao.RawAoError):
+            ctx.trace_rays(np.zeros((1, 8), np.float32), any_hit=True)
+    finally:
+        ctx.set_option(raw_ao.OPT_TRACE_KERNEL, raw_ao.TRACE_WIDE)
+
+
 def test_repeatable(ctx):
     """Two runs give the same bytes (no scheduling-dependent results)."""
     tris, pos, nrm, mask = big_frame(128, 72, seed=3)
