@@ -40,6 +40,11 @@ def build(verbose=False):
     out += run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-Wall", "host/raw_ao_host.cpp",
                 "-L" + BUILD, "-lraw_ao_hip", "-Wl,-rpath,$ORIGIN", "-o", tmp], HERE)
     os.replace(tmp, host_out)
+    # the C++ host interface's frame test (tests/cpp), linked to the host library
+    exe = os.path.join(BUILD, "host_interface_frame")
+    out += run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "../tests/cpp/host_interface_frame.cpp",
+                "-L" + BUILD, "-lraw_ao_host", "-lraw_ao_hip", "-Wl,-rpath,$ORIGIN", "-o", exe + ".tmp"], HERE)
+    os.replace(exe + ".tmp", exe)
     if verbose and out.strip():
         print(out)
     return hip_out, host_out

@@ -244,6 +244,22 @@ def test_option_change_without_rebuild_is_an_error(ctx):
         ctx.set_option(raw_ao.OPT_TRACE_KERNEL, raw_ao.TRACE_WIDE)
 
 
+def test_cpp_host_interface_default_frame(tmp_path):
+    """The C++ host interface (raw::computeRTAO / computeSSAO / reconcile, as render.cpp
+    calls them) reproduces the default render's golden bytes and known reconcile."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(raw_ao.LIB_PATH), "host_interface_frame")
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    f = CASES["default"]["files"]
+    assert sha((tmp_path / "ao_rt.pfm").read_bytes()) == f["ao_rt.pfm"]
+    assert sha((tmp_path / "ao_ss.pfm").read_bytes()) == f["ao_ss.pfm"]
+    assert "pixels=37996 rmse=0.129436031" in r.stdout
+    cert = (tmp_path / "certificate.json").read_text()
+    assert '"verdict":"refuted"' in cert and '["rmse","0.1294"]' in cert
+
+
 def test_repeatable(ctx):
     """Two runs give the same bytes (no scheduling-dependent results)."""
     tris, pos, nrm, mask = big_frame(128, 72, seed=3)
