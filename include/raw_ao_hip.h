@@ -136,6 +136,50 @@ int raw_ao_angle_table(raw_ao_ctx* ctx, float* sin_out, float* cos_out);
 /* Bytes of the context's device-resident BVH (nodes + triangles) in its traversal form. */
 int raw_ao_bvh_bytes(raw_ao_ctx* ctx, uint64_t* node_bytes, uint64_t* tri_bytes, uint32_t* node_record_bytes);
 
+/* ---- the learned AO surrogate --------------------------------------------------------
+ * No reference function: the reference's only learned precedent is its roughness network
+ * (scripts/rr_model.py:43-80, src/rhi/vulkan/vk_rrnet.cpp), whose interface this follows:
+ * fp32 master parameters owned by the caller's file, a forward pass on the device, a
+ * training step in the caller's loop. The network: a 4-level U-Net (32/64/96/128 channels,
+ * two 3x3 convolutions with ReLU a level, 2x2 average pooling, nearest 2x upsampling with
+ * skip concatenation, a 1x1 head) over RAW_AO_NET_FEATURES bf16 channels a pixel, bf16
+ * weights and activations on MFMA with fp32 accumulation. Images are NHWC, w and h
+ * multiples of 8. Parameters: per layer (in order) [cout][9*cin] weights with
+ * k = (ky*3 + kx)*cin + ci, then cout biases; the head's 32 weights and bias last. */
+#define RAW_AO_NET_FEATURES 8
+int raw_ao_net_param_count(raw_ao_ctx* ctx, uint64_t* count);
+/* Loads fp32 master parameters and resets the optimizer state. */
+int raw_ao_net_set_params(raw_ao_ctx* ctx, const float* params, uint64_t count);
+int raw_ao_net_get_params(raw_ao_ctx* ctx, float* params, uint64_t count);
+/* The gradients of the last training step (fp32, the parameters' layout). */
+int raw_ao_net_get_grads(raw_ao_ctx* ctx, float* grads, uint64_t count);
+/* Features of one w x h G-buffer (computeRTAO's inputs plus SSAO and, for the denoiser,
+ * a low-spp RT AO; d_noisy may be NULL): covered, distance to eye / 20, octahedral
+ * normal (2), SSAO, noisy AO, normal y, noisy-AO present; zeros off the mask. */
+int raw_ao_net_features_dev(raw_ao_ctx* ctx, const float* d_pos, const float* d_nrm, const uint8_t* d_mask,
+                            const float* d_ssao, const float* d_noisy, const float eye[3], int w, int h,
+                            uint16_t* d_x, void* stream);
+/* AO of n stacked images: clamp(network, 0, 1) where covered, 1 elsewhere. */
+int raw_ao_net_forward_dev(raw_ao_ctx* ctx, const uint16_t* d_x, const uint8_t* d_mask, int n, int w, int h,
+                           float* d_ao, void* stream);
+/* Forward, backward of the mean squared error against d_target over covered pixels, and
+ * (lr > 0) one Adam step (beta 0.9, 0.999, eps 1e-8) on the master parameters. */
+int raw_ao_net_train_step_dev(raw_ao_ctx* ctx, const uint16_t* d_x, const uint8_t* d_mask, const float* d_target,
+                              int n, int w, int h, float lr, void* stream);
+/* The loss of the last training step (waits for the device). */
+int raw_ao_net_loss(raw_ao_ctx* ctx, float* loss);
+/* Verification entry points: a layer's bf16 output from the last forward pass (count =
+ * n * (h >> level) * (w >> level) * cout), and the single-layer kernels: a 3x3 convolution
+ * (zero padding) of in0 (c0 channels) concatenated with in1 upsampled 2x (c1 channels, may be
+ * 0) by w [cout][kpad] bf16, + bias (may be NULL), ReLU when relu != 0; and its weight
+ * gradient against the output grad dy: gw [cout][9*(c0+c1)], gb [cout] (fp32). */
+int raw_ao_net_read_act(raw_ao_ctx* ctx, int layer, uint16_t* out, uint64_t count);
+int raw_ao_conv3x3_dev(raw_ao_ctx* ctx, const uint16_t* d_in0, const uint16_t* d_in1, int c0, int c1,
+                       const uint16_t* d_w, int kpad, const float* d_bias, int cout, int relu, int n, int w, int h,
+                       uint16_t* d_out, void* stream);
+int raw_ao_conv3x3_wgrad_dev(raw_ao_ctx* ctx, const uint16_t* d_dy, const uint16_t* d_in0, const uint16_t* d_in1,
+                             int c0, int c1, int cout, int n, int w, int h, float* d_gw, float* d_gb, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

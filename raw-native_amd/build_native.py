@@ -12,7 +12,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 BUILD = os.path.join(HERE, "build")
-HIP_SOURCES = ["csrc/capi.hip", "csrc/ploc.hip", "csrc/trace.hip", "csrc/bvh4.hip", "csrc/ssao.hip", "csrc/trig_table.cpp"]
+HIP_SOURCES = ["csrc/capi.hip", "csrc/ploc.hip", "csrc/trace.hip", "csrc/bvh4.hip", "csrc/ssao.hip", "csrc/unet.hip",
+               "csrc/trig_table.cpp"]
 ARCH = os.environ.get("RAW_AO_ARCH", "gfx950")
 
 HIPCC_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",

@@ -6,10 +6,6 @@
 #include <string>
 #include <vector>
 
-struct raw_ao_ctx {
-    raw_ao::Ctx c;
-};
-
 namespace raw_ao {
 
 bool hip_ok(Ctx& c, hipError_t e, const char* what) {
@@ -96,6 +92,7 @@ void raw_ao_close(raw_ao_ctx* x) {
     if (c.stream) {
         (void)hipSetDevice(c.device);
         (void)hipStreamSynchronize(c.stream);
+        raw_ao::net_release(c);
         raw_ao::DevBuf* bufs[] = {&c.tris_in, &c.keys, &c.vals, &c.keys2, &c.vals2, &c.hist, &c.rs_tot, &c.boxes, &c.links,
                                   &c.clus_a, &c.clus_b, &c.nn, &c.flags, &c.blk, &c.state, &c.g4, &c.bounds_part,
                                   &c.nodes, &c.tris_sorted, &c.nodes4, &c.wfront_a, &c.wfront_b, &c.wblk, &c.wstate, &c.st_pos, &c.st_nrm, &c.st_mask, &c.st_out, &c.st_out2,

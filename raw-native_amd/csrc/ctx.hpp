@@ -27,6 +27,8 @@ struct PlocState {
     uint32_t pad[2];
 };
 
+struct UNet;
+
 struct TraceCounters {
     unsigned long long visits, tests, rays, overflows;
 };
@@ -74,6 +76,8 @@ struct Ctx {
 
     hipEvent_t ev[8]{};
     float ms_build{0}, ms_trace{0}, ms_ssao{0}, ms_error{0};
+
+    UNet* net{nullptr};          // the learned AO surrogate, made on first use
 };
 
 bool ensure(Ctx& c, DevBuf& b, size_t bytes);
@@ -97,7 +101,14 @@ int ssao(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, 
 int error_map(Ctx& c, const float* d_a, const float* d_t, const uint8_t* d_mask, int w, int h, float* d_err,
               float* d_max, hipStream_t s);
 
+void net_release(Ctx& c);
+bool conv_shape_ok(int c0, int c1, int cout);
+
 // Host-libm correction table (trig_table.cpp): 2^23 bytes, one nibble per index.
 bool build_trig_table(std::vector<uint8_t>& out, std::string& err);
 
 }  // namespace raw_ao
+
+struct raw_ao_ctx {
+    raw_ao::Ctx c;
+};

@@ -1,0 +1,851 @@
+// The learned AO surrogate on gfx950: a 4-level convolutional U-Net (32/64/96/128
+// channels, two 3x3 convolutions a level, ReLU, 2x2 average pooling down, nearest 2x
+// upsampling and skip concatenation up, a 1x1 head) from G-buffer features to AO, trained
+// and run with BF16 activations and weights on MFMA with FP32 accumulation.
+//
+// Layout: activations NHWC bf16 (images stacked, pixel index = (n*H + y)*W + x); a layer's
+// weights as [Cout][Kpad] bf16 with k = tap*Cin + ci (tap = ky*3 + kx, offset (ky-1, kx-1)),
+// Kpad = K rounded up to 32. FP32 master parameters: per layer [Cout][K] weights, then Cout
+// biases; the head's 32 weights and bias last.
+//
+// Convolution = implicit GEMM on v_mfma_f32_16x16x32_bf16 (lane l of a wave holds A row
+// l%16, B column l%16, k 8*(l/16)..+7; D column l%16, rows 4*(l/16)..+3):
+//   forward     A = weights (cout x k), B = input patches (k x pixel): a wave computes up to
+//               128 output channels for 32 pixels; the decoder's skip + 2x upsampled
+//               concatenation is read in the B gather, never materialized; bias and ReLU fused.
+//   input grad  the same kernel over the flipped, transposed weights.
+//   weight grad A = output grads (cout x pixel), B = input patches (pixel x k), both staged
+//               through LDS so lanes read 8 pixels contiguously; per-chunk partial sums
+//               reduced over chunks in a fixed order (deterministic). The bias is the weight
+//               of a constant-1 input column (k = K).
+#include "../../include/raw_ao_hip.h"
+#include "unet.hpp"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+namespace raw_ao {
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+typedef unsigned short bf16_t;   // storage type
+
+__device__ __forceinline__ bf16_t f2bf(float f) {   // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return bf16_t(0x7fc0);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return bf16_t(u >> 16);
+}
+__device__ __forceinline__ float bf2f(bf16_t b) { return __uint_as_float(uint32_t(b) << 16); }
+
+union V8 {
+    uint4 u;
+    bf16x8 v;
+};
+
+// ---- forward / input-grad convolution -------------------------------------------
+struct ConvArgs {
+    const bf16_t* in0;   // N,H,W,C0
+    const bf16_t* in1;   // N,H/2,W/2,C1 (upsampled 2x on read) or null
+    const bf16_t* w;     // [Cout][Kpad]
+    const float* bias;   // [Cout] or null
+    bf16_t* out;         // N,H,W,Cout
+    int n, h, wd, c0, c1, cout, kpad, relu;
+    int co0;             // first output channel of this launch's group
+};
+
+// One wave: 32 pixels (2 B tiles) x CT*16 output channels starting at co0.
+template <int CT>
+__global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int wave = int((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int npix = a.n * a.h * a.wd;
+    const int pbase = wave * 32;
+    if (pbase >= npix) return;
+    const int cin = a.c0 + a.c1, K = 9 * cin;
+    const int g = lane >> 4, r = lane & 15;
+    int py[2], px[2], pn[2];
+    bool pvalid[2];
+    for (int t = 0; t < 2; ++t) {
+        const int p = pbase + t * 16 + r;
+        pvalid[t] = p < npix;
+        const int pp = pvalid[t] ? p : 0;
+        px[t] = pp % a.wd;
+        const int q = pp / a.wd;
+        py[t] = q % a.h;
+        pn[t] = q / a.h;
+    }
+    f32x4 acc[CT][2];
+    for (int i = 0; i < CT; ++i)
+        for (int t = 0; t < 2; ++t) acc[i][t] = f32x4{0, 0, 0, 0};
+    const int hh = a.h >> 1, hw = a.wd >> 1;
+    for (int kc = 0; kc < a.kpad; kc += 32) {
+        const int k = kc + 8 * g;
+        V8 bv[2];
+        const bool kin = k < K;
+        const int tap = kin ? k / cin : 0, ci = kin ? k - tap * cin : 0;
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        for (int t = 0; t < 2; ++t) {
+            bv[t].u = make_uint4(0, 0, 0, 0);
+            const int y = py[t] + dy, x = px[t] + dx;
+            if (!kin || !pvalid[t] || y < 0 || y >= a.h || x < 0 || x >= a.wd) continue;
+            const bf16_t* src;
+            if (ci < a.c0) src = a.in0 + (size_t(pn[t] * a.h + y) * a.wd + x) * a.c0 + ci;
+            else src = a.in1 + (size_t(pn[t] * hh + (y >> 1)) * hw + (x >> 1)) * a.c1 + (ci - a.c0);
+            bv[t].u = *reinterpret_cast<const uint4*>(src);
+        }
+        for (int i = 0; i < CT; ++i) {
+            V8 av;
+            const int co = a.co0 + i * 16 + r;
+            av.u = *reinterpret_cast<const uint4*>(a.w + size_t(co) * a.kpad + k);
+            for (int t = 0; t < 2; ++t) acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av.v, bv[t].v, acc[i][t], 0, 0, 0);
+        }
+    }
+    for (int t = 0; t < 2; ++t) {
+        if (!pvalid[t]) continue;
+        const size_t p = size_t(pbase + t * 16 + r);
+        for (int i = 0; i < CT; ++i) {
+            const int co = a.co0 + i * 16 + 4 * g;
+            float v[4];
+            for (int j = 0; j < 4; ++j) {
+                v[j] = acc[i][t][j] + (a.bias ? a.bias[co + j] : 0.0f);
+                if (a.relu && v[j] < 0.0f) v[j] = 0.0f;
+            }
+            const uint32_t lo = uint32_t(f2bf(v[0])) | (uint32_t(f2bf(v[1])) << 16);
+            const uint32_t hi = uint32_t(f2bf(v[2])) | (uint32_t(f2bf(v[3])) << 16);
+            *reinterpret_cast<uint2*>(a.out + p * a.cout + co) = make_uint2(lo, hi);
+        }
+    }
+}
+
+// ---- weight gradient ----------------------------------------------------------------
+// dW[co][k] = sum_p dy[p][co] * X[p][k], X the forward's input patches (k < K) and 1 at
+// k == K (the bias column). A block: one 64-wide k tile, all cout (<= 128), a chunk of
+// pixels; its 4 waves take 32 pixels each per step, then add their tiles in LDS in wave
+// order; the block's partial goes to part[chunk][cout][kpg] (kpg = K + 1 rounded up to 64).
+struct WgradArgs {
+    const bf16_t* dy;    // N,H,W,Cout
+    const bf16_t* in0;
+    const bf16_t* in1;
+    float* part;
+    int n, h, wd, c0, c1, cout, kpg, chunk;
+};
+template <int CT>
+__global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
+    __shared__ bf16_t sa[4][CT * 16][40];   // per wave: [cout][32 pixels] (+8 pad)
+    __shared__ bf16_t sb[4][64][40];        // per wave: [k][32 pixels] (+8 pad)
+    __shared__ float red[CT * 16 * 64];     // the four waves' sums
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int g = lane >> 4, r = lane & 15;
+    const int ktile = blockIdx.x, chunk = blockIdx.y;
+    const int npix = a.n * a.h * a.wd;
+    const int cin = a.c0 + a.c1, K = 9 * cin;
+    const int hh = a.h >> 1, hw = a.wd >> 1;
+    f32x4 acc[CT][4];
+    for (int i = 0; i < CT; ++i)
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
+    const int p0 = chunk * a.chunk, p1 = min(npix, p0 + a.chunk);
+    for (int pb0 = p0; pb0 < p1; pb0 += 4 * 32) {
+        const int pb = pb0 + wv * 32;
+        // dy: 32 pixels x (CT*16) channels -> sa[wv][co][p]
+        for (int e = lane; e < 32 * CT * 2; e += 64) {   // units of 8 channels
+            const int pl = e / (CT * 2), cg = e - pl * (CT * 2);
+            const int p = pb + pl;
+            V8 v;
+            v.u = make_uint4(0, 0, 0, 0);
+            if (p < p1) v.u = *reinterpret_cast<const uint4*>(a.dy + size_t(p) * a.cout + cg * 8);
+            const bf16_t* s = reinterpret_cast<const bf16_t*>(&v.u);
+            for (int j = 0; j < 8; ++j) sa[wv][cg * 8 + j][pl] = s[j];
+        }
+        // X patches: 32 pixels x 64 k -> sb[wv][k][p]
+        for (int e = lane; e < 32 * 8; e += 64) {          // units of 8 k
+            const int pl = e >> 3, kg = e & 7;
+            const int p = pb + pl;
+            const int k = ktile * 64 + kg * 8;
+            V8 v;
+            v.u = make_uint4(0, 0, 0, 0);
+            if (p < p1) {
+                if (k < K) {
+                    const int tap = k / cin, ci = k - tap * cin;
+                    const int x0 = p % a.wd, q = p / a.wd, y0 = q % a.h, nn = q / a.h;
+                    const int y = y0 + tap / 3 - 1, x = x0 + tap % 3 - 1;
+                    if (y >= 0 && y < a.h && x >= 0 && x < a.wd) {
+                        const bf16_t* src = ci < a.c0 ? a.in0 + (size_t(nn * a.h + y) * a.wd + x) * a.c0 + ci
+                                                      : a.in1 + (size_t(nn * hh + (y >> 1)) * hw + (x >> 1)) * a.c1 + (ci - a.c0);
+                        v.u = *reinterpret_cast<const uint4*>(src);
+                    }
+                } else if (k == K) {
+                    v.u.x = 0x3f80u;   // bf16 1.0: the bias column
+                }
+            }
+            const bf16_t* s = reinterpret_cast<const bf16_t*>(&v.u);
+            for (int j = 0; j < 8; ++j) sb[wv][kg * 8 + j][pl] = s[j];
+        }
+        __syncthreads();
+        // A = dy^T (cout x 32 pixels), B = X (32 pixels x 64 k)
+        V8 bv[4];
+        for (int j = 0; j < 4; ++j) bv[j].u = *reinterpret_cast<const uint4*>(&sb[wv][j * 16 + r][8 * g]);
+        for (int i = 0; i < CT; ++i) {
+            V8 av;
+            av.u = *reinterpret_cast<const uint4*>(&sa[wv][i * 16 + r][8 * g]);
+            for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av.v, bv[j].v, acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    for (int w = 0; w < 4; ++w) {
+        if (wv == w)
+            for (int i = 0; i < CT; ++i)
+                for (int j = 0; j < 4; ++j)
+                    for (int q = 0; q < 4; ++q) {
+                        const int co = i * 16 + 4 * g + q, kk = j * 16 + r;
+                        if (w == 0) red[co * 64 + kk] = acc[i][j][q];
+                        else red[co * 64 + kk] += acc[i][j][q];
+                    }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < CT * 16 * 64; e += 256) {
+        const int co = e / 64, kk = e - co * 64;
+        a.part[(size_t(chunk) * a.cout + co) * a.kpg + ktile * 64 + kk] = red[e];
+    }
+}
+
+// grad[co][k] (bias grad at k == K) = sum over chunks of the partials, in chunk order.
+__global__ void k_wgrad_reduce(const float* __restrict__ part, int nchunks, int cout, int kpg, int K, float* __restrict__ gw,
+                               float* __restrict__ gb) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= cout * (K + 1)) return;
+    const int co = e / (K + 1), k = e - co * (K + 1);
+    float s = 0.0f;
+    for (int c = 0; c < nchunks; ++c) s += part[(size_t(c) * cout + co) * kpg + k];
+    if (k < K) gw[size_t(co) * K + k] = s;
+    else gb[co] = s;
+}
+
+// ---- elementwise pieces ---------------------------------------------------------------
+__global__ void k_relu_bwd(const bf16_t* __restrict__ y, const bf16_t* __restrict__ dy, bf16_t* __restrict__ dpre, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dpre[i] = bf2f(y[i]) > 0.0f ? dy[i] : bf16_t(0);
+}
+// 2x2 average pool of an h x w level (h, w even)
+__global__ void k_pool_fwd(const bf16_t* __restrict__ in, bf16_t* __restrict__ out, int n, int h, int w, int c) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int ho = h / 2, wo = w / 2;
+    if (i >= size_t(n) * ho * wo * c) return;
+    const int ch = int(i % c);
+    const size_t q = i / c;
+    const int x = int(q % wo), y = int((q / wo) % ho), nn = int(q / (size_t(wo) * ho));
+    const size_t b = (size_t(nn * h + 2 * y) * w + 2 * x) * c + ch;
+    const float s = (bf2f(in[b]) + bf2f(in[b + c])) + (bf2f(in[b + size_t(w) * c]) + bf2f(in[b + size_t(w) * c + c]));
+    out[i] = f2bf(s * 0.25f);
+}
+// grad of an encoder output that feeds both a skip and a pool:
+// d0 = dx[..., :c0] (the skip half of a decoder's input grad) + 0.25 * dpool (upsampled).
+__global__ void k_skip_grad(const bf16_t* __restrict__ dx, int cx, int c0, const bf16_t* __restrict__ dpool,
+                            bf16_t* __restrict__ d0, int n, int h, int w) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(n) * h * w * c0) return;
+    const int ch = int(i % c0);
+    const size_t p = i / c0;
+    const int x = int(p % w), y = int((p / w) % h), nn = int(p / (size_t(w) * h));
+    const float v = bf2f(dx[p * cx + ch]) + 0.25f * bf2f(dpool[(size_t(nn * (h / 2) + y / 2) * (w / 2) + x / 2) * c0 + ch]);
+    d0[i] = f2bf(v);
+}
+// grad of the lower level read through the 2x upsampling: the sum of its 4 children.
+__global__ void k_up_grad(const bf16_t* __restrict__ dx, int cx, int c0, int c1, bf16_t* __restrict__ dlow, int n, int h, int w) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int ho = h / 2, wo = w / 2;
+    if (i >= size_t(n) * ho * wo * c1) return;
+    const int ch = int(i % c1);
+    const size_t q = i / c1;
+    const int x = int(q % wo), y = int((q / wo) % ho), nn = int(q / (size_t(wo) * ho));
+    const size_t b = (size_t(nn * h + 2 * y) * w + 2 * x) * cx + c0 + ch;
+    const float s = (bf2f(dx[b]) + bf2f(dx[b + cx])) + (bf2f(dx[b + size_t(w) * cx]) + bf2f(dx[b + size_t(w) * cx + cx]));
+    dlow[i] = f2bf(s);
+}
+
+// The feature channels of one image (kNetFeatures bf16 a pixel).
+__global__ void k_features(const float* __restrict__ pos, const float* __restrict__ nrm, const uint8_t* __restrict__ mask,
+                           const float* __restrict__ ssao, const float* __restrict__ noisy, float ex, float ey, float ez,
+                           size_t np, bf16_t* __restrict__ x) {
+    const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (mask[p]) {
+        const float dx = pos[3 * p] - ex, dy = pos[3 * p + 1] - ey, dz = pos[3 * p + 2] - ez;
+        const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+        const float nx = nrm[3 * p], ny = nrm[3 * p + 1], nz = nrm[3 * p + 2];
+        const float l1 = fabsf(nx) + fabsf(ny) + fabsf(nz);
+        float ox = l1 > 0.0f ? nx / l1 : 0.0f, oy = l1 > 0.0f ? nz / l1 : 0.0f;   // octahedral about +y
+        if (ny < 0.0f) {
+            const float tx = (1.0f - fabsf(oy)) * (ox >= 0.0f ? 1.0f : -1.0f);
+            const float ty = (1.0f - fabsf(ox)) * (oy >= 0.0f ? 1.0f : -1.0f);
+            ox = tx;
+            oy = ty;
+        }
+        f[0] = 1.0f;
+        f[1] = d * 0.05f;
+        f[2] = ox;
+        f[3] = oy;
+        f[4] = ssao[p];
+        f[5] = noisy ? noisy[p] : 0.0f;
+        f[6] = ny;
+        f[7] = noisy ? 1.0f : 0.0f;
+    }
+    uint32_t pk[4];
+    for (int j = 0; j < 4; ++j) pk[j] = uint32_t(f2bf(f[2 * j])) | (uint32_t(f2bf(f[2 * j + 1])) << 16);
+    *reinterpret_cast<uint4*>(x + p * 8) = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+}
+
+// head: pred = b + sum_c w[c] x[p][c] over 32 channels in channel order; ao = clamp(pred,
+// 0, 1) on covered pixels and 1 elsewhere (the reference's value for uncovered pixels).
+__global__ void k_head(const bf16_t* __restrict__ x, const float* __restrict__ hw, const uint8_t* __restrict__ mask, size_t np,
+                       float* __restrict__ pred, float* __restrict__ ao) {
+    const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const uint4* xv = reinterpret_cast<const uint4*>(x + p * 32);
+    float s = hw[32];
+    for (int q = 0; q < 4; ++q) {
+        const uint4 u = xv[q];
+        const uint32_t wds[4] = {u.x, u.y, u.z, u.w};
+        for (int j = 0; j < 4; ++j) {
+            s += hw[8 * q + 2 * j] * __uint_as_float(wds[j] << 16);
+            s += hw[8 * q + 2 * j + 1] * __uint_as_float(wds[j] & 0xffff0000u);
+        }
+    }
+    if (pred) pred[p] = s;
+    if (ao) ao[p] = mask[p] ? fminf(fmaxf(s, 0.0f), 1.0f) : 1.0f;
+}
+
+__global__ void k_zero_u32(uint32_t* p) { p[threadIdx.x] = 0; }
+__global__ __launch_bounds__(256) void k_mask_count(const uint8_t* __restrict__ mask, size_t np, uint32_t* cnt) {
+    __shared__ uint32_t s;
+    if (threadIdx.x == 0) s = 0;
+    __syncthreads();
+    uint32_t c = 0;
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < np; i += size_t(gridDim.x) * 256) c += mask[i] != 0;
+    atomicAdd(&s, c);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(cnt, s);
+}
+
+// Mean squared error over covered pixels: dL/dpred = 2 (pred - target) / count; the head's
+// input grad dx[p][c] = dL/dpred * w[c]; per-block sums of the head's weight and bias grads
+// and of the loss.
+__global__ __launch_bounds__(256) void k_head_bwd(const bf16_t* __restrict__ x, const float* __restrict__ hw,
+                                                  const float* __restrict__ pred, const float* __restrict__ target,
+                                                  const uint8_t* __restrict__ mask, size_t np, const uint32_t* __restrict__ cnt,
+                                                  bf16_t* __restrict__ dx, float* __restrict__ part) {
+    __shared__ float red[34][256];
+    const size_t p = size_t(blockIdx.x) * 256 + threadIdx.x;
+    const float inv = 1.0f / float(max(*cnt, 1u));
+    float g = 0.0f, l = 0.0f;
+    if (p < np && mask[p]) {
+        const float e = pred[p] - target[p];
+        l = e * e * inv;
+        g = 2.0f * e * inv;
+    }
+    for (int c = 0; c < 32; ++c) {
+        const float xv = p < np ? bf2f(x[p * 32 + c]) : 0.0f;
+        if (p < np) dx[p * 32 + c] = f2bf(g * hw[c]);
+        red[c][threadIdx.x] = g * xv;
+    }
+    red[32][threadIdx.x] = g;
+    red[33][threadIdx.x] = l;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s)
+            for (int c = 0; c < 34; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 34) part[size_t(blockIdx.x) * 34 + threadIdx.x] = red[threadIdx.x][0];
+}
+__global__ void k_head_reduce(const float* __restrict__ part, int nblk, float* __restrict__ ghead, float* __restrict__ loss) {
+    const int c = threadIdx.x;
+    if (c >= 34) return;
+    float s = 0.0f;
+    for (int b = 0; b < nblk; ++b) s += part[size_t(b) * 34 + c];
+    if (c < 33) ghead[c] = s;
+    else *loss = s;
+}
+
+// Adam (Kingma and Ba, 2015) on the fp32 master parameters; c1, c2 the bias corrections.
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v, size_t n,
+                       float lr, float b1, float b2, float eps, float c1, float c2) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// fp32 master [Cout][K] -> bf16 forward weights [Cout][Kpad] and input-grad weights
+// [Cin][Kpad_t] with wt[ci][t * Cout + co] = w[co][(8 - t) * Cin + ci].
+__global__ void k_pack(const float* __restrict__ m, int cout, int cin, int kpad, int kpad_t, bf16_t* __restrict__ w,
+                       bf16_t* __restrict__ wt) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int K = 9 * cin, Kt = 9 * cout;
+    if (e < cout * kpad) {
+        const int co = e / kpad, k = e - co * kpad;
+        w[e] = k < K ? f2bf(m[size_t(co) * K + k]) : bf16_t(0);
+    }
+    if (e < cin * kpad_t) {
+        const int ci = e / kpad_t, kt = e - ci * kpad_t;
+        bf16_t v = 0;
+        if (kt < Kt) {
+            const int tp = kt / cout, co = kt - tp * cout;
+            v = f2bf(m[size_t(co) * K + (8 - tp) * cin + ci]);
+        }
+        wt[e] = v;
+    }
+}
+
+unsigned cdivu(size_t a, unsigned b) { return unsigned((a + b - 1) / b); }
+
+template <int CT> void launch_conv(const ConvArgs& a, hipStream_t s) {
+    const size_t npix = size_t(a.n) * a.h * a.wd;
+    const size_t waves = (npix + 31) / 32;
+    k_conv<CT><<<cdivu(waves * 64, 256), 256, 0, s>>>(a);
+}
+
+// e1a e1b | e2a e2b | e3a e3b | b4a b4b | d3a d3b | d2a d2b | d1a d1b
+struct LayerDef { int c0, c1, cout, level; };
+const LayerDef kLayers[kNetLayers] = {{8, 0, 32, 0},   {32, 0, 32, 0},  {32, 0, 64, 1},   {64, 0, 64, 1}, {64, 0, 96, 2},
+                                      {96, 0, 96, 2},  {96, 0, 128, 3}, {128, 0, 128, 3}, {96, 128, 96, 2}, {96, 0, 96, 2},
+                                      {64, 96, 64, 1}, {64, 0, 64, 1},  {32, 64, 32, 0},  {32, 0, 32, 0}};
+int round32(int k) { return (k + 31) / 32 * 32; }
+
+}  // namespace
+
+bool conv_shape_ok(int c0, int c1, int cout) {
+    return c0 > 0 && c0 % 8 == 0 && c1 >= 0 && c1 % 8 == 0 && cout > 0 && cout % 16 == 0;
+}
+
+bool conv3x3(Ctx& c, const bf16_t* in0, const bf16_t* in1, int c0, int c1, const bf16_t* w, int kpad, const float* bias,
+             int cout, int relu, int n, int h, int wd, bf16_t* out, hipStream_t s) {
+    ConvArgs a{in0, in1, w, bias, out, n, h, wd, c0, c1, cout, kpad, relu, 0};
+    for (int co0 = 0; co0 < cout; co0 += 128) {
+        a.co0 = co0;
+        switch (std::min(8, (cout - co0) / 16)) {
+            case 1: launch_conv<1>(a, s); break;
+            case 2: launch_conv<2>(a, s); break;
+            case 3: launch_conv<3>(a, s); break;
+            case 4: launch_conv<4>(a, s); break;
+            case 5: launch_conv<5>(a, s); break;
+            case 6: launch_conv<6>(a, s); break;
+            case 7: launch_conv<7>(a, s); break;
+            default: launch_conv<8>(a, s); break;
+        }
+    }
+    return hip_ok(c, hipGetLastError(), "conv3x3");
+}
+
+bool conv3x3_wgrad(Ctx& c, DevBuf& part, const bf16_t* dy, const bf16_t* in0, const bf16_t* in1, int c0, int c1, int cout,
+                   int n, int h, int wd, float* gw, float* gb, hipStream_t s) {
+    const int K = 9 * (c0 + c1);
+    const int kpg = (K + 1 + 63) / 64 * 64;
+    const size_t px = size_t(n) * h * wd;
+    // chunks: enough blocks to fill the chip, with at least 128 pixels each
+    const int ktiles = kpg / 64;
+    int chunk = 128;
+    while (size_t(ktiles) * ((px + chunk - 1) / chunk) > 2048 && chunk < (1 << 20)) chunk *= 2;
+    const int nch = int((px + chunk - 1) / chunk);
+    if (!ensure(c, part, size_t(nch) * cout * kpg * 4)) return false;
+    WgradArgs wa{dy, in0, in1, part.as<float>(), n, h, wd, c0, c1, cout, kpg, chunk};
+    const dim3 grid{unsigned(ktiles), unsigned(nch), 1u};
+    switch (cout / 16) {
+        case 2: k_wgrad<2><<<grid, 256, 0, s>>>(wa); break;
+        case 4: k_wgrad<4><<<grid, 256, 0, s>>>(wa); break;
+        case 6: k_wgrad<6><<<grid, 256, 0, s>>>(wa); break;
+        case 8: k_wgrad<8><<<grid, 256, 0, s>>>(wa); break;
+        default: c.err = "conv3x3 weight grad: output channels must be 32, 64, 96 or 128"; return false;
+    }
+    k_wgrad_reduce<<<cdivu(size_t(cout) * (K + 1), 256), 256, 0, s>>>(part.as<float>(), nch, cout, kpg, K, gw, gb);
+    return hip_ok(c, hipGetLastError(), "conv3x3 weight grad");
+}
+
+size_t UNet::param_count() const {
+    size_t n = 0;
+    for (const auto& L : kLayers) n += size_t(L.cout) * 9 * (L.c0 + L.c1) + L.cout;
+    return n + 33;
+}
+
+bool UNet::init(Ctx& c) {
+    ctx = &c;
+    size_t off = 0, wb = 0, wtb = 0;
+    for (int i = 0; i < kNetLayers; ++i) {
+        const LayerDef& L = kLayers[i];
+        Layer& l = layers[i];
+        l.c0 = L.c0;
+        l.c1 = L.c1;
+        l.cout = L.cout;
+        l.level = L.level;
+        l.K = 9 * (L.c0 + L.c1);
+        l.kpad = round32(l.K);
+        l.kpad_t = round32(9 * L.cout);
+        l.woff = off;
+        off += size_t(l.cout) * l.K;
+        l.boff = off;
+        off += size_t(l.cout);
+        l.wofs16 = wb;
+        wb += size_t(l.cout) * l.kpad;
+        l.wtofs16 = wtb;
+        wtb += size_t(l.c0 + l.c1) * l.kpad_t;
+    }
+    head_off = off;
+    const size_t np = param_count();
+    if (!ensure(c, params, 4 * np) || !ensure(c, grads, 4 * np) || !ensure(c, adam_m, 4 * np) || !ensure(c, adam_v, 4 * np) ||
+        !ensure(c, w16, 2 * wb) || !ensure(c, wt16, 2 * wtb))
+        return false;
+    return hip_ok(c, hipMemset(params.p, 0, 4 * np), "net init") && hip_ok(c, hipMemset(grads.p, 0, 4 * np), "net init") &&
+           hip_ok(c, hipMemset(adam_m.p, 0, 4 * np), "net init") && hip_ok(c, hipMemset(adam_v.p, 0, 4 * np), "net init");
+}
+
+bool UNet::pack(hipStream_t s) {
+    for (auto& l : layers) {
+        const int cin = l.c0 + l.c1;
+        const int nel = std::max(l.cout * l.kpad, cin * l.kpad_t);
+        k_pack<<<cdivu(size_t(nel), 256), 256, 0, s>>>(params.as<float>() + l.woff, l.cout, cin, l.kpad, l.kpad_t,
+                                                       w16.as<bf16_t>() + l.wofs16, wt16.as<bf16_t>() + l.wtofs16);
+    }
+    return hip_ok(*ctx, hipGetLastError(), "net pack");
+}
+
+bool UNet::ensure_acts(int n, int h, int w, bool grad) {
+    Ctx& c = *ctx;
+    const size_t np = size_t(n) * h * w;
+    if (!ensure(c, pred, np * 4)) return false;
+    for (int i = 0; i < kNetLayers; ++i) {
+        const size_t px = size_t(n) * (h >> layers[i].level) * (w >> layers[i].level);
+        if (!ensure(c, act[i], px * 2 * layers[i].cout)) return false;
+    }
+    for (int lv = 0; lv < 3; ++lv) {
+        const size_t px = size_t(n) * (h >> (lv + 1)) * (w >> (lv + 1));
+        if (!ensure(c, pooled[lv], px * 2 * layers[2 * lv + 1].cout)) return false;
+    }
+    if (grad) {
+        for (int lv = 0; lv < 4; ++lv) {
+            // the widest gradient of a level: a decoder input (skip + upsampled) or an output
+            int cmax = 0;
+            for (const auto& l : layers)
+                if (l.level == lv) cmax = std::max(cmax, std::max(l.c0 + l.c1, l.cout));
+            const size_t px = size_t(n) * (h >> lv) * (w >> lv);
+            for (auto& b : lvbuf[lv])
+                if (!ensure(c, b, px * 2 * cmax)) return false;
+        }
+        if (!ensure(c, headpart, (np / 256 + 2) * 34 * 4) || !ensure(c, lossbuf, 16) || !ensure(c, cnt, 16)) return false;
+    }
+    an = n;
+    ah = h;
+    aw = w;
+    return true;
+}
+
+bool UNet::features(const float* pos, const float* nrm, const uint8_t* mask, const float* ssao, const float* noisy,
+                    const float eye[3], int h, int w, bf16_t* x, hipStream_t s) {
+    const size_t np = size_t(h) * w;
+    k_features<<<cdivu(np, 256), 256, 0, s>>>(pos, nrm, mask, ssao, noisy, eye[0], eye[1], eye[2], np, x);
+    return hip_ok(*ctx, hipGetLastError(), "net features");
+}
+
+namespace {
+// the forward input(s) of layer i
+void layer_inputs(const UNet& u, int i, const bf16_t* x, const bf16_t*& in0, const bf16_t*& in1) {
+    in1 = nullptr;
+    if (i == 0) in0 = x;
+    else if (i == 2 || i == 4 || i == 6) in0 = u.pooled[i / 2 - 1].as<bf16_t>();
+    else if (i == 8 || i == 10 || i == 12) {
+        in0 = u.act[i == 8 ? 5 : (i == 10 ? 3 : 1)].as<bf16_t>();
+        in1 = u.act[i - 1].as<bf16_t>();
+    } else in0 = u.act[i - 1].as<bf16_t>();
+}
+}  // namespace
+
+bool UNet::forward(const bf16_t* x, const uint8_t* mask, int n, int h, int w, float* ao_out, hipStream_t s) {
+    if (!ensure_acts(n, h, w, false)) return false;
+    Ctx& c = *ctx;
+    const size_t np = size_t(n) * h * w;
+    for (int i = 0; i < kNetLayers; ++i) {
+        Layer& l = layers[i];
+        const int lh = h >> l.level, lw = w >> l.level;
+        if (i == 2 || i == 4 || i == 6) {   // pool the previous level's last output
+            const int cp = layers[i - 1].cout;
+            k_pool_fwd<<<cdivu(size_t(n) * lh * lw * cp, 256), 256, 0, s>>>(act[i - 1].as<bf16_t>(), pooled[i / 2 - 1].as<bf16_t>(),
+                                                                             n, 2 * lh, 2 * lw, cp);
+        }
+        const bf16_t *in0, *in1;
+        layer_inputs(*this, i, x, in0, in1);
+        if (!conv3x3(c, in0, in1, l.c0, l.c1, w16.as<bf16_t>() + l.wofs16, l.kpad, params.as<float>() + l.boff, l.cout, 1, n, lh,
+                     lw, act[i].as<bf16_t>(), s))
+            return false;
+    }
+    k_head<<<cdivu(np, 256), 256, 0, s>>>(act[kNetLayers - 1].as<bf16_t>(), params.as<float>() + head_off, mask, np,
+                                          pred.as<float>(), ao_out);
+    return hip_ok(c, hipGetLastError(), "net forward");
+}
+
+bool UNet::backward(const bf16_t* x, const uint8_t* mask, const float* target, int n, int h, int w, hipStream_t s) {
+    Ctx& c = *ctx;
+    if (!ensure_acts(n, h, w, true)) return false;
+    const size_t np = size_t(n) * h * w;
+    const unsigned hb = cdivu(np, 256);
+    float* gr = grads.as<float>();
+    k_zero_u32<<<1, 4, 0, s>>>(cnt.as<uint32_t>());
+    k_mask_count<<<std::min(1024u, hb), 256, 0, s>>>(mask, np, cnt.as<uint32_t>());
+    k_head_bwd<<<hb, 256, 0, s>>>(act[kNetLayers - 1].as<bf16_t>(), params.as<float>() + head_off, pred.as<float>(), target,
+                                  mask, np, cnt.as<uint32_t>(), lvbuf[0][0].as<bf16_t>(), headpart.as<float>());
+    k_head_reduce<<<1, 64, 0, s>>>(headpart.as<float>(), int(hb), gr + head_off, lossbuf.as<float>());
+    // Three rotating buffers a level: the grad arriving at a layer's output (cur), its
+    // pre-activation grad (cur+1) and the grad it passes to its input (cur+2).
+    int cur[4] = {0, 0, 0, 0};
+    bf16_t* dy = lvbuf[0][0].as<bf16_t>();
+    for (int i = kNetLayers - 1; i >= 0; --i) {
+        Layer& l = layers[i];
+        const int lv = l.level, lh = h >> lv, lw = w >> lv;
+        const size_t px = size_t(n) * lh * lw;
+        const int cin = l.c0 + l.c1;
+        bf16_t* dpre = lvbuf[lv][(cur[lv] + 1) % 3].as<bf16_t>();
+        k_relu_bwd<<<cdivu(px * l.cout, 256), 256, 0, s>>>(act[i].as<bf16_t>(), dy, dpre, px * l.cout);
+        const bf16_t *in0, *in1;
+        layer_inputs(*this, i, x, in0, in1);
+        if (!conv3x3_wgrad(c, wpart, dpre, in0, in1, l.c0, l.c1, l.cout, n, lh, lw, gr + l.woff, gr + l.boff, s)) return false;
+        if (i == 0) break;   // the input features take no gradient
+        bf16_t* dx = lvbuf[lv][(cur[lv] + 2) % 3].as<bf16_t>();
+        if (!conv3x3(c, dpre, nullptr, l.cout, 0, wt16.as<bf16_t>() + l.wtofs16, l.kpad_t, nullptr, cin, 0, n, lh, lw, dx, s))
+            return false;
+        cur[lv] = (cur[lv] + 2) % 3;
+        if (i == 8 || i == 10 || i == 12) {
+            // the upsampled half goes to the lower level's last layer; the skip half waits
+            // for the encoder output it came from
+            const int llv = lv + 1;
+            bf16_t* dlow = lvbuf[llv][(cur[llv] + 1) % 3].as<bf16_t>();
+            const size_t nl = size_t(n) * (h >> llv) * (w >> llv) * l.c1;
+            k_up_grad<<<cdivu(nl, 256), 256, 0, s>>>(dx, cin, l.c0, l.c1, dlow, n, lh, lw);
+            cur[llv] = (cur[llv] + 1) % 3;
+            skip_dx[lv] = dx;
+            skip_cx[lv] = cin;
+            dy = dlow;
+        } else if (i == 2 || i == 4 || i == 6) {
+            // dx is the pooled tensor's grad; the encoder output it pooled also fed a skip
+            const int plv = lv - 1;
+            const int c0 = layers[i - 1].cout;
+            bf16_t* dsk = lvbuf[plv][(cur[plv] + 1) % 3].as<bf16_t>();
+            const size_t ns = size_t(n) * (h >> plv) * (w >> plv) * c0;
+            k_skip_grad<<<cdivu(ns, 256), 256, 0, s>>>(skip_dx[plv], skip_cx[plv], c0, dx, dsk, n, h >> plv, w >> plv);
+            cur[plv] = (cur[plv] + 1) % 3;
+            dy = dsk;
+        } else {
+            dy = dx;
+        }
+    }
+    return hip_ok(c, hipGetLastError(), "net backward");
+}
+
+bool UNet::adam(float lr, hipStream_t s) {
+    ++step;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - std::pow(b1, float(step)), c2 = 1.0f - std::pow(b2, float(step));
+    const size_t np = param_count();
+    k_adam<<<cdivu(np, 256), 256, 0, s>>>(params.as<float>(), grads.as<float>(), adam_m.as<float>(), adam_v.as<float>(), np, lr,
+                                          b1, b2, 1e-8f, c1, c2);
+    return hip_ok(*ctx, hipGetLastError(), "adam") && pack(s);
+}
+
+void UNet::release() {
+    auto drop = [](DevBuf& b) {
+        if (b.p) (void)hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+    };
+    DevBuf* bufs[] = {&params, &grads, &adam_m, &adam_v, &w16, &wt16, &pred, &headpart, &lossbuf, &cnt, &wpart};
+    for (auto* b : bufs) drop(*b);
+    for (auto& lv : lvbuf)
+        for (auto& b : lv) drop(b);
+    for (auto& b : act) drop(b);
+    for (auto& b : pooled) drop(b);
+}
+
+void net_release(Ctx& c) {
+    if (!c.net) return;
+    c.net->release();
+    delete c.net;
+    c.net = nullptr;
+}
+
+}  // namespace raw_ao
+
+// ---- C ABI ------------------------------------------------------------------------------
+using raw_ao::Ctx;
+using raw_ao::UNet;
+namespace {
+int nfail(Ctx& c, int code, const char* msg) {
+    if (c.err.empty()) c.err = msg;
+    return code;
+}
+bool nbegin(Ctx& c) {
+    c.err.clear();
+    return raw_ao::hip_ok(c, hipSetDevice(c.device), "hipSetDevice");
+}
+hipStream_t npick(Ctx& c, void* s) { return s ? static_cast<hipStream_t>(s) : c.stream; }
+UNet* net_of(Ctx& c) {
+    if (c.net) return c.net;
+    UNet* u = new (std::nothrow) UNet;
+    if (!u) return nullptr;
+    if (!u->init(c)) {
+        u->release();
+        delete u;
+        return nullptr;
+    }
+    c.net = u;
+    return u;
+}
+bool net_shape_ok(int n, int w, int h) {
+    return n > 0 && w >= 8 && h >= 8 && w % 8 == 0 && h % 8 == 0 && int64_t(n) * w * h <= (int64_t(1) << 28);
+}
+int net_read(raw_ao_ctx* x, float* out, uint64_t count, bool grads) {
+    if (!x || !out) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!nbegin(c)) return RAW_AO_ERR_DEVICE;
+    UNet* u = net_of(c);
+    if (!u) return nfail(c, RAW_AO_ERR_DEVICE, "network allocation failed");
+    const size_t np = u->param_count();
+    if (count != np) return nfail(c, RAW_AO_ERR_ARG, "parameter count mismatch");
+    if (!raw_ao::hip_ok(c, hipDeviceSynchronize(), "net read") ||
+        !raw_ao::hip_ok(c, hipMemcpy(out, grads ? u->grads.p : u->params.p, 4 * np, hipMemcpyDeviceToHost), "net read"))
+        return RAW_AO_ERR_DEVICE;
+    return RAW_AO_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int raw_ao_net_param_count(raw_ao_ctx* x, uint64_t* count) {
+    if (!x || !count) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!nbegin(c)) return RAW_AO_ERR_DEVICE;
+    UNet* u = net_of(c);
+    if (!u) return nfail(c, RAW_AO_ERR_DEVICE, "network allocation failed");
+    *count = u->param_count();
+    return RAW_AO_OK;
+}
+
+int raw_ao_net_set_params(raw_ao_ctx* x, const float* params, uint64_t count) {
+    if (!x || !params) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!nbegin(c)) return RAW_AO_ERR_DEVICE;
+    UNet* u = net_of(c);
+    if (!u) return nfail(c, RAW_AO_ERR_DEVICE, "network allocation failed");
+    const size_t np = u->param_count();
+    if (count != np) return nfail(c, RAW_AO_ERR_ARG, "parameter count mismatch");
+    if (!raw_ao::hip_ok(c, hipDeviceSynchronize(), "param upload") ||
+        !raw_ao::hip_ok(c, hipMemcpy(u->params.p, params, 4 * np, hipMemcpyHostToDevice), "param upload") ||
+        !raw_ao::hip_ok(c, hipMemset(u->adam_m.p, 0, 4 * np), "adam reset") ||
+        !raw_ao::hip_ok(c, hipMemset(u->adam_v.p, 0, 4 * np), "adam reset") || !u->pack(c.stream) ||
+        !raw_ao::hip_ok(c, hipStreamSynchronize(c.stream), "param pack"))
+        return RAW_AO_ERR_DEVICE;
+    u->step = 0;
+    return RAW_AO_OK;
+}
+
+int raw_ao_net_get_params(raw_ao_ctx* x, float* params, uint64_t count) { return net_read(x, params, count, false); }
+int raw_ao_net_get_grads(raw_ao_ctx* x, float* grads, uint64_t count) { return net_read(x, grads, count, true); }
+
+int raw_ao_net_features_dev(raw_ao_ctx* x, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, const float* d_ssao,
+                            const float* d_noisy, const float eye[3], int w, int h, uint16_t* d_x, void* stream) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!d_pos || !d_nrm || !d_mask || !d_ssao || !eye || !d_x || w <= 0 || h <= 0 || int64_t(w) * h > (int64_t(1) << 28))
+        return nfail(c, RAW_AO_ERR_ARG, "invalid feature arguments");
+    if (!nbegin(c)) return RAW_AO_ERR_DEVICE;
+    UNet* u = net_of(c);
+    if (!u) return nfail(c, RAW_AO_ERR_DEVICE, "network allocation failed");
+    return u->features(d_pos, d_nrm, d_mask, d_ssao, d_noisy, eye, h, w, d_x, npick(c, stream)) ? RAW_AO_OK : RAW_AO_ERR_DEVICE;
+}
+
+int raw_ao_net_forward_dev(raw_ao_ctx* x, const uint16_t* d_x, const uint8_t* d_mask, int n, int w, int h, float* d_ao,
+                           void* stream) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!d_x || !d_mask || !net_shape_ok(n, w, h)) return nfail(c, RAW_AO_ERR_ARG, "invalid network input (w, h multiples of 8)");
+    if (!nbegin(c)) return RAW_AO_ERR_DEVICE;
+    UNet* u = net_of(c);
+    if (!u) return nfail(c, RAW_AO_ERR_DEVICE, "network allocation failed");
+    return u->forward(d_x, d_mask, n, h, w, d_ao, npick(c, stream)) ? RAW_AO_OK : RAW_AO_ERR_DEVICE;
+}
+
+int raw_ao_net_train_step_dev(raw_ao_ctx* x, const uint16_t* d_x, const uint8_t* d_mask, const float* d_target, int n, int w,
+                              int h, float lr, void* stream) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!d_x || !d_mask || !d_target || !net_shape_ok(n, w, h) || !(lr >= 0.0f))
+        return nfail(c, RAW_AO_ERR_ARG, "invalid training arguments");
+    if (!nbegin(c)) return RAW_AO_ERR_DEVICE;
+    UNet* u = net_of(c);
+    if (!u) return nfail(c, RAW_AO_ERR_DEVICE, "network allocation failed");
+    const hipStream_t s = npick(c, stream);
+    if (!u->forward(d_x, d_mask, n, h, w, nullptr, s) || !u->backward(d_x, d_mask, d_target, n, h, w, s)) return RAW_AO_ERR_DEVICE;
+    if (lr > 0.0f && !u->adam(lr, s)) return RAW_AO_ERR_DEVICE;
+    return RAW_AO_OK;
+}
+
+int raw_ao_net_loss(raw_ao_ctx* x, float* loss) {
+    if (!x || !loss) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!nbegin(c)) return RAW_AO_ERR_DEVICE;
+    if (!c.net || !c.net->lossbuf.p) return nfail(c, RAW_AO_ERR_ARG, "no training step has run");
+    if (!raw_ao::hip_ok(c, hipDeviceSynchronize(), "loss read") ||
+        !raw_ao::hip_ok(c, hipMemcpy(loss, c.net->lossbuf.p, 4, hipMemcpyDeviceToHost), "loss read"))
+        return RAW_AO_ERR_DEVICE;
+    return RAW_AO_OK;
+}
+
+int raw_ao_net_read_act(raw_ao_ctx* x, int layer, uint16_t* out, uint64_t count) {
+    if (!x || !out) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!nbegin(c)) return RAW_AO_ERR_DEVICE;
+    if (!c.net || c.net->an == 0 || layer < 0 || layer >= raw_ao::kNetLayers) return nfail(c, RAW_AO_ERR_ARG, "no such activation");
+    const UNet& u = *c.net;
+    const auto& l = u.layers[layer];
+    const size_t n = size_t(u.an) * (u.ah >> l.level) * (u.aw >> l.level) * l.cout;
+    if (count != n) return nfail(c, RAW_AO_ERR_ARG, "activation size mismatch");
+    if (!raw_ao::hip_ok(c, hipDeviceSynchronize(), "act read") ||
+        !raw_ao::hip_ok(c, hipMemcpy(out, u.act[layer].p, 2 * n, hipMemcpyDeviceToHost), "act read"))
+        return RAW_AO_ERR_DEVICE;
+    return RAW_AO_OK;
+}
+
+int raw_ao_conv3x3_dev(raw_ao_ctx* x, const uint16_t* d_in0, const uint16_t* d_in1, int c0, int c1, const uint16_t* d_w, int kpad,
+                       const float* d_bias, int cout, int relu, int n, int w, int h, uint16_t* d_out, void* stream) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!d_in0 || (c1 > 0 && !d_in1) || !d_w || !d_out || !raw_ao::conv_shape_ok(c0, c1, cout) || kpad < 9 * (c0 + c1) ||
+        kpad % 32 != 0 || n <= 0 || w <= 0 || h <= 0 || (c1 > 0 && (w % 2 || h % 2)) || int64_t(n) * w * h > (int64_t(1) << 28))
+        return nfail(c, RAW_AO_ERR_ARG, "invalid convolution arguments");
+    if (!nbegin(c)) return RAW_AO_ERR_DEVICE;
+    return raw_ao::conv3x3(c, d_in0, d_in1, c0, c1, d_w, kpad, d_bias, cout, relu, n, h, w, d_out, npick(c, stream))
+               ? RAW_AO_OK
+               : RAW_AO_ERR_DEVICE;
+}
+
+int raw_ao_conv3x3_wgrad_dev(raw_ao_ctx* x, const uint16_t* d_dy, const uint16_t* d_in0, const uint16_t* d_in1, int c0, int c1,
+                             int cout, int n, int w, int h, float* d_gw, float* d_gb, void* stream) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!d_dy || !d_in0 || (c1 > 0 && !d_in1) || !d_gw || !d_gb || !raw_ao::conv_shape_ok(c0, c1, cout) ||
+        (cout != 32 && cout != 64 && cout != 96 && cout != 128) || n <= 0 || w <= 0 || h <= 0 || (c1 > 0 && (w % 2 || h % 2)) ||
+        int64_t(n) * w * h > (int64_t(1) << 28))
+        return nfail(c, RAW_AO_ERR_ARG, "invalid weight-gradient arguments");
+    if (!nbegin(c)) return RAW_AO_ERR_DEVICE;
+    UNet* u = net_of(c);
+    if (!u) return nfail(c, RAW_AO_ERR_DEVICE, "network allocation failed");
+    return raw_ao::conv3x3_wgrad(c, u->wpart, d_dy, d_in0, d_in1, c0, c1, cout, n, h, w, d_gw, d_gb, npick(c, stream))
+               ? RAW_AO_OK
+               : RAW_AO_ERR_DEVICE;
+}
+
+}  // extern "C"

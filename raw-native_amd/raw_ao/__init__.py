@@ -30,6 +30,7 @@ _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_u16p = np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS")
 _vp = C.c_void_p
 _lib = None
 
@@ -58,6 +59,19 @@ SIGNATURES = [
                                  C.POINTER(C.c_float)]),
     ("raw_ao_angle_table", C.c_int, [_vp, _f32p, _f32p]),
     ("raw_ao_bvh_bytes", C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    ("raw_ao_net_param_count", C.c_int, [_vp, C.POINTER(C.c_uint64)]),
+    ("raw_ao_net_set_params", C.c_int, [_vp, _f32p, C.c_uint64]),
+    ("raw_ao_net_get_params", C.c_int, [_vp, _f32p, C.c_uint64]),
+    ("raw_ao_net_get_grads", C.c_int, [_vp, _f32p, C.c_uint64]),
+    ("raw_ao_net_features_dev", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _f32p, C.c_int, C.c_int, _vp, _vp]),
+    ("raw_ao_net_forward_dev", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp]),
+    ("raw_ao_net_train_step_dev", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp]),
+    ("raw_ao_net_loss", C.c_int, [_vp, C.POINTER(C.c_float)]),
+    ("raw_ao_net_read_act", C.c_int, [_vp, C.c_int, _u16p, C.c_uint64]),
+    ("raw_ao_conv3x3_dev", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int,
+                                     C.c_int, C.c_int, _vp, _vp]),
+    ("raw_ao_conv3x3_wgrad_dev", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                           _vp, _vp, _vp]),
 ]
 
 
@@ -87,6 +101,11 @@ def _ptr(t):
     if not t.is_contiguous():
         raise ValueError("tensor must be contiguous")
     return C.c_void_p(t.data_ptr())
+
+
+def _optr(t):
+    """_ptr, or NULL for None."""
+    return _vp(None) if t is None else _ptr(t)
 
 
 class Context:
@@ -225,3 +244,54 @@ class Context:
         nb, tb, rec = C.c_uint64(), C.c_uint64(), C.c_uint32()
         self._check(lib().raw_ao_bvh_bytes(self._h, C.byref(nb), C.byref(tb), C.byref(rec)))
         return nb.value, tb.value, rec.value
+
+    # ---- learned AO surrogate ------------------------------------------------------
+    def net_param_count(self):
+        n = C.c_uint64()
+        self._check(lib().raw_ao_net_param_count(self._h, C.byref(n)))
+        return n.value
+
+    def net_set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32).ravel()
+        self._check(lib().raw_ao_net_set_params(self._h, p, p.size))
+
+    def net_get_params(self):
+        p = np.zeros(self.net_param_count(), np.float32)
+        self._check(lib().raw_ao_net_get_params(self._h, p, p.size))
+        return p
+
+    def net_get_grads(self):
+        g = np.zeros(self.net_param_count(), np.float32)
+        self._check(lib().raw_ao_net_get_grads(self._h, g, g.size))
+        return g
+
+    def net_features_dev(self, position, normal, mask, ssao, noisy, eye, w, h, out, stream=None):
+        """out: (h, w, 8) bf16 bits as a torch int16 tensor."""
+        self._check(lib().raw_ao_net_features_dev(self._h, _ptr(position), _ptr(normal), _ptr(mask), _ptr(ssao),
+                                                  _optr(noisy), np.ascontiguousarray(eye, np.float32), w, h, _ptr(out),
+                                                  _vp(stream)))
+
+    def net_forward_dev(self, x, mask, n, w, h, ao, stream=None):
+        self._check(lib().raw_ao_net_forward_dev(self._h, _ptr(x), _ptr(mask), n, w, h, _ptr(ao), _vp(stream)))
+
+    def net_train_step_dev(self, x, mask, target, n, w, h, lr, stream=None):
+        self._check(lib().raw_ao_net_train_step_dev(self._h, _ptr(x), _ptr(mask), _ptr(target), n, w, h, float(lr),
+                                                    _vp(stream)))
+
+    def net_loss(self):
+        v = C.c_float()
+        self._check(lib().raw_ao_net_loss(self._h, C.byref(v)))
+        return v.value
+
+    def net_read_act(self, layer, count):
+        out = np.zeros(count, np.uint16)
+        self._check(lib().raw_ao_net_read_act(self._h, layer, out, count))
+        return out
+
+    def conv3x3_dev(self, in0, in1, c0, c1, w, kpad, bias, cout, relu, n, width, height, out, stream=None):
+        self._check(lib().raw_ao_conv3x3_dev(self._h, _ptr(in0), _optr(in1), c0, c1, _ptr(w), kpad, _optr(bias), cout,
+                                             int(relu), n, width, height, _ptr(out), _vp(stream)))
+
+    def conv3x3_wgrad_dev(self, dy, in0, in1, c0, c1, cout, n, width, height, gw, gb, stream=None):
+        self._check(lib().raw_ao_conv3x3_wgrad_dev(self._h, _ptr(dy), _ptr(in0), _optr(in1), c0, c1, cout, n, width,
+                                                   height, _ptr(gw), _ptr(gb), _vp(stream)))

@@ -1,0 +1,259 @@
+"""The learned-AO network's HIP kernels against the float64 oracle (oracle/unet_oracle.py),
+through the C ABI: the single-layer convolution and weight-gradient kernels, each network
+layer's forward output from the GPU's own inputs, the head bit for bit, the gradients of
+a training step, the Adam update bit for bit, and a loss that falls in training.
+
+Tolerances: a convolution output may differ from the oracle's bf16 rounding of the exact
+sum by one bf16 ulp (2^-7 relative), or by 1e-6 of the sum of |terms| where the sum is
+near zero (the GPU sums in float32 in another order); a weight gradient by 1e-5 of the sum
+of |terms|."""
+import numpy as np
+import pytest
+import torch
+
+import unet_oracle as U
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    a = np.ascontiguousarray(a)
+    if a.dtype == np.uint16:
+        a = a.view(np.int16)
+    return torch.from_numpy(a).cuda()
+
+
+def host16(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy().view(np.uint16)
+
+
+def round32(k):
+    return (k + 31) // 32 * 32
+
+
+def bf16_case(r, shape, scale=1.0):
+    return U.bf16_bits(r.normal(0, scale, shape).astype(np.float32))
+
+
+def check_conv(gpu, ref, absum):
+    gap = np.abs(gpu - ref)
+    bound = 2.0 ** -7 * np.abs(ref) + 1e-6 * absum
+    assert (gap <= bound).all(), float((gap - bound).max())
+    assert (gpu == ref).mean() >= 0.99
+
+
+CONV_CASES = [  # n, h, w, c0, c1, cout, relu
+    (1, 8, 8, 8, 0, 32, 1),
+    (2, 10, 6, 32, 64, 32, 1),
+    (1, 12, 20, 96, 128, 96, 1),
+    (1, 6, 10, 96, 0, 224, 0),
+    (3, 5, 7, 16, 0, 48, 0),
+    (1, 16, 16, 128, 0, 128, 1),
+]
+
+
+@pytest.mark.parametrize("case", CONV_CASES)
+def test_conv3x3_matches_oracle(ctx, case):
+    n, h, w, c0, c1, cout, relu = case
+    print("conv case", case, flush=True)
+    r = np.random.default_rng(hash(case) % (1 << 32))
+    k = 9 * (c0 + c1)
+    kpad = round32(k)
+    x0 = bf16_case(r, (n, h, w, c0))
+    x1 = bf16_case(r, (n, h // 2, w // 2, c1)) if c1 else None
+    wt = bf16_case(r, (cout, k), np.sqrt(2.0 / k))
+    wpad = np.zeros((cout, kpad), np.uint16)
+    wpad[:, :k] = wt
+    bias = r.normal(0, 0.1, cout).astype(np.float32)
+    out = torch.zeros(n * h * w * cout, dtype=torch.int16, device="cuda")
+    keep = [dev(x0), dev(x1) if c1 else None, dev(wpad), dev(bias)]
+    ctx.conv3x3_dev(keep[0], keep[1], c0, c1, keep[2], kpad, keep[3], cout, relu, n, w, h, out)
+    gpu = U.from_bits(host16(out)).reshape(n, h, w, cout)
+    f0 = U.from_bits(x0)
+    f1 = U.from_bits(x1) if c1 else None
+    ref = U.conv3x3(f0, f1, U.from_bits(wt), bias.astype(np.float64), relu, exact=False)
+    xin = f0 if f1 is None else np.concatenate([f0, U.upsample2(f1)], 3)
+    absum = (np.abs(U.im2col(xin)) @ np.abs(U.from_bits(wt)).T).reshape(ref.shape) + np.abs(bias)
+    check_conv(gpu, ref, absum)
+
+
+WGRAD_CASES = [  # n, h, w, c0, c1, cout
+    (1, 8, 8, 8, 0, 32),
+    (2, 10, 6, 32, 64, 32),
+    (1, 12, 20, 96, 128, 96),
+    (1, 16, 16, 128, 0, 128),
+    (3, 9, 7, 64, 0, 64),
+    (1, 96, 80, 32, 0, 64),
+]
+
+
+@pytest.mark.parametrize("case", WGRAD_CASES)
+def test_conv3x3_wgrad_matches_oracle(ctx, case):
+    n, h, w, c0, c1, cout = case
+    print("wgrad case", case, flush=True)
+    r = np.random.default_rng(hash(case) % (1 << 32) + 1)
+    k = 9 * (c0 + c1)
+    x0 = bf16_case(r, (n, h, w, c0))
+    x1 = bf16_case(r, (n, h // 2, w // 2, c1)) if c1 else None
+    dy = bf16_case(r, (n, h, w, cout), 0.01)
+    gw = torch.zeros(cout * k, dtype=torch.float32, device="cuda")
+    gb = torch.zeros(cout, dtype=torch.float32, device="cuda")
+    keep = [dev(dy), dev(x0), dev(x1) if c1 else None]
+    ctx.conv3x3_wgrad_dev(keep[0], keep[1], keep[2], c0, c1, cout, n, w, h, gw, gb)
+    torch.cuda.synchronize()
+    f0 = U.from_bits(x0)
+    f1 = U.from_bits(x1) if c1 else None
+    fdy = U.from_bits(dy)
+    rw, rb = U.conv3x3_wgrad(fdy, f0, f1)
+    xin = f0 if f1 is None else np.concatenate([f0, U.upsample2(f1)], 3)
+    aw = np.abs(fdy.reshape(-1, cout)).T @ np.abs(U.im2col(xin))
+    ab = np.abs(fdy.reshape(-1, cout)).sum(0)
+    assert (np.abs(gw.cpu().numpy().reshape(cout, k) - rw) <= 1e-5 * aw + 1e-30).all()
+    assert (np.abs(gb.cpu().numpy() - rb) <= 1e-5 * ab + 1e-30).all()
+
+
+def random_params(seed):
+    r = np.random.default_rng(seed)
+    layers = []
+    for c0, c1, co, _ in U.LAYERS:
+        k = 9 * (c0 + c1)
+        layers.append((r.normal(0, np.sqrt(2.0 / k), (co, k)), r.normal(0, 0.05, co)))
+    return U.pack(layers, (r.normal(0, 0.2, 32), 0.3)).astype(np.float32)
+
+
+def net_case(seed, n, h, w):
+    r = np.random.default_rng(seed)
+    x = r.normal(0, 1, (n, h, w, U.FEATURES))
+    mask = (r.random((n, h, w)) > 0.2).astype(np.uint8)
+    x[mask == 0] = 0.0
+    target = r.random((n, h, w)).astype(np.float32)
+    return U.bf16_bits(x.astype(np.float32)), mask, target
+
+
+def gpu_cache(ctx, params, xb, mask, n, h, w):
+    """The oracle's backward cache filled with the GPU's own activations."""
+    layers, head = U.unpack(params.astype(np.float64))
+    acts = []
+    for i, (c0, c1, co, lv) in enumerate(U.LAYERS):
+        cnt = n * (h >> lv) * (w >> lv) * co
+        acts.append(U.from_bits(ctx.net_read_act(i, cnt)).reshape(n, h >> lv, w >> lv, co))
+    pooled = {i: U.pool2(acts[s], exact=False) for i, s in U.POOL_INTO.items()}
+    return dict(layers=[(U.bf16(wt), b) for wt, b in layers], head=head, acts=acts, pooled=pooled,
+                x=U.from_bits(xb), mask=mask, exact=False)
+
+
+def head_pred(cache):
+    hw, hb = cache["head"]
+    a = cache["acts"][-1].astype(np.float32)
+    s = np.full(a.shape[:3], np.float32(hb), np.float32)
+    for c in range(32):
+        s = s + np.float32(hw[c]) * a[..., c]
+    return s
+
+
+@pytest.mark.parametrize("shape", [(2, 16, 24), (1, 64, 96)])
+def test_net_forward_layers_match_oracle(ctx, shape):
+    n, h, w = shape
+    print("net forward", shape, flush=True)
+    params = random_params(20)
+    xb, mask, _ = net_case(21, n, h, w)
+    ctx.net_set_params(params)
+    ao = torch.zeros(n * h * w, dtype=torch.float32, device="cuda")
+    keep = [dev(xb), dev(mask)]
+    ctx.net_forward_dev(keep[0], keep[1], n, w, h, ao)
+    cache = gpu_cache(ctx, params, xb, mask, n, h, w)
+    acts = cache["acts"]
+    for i, (c0, c1, co, lv) in enumerate(U.LAYERS):
+        in0, in1 = U.layer_inputs(cache, i)
+        wt, b = cache["layers"][i]
+        ref = U.conv3x3(in0, in1, wt, b, True, exact=False)
+        xin = in0 if in1 is None else np.concatenate([in0, U.upsample2(in1)], 3)
+        absum = (np.abs(U.im2col(xin)) @ np.abs(wt).T).reshape(ref.shape) + np.abs(b)
+        check_conv(acts[i], ref, absum)
+    pred = head_pred(cache)
+    want = np.where(mask != 0, np.clip(pred, 0, 1), 1).astype(np.float32)
+    assert np.array_equal(ao.cpu().numpy().reshape(n, h, w).view(np.uint32), want.view(np.uint32))
+
+
+def test_net_gradients_match_oracle(ctx):
+    """GPU gradients of one step against the oracle's emulated backward over the GPU's own
+    activations (per layer: relative L2 error <= 1e-2), and against the float64 definition
+    (cosine >= 0.99: the size of bf16 training's rounding)."""
+    n, h, w = 2, 32, 48
+    params = random_params(30)
+    xb, mask, target = net_case(31, n, h, w)
+    ctx.net_set_params(params)
+    keep = [dev(xb), dev(mask), dev(target)]
+    ctx.net_train_step_dev(keep[0], keep[1], keep[2], n, w, h, 0.0)
+    g = ctx.net_get_grads().astype(np.float64)
+    cache = gpu_cache(ctx, params, xb, mask, n, h, w)
+    pred = head_pred(cache).astype(np.float64)
+    gref, lref = U.backward(pred, target.astype(np.float64), cache)
+    assert abs(ctx.net_loss() - lref) <= 1e-5 * lref
+    off = 0
+    errs = []
+    for c0, c1, co, _ in U.LAYERS + [(32, 0, 0, 0)]:
+        size = co * 9 * (c0 + c1) + co if co else 33
+        a, b = g[off:off + size], gref[off:off + size]
+        errs.append(np.linalg.norm(a - b) / np.linalg.norm(b))
+        off += size
+    print("per-layer relative gradient error", ["%.2e" % e for e in errs])
+    assert max(errs) <= 1e-2
+    pe, ce = U.forward(params.astype(np.float64), U.from_bits(xb), mask, exact=True)
+    ge, _ = U.backward(pe, target.astype(np.float64), ce)
+    assert g @ ge / np.linalg.norm(g) / np.linalg.norm(ge) >= 0.99
+
+
+def test_adam_step_bit_exact(ctx):
+    n, h, w = 1, 16, 16
+    params = random_params(40)
+    xb, mask, target = net_case(41, n, h, w)
+    ctx.net_set_params(params)
+    lr = np.float32(1e-3)
+    keep = [dev(xb), dev(mask), dev(target)]
+    ctx.net_train_step_dev(keep[0], keep[1], keep[2], n, w, h, float(lr))
+    g = ctx.net_get_grads()
+    p1 = ctx.net_get_params()
+    b1, b2, one = np.float32(0.9), np.float32(0.999), np.float32(1)
+    m = (one - b1) * g
+    v = ((one - b2) * g) * g
+    c1, c2 = one - b1, one - b2
+    want = params - (lr * (m / c1)) / (np.sqrt(v / c2) + np.float32(1e-8))
+    assert np.array_equal(p1.view(np.uint32), want.astype(np.float32).view(np.uint32))
+
+
+def test_training_lowers_the_loss(ctx):
+    """40 Adam steps on one batch fit a smooth function of the features."""
+    n, h, w = 2, 32, 32
+    r = np.random.default_rng(50)
+    x = r.normal(0, 1, (n, h, w, U.FEATURES)).astype(np.float32)
+    mask = np.ones((n, h, w), np.uint8)
+    target = (0.5 + 0.4 * np.tanh(x[..., 1] - 0.5 * x[..., 4])).astype(np.float32)
+    ctx.net_set_params(random_params(51))
+    tx, tm, tt = dev(U.bf16_bits(x)), dev(mask), dev(target)
+    losses = []
+    for _ in range(40):
+        ctx.net_train_step_dev(tx, tm, tt, n, w, h, 2e-3)
+        losses.append(ctx.net_loss())
+    print("losses", losses[0], losses[-1])
+    assert losses[-1] < 0.3 * losses[0]
+
+
+def test_features_match_oracle(ctx):
+    h, w = 24, 40
+    r = np.random.default_rng(60)
+    pos = r.normal(0, 5, (h, w, 3)).astype(np.float32)
+    nrm = r.normal(size=(h, w, 3))
+    nrm = (nrm / np.linalg.norm(nrm, axis=2, keepdims=True)).astype(np.float32)
+    mask = (r.random((h, w)) > 0.3).astype(np.uint8)
+    ssao = r.random((h, w)).astype(np.float32)
+    noisy = r.random((h, w)).astype(np.float32)
+    eye = np.array([1.0, 2.0, 3.0], np.float32)
+    for nz in (None, noisy):
+        out = torch.zeros(h * w * 8, dtype=torch.int16, device="cuda")
+        keep = [dev(pos), dev(nrm), dev(mask), dev(ssao), None if nz is None else dev(nz)]
+        ctx.net_features_dev(keep[0], keep[1], keep[2], keep[3], keep[4], eye, w, h, out)
+        gpu = U.from_bits(host16(out)).reshape(h, w, 8)
+        ref = U.features(pos, nrm, mask, ssao, nz, eye)
+        assert (np.abs(gpu - U.bf16(ref)) <= 2.0 ** -7 * np.abs(ref) + 1e-7).all()
