@@ -68,7 +68,113 @@ def cpu_baseline(tris, pos, nrm, mask, spp, radius, row_step):
             "build_s": build_s}
 
 
+WEIGHTS = os.path.join(ROOT, "raw-native_amd", "weights")
+
+
+def _timed(torch, fn, steps, warmup=2):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = None
+    for _ in range(steps):
+        r = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps, r
+
+
+def _rmse(ctx, raw_ao, torch, approx, truth, mask_t, mask_np, W, H, stream):
+    """RMSE (the reconcile's float64 row-major sum) and maximum error over covered pixels."""
+    err = torch.empty_like(approx)
+    mx = torch.zeros(4, dtype=torch.float32, device=approx.device)
+    ctx.error_map_dev(approx, truth, mask_t, W, H, err, mx, stream)
+    rmse, _ = raw_ao.host.rmse_of(err.cpu().numpy(), mask_np)
+    return float(rmse), float(mx[0].item())
+
+
+def learned_surrogate(ctx, raw_ao, torch, d_pos, d_nrm, d_mask, d_rt, d_ss, mask, eye, W, H, spp, radius, steps, trace_ms,
+                      frame_ms, stream):
+    """The learned AO surrogate on the same frame: network-only and probe-and-fallback frames,
+    timed end to end, with RMSE / max error against this frame's 64-spp ray-traced AO."""
+    path = os.path.join(WEIGHTS, "ao_surrogate.raow")
+    if not os.path.exists(path):
+        return {"skipped": "no trained weights at raw-native_amd/weights/ao_surrogate.raow"}
+    import raw_ao.learned as L
+    _, meta = L.load_weights(path)
+    ctx.net_load(path)
+    d_l = torch.empty_like(d_rt)
+    args = (d_pos, d_nrm, d_mask, None, eye, W, H, d_l)
+    ms_net, _ = _timed(torch, lambda: ctx.learned_ao_dev(*args, rt_radius=radius, stream=stream), steps)
+    rmse, mx = _rmse(ctx, raw_ao, torch, d_l, d_rt, d_mask, mask, W, H, stream)
+    ss_rmse, ss_max = _rmse(ctx, raw_ao, torch, d_ss, d_rt, d_mask, mask, W, H, stream)
+    res = {"weights": os.path.basename(path), "trained_on": "%d views of %d seeded scenes (seeds %s), %s"
+           % (meta["scenes"] * meta["views_per_scene"], meta["scenes"], meta["scene_seeds"], "held-out scene"),
+           "ms_per_frame": ms_net, "rmse_vs_rt": rmse, "max_error_vs_rt": mx,
+           "ablation_ssao_rmse_vs_rt": ss_rmse, "ablation_ssao_max_error_vs_rt": ss_max,
+           "rt64_trace_ms": trace_ms, "rt64_frame_ms": frame_ms, "speedup_vs_rt64_trace": trace_ms / ms_net,
+           "meets_rmse_0.02": rmse <= 0.02}
+    stride, thr = ARGS.probe_stride, ARGS.fallback_rmse
+    if stride > 0:
+        ms_probe, rep = _timed(torch, lambda: ctx.learned_ao_dev(*args, rt_radius=radius, probe_stride=stride,
+                                                                  fallback_rmse=1e9, stream=stream), steps)
+        ms_fb, rep_fb = _timed(torch, lambda: ctx.learned_ao_dev(*args, rt_radius=radius, probe_stride=stride,
+                                                                  fallback_rmse=thr, stream=stream), max(2, steps // 2))
+        rmse_fb, mx_fb = _rmse(ctx, raw_ao, torch, d_l, d_rt, d_mask, mask, W, H, stream)
+        res["probe"] = {"stride": stride, "ms_per_frame_with_probe": ms_probe, "probe_pixels": rep["probe_pixels"],
+                        "probe_rmse": rep["probe_rmse"], "fallback_rmse": thr, "fell_back": rep_fb["fell_back"],
+                        "ms_per_frame_delivered": ms_fb, "delivered_rmse_vs_rt": rmse_fb,
+                        "delivered_max_error_vs_rt": mx_fb}
+    log("learned surrogate: %.3f ms/frame, RMSE %.4f vs RT (SSAO %.4f)" % (ms_net, rmse, ss_rmse))
+    return res
+
+
+def denoise_4k(ctx, raw_ao, torch, steps, stream):
+    """4 rays/pixel ray-traced AO at 3840 x 2160 on the seeded ~2M-triangle scene, denoised by
+    the network, against a 256-spp ray-traced reference; beside plain 64-spp ray tracing."""
+    import numpy as np
+    path = os.path.join(WEIGHTS, "ao_denoiser.raow")
+    if not os.path.exists(path):
+        return {"skipped": "no trained weights at raw-native_amd/weights/ao_denoiser.raow"}
+    W, H, radius = 3840, 2160, 1.0
+    spec = raw_ao.host.SCENE_2M
+    vpos, vnrm, idx, _ = raw_ao.host.scene_seeded(spec)
+    tris = raw_ao.host.mesh_tris(vpos, idx)
+    eye = raw_ao.host.orbit_camera(1)
+    _, pos, nrm, mask = raw_ao.host.rasterize(vpos, vnrm, idx, eye, (0, 2, 0), (0, 1, 0), 0.9,
+                                              float(np.float32(W) / np.float32(H)), 0.1, 200.0, W, H)
+    dev = torch.device("cuda:0")
+    d_tris = torch.from_numpy(tris).to(dev)
+    d_pos, d_nrm, d_mask = (torch.from_numpy(x).to(dev) for x in (pos, nrm, mask))
+    img = lambda: torch.empty((H, W), dtype=torch.float32, device=dev)
+    d_ref, d_64, d_4, d_dn = img(), img(), img(), img()
+    ctx.net_load(path)
+    ms_build, _ = _timed(torch, lambda: ctx.build_ploc_dev(d_tris, 16, stream), 3, 1)
+    ctx.compute_rtao_dev(d_pos, d_nrm, d_mask, W, H, 256, radius, d_ref, stream)
+    ms_64, _ = _timed(torch, lambda: ctx.compute_rtao_dev(d_pos, d_nrm, d_mask, W, H, 64, radius, d_64, stream), steps)
+
+    def learned():
+        ctx.compute_rtao_dev(d_pos, d_nrm, d_mask, W, H, 4, radius, d_4, stream)
+        return ctx.learned_ao_dev(d_pos, d_nrm, d_mask, d_4, eye, W, H, d_dn, rt_radius=radius, stream=stream)
+
+    ms_dn, _ = _timed(torch, learned, steps)
+    ms_4, _ = _timed(torch, lambda: ctx.compute_rtao_dev(d_pos, d_nrm, d_mask, W, H, 4, radius, d_4, stream), steps)
+    q = {k: _rmse(ctx, raw_ao, torch, v, d_ref, d_mask, mask, W, H, stream) for k, v in
+         (("denoised", d_dn), ("rt64", d_64), ("rt4", d_4))}
+    log("denoise 4K: %.2f ms/frame (RMSE %.4f vs 256 spp) vs 64-spp RT %.2f ms (RMSE %.4f)"
+        % (ms_dn, q["denoised"][0], ms_64, q["rt64"][0]))
+    return {"workload": "seeded %d-triangle scene, %dx%d, 4-spp ray-traced AO + learned denoise vs 64-spp ray tracing, "
+                        "both against 256-spp ray tracing" % (tris.shape[0], W, H),
+            "covered_pixels": int(mask.sum()), "ploc_build_ms": ms_build,
+            "ms_per_frame_denoised": ms_dn, "ms_rt4_trace": ms_4, "ms_per_frame_rt64": ms_64,
+            "speedup_vs_rt64": ms_64 / ms_dn,
+            "rmse_vs_rt256": {k: v[0] for k, v in q.items()}, "max_error_vs_rt256": {k: v[1] for k, v in q.items()}}
+
+
+ARGS = None
+
+
 def main():
+    global ARGS
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
@@ -83,7 +189,12 @@ def main():
                     "(the known-size kernel that scales the HBM counters)")
     ap.add_argument("--kernel", choices=["wide", "binary"], default="wide", help="traversal form")
     ap.add_argument("--tri-q", type=int, default=0, help="wide kernel's triangle-phase threshold (1..64; 0: default)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned AO surrogate at 1080p")
+    ap.add_argument("--no-denoise", action="store_true", help="skip the 4-spp + learned denoise workload at 4K")
+    ap.add_argument("--probe-stride", type=int, default=8, help="learned frame: ray-traced probe of every n-th pixel")
+    ap.add_argument("--fallback-rmse", type=float, default=0.02, help="learned frame: ray trace above this probe RMSE")
     a = ap.parse_args()
+    ARGS = a
     if a.gpus != 1:
         print("bench.py measures one GPU: the AO path's multi-GPU form (replicated BVH, row-split frame) is "
               "not built in this tree; run with --gpus 1", file=sys.stderr)
@@ -157,6 +268,12 @@ def main():
     wall = time.perf_counter() - t0
     log("timed %d frames in %.3f s" % (a.steps, wall))
 
+    frame_ms = wall * 1e3 / a.steps
+    learned = None
+    if not a.no_learned:
+        learned = learned_surrogate(ctx, raw_ao, torch, d_pos, d_nrm, d_mask, d_ao, d_ss, mask, eye, W, H, spp, radius,
+                                    a.steps, float(np.mean(ms["trace"])), frame_ms, stream)
+
     rays = covered * spp
     value = rays * a.steps / wall
     trace_ms = float(np.mean(ms["trace"]))
@@ -182,6 +299,10 @@ def main():
                      "kernel": "k_trace_ao", "algorithmic_bytes_per_launch": alg_bytes,
                      "node_visits": visits, "triangle_tests": tests},
     }
+    if learned is not None:
+        out["learned_ao"] = learned
+    if not a.no_denoise:
+        out["denoise_4k"] = denoise_4k(ctx, raw_ao, torch, a.steps, stream)
     if a.calib_copy:
         n = a.calib_copy * (1 << 20) // 4
         x = torch.empty(n, dtype=torch.float32, device=dev).fill_(1.0)

@@ -59,7 +59,11 @@ enum {
     /* Wide kernel tuning, 1..64 (default 16): a wave runs its triangle phase once this
      * many 64ths of its live lanes wait on leaf triangles (or none can take a node step).
      * Results are the same for every value; only the speed differs. */
-    RAW_AO_OPT_TRI_PHASE = 3
+    RAW_AO_OPT_TRI_PHASE = 3,
+    /* The network's convolution kernel: 0 (default) the tap-major kernel with LDS-staged
+     * weights wherever a layer's input channels are a multiple of 32, 1 the general
+     * kernel everywhere. The sums run in the same order: results are identical. */
+    RAW_AO_OPT_CONV_KERNEL = 4
 };
 enum {
     RAW_AO_TRACE_WIDE = 0,     /* 4-wide nodes, LDS stack, triangle tests batched per wave */
@@ -136,6 +140,50 @@ int raw_ao_angle_table(raw_ao_ctx* ctx, float* sin_out, float* cos_out);
 /* Bytes of the context's device-resident BVH (nodes + triangles) in its traversal form. */
 int raw_ao_bvh_bytes(raw_ao_ctx* ctx, uint64_t* node_bytes, uint64_t* tri_bytes, uint32_t* node_record_bytes);
 
+/* ---- batched AO: many views of several scenes ---------------------------------------
+ * No reference function (the reference renders one view of one scene per call,
+ * src/renderer/render.cpp:26-54); each view's outputs are exactly computeRTAO's and
+ * computeSSAO's for that view, and the reconcile's error map (src/cert/reconcile.cpp:4-21).
+ * Scenes are built (eagerly, recording their PLOC and collapse grids) when added; a run
+ * traces every view's rays in ONE persistent launch whose waves take 64-ray groups from a
+ * global queue, then finishes each view's AO, SSAO and reconcile on the device. The
+ * caller's device arrays must outlive the batch. The per-view RMSE sums squares in float64
+ * on the device in a fixed order (not the reference's row-major order: the float RMSE can
+ * differ in its last bit). */
+typedef struct raw_ao_batch raw_ao_batch;
+int raw_ao_batch_open(int device_ordinal, raw_ao_batch** out);
+void raw_ao_batch_close(raw_ao_batch* batch);
+const char* raw_ao_batch_last_error(const raw_ao_batch* batch);
+/* d_tris9: ntri triangles (9 floats) on the device, read again by every rebuild. */
+int raw_ao_batch_add_scene(raw_ao_batch* batch, const float* d_tris9, uint32_t ntri, int radius, int* scene_id);
+/* One view of a scene: G-buffer in, d_ao out (spp rays of length radius a covered pixel);
+ * d_ssao (ssao_samples, ssao_radius) and d_err (|ssao - ao| where covered) may be NULL. */
+int raw_ao_batch_add_view(raw_ao_batch* batch, int scene, const float* d_pos, const float* d_nrm, const uint8_t* d_mask,
+                          int w, int h, int spp, float radius, int ssao_samples, float ssao_radius, float* d_ao,
+                          float* d_ssao, float* d_err, int* view_id);
+/* Launch parameters: 64-ray groups a wave takes from the queue at a time, the triangle
+ * phase threshold (of 64 lanes), persistent blocks per compute unit. Results never depend
+ * on them. */
+int raw_ao_batch_set_tuning(raw_ao_batch* batch, int chunk, int tri_q, int blocks_per_cu);
+int raw_ao_batch_get_tuning(raw_ao_batch* batch, int* chunk, int* tri_q, int* blocks_per_cu);
+/* Enqueues one batched frame (rebuild != 0: first rebuild every scene from its triangles
+ * with the recorded grids) on stream (NULL: the batch's own). */
+int raw_ao_batch_run(raw_ao_batch* batch, int rebuild, void* stream);
+/* Records one batched frame (with the scenes' builds when with_builds != 0) into a hipGraph;
+ * raw_ao_batch_launch replays it. After replays with builds, raw_ao_batch_check confirms
+ * each rebuilt tree is the recorded one (RAW_AO_ERR_BUILD if a scene changed shape). */
+int raw_ao_batch_capture(raw_ao_batch* batch, int with_builds);
+int raw_ao_batch_launch(raw_ao_batch* batch, void* stream);
+int raw_ao_batch_check(raw_ao_batch* batch);
+/* Per view (arrays of the view count): RMSE and max error of SSAO against RT AO over the
+ * covered pixels, covered pixels; *rays = the rays traced. Waits for the device. */
+int raw_ao_batch_results(raw_ao_batch* batch, float* rmse, float* max_error, uint32_t* pixels, uint64_t* rays);
+/* Autotuner: times eager runs over queue chunks, triangle phase and persistent blocks
+ * (coordinate descent from the defaults) and keeps the fastest; with cache_path, a
+ * result for the same device and workload is read from that text file instead, and new
+ * results are appended to it. */
+int raw_ao_batch_tune(raw_ao_batch* batch, const char* cache_path, float* best_ms);
+
 /* ---- the learned AO surrogate --------------------------------------------------------
  * No reference function: the reference's only learned precedent is its roughness network
  * (scripts/rr_model.py:43-80, src/rhi/vulkan/vk_rrnet.cpp), whose interface this follows:
@@ -168,6 +216,34 @@ int raw_ao_net_train_step_dev(raw_ao_ctx* ctx, const uint16_t* d_x, const uint8_
                               int n, int w, int h, float lr, void* stream);
 /* The loss of the last training step (waits for the device). */
 int raw_ao_net_loss(raw_ao_ctx* ctx, float* loss);
+/* Loads a weights file written by raw_ao.learned.save_weights: "RAOW", uint32 version 1,
+ * uint64 parameter count, uint32 metadata length, the fp32 parameters (little endian), the
+ * metadata (JSON text). */
+int raw_ao_net_load(raw_ao_ctx* ctx, const char* path);
+
+/* The learned AO frame, for callers of computeRTAO (raw/renderer/ray_ao.hpp:6-8) that
+ * accept a learned answer judged against ray tracing: SSAO (ssao_samples, ssao_radius, as
+ * computeSSAO), the features (d_noisy: a low-spp computeRTAO result for the denoiser form,
+ * or NULL) and the network into d_ao; w, h multiples of 8. With probe_stride > 0, rt_spp-spp
+ * ray-traced AO (rt_radius) of every probe_stride-th pixel in x and y of the context's built
+ * scene estimates the frame's RMSE against ray tracing (the report; this waits for the
+ * device), and when it exceeds fallback_rmse the frame is ray traced instead (fell_back).
+ * The probe traces its own sub-image, so its rays differ from a full frame's at the same
+ * pixels: the estimate is statistical. */
+typedef struct raw_ao_learned_report {
+    uint32_t probe_pixels;   /* covered probe pixels */
+    float probe_rmse;        /* the network against rt_spp ray tracing on them */
+    float probe_max_error;
+    int32_t fell_back;       /* 1: d_ao holds the ray-traced frame */
+} raw_ao_learned_report;
+int raw_ao_learned_ao_dev(raw_ao_ctx* ctx, const float* d_pos, const float* d_nrm, const uint8_t* d_mask,
+                          const float* d_noisy, const float eye[3], int w, int h, int ssao_samples, float ssao_radius,
+                          int rt_spp, float rt_radius, int probe_stride, float fallback_rmse, float* d_ao,
+                          raw_ao_learned_report* report, void* stream);
+int raw_ao_learned_ao(raw_ao_ctx* ctx, const float* pos, const float* nrm, const uint8_t* mask, const float* noisy,
+                      const float eye[3], int w, int h, int ssao_samples, float ssao_radius, int rt_spp, float rt_radius,
+                      int probe_stride, float fallback_rmse, float* ao, raw_ao_learned_report* report);
+
 /* Verification entry points: a layer's bf16 output from the last forward pass (count =
  * n * (h >> level) * (w >> level) * cout), and the single-layer kernels: a 3x3 convolution
  * (zero padding) of in0 (c0 channels) concatenated with in1 upsampled 2x (c1 channels, may be

@@ -116,6 +116,14 @@ AO_HD void trig_adjust(uint32_t nib, float& s, float& c) {
     c = bits_step(c, dc);
     s = bits_step(s, ds);
 }
+// The host-libm correction table: a bitmap of the 2^24 angle indices whose host sin or
+// cos differs from the correctly rounded value (2 MB: it stays in L2), then one nibble an
+// index (8 MB), read only where the bit is set (about 1.3% of the angles).
+constexpr uint32_t kTrigBitmapBytes = 1u << 21;
+AO_HD void trig_fix(const uint8_t* t, uint32_t k, float& s, float& c) {
+    const uint32_t word = reinterpret_cast<const uint32_t*>(t)[k >> 5];
+    if ((word >> (k & 31u)) & 1u) trig_adjust((t[kTrigBitmapBytes + (k >> 1)] >> ((k & 1u) * 4)) & 15u, s, c);
+}
 
 // ---- watertight ray/triangle test (raw/math/primitives.hpp:17-64) ------------------
 // The part that depends only on the ray direction is computed once per ray.

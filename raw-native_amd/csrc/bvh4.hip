@@ -18,6 +18,7 @@
 // lanes wait on leaves or none can step, so node and triangle code are not interleaved
 // lane by lane.
 #include "ao_math.hpp"
+#include "batch.hpp"
 #include "ctx.hpp"
 #include <algorithm>
 #include <cfloat>
@@ -43,10 +44,6 @@ __device__ __forceinline__ float pad_of(float l, float h) {
 __device__ __forceinline__ float box_area(const float* b) {
     const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
     return 2.0f * (dy * (dx + dz) + dz * dx);
-}
-__device__ __forceinline__ unsigned long long lanemask_lt() {
-    const unsigned lane = __lane_id();
-    return lane ? (~0ull >> (64 - lane)) : 0ull;
 }
 
 // Up to 4 slots (binary node ids) of the wide node rooted at binary inner node f.
@@ -282,7 +279,8 @@ __device__ __forceinline__ bool tri_hit(const Ray& r, const float4* __restrict__
 // would overflow (the caller answers that ray by the scan).
 template <bool COUNT>
 __device__ int occluded4(const Ray& r, float tmax, const Node4* __restrict__ nodes, const float4* __restrict__ ts,
-                         uint32_t ntri, uint32_t* lds_col, uint32_t tri_q, uint32_t& visits, uint32_t& tests) {
+                         uint32_t ntri, uint32_t* lds_col, uint32_t tri_q, uint32_t& visits, uint32_t& tests,
+                         bool active = true) {
     const float reach = tmax * kReach;
     const uint32_t ox = r.inv.x >= 0.0f ? 0u : 48u, oy = r.inv.y >= 0.0f ? 0u : 48u, oz = r.inv.z >= 0.0f ? 0u : 48u;
     const char* const base = reinterpret_cast<const char*>(nodes);
@@ -292,7 +290,7 @@ __device__ int occluded4(const Ray& r, float tmax, const Node4* __restrict__ nod
     uint32_t pend = 0;                // pending leaf slots of `leafref`
     int4 leafref = make_int4(0, 0, 0, 0);
     int result = 0;
-    bool live = true;
+    bool live = active;   // an inactive lane takes part in the wave's votes only
     // A lane's node steps are bounded by the tree's record count; more means a malformed
     // tree, and the lane stops with -1 (answered by the scan). Every iteration some live
     // lane steps or tests, so the loop ends.
@@ -390,7 +388,7 @@ __global__ __launch_bounds__(BT) void k_trace_ao4(TraceArgs4 a) {
     const float u1 = float(k1) / float(0x1000000), u2 = float(k2) / float(0x1000000);
     float sphi, cphi;
     sincos_base(6.2831853f * u2, sphi, cphi);
-    if (a.trig) trig_adjust((a.trig[k2 >> 1] >> ((k2 & 1u) * 4)) & 15u, sphi, cphi);
+    if (a.trig) trig_fix(a.trig, k2, sphi, cphi);
     const V3 dir = ao_dir(u1, sphi, cphi, n, t, b);
     bool occ = false;
     uint32_t visits = 0, tests = 0;
@@ -486,6 +484,110 @@ __global__ __launch_bounds__(BT) void k_trace_rays4(const float* __restrict__ ra
 
 uint32_t cdiv(uint64_t a, uint32_t b) { return uint32_t((a + b - 1) / b); }
 
+
+// ---- batched AO: one persistent launch over many views ------------------------------------
+__global__ __launch_bounds__(BT) void k_batch_clear(BatchDev d) {
+    const uint32_t stride = gridDim.x * BT;
+    for (uint32_t i = blockIdx.x * BT + threadIdx.x; i < d.total_pix; i += stride) d.cnt[i] = 0;
+    if (blockIdx.x == 0) {
+        for (uint32_t v = threadIdx.x; v < d.nviews; v += BT) d.ncov[v] = 0;
+        if (threadIdx.x == 0) *d.head = 0;
+    }
+}
+// Covered pixels of view blockIdx.y, listed at the view's pix_base: one atomic a wave, so
+// a wave's pixels stay contiguous and in order.
+__global__ __launch_bounds__(BT) void k_batch_compact(BatchDev d) {
+    const uint32_t v = blockIdx.y;
+    const BatchView& bv = d.views[v];
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t i0 = blockIdx.x * BT; i0 < bv.npix; i0 += gridDim.x * BT) {
+        const uint32_t i = i0 + threadIdx.x;
+        const bool cov = i < bv.npix && bv.mask[i] != 0;
+        const unsigned long long b = __ballot(cov);
+        if (b == 0) continue;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&d.ncov[v], uint32_t(__popcll(b)));
+        base = __shfl(base, 0);
+        if (cov) d.cov[bv.pix_base + base + uint32_t(__popcll(b & ((1ull << lane) - 1ull)))] = i;
+    }
+}
+// First 64-ray group of each view (its rays padded to whole groups), and the total.
+__global__ void k_batch_plan(BatchDev d) {
+    if (threadIdx.x != 0) return;
+    uint64_t g = 0;
+    for (uint32_t v = 0; v < d.nviews; ++v) {
+        d.gbase[v] = uint32_t(g);
+        g += (uint64_t(d.ncov[v]) * d.views[v].spp + 63u) / 64u;
+    }
+    d.gbase[d.nviews] = uint32_t(g);
+}
+// Persistent: each wave takes `chunk` groups from the queue until it is empty. A group is
+// 64 consecutive rays of one view (pixel-major, the samples of a pixel adjacent), one ray
+// a lane; a pixel's open rays are summed in the wave and added once.
+__global__ __launch_bounds__(BT) void k_batch_trace(BatchDev d) {
+    __shared__ uint32_t stk[STACK_LDS * BT];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t total = d.gbase[d.nviews];
+    for (;;) {
+        uint32_t g0 = 0;
+        if (lane == 0) g0 = atomicAdd(d.head, d.chunk);
+        g0 = __shfl(g0, 0);
+        if (g0 >= total) break;
+        const uint32_t g1 = min(total, g0 + d.chunk);
+        for (uint32_t g = g0; g < g1; ++g) {
+            uint32_t lo = 0, hi = d.nviews;   // the view: the last v with gbase[v] <= g
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (d.gbase[mid] <= g) lo = mid;
+                else hi = mid;
+            }
+            const BatchView& bv = d.views[lo];
+            const BatchScene& sc = d.scenes[bv.scene];
+            const uint64_t rr = uint64_t(g - d.gbase[lo]) * 64u + lane;
+            const bool active = rr < uint64_t(d.ncov[lo]) * bv.spp;
+            const uint32_t ci = active ? uint32_t(rr / bv.spp) : 0u;
+            const uint32_t s = active ? uint32_t(rr - uint64_t(ci) * bv.spp) : 0u;
+            const uint32_t pix = active ? d.cov[bv.pix_base + ci] : 0u;
+            bool occ = false;
+            if (sc.ntri) {
+                const V3 p = v3(bv.pos[3 * size_t(pix)], bv.pos[3 * size_t(pix) + 1], bv.pos[3 * size_t(pix) + 2]);
+                const V3 n = v3(bv.nrm[3 * size_t(pix)], bv.nrm[3 * size_t(pix) + 1], bv.nrm[3 * size_t(pix) + 2]);
+                const uint32_t y = pix / bv.w, x = pix - y * bv.w;
+                V3 t, b;
+                basis(n, t, b);
+                const V3 origin = add(p, scl(n, 1e-3f));
+                const uint32_t k1 = hash24(x, y, 2 * s), k2 = hash24(x, y, 2 * s + 1);
+                const float u1 = float(k1) / float(0x1000000), u2 = float(k2) / float(0x1000000);
+                float sphi, cphi;
+                sincos_base(6.2831853f * u2, sphi, cphi);
+                if (d.trig) trig_fix(d.trig, k2, sphi, cphi);
+                const V3 dir = ao_dir(u1, sphi, cphi, n, t, b);
+                const Ray r = make_ray(origin, dir);
+                uint32_t visits = 0, tests = 0;
+                const int res = occluded4<false>(r, bv.radius, static_cast<const Node4*>(sc.nodes), sc.tris, sc.ntri,
+                                                 stk + threadIdx.x, d.tri_q, visits, tests, active);
+                occ = res < 0 ? scan_occluded(r, bv.radius, sc.tris, sc.ntri) : res == 1;
+            }
+            // segments of equal ci: a lane starts one where ci changes
+            const uint32_t prev = __shfl_up(ci, 1);
+            const bool start = active && (lane == 0 || prev != ci);
+            const unsigned long long sb = __ballot(start), ob = __ballot(active && !occ);
+            if (start) {
+                const unsigned long long after = lane == 63 ? 0ull : (sb >> (lane + 1)) << (lane + 1);
+                const uint32_t end = after ? uint32_t(__builtin_ctzll(after)) : 64u;
+                const unsigned long long seg = (end == 64 ? ~0ull : ((1ull << end) - 1ull)) & ~((1ull << lane) - 1ull);
+                const uint32_t open = uint32_t(__popcll(ob & seg));
+                if (open) atomicAdd(&d.cnt[bv.pix_base + pix], open);
+            }
+        }
+    }
+}
+// AO = open / spp where covered, 1 elsewhere (k_ao_finish's formula).
+__global__ __launch_bounds__(BT) void k_batch_finish(BatchDev d) {
+    const BatchView& bv = d.views[blockIdx.y];
+    for (uint32_t i = blockIdx.x * BT + threadIdx.x; i < bv.npix; i += gridDim.x * BT)
+        bv.ao[i] = bv.mask[i] ? float(d.cnt[bv.pix_base + i]) / float(bv.spp) : 1.0f;
+}
 }  // namespace
 
 int wide_build(Ctx& c, hipStream_t s) {
@@ -510,8 +612,10 @@ int wide_build(Ctx& c, hipStream_t s) {
     uint32_t *fin = c.wfront_a.as<uint32_t>(), *fout = c.wfront_b.as<uint32_t>();
     uint64_t ub = 1;   // upper bound of this level's size (grids are sized by it)
     int parity = 0;
+    c.plan_wide_nb.clear();
     for (int level = 0;; ++level) {
         const uint32_t nb = cdiv(ub, BT);
+        c.plan_wide_nb.push_back(nb);
         k_wide_count<<<nb, BT, 0, s>>>(d_st + parity, fin, c.boxes.as<float>(), c.links.as<int32_t>(), n, c.wblk.as<uint32_t>());
         k_wide_scan<<<1, 1024, 0, s>>>(d_st + parity, d_st + (parity ^ 1), c.wblk.as<uint32_t>(), nb);
         k_wide_write<<<nb, BT, 0, s>>>(d_st + parity, fin, fout, c.wblk.as<uint32_t>(), c.boxes.as<float>(),
@@ -528,6 +632,38 @@ int wide_build(Ctx& c, hipStream_t s) {
             ub = h.count;
         }
         if (level > 4 * 64) { c.err = "wide collapse did not finish"; return -2; }
+    }
+    return hip_ok(c, hipGetLastError(), "wide build") ? 0 : -2;
+}
+
+namespace {
+__global__ void k_wide_init(WideState* st, uint32_t* frontier, uint32_t root) {
+    st[0] = WideState{1, 0, 1, 0};
+    frontier[0] = root;
+}
+}  // namespace
+
+// The collapse with the level grids of the last eager wide_build of this tree: no host
+// copies or waits, so a stream capture can record it.
+int wide_build_fixed(Ctx& c, hipStream_t s) {
+    const uint32_t n = c.ntri;
+    if (n == 0) return 0;
+    if (n == 1) {
+        k_wide_single<<<1, 64, 0, s>>>(c.boxes.as<float>(), c.nodes4.as<Node4>());
+        return hip_ok(c, hipGetLastError(), "wide launch") ? 0 : -2;
+    }
+    if (c.plan_wide_nb.empty()) { c.err = "no recorded wide collapse plan"; return -1; }
+    WideState* d_st = c.wstate.as<WideState>();
+    uint32_t *fin = c.wfront_a.as<uint32_t>(), *fout = c.wfront_b.as<uint32_t>();
+    k_wide_init<<<1, 1, 0, s>>>(d_st, fin, uint32_t(c.root));
+    int parity = 0;
+    for (const uint32_t nb : c.plan_wide_nb) {
+        k_wide_count<<<nb, BT, 0, s>>>(d_st + parity, fin, c.boxes.as<float>(), c.links.as<int32_t>(), n, c.wblk.as<uint32_t>());
+        k_wide_scan<<<1, 1024, 0, s>>>(d_st + parity, d_st + (parity ^ 1), c.wblk.as<uint32_t>(), nb);
+        k_wide_write<<<nb, BT, 0, s>>>(d_st + parity, fin, fout, c.wblk.as<uint32_t>(), c.boxes.as<float>(),
+                                       c.links.as<int32_t>(), n, c.nodes4.as<Node4>());
+        parity ^= 1;
+        std::swap(fin, fout);
     }
     return hip_ok(c, hipGetLastError(), "wide build") ? 0 : -2;
 }
@@ -568,6 +704,15 @@ int trace_rays4(Ctx& c, const float* d_rays, uint32_t n, int any, float far_scal
     k_trace_rays4<<<cdiv(n, BT), BT, 0, s>>>(d_rays, n, any, far_scale, c.nodes4.as<Node4>(), c.tris_sorted.as<float4>(),
                                               c.ntri, d_out);
     return hip_ok(c, hipGetLastError(), "trace_rays launch") ? 0 : -2;
+}
+
+int batch_trace_enqueue(const BatchDev& d, uint32_t persistent_blocks, hipStream_t s) {
+    k_batch_clear<<<1024, BT, 0, s>>>(d);
+    k_batch_compact<<<dim3(256, d.nviews, 1), BT, 0, s>>>(d);
+    k_batch_plan<<<1, 64, 0, s>>>(d);
+    k_batch_trace<<<persistent_blocks, BT, 0, s>>>(d);
+    k_batch_finish<<<dim3(256, d.nviews, 1), BT, 0, s>>>(d);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 }  // namespace raw_ao

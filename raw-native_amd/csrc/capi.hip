@@ -120,6 +120,11 @@ int raw_ao_set_option(raw_ao_ctx* x, int option, int value) {
         c.trace_kernel = value;
         return RAW_AO_OK;
     }
+    if (option == RAW_AO_OPT_CONV_KERNEL) {
+        if (value != 0 && value != 1) return fail(c, RAW_AO_ERR_ARG, "convolution kernel must be 0 or 1");
+        c.conv_kernel = value;
+        return RAW_AO_OK;
+    }
     if (option == RAW_AO_OPT_TRI_PHASE) {
         if (value < 1 || value > 64) return fail(c, RAW_AO_ERR_ARG, "triangle phase threshold must be in 1..64");
         c.tri_q = value;

@@ -43,7 +43,7 @@ struct Ctx {
     int tri_q{16};               // wide kernel: triangle phase when tri_q/64 of live lanes wait
     bool count_next{false};
 
-    // host-libm sin/cos correction table, one nibble per 24-bit angle index
+    // host-libm sin/cos correction table: a 2 MB bitmap of the corrected angle indices, then one nibble an index
     uint8_t* d_trig{nullptr};
     bool trig_ready{false};
 
@@ -78,6 +78,13 @@ struct Ctx {
     float ms_build{0}, ms_trace{0}, ms_ssao{0}, ms_error{0};
 
     UNet* net{nullptr};          // the learned AO surrogate, made on first use
+    int conv_kernel{0};          // 0: tap-major LDS kernel where it applies, 1: the general one only
+
+    // grids of the last eager build, replayed by the capturable build (ploc_build_fixed,
+    // wide_build_fixed)
+    std::vector<uint32_t> plan_ploc_nb, plan_wide_nb;
+    uint32_t plan_n{0};
+    int plan_radius{0};
 };
 
 bool ensure(Ctx& c, DevBuf& b, size_t bytes);
@@ -92,19 +99,25 @@ int trace_ao(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_ma
 int angle_table(Ctx& c, float* d_sin, float* d_cos, hipStream_t s);
 int ao_finish(Ctx& c, const uint8_t* d_mask, const uint32_t* d_cnt, uint32_t npix, uint32_t spp, float* d_ao, hipStream_t s);
 int wide_build(Ctx& c, hipStream_t s);
+int ploc_build_fixed(Ctx& c, const float* d_tris, hipStream_t s);
+int ploc_fixed_check(Ctx& c);
+int wide_build_fixed(Ctx& c, hipStream_t s);
 int trace_ao4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int spp,
               float radius, float* d_ao, hipStream_t s);
 int trace_rays4(Ctx& c, const float* d_rays, uint32_t n, int any, float far_scale, float* d_out, hipStream_t s);
 int trace_rays(Ctx& c, const float* d_rays, uint32_t n, int any, float far_scale, float* d_out, hipStream_t s);
 int ssao(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int samples,
          float radius, float* d_ao, hipStream_t s);
+// k_ssao without event records (stream-capture safe); trig may be null (correctly rounded).
+void ssao_enqueue(const uint8_t* trig, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int samples,
+                  float radius, float* d_ao, hipStream_t s);
 int error_map(Ctx& c, const float* d_a, const float* d_t, const uint8_t* d_mask, int w, int h, float* d_err,
               float* d_max, hipStream_t s);
 
 void net_release(Ctx& c);
 bool conv_shape_ok(int c0, int c1, int cout);
 
-// Host-libm correction table (trig_table.cpp): 2^23 bytes, one nibble per index.
+// Host-libm correction table (trig_table.cpp): a 2^21-byte bitmap, then 2^23 bytes of nibbles.
 bool build_trig_table(std::vector<uint8_t>& out, std::string& err);
 
 }  // namespace raw_ao

@@ -519,9 +519,11 @@ int ploc_build(Ctx& c, const float* d_tris, uint32_t n, int radius, hipStream_t 
     uint32_t m = n;
     int parity = 0;
     uint32_t *cin = c.clus_a.as<uint32_t>(), *cout = c.clus_b.as<uint32_t>();
+    c.plan_ploc_nb.clear();
     while (m > uint32_t(TAIL_M)) {
         const uint32_t nb = cdiv(m, BT);
         for (int r = 0; r < kBatch; ++r) {
+            c.plan_ploc_nb.push_back(nb);
             k_ploc_nn<<<nb, BT, 0, s>>>(d_st + parity, radius, c.boxes.as<float>(), cin, c.nn.as<uint32_t>(),
                                         c.flags.as<uint8_t>(), c.blk.as<uint32_t>(), nb);
             k_ploc_scan<<<1, 1024, 0, s>>>(d_st + parity, d_st + (parity ^ 1), c.blk.as<uint32_t>(), nb);
@@ -544,6 +546,69 @@ int ploc_build(Ctx& c, const float* d_tris, uint32_t n, int radius, hipStream_t 
     if (c.h_state->stalled || c.h_state->m != 1) { c.err = "PLOC round made no merge (degenerate input)"; return -3; }
     c.rounds = c.h_state->rounds;
     c.root = int32_t(c.h_state->root);
+    c.plan_n = n;
+    c.plan_radius = radius;
+    return 0;
+}
+
+namespace {
+__global__ void k_ploc_init(PlocState* st, uint32_t n) {
+    PlocState init{};
+    init.m = n;
+    init.next_id = n;
+    st[0] = init;
+}
+}  // namespace
+
+// The build with the round grids of the last eager ploc_build of the same triangles: no
+// host copies or waits, so a stream capture can record it. PLOC is deterministic, so the
+// same input takes the same rounds; ploc_fixed_check() confirms it after the run.
+int ploc_build_fixed(Ctx& c, const float* d_tris, hipStream_t s) {
+    const uint32_t n = c.ntri;
+    const int radius = c.radius;
+    if (n == 0) return 0;
+    if (c.plan_n != n || c.plan_radius != radius || c.root < 0) { c.err = "no recorded PLOC plan for this scene"; return -1; }
+    const uint32_t nblk_rs = cdiv(n, RS_TILE);
+    const uint32_t nblk = cdiv(n, BT);
+    const uint32_t nparts = std::min<uint32_t>(1024, nblk);
+    k_bounds_part<<<nparts, BT, 0, s>>>(d_tris, n, c.bounds_part.as<float>());
+    k_bounds_final<<<1, BT, 0, s>>>(c.bounds_part.as<float>(), int(nparts), c.g4.as<float>());
+    k_morton<<<nblk, BT, 0, s>>>(d_tris, n, c.g4.as<float>(), c.keys.as<uint32_t>(), c.vals.as<uint32_t>());
+    uint32_t *ki = c.keys.as<uint32_t>(), *vi = c.vals.as<uint32_t>(), *ko = c.keys2.as<uint32_t>(), *vo = c.vals2.as<uint32_t>();
+    for (int shift = 0; shift < 30; shift += 8) {
+        k_radix_hist<<<nblk_rs, BT, 0, s>>>(ki, n, shift, c.hist.as<uint32_t>(), nblk_rs);
+        k_radix_colscan<<<256, BT, 0, s>>>(c.hist.as<uint32_t>(), nblk_rs, c.rs_tot.as<uint32_t>());
+        k_radix_scatter<<<nblk_rs, BT, 0, s>>>(ki, vi, ko, vo, n, shift, c.hist.as<uint32_t>(), nblk_rs, c.rs_tot.as<uint32_t>());
+        std::swap(ki, ko);
+        std::swap(vi, vo);
+    }
+    k_leaves<<<nblk, BT, 0, s>>>(d_tris, vi, n, c.boxes.as<float>(), c.links.as<int32_t>(), c.clus_a.as<uint32_t>());
+    PlocState* d_st = c.state.as<PlocState>();
+    k_ploc_init<<<1, 1, 0, s>>>(d_st, n);
+    int parity = 0;
+    uint32_t *cin = c.clus_a.as<uint32_t>(), *cout = c.clus_b.as<uint32_t>();
+    for (const uint32_t nb : c.plan_ploc_nb) {
+        k_ploc_nn<<<nb, BT, 0, s>>>(d_st + parity, radius, c.boxes.as<float>(), cin, c.nn.as<uint32_t>(), c.flags.as<uint8_t>(),
+                                    c.blk.as<uint32_t>(), nb);
+        k_ploc_scan<<<1, 1024, 0, s>>>(d_st + parity, d_st + (parity ^ 1), c.blk.as<uint32_t>(), nb);
+        k_ploc_merge<<<nb, BT, 0, s>>>(d_st + parity, c.nn.as<uint32_t>(), c.flags.as<uint8_t>(), c.blk.as<uint32_t>(), nb, cin,
+                                       cout, c.boxes.as<float>(), c.links.as<int32_t>());
+        parity ^= 1;
+        std::swap(cin, cout);
+    }
+    k_ploc_tail<<<1, TAIL_T, 0, s>>>(d_st + parity, d_st + 2, radius, cin, c.boxes.as<float>(), c.links.as<int32_t>());
+    return hip_ok(c, hipGetLastError(), "PLOC launch") ? 0 : -2;
+}
+
+// After a ploc_build_fixed has run: the tree is the recorded one (same root and rounds).
+int ploc_fixed_check(Ctx& c) {
+    if (c.ntri == 0) return 0;
+    PlocState st{};
+    if (!hip_ok(c, hipMemcpy(&st, c.state.as<PlocState>() + 2, sizeof st, hipMemcpyDeviceToHost), "state read")) return -2;
+    if (st.stalled || st.m != 1 || int32_t(st.root) != c.root || st.rounds != c.rounds) {
+        c.err = "the replayed PLOC build differs from the recorded one (the scene changed: build it again)";
+        return -3;
+    }
     return 0;
 }
 

@@ -27,7 +27,7 @@ __global__ __launch_bounds__(TX* TY) void k_ssao(const float* __restrict__ pos, 
         const float rad = (0.3f + 0.7f * hash01(uint32_t(x), uint32_t(y), uint32_t(s + 97))) * float(R);
         float sa, ca;
         sincos_base(a, sa, ca);
-        if (trig) trig_adjust((trig[ka >> 1] >> ((ka & 1u) * 4)) & 15u, sa, ca);
+        if (trig) trig_fix(trig, ka, sa, ca);
         const int sx = x + lround_f(ca * rad);
         const int sy = y + lround_f(sa * rad);
         if (sx < 0 || sy < 0 || sx >= w || sy >= h) continue;
@@ -63,11 +63,16 @@ __global__ __launch_bounds__(256) void k_error_map(const float* __restrict__ a, 
 
 }  // namespace
 
+void ssao_enqueue(const uint8_t* trig, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int samples,
+                  float radius, float* d_ao, hipStream_t s) {
+    const dim3 grid((w + TX - 1) / TX, (h + TY - 1) / TY), block(TX, TY);
+    k_ssao<<<grid, block, 0, s>>>(d_pos, d_nrm, d_mask, trig, w, h, samples, radius, d_ao);
+}
+
 int ssao(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int samples, float radius,
          float* d_ao, hipStream_t s) {
-    const dim3 grid((w + TX - 1) / TX, (h + TY - 1) / TY), block(TX, TY);
     (void)hipEventRecord(c.ev[4], s);
-    k_ssao<<<grid, block, 0, s>>>(d_pos, d_nrm, d_mask, c.host_trig ? c.d_trig : nullptr, w, h, samples, radius, d_ao);
+    ssao_enqueue(c.host_trig ? c.d_trig : nullptr, d_pos, d_nrm, d_mask, w, h, samples, radius, d_ao, s);
     (void)hipEventRecord(c.ev[5], s);
     return hip_ok(c, hipGetLastError(), "ssao launch") ? 0 : -2;
 }

@@ -184,7 +184,7 @@ __global__ __launch_bounds__(BT) void k_trace_ao(TraceArgs a) {
     const float u1 = float(k1) / float(0x1000000), u2 = float(k2) / float(0x1000000);
     float sphi, cphi;
     sincos_base(6.2831853f * u2, sphi, cphi);
-    if (a.trig) trig_adjust((a.trig[k2 >> 1] >> ((k2 & 1u) * 4)) & 15u, sphi, cphi);
+    if (a.trig) trig_fix(a.trig, k2, sphi, cphi);
     const V3 dir = ao_dir(u1, sphi, cphi, n, t, b);
     bool occ = false;
     uint32_t visits = 0, tests = 0;
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(BT) void k_angles(const uint8_t* __restrict__ trig,
     if (k >= (1u << 24)) return;
     float s, c;
     sincos_base(6.2831853f * (float(k) / float(0x1000000)), s, c);
-    if (trig) trig_adjust((trig[k >> 1] >> ((k & 1u) * 4)) & 15u, s, c);
+    if (trig) trig_fix(trig, k, s, c);
     so[k] = s;
     co[k] = c;
 }

@@ -22,8 +22,10 @@
 #include "unet.hpp"
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <new>
+#include <vector>
 
 namespace raw_ao {
 namespace {
@@ -104,6 +106,87 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
         }
     }
     for (int t = 0; t < 2; ++t) {
+        if (!pvalid[t]) continue;
+        const size_t p = size_t(pbase + t * 16 + r);
+        for (int i = 0; i < CT; ++i) {
+            const int co = a.co0 + i * 16 + 4 * g;
+            float v[4];
+            for (int j = 0; j < 4; ++j) {
+                v[j] = acc[i][t][j] + (a.bias ? a.bias[co + j] : 0.0f);
+                if (a.relu && v[j] < 0.0f) v[j] = 0.0f;
+            }
+            const uint32_t lo = uint32_t(f2bf(v[0])) | (uint32_t(f2bf(v[1])) << 16);
+            const uint32_t hi = uint32_t(f2bf(v[2])) | (uint32_t(f2bf(v[3])) << 16);
+            *reinterpret_cast<uint2*>(a.out + p * a.cout + co) = make_uint2(lo, hi);
+        }
+    }
+}
+
+// The same convolution for cin a multiple of 32 (every layer but the first): a 32-wide k
+// step then lies inside one tap, so the tap's pixel offset is wave-uniform and kept
+// incrementally (no per-lane divisions), and the 4 waves of a block share the weights
+// through LDS, staged KS columns at a time. A wave computes PT tiles of 16 pixels. The
+// MFMA sums run over k in the same order as k_conv, so the outputs are identical.
+template <int CT, int PT>
+__global__ __launch_bounds__(256) void k_conv_t(ConvArgs a) {
+    constexpr int KS = 128;
+    constexpr int RW = KS / 8 + 1;   // 16-byte units a staged weight row (+1 against bank conflicts)
+    __shared__ uint4 wl[CT * 16 * RW];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int g = lane >> 4, r = lane & 15;
+    const int npix = a.n * a.h * a.wd;
+    const int pbase = (int(blockIdx.x) * 4 + wv) * (16 * PT);
+    const int cin = a.c0 + a.c1;
+    const int hh = a.h >> 1, hw = a.wd >> 1;
+    int py[PT], px[PT], pn[PT];
+    bool pvalid[PT];
+    for (int t = 0; t < PT; ++t) {
+        const int p = pbase + t * 16 + r;
+        pvalid[t] = p < npix;
+        const int pp = pvalid[t] ? p : 0;
+        px[t] = pp % a.wd;
+        const int q = pp / a.wd;
+        py[t] = q % a.h;
+        pn[t] = q / a.h;
+    }
+    f32x4 acc[CT][PT];
+    for (int i = 0; i < CT; ++i)
+        for (int t = 0; t < PT; ++t) acc[i][t] = f32x4{0, 0, 0, 0};
+    int tap = 0, cb = 0;   // the tap and first channel of the current 32-wide k step
+    for (int ks = 0; ks < a.kpad; ks += KS) {
+        const int kn = min(KS, a.kpad - ks);
+        __syncthreads();
+        for (int e = threadIdx.x; e < CT * 16 * (kn / 8); e += 256) {
+            const int co = e / (kn / 8), kk = e - co * (kn / 8);
+            wl[co * RW + kk] = *reinterpret_cast<const uint4*>(a.w + size_t(a.co0 + co) * a.kpad + ks + kk * 8);
+        }
+        __syncthreads();
+        for (int kc = 0; kc < kn; kc += 32) {
+            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+            const int ci = cb + 8 * g;
+            const bool lo = cb < a.c0;   // wave-uniform: c0 is a multiple of 32
+            V8 bv[PT];
+            for (int t = 0; t < PT; ++t) {
+                bv[t].u = make_uint4(0, 0, 0, 0);
+                const int y = py[t] + dy, x = px[t] + dx;
+                if (!pvalid[t] || y < 0 || y >= a.h || x < 0 || x >= a.wd) continue;
+                const bf16_t* src = lo ? a.in0 + (size_t(pn[t] * a.h + y) * a.wd + x) * a.c0 + ci
+                                       : a.in1 + (size_t(pn[t] * hh + (y >> 1)) * hw + (x >> 1)) * a.c1 + (ci - a.c0);
+                bv[t].u = *reinterpret_cast<const uint4*>(src);
+            }
+            for (int i = 0; i < CT; ++i) {
+                V8 av;
+                av.u = wl[(i * 16 + r) * RW + (kc >> 3) + g];
+                for (int t = 0; t < PT; ++t) acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av.v, bv[t].v, acc[i][t], 0, 0, 0);
+            }
+            cb += 32;
+            if (cb == cin) {
+                cb = 0;
+                ++tap;
+            }
+        }
+    }
+    for (int t = 0; t < PT; ++t) {
         if (!pvalid[t]) continue;
         const size_t p = size_t(pbase + t * 16 + r);
         for (int i = 0; i < CT; ++i) {
@@ -407,6 +490,12 @@ __global__ void k_pack(const float* __restrict__ m, int cout, int cin, int kpad,
 
 unsigned cdivu(size_t a, unsigned b) { return unsigned((a + b - 1) / b); }
 
+constexpr int kConvPT = 4;
+template <int CT> void launch_conv_t(const ConvArgs& a, hipStream_t s) {
+    const size_t npix = size_t(a.n) * a.h * a.wd;
+    const size_t waves = (npix + 16 * kConvPT - 1) / (16 * kConvPT);
+    k_conv_t<CT, kConvPT><<<cdivu(waves, 4), 256, 0, s>>>(a);
+}
 template <int CT> void launch_conv(const ConvArgs& a, hipStream_t s) {
     const size_t npix = size_t(a.n) * a.h * a.wd;
     const size_t waves = (npix + 31) / 32;
@@ -429,8 +518,22 @@ bool conv_shape_ok(int c0, int c1, int cout) {
 bool conv3x3(Ctx& c, const bf16_t* in0, const bf16_t* in1, int c0, int c1, const bf16_t* w, int kpad, const float* bias,
              int cout, int relu, int n, int h, int wd, bf16_t* out, hipStream_t s) {
     ConvArgs a{in0, in1, w, bias, out, n, h, wd, c0, c1, cout, kpad, relu, 0};
+    const bool tapwise = c.conv_kernel != 1 && (c0 + c1) % 32 == 0 && c0 % 32 == 0 && kpad == 9 * (c0 + c1);
     for (int co0 = 0; co0 < cout; co0 += 128) {
         a.co0 = co0;
+        if (tapwise) {
+            switch (std::min(8, (cout - co0) / 16)) {
+                case 1: launch_conv_t<1>(a, s); break;
+                case 2: launch_conv_t<2>(a, s); break;
+                case 3: launch_conv_t<3>(a, s); break;
+                case 4: launch_conv_t<4>(a, s); break;
+                case 5: launch_conv_t<5>(a, s); break;
+                case 6: launch_conv_t<6>(a, s); break;
+                case 7: launch_conv_t<7>(a, s); break;
+                default: launch_conv_t<8>(a, s); break;
+            }
+            continue;
+        }
         switch (std::min(8, (cout - co0) / 16)) {
             case 1: launch_conv<1>(a, s); break;
             case 2: launch_conv<2>(a, s); break;
@@ -662,12 +765,136 @@ void UNet::release() {
         b.p = nullptr;
         b.cap = 0;
     };
-    DevBuf* bufs[] = {&params, &grads, &adam_m, &adam_v, &w16, &wt16, &pred, &headpart, &lossbuf, &cnt, &wpart};
+    DevBuf* bufs[] = {&params, &grads, &adam_m, &adam_v, &w16, &wt16, &pred, &headpart, &lossbuf, &cnt, &wpart,
+                      &f_ssao, &f_x, &pr_pos, &pr_nrm, &pr_mask, &pr_rt, &pr_net, &pr_stats};
     for (auto* b : bufs) drop(*b);
+    if (h_stats) (void)hipHostFree(h_stats);
+    h_stats = nullptr;
     for (auto& lv : lvbuf)
         for (auto& b : lv) drop(b);
     for (auto& b : act) drop(b);
     for (auto& b : pooled) drop(b);
+}
+
+namespace {
+// The probe sub-image: every stride-th pixel in x and y (from stride/2), its G-buffer and
+// the network's AO there.
+__global__ void k_probe_gather(const float* __restrict__ pos, const float* __restrict__ nrm, const uint8_t* __restrict__ mask,
+                               const float* __restrict__ net, int w, int stride, int pw, int ph, float* __restrict__ ppos,
+                               float* __restrict__ pnrm, uint8_t* __restrict__ pmask, float* __restrict__ pnet) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pw * ph) return;
+    const int px = i % pw, py = i / pw;
+    const size_t src = size_t(py * stride + stride / 2) * w + size_t(px * stride + stride / 2);
+    for (int k = 0; k < 3; ++k) {
+        ppos[3 * i + k] = pos[3 * src + k];
+        pnrm[3 * i + k] = nrm[3 * src + k];
+    }
+    pmask[i] = mask[src];
+    pnet[i] = net[src];
+}
+// One block: the sum of squared differences (float64), the largest difference and the
+// covered count over the probe pixels, in a fixed order.
+__global__ __launch_bounds__(1024) void k_probe_stats(const float* __restrict__ rt, const float* __restrict__ net,
+                                                      const uint8_t* __restrict__ mask, int n, double* __restrict__ out) {
+    __shared__ double ss[1024];
+    __shared__ float mx[1024];
+    __shared__ unsigned cn[1024];
+    double s = 0.0;
+    float m = 0.0f;
+    unsigned c = 0;
+    for (int i = threadIdx.x; i < n; i += 1024)
+        if (mask[i]) {
+            const float d = fabsf(net[i] - rt[i]);
+            s += double(d) * double(d);
+            m = fmaxf(m, d);
+            ++c;
+        }
+    ss[threadIdx.x] = s;
+    mx[threadIdx.x] = m;
+    cn[threadIdx.x] = c;
+    __syncthreads();
+    for (int k = 512; k > 0; k >>= 1) {
+        if (int(threadIdx.x) < k) {
+            ss[threadIdx.x] += ss[threadIdx.x + k];
+            mx[threadIdx.x] = fmaxf(mx[threadIdx.x], mx[threadIdx.x + k]);
+            cn[threadIdx.x] += cn[threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = ss[0];
+        out[1] = double(mx[0]);
+        out[2] = double(cn[0]);
+    }
+}
+}  // namespace
+
+int trace_built(Ctx& c, const float* pos, const float* nrm, const uint8_t* mask, int w, int h, int spp, float radius, float* ao,
+                hipStream_t s) {
+    if (c.root < 0) {
+        c.err = "no BVH: call raw_ao_build_ploc first";
+        return -1;
+    }
+    if (c.trace_kernel != c.built_kernel) {
+        c.err = "the trace kernel option changed after the build: build again";
+        return -1;
+    }
+    if (c.host_trig) {
+        const int r = trig_table_upload(c);
+        if (r) return r;
+    }
+    return c.built_kernel == RAW_AO_TRACE_BINARY ? trace_ao(c, pos, nrm, mask, w, h, spp, radius, ao, s)
+                                                 : trace_ao4(c, pos, nrm, mask, w, h, spp, radius, ao, s);
+}
+
+int learned_frame(Ctx& c, UNet& u, const LearnedArgs& a, raw_ao_learned_report* rep, hipStream_t s) {
+    const size_t np = size_t(a.w) * a.h;
+    if (!ensure(c, u.f_ssao, 4 * np) || !ensure(c, u.f_x, 2 * kNetFeatures * np)) return -2;
+    if (c.host_trig) {
+        const int r = trig_table_upload(c);
+        if (r) return r;
+    }
+    int r = ssao(c, a.pos, a.nrm, a.mask, a.w, a.h, a.ssao_samples, a.ssao_radius, u.f_ssao.as<float>(), s);
+    if (r) return r;
+    if (!u.features(a.pos, a.nrm, a.mask, u.f_ssao.as<float>(), a.noisy, a.eye, a.h, a.w, u.f_x.as<bf16_t>(), s) ||
+        !u.forward(u.f_x.as<bf16_t>(), a.mask, 1, a.h, a.w, a.ao, s))
+        return -2;
+    raw_ao_learned_report out{};
+    if (a.probe_stride > 0) {
+        const int st = a.probe_stride, pw = a.w / st, ph = a.h / st;
+        const size_t pn = size_t(pw) * ph;
+        if (pn == 0) {
+            c.err = "the probe stride exceeds the image";
+            return -1;
+        }
+        if (!ensure(c, u.pr_pos, 12 * pn) || !ensure(c, u.pr_nrm, 12 * pn) || !ensure(c, u.pr_mask, pn) ||
+            !ensure(c, u.pr_rt, 4 * pn) || !ensure(c, u.pr_net, 4 * pn) || !ensure(c, u.pr_stats, 32))
+            return -2;
+        if (!u.h_stats && !hip_ok(c, hipHostMalloc(reinterpret_cast<void**>(&u.h_stats), 32), "pinned stats")) return -2;
+        k_probe_gather<<<cdivu(pn, 256), 256, 0, s>>>(a.pos, a.nrm, a.mask, a.ao, a.w, st, pw, ph, u.pr_pos.as<float>(),
+                                                      u.pr_nrm.as<float>(), u.pr_mask.as<uint8_t>(), u.pr_net.as<float>());
+        r = trace_built(c, u.pr_pos.as<float>(), u.pr_nrm.as<float>(), u.pr_mask.as<uint8_t>(), pw, ph, a.rt_spp, a.rt_radius,
+                        u.pr_rt.as<float>(), s);
+        if (r) return r;
+        k_probe_stats<<<1, 1024, 0, s>>>(u.pr_rt.as<float>(), u.pr_net.as<float>(), u.pr_mask.as<uint8_t>(), int(pn),
+                                         u.pr_stats.as<double>());
+        if (!hip_ok(c, hipMemcpyAsync(u.h_stats, u.pr_stats.p, 24, hipMemcpyDeviceToHost, s), "probe stats") ||
+            !hip_ok(c, hipStreamSynchronize(s), "probe"))
+            return -2;
+        const double* hs = reinterpret_cast<const double*>(u.h_stats);
+        out.probe_pixels = uint32_t(hs[2]);
+        out.probe_rmse = out.probe_pixels ? float(std::sqrt(hs[0] / hs[2])) : 0.0f;
+        out.probe_max_error = float(hs[1]);
+        if (out.probe_rmse > a.fallback_rmse) {
+            r = trace_built(c, a.pos, a.nrm, a.mask, a.w, a.h, a.rt_spp, a.rt_radius, a.ao, s);
+            if (r) return r;
+            out.fell_back = 1;
+        }
+    }
+    if (!hip_ok(c, hipGetLastError(), "learned frame")) return -2;
+    if (rep) *rep = out;
+    return 0;
 }
 
 void net_release(Ctx& c) {
@@ -846,6 +1073,74 @@ int raw_ao_conv3x3_wgrad_dev(raw_ao_ctx* x, const uint16_t* d_dy, const uint16_t
     return raw_ao::conv3x3_wgrad(c, u->wpart, d_dy, d_in0, d_in1, c0, c1, cout, n, h, w, d_gw, d_gb, npick(c, stream))
                ? RAW_AO_OK
                : RAW_AO_ERR_DEVICE;
+}
+
+int raw_ao_net_load(raw_ao_ctx* x, const char* path) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    c.err.clear();
+    if (!path) return nfail(c, RAW_AO_ERR_ARG, "null path");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return nfail(c, RAW_AO_ERR_ARG, "cannot open the weights file");
+    char magic[4] = {0, 0, 0, 0};
+    uint32_t version = 0, meta_len = 0;
+    uint64_t count = 0;
+    bool ok = std::fread(magic, 1, 4, f) == 4 && std::fread(&version, 4, 1, f) == 1 && std::fread(&count, 8, 1, f) == 1 &&
+              std::fread(&meta_len, 4, 1, f) == 1 && std::memcmp(magic, "RAOW", 4) == 0 && version == 1 &&
+              count > 0 && count < (uint64_t(1) << 28);
+    std::vector<float> p;
+    if (ok) {
+        p.resize(size_t(count));
+        ok = std::fread(p.data(), 4, p.size(), f) == p.size();
+    }
+    std::fclose(f);
+    if (!ok) return nfail(c, RAW_AO_ERR_ARG, "not a version-1 RAOW weights file");
+    return raw_ao_net_set_params(x, p.data(), count);
+}
+
+int raw_ao_learned_ao_dev(raw_ao_ctx* x, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, const float* d_noisy,
+                          const float eye[3], int w, int h, int ssao_samples, float ssao_radius, int rt_spp, float rt_radius,
+                          int probe_stride, float fallback_rmse, float* d_ao, raw_ao_learned_report* report, void* stream) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!d_pos || !d_nrm || !d_mask || !eye || !d_ao || !net_shape_ok(1, w, h) || ssao_samples < 0 || ssao_samples > 4096 ||
+        probe_stride < 0 || (probe_stride > 0 && (rt_spp < 1 || rt_spp > 4096 || !(rt_radius > 0.0f))))
+        return nfail(c, RAW_AO_ERR_ARG, "invalid learned AO arguments (w, h multiples of 8)");
+    if (!nbegin(c)) return RAW_AO_ERR_DEVICE;
+    UNet* u = net_of(c);
+    if (!u) return nfail(c, RAW_AO_ERR_DEVICE, "network allocation failed");
+    const raw_ao::LearnedArgs a{d_pos, d_nrm, d_mask, d_noisy, eye, w, h, ssao_samples, ssao_radius, rt_spp, rt_radius,
+                                probe_stride, fallback_rmse, d_ao};
+    const int r = raw_ao::learned_frame(c, *u, a, report, npick(c, stream));
+    if (r == -1) return RAW_AO_ERR_ARG;
+    return r ? RAW_AO_ERR_DEVICE : RAW_AO_OK;
+}
+
+int raw_ao_learned_ao(raw_ao_ctx* x, const float* pos, const float* nrm, const uint8_t* mask, const float* noisy,
+                      const float eye[3], int w, int h, int ssao_samples, float ssao_radius, int rt_spp, float rt_radius,
+                      int probe_stride, float fallback_rmse, float* ao, raw_ao_learned_report* report) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!pos || !nrm || !mask || !ao || !net_shape_ok(1, w, h)) return nfail(c, RAW_AO_ERR_ARG, "bad image arguments");
+    if (!nbegin(c)) return RAW_AO_ERR_DEVICE;
+    const size_t np = size_t(w) * size_t(h);
+    if (!raw_ao::ensure(c, c.st_pos, 12 * np) || !raw_ao::ensure(c, c.st_nrm, 12 * np) || !raw_ao::ensure(c, c.st_mask, np) ||
+        !raw_ao::ensure(c, c.st_out, 4 * np) || (noisy && !raw_ao::ensure(c, c.st_out2, 4 * np)))
+        return RAW_AO_ERR_DEVICE;
+    hipStream_t s = c.stream;
+    if (!raw_ao::hip_ok(c, hipMemcpyAsync(c.st_pos.p, pos, 12 * np, hipMemcpyHostToDevice, s), "upload") ||
+        !raw_ao::hip_ok(c, hipMemcpyAsync(c.st_nrm.p, nrm, 12 * np, hipMemcpyHostToDevice, s), "upload") ||
+        !raw_ao::hip_ok(c, hipMemcpyAsync(c.st_mask.p, mask, np, hipMemcpyHostToDevice, s), "upload") ||
+        (noisy && !raw_ao::hip_ok(c, hipMemcpyAsync(c.st_out2.p, noisy, 4 * np, hipMemcpyHostToDevice, s), "upload")))
+        return RAW_AO_ERR_DEVICE;
+    const int r = raw_ao_learned_ao_dev(x, c.st_pos.as<float>(), c.st_nrm.as<float>(), c.st_mask.as<uint8_t>(),
+                                        noisy ? c.st_out2.as<float>() : nullptr, eye, w, h, ssao_samples, ssao_radius, rt_spp,
+                                        rt_radius, probe_stride, fallback_rmse, c.st_out.as<float>(), report, nullptr);
+    if (r) return r;
+    if (!raw_ao::hip_ok(c, hipMemcpyAsync(ao, c.st_out.p, 4 * np, hipMemcpyDeviceToHost, s), "download") ||
+        !raw_ao::hip_ok(c, hipStreamSynchronize(s), "learned frame"))
+        return RAW_AO_ERR_DEVICE;
+    return RAW_AO_OK;
 }
 
 }  // extern "C"

@@ -1,5 +1,6 @@
 // The learned AO surrogate's network state (kernels and host driver in unet.hip).
 #pragma once
+#include "../../include/raw_ao_hip.h"
 #include "ctx.hpp"
 
 namespace raw_ao {
@@ -25,12 +26,15 @@ struct UNet {
     // activations for the current shape
     int an{0}, ah{0}, aw{0};
     bool have_grad_bufs{false};
-    DevBuf x0, pred, headpart, lossbuf, cnt, wpart;
+    DevBuf pred, headpart, lossbuf, cnt, wpart;
     DevBuf act[kNetLayers];
     DevBuf pooled[3];
     DevBuf lvbuf[4][3];                     // backward: three rotating grad buffers a level
     const unsigned short* skip_dx[3]{};     // decoder input grads whose first channels feed a skip
     int skip_cx[3]{};
+    // the learned frame: SSAO, features, the probe sub-image and its statistics
+    DevBuf f_ssao, f_x, pr_pos, pr_nrm, pr_mask, pr_rt, pr_net, pr_stats;
+    float* h_stats{nullptr};                // pinned: sum of squares (as 2 floats of a double), max, count
 
     size_t param_count() const;
     bool init(Ctx& c);
@@ -45,6 +49,28 @@ struct UNet {
     bool adam(float lr, hipStream_t s);
     void release();
 };
+
+struct LearnedArgs {
+    const float* pos;
+    const float* nrm;
+    const uint8_t* mask;
+    const float* noisy;
+    const float* eye;
+    int w, h, ssao_samples;
+    float ssao_radius;
+    int rt_spp;
+    float rt_radius;
+    int probe_stride;
+    float fallback_rmse;
+    float* ao;
+};
+// SSAO, features and the network into a.ao; with a probe stride, a 64-spp ray-traced
+// probe of every stride-th pixel estimates the frame's error, and above fallback_rmse the
+// frame is ray traced instead.
+int learned_frame(Ctx& c, UNet& u, const LearnedArgs& a, raw_ao_learned_report* rep, hipStream_t s);
+// trace_ao / trace_ao4 of the built scene, with the checks of the C ABI
+int trace_built(Ctx& c, const float* pos, const float* nrm, const uint8_t* mask, int w, int h, int spp, float radius, float* ao,
+                hipStream_t s);
 
 // Single-layer kernels for the tests and the host driver.
 bool conv3x3(Ctx& c, const unsigned short* in0, const unsigned short* in1, int c0, int c1, const unsigned short* w, int kpad,

@@ -23,7 +23,7 @@ PKG = os.path.dirname(HERE)
 LIB_PATH = os.path.join(PKG, "build", "libraw_ao_hip.so")
 
 OK, ERR_ARG, ERR_DEVICE, ERR_BUILD, ERR_NO_DEVICE = 0, -1, -2, -3, -4
-OPT_HOST_LIBM_TRIG, OPT_TRACE_KERNEL, OPT_TRI_PHASE = 1, 2, 3
+OPT_HOST_LIBM_TRIG, OPT_TRACE_KERNEL, OPT_TRI_PHASE, OPT_CONV_KERNEL = 1, 2, 3, 4
 TRACE_WIDE, TRACE_BINARY = 0, 1
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
@@ -59,6 +59,20 @@ SIGNATURES = [
                                  C.POINTER(C.c_float)]),
     ("raw_ao_angle_table", C.c_int, [_vp, _f32p, _f32p]),
     ("raw_ao_bvh_bytes", C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    ("raw_ao_batch_open", C.c_int, [C.c_int, C.POINTER(_vp)]),
+    ("raw_ao_batch_close", None, [_vp]),
+    ("raw_ao_batch_last_error", C.c_char_p, [_vp]),
+    ("raw_ao_batch_add_scene", C.c_int, [_vp, _vp, C.c_uint32, C.c_int, C.POINTER(C.c_int)]),
+    ("raw_ao_batch_add_view", C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int,
+                                        C.c_float, _vp, _vp, _vp, C.POINTER(C.c_int)]),
+    ("raw_ao_batch_set_tuning", C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    ("raw_ao_batch_get_tuning", C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("raw_ao_batch_run", C.c_int, [_vp, C.c_int, _vp]),
+    ("raw_ao_batch_capture", C.c_int, [_vp, C.c_int]),
+    ("raw_ao_batch_launch", C.c_int, [_vp, _vp]),
+    ("raw_ao_batch_check", C.c_int, [_vp]),
+    ("raw_ao_batch_results", C.c_int, [_vp, _f32p, _f32p, _u32p, C.POINTER(C.c_uint64)]),
+    ("raw_ao_batch_tune", C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_float)]),
     ("raw_ao_net_param_count", C.c_int, [_vp, C.POINTER(C.c_uint64)]),
     ("raw_ao_net_set_params", C.c_int, [_vp, _f32p, C.c_uint64]),
     ("raw_ao_net_get_params", C.c_int, [_vp, _f32p, C.c_uint64]),
@@ -67,12 +81,27 @@ SIGNATURES = [
     ("raw_ao_net_forward_dev", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp]),
     ("raw_ao_net_train_step_dev", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp]),
     ("raw_ao_net_loss", C.c_int, [_vp, C.POINTER(C.c_float)]),
+    ("raw_ao_net_load", C.c_int, [_vp, C.c_char_p]),
+    ("raw_ao_learned_ao_dev", C.c_int, [_vp, _vp, _vp, _vp, _vp, _f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int,
+                                        C.c_float, C.c_int, C.c_float, _vp, C.c_void_p, _vp]),
+    ("raw_ao_learned_ao", C.c_int, [_vp, _f32p, _f32p, _u8p, C.c_void_p, _f32p, C.c_int, C.c_int, C.c_int, C.c_float,
+                                    C.c_int, C.c_float, C.c_int, C.c_float, _f32p, C.c_void_p]),
     ("raw_ao_net_read_act", C.c_int, [_vp, C.c_int, _u16p, C.c_uint64]),
     ("raw_ao_conv3x3_dev", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int,
                                      C.c_int, C.c_int, _vp, _vp]),
     ("raw_ao_conv3x3_wgrad_dev", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                            _vp, _vp, _vp]),
 ]
+
+
+class LearnedReport(C.Structure):
+    """raw_ao_learned_report"""
+    _fields_ = [("probe_pixels", C.c_uint32), ("probe_rmse", C.c_float), ("probe_max_error", C.c_float),
+                ("fell_back", C.c_int32)]
+
+    def as_dict(self):
+        return dict(probe_pixels=self.probe_pixels, probe_rmse=self.probe_rmse, probe_max_error=self.probe_max_error,
+                    fell_back=bool(self.fell_back))
 
 
 class RawAoError(RuntimeError):
@@ -295,3 +324,115 @@ class Context:
     def conv3x3_wgrad_dev(self, dy, in0, in1, c0, c1, cout, n, width, height, gw, gb, stream=None):
         self._check(lib().raw_ao_conv3x3_wgrad_dev(self._h, _ptr(dy), _ptr(in0), _optr(in1), c0, c1, cout, n, width,
                                                    height, _ptr(gw), _ptr(gb), _vp(stream)))
+
+    def net_load(self, path):
+        self._check(lib().raw_ao_net_load(self._h, os.fsencode(path)))
+
+    def learned_ao_dev(self, position, normal, mask, noisy, eye, w, h, ao, ssao_samples=24, ssao_radius=1.0, rt_spp=64,
+                       rt_radius=1.0, probe_stride=0, fallback_rmse=0.05, stream=None):
+        """The learned frame into ao (torch); returns the report (meaningful with a probe)."""
+        rep = LearnedReport()
+        self._check(lib().raw_ao_learned_ao_dev(self._h, _ptr(position), _ptr(normal), _ptr(mask), _optr(noisy),
+                                                np.ascontiguousarray(eye, np.float32), w, h, ssao_samples, ssao_radius,
+                                                rt_spp, rt_radius, probe_stride, fallback_rmse, _ptr(ao), C.byref(rep),
+                                                _vp(stream)))
+        return rep.as_dict()
+
+    def learned_ao(self, position, normal, mask, eye, noisy=None, ssao_samples=24, ssao_radius=1.0, rt_spp=64,
+                   rt_radius=1.0, probe_stride=0, fallback_rmse=0.05):
+        mask = np.ascontiguousarray(mask, np.uint8)
+        h, w = mask.shape
+        ao = np.zeros((h, w), np.float32)
+        rep = LearnedReport()
+        nz = None if noisy is None else np.ascontiguousarray(noisy, np.float32)
+        self._check(lib().raw_ao_learned_ao(self._h, np.ascontiguousarray(position, np.float32),
+                                            np.ascontiguousarray(normal, np.float32), mask,
+                                            None if nz is None else nz.ctypes.data_as(C.c_void_p),
+                                            np.ascontiguousarray(eye, np.float32), w, h, ssao_samples, ssao_radius,
+                                            rt_spp, rt_radius, probe_stride, fallback_rmse, ao, C.byref(rep)))
+        return ao, rep.as_dict()
+
+
+class Batch:
+    """raw_ao_batch: many views of several scenes, one persistent traversal launch. The
+    torch tensors handed in must stay alive while the batch uses them (it keeps them)."""
+
+    def __init__(self, device=0):
+        self._h = _vp()
+        self._keep = []
+        self.nviews = 0
+        L = lib()
+        rc = L.raw_ao_batch_open(device, C.byref(self._h))
+        if rc != OK:
+            msg = L.raw_ao_batch_last_error(self._h).decode() if self._h else "open failed"
+            self.close()
+            raise RawAoError(rc, msg)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().raw_ao_batch_close(self._h)
+            self._h = None
+        self._keep = []
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != OK:
+            raise RawAoError(rc, lib().raw_ao_batch_last_error(self._h).decode())
+
+    def add_scene(self, tris9_tensor, radius=16, ntri=None):
+        sid = C.c_int()
+        self._keep.append(tris9_tensor)
+        n = tris9_tensor.numel() // 9 if ntri is None else ntri
+        self._check(lib().raw_ao_batch_add_scene(self._h, _ptr(tris9_tensor), n, radius, C.byref(sid)))
+        return sid.value
+
+    def add_view(self, scene, position, normal, mask, w, h, spp, radius, ao, ssao=None, err=None, ssao_samples=24,
+                 ssao_radius=None):
+        vid = C.c_int()
+        self._keep += [position, normal, mask, ao, ssao, err]
+        self._check(lib().raw_ao_batch_add_view(self._h, scene, _ptr(position), _ptr(normal), _ptr(mask), w, h, spp, radius,
+                                                ssao_samples, radius if ssao_radius is None else ssao_radius, _ptr(ao),
+                                                _optr(ssao), _optr(err), C.byref(vid)))
+        self.nviews = vid.value + 1
+        return vid.value
+
+    def set_tuning(self, chunk, tri_q, blocks_per_cu):
+        self._check(lib().raw_ao_batch_set_tuning(self._h, chunk, tri_q, blocks_per_cu))
+
+    def tuning(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        self._check(lib().raw_ao_batch_get_tuning(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def run(self, rebuild=False, stream=None):
+        self._check(lib().raw_ao_batch_run(self._h, int(bool(rebuild)), _vp(stream)))
+
+    def capture(self, with_builds=True):
+        self._check(lib().raw_ao_batch_capture(self._h, int(bool(with_builds))))
+
+    def launch(self, stream=None):
+        self._check(lib().raw_ao_batch_launch(self._h, _vp(stream)))
+
+    def check(self):
+        self._check(lib().raw_ao_batch_check(self._h))
+
+    def results(self):
+        n = max(self.nviews, 1)
+        rmse, mx = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        px = np.zeros(n, np.uint32)
+        rays = C.c_uint64()
+        self._check(lib().raw_ao_batch_results(self._h, rmse, mx, px, C.byref(rays)))
+        return rmse[:self.nviews], mx[:self.nviews], px[:self.nviews], rays.value
+
+    def tune(self, cache_path=None):
+        ms = C.c_float()
+        self._check(lib().raw_ao_batch_tune(self._h, None if cache_path is None else os.fsencode(cache_path), C.byref(ms)))
+        return ms.value

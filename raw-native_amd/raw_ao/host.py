@@ -86,6 +86,8 @@ SceneSpec = collections.namedtuple("SceneSpec", "seed objects p_sphere subdiv sm
 # ~1M triangles: a 40 x 40 ground quad and 2,000 boxes and icospheres (subdivision 3,
 # 1,280 triangles) in a 40 x 40 x 10 volume, sizes log-uniform in [0.05, 2.0].
 SCENE_1M = SceneSpec(0x5EED, 2000, 0.385, 3, 0.05, 2.0, 40.0, 10.0)
+# ~2M triangles: the same family with 4,000 objects (the 4K low-spp denoise workload).
+SCENE_2M = SceneSpec(0x5EED2, 4000, 0.385, 3, 0.05, 2.0, 40.0, 10.0)
 
 
 def scene_seeded(spec=SCENE_1M):

@@ -1,0 +1,71 @@
+"""Ablation of the learned part with its committed weights, on views no training run saw
+(the 1M- and 2M-triangle workload scenes, seeds 0x5EED and 0x5EED2; training used seeds
+1000-1059): what the network gains over the method without it.
+
+  surrogate  RMSE against 64-spp ray tracing: the network's AO vs SSAO (its input), at
+             1920 x 1080 on the ~1M-triangle scene's seeded orbit view.
+  denoiser   RMSE against 256-spp ray tracing: the network's AO from 4-spp ray tracing vs
+             the 4-spp ray tracing itself, at 3840 x 2160 on the ~2M-triangle scene.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import raw_ao
+import raw_ao.learned as L
+
+pytestmark = pytest.mark.gpu
+
+
+def frame(spec, w, h):
+    vpos, vnrm, idx, _ = raw_ao.host.scene_seeded(spec)
+    eye = raw_ao.host.orbit_camera(1)
+    _, pos, nrm, mask = raw_ao.host.rasterize(vpos, vnrm, idx, eye, (0, 2, 0), (0, 1, 0), 0.9,
+                                              float(np.float32(w) / np.float32(h)), 0.1, 200.0, w, h)
+    dev = torch.device("cuda:0")
+    return (raw_ao.host.mesh_tris(vpos, idx), eye, mask, *(torch.from_numpy(x).to(dev) for x in (pos, nrm, mask)))
+
+
+def rmse(ctx, a, b, mask_t, mask, w, h):
+    err = torch.empty_like(a)
+    mx = torch.zeros(4, device=a.device)
+    ctx.error_map_dev(a, b, mask_t, w, h, err, mx)
+    return float(raw_ao.host.rmse_of(err.cpu().numpy(), mask)[0])
+
+
+@pytest.mark.skipif(not os.path.exists(L.SURROGATE), reason="no trained surrogate weights")
+def test_surrogate_beats_ssao_on_a_held_out_view(ctx):
+    w, h = 1920, 1080
+    tris, eye, mask, P, N, M = frame(raw_ao.host.SCENE_1M, w, h)
+    ctx.build_ploc(tris)
+    img = lambda: torch.empty((h, w), dtype=torch.float32, device="cuda")
+    rt, ss, lao = img(), img(), img()
+    ctx.compute_rtao_dev(P, N, M, w, h, 64, 1.0, rt)
+    ctx.compute_ssao_dev(P, N, M, w, h, 24, 1.0, ss)
+    ctx.net_load(L.SURROGATE)
+    rep = ctx.learned_ao_dev(P, N, M, None, eye, w, h, lao, rt_radius=1.0, probe_stride=8, fallback_rmse=1e9)
+    torch.cuda.synchronize()
+    r_net, r_ss = rmse(ctx, lao, rt, M, mask, w, h), rmse(ctx, ss, rt, M, mask, w, h)
+    print("surrogate RMSE %.4f vs SSAO %.4f (probe estimate %.4f)" % (r_net, r_ss, rep["probe_rmse"]))
+    assert r_net < 0.7 * r_ss
+    # the probe's estimate of the frame's error is within a third of the truth
+    assert abs(rep["probe_rmse"] - r_net) < r_net / 3
+
+
+@pytest.mark.skipif(not os.path.exists(L.DENOISER), reason="no trained denoiser weights")
+def test_denoiser_beats_low_spp_ray_tracing(ctx):
+    w, h = 3840, 2160
+    tris, eye, mask, P, N, M = frame(raw_ao.host.SCENE_2M, w, h)
+    ctx.build_ploc(tris)
+    img = lambda: torch.empty((h, w), dtype=torch.float32, device="cuda")
+    ref, rt4, dn = img(), img(), img()
+    ctx.compute_rtao_dev(P, N, M, w, h, 256, 1.0, ref)
+    ctx.compute_rtao_dev(P, N, M, w, h, 4, 1.0, rt4)
+    ctx.net_load(L.DENOISER)
+    ctx.learned_ao_dev(P, N, M, rt4, eye, w, h, dn, rt_radius=1.0)
+    torch.cuda.synchronize()
+    r_dn, r_4 = rmse(ctx, dn, ref, M, mask, w, h), rmse(ctx, rt4, ref, M, mask, w, h)
+    print("denoised RMSE %.4f vs 4-spp %.4f" % (r_dn, r_4))
+    assert r_dn < 0.5 * r_4

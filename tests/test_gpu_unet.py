@@ -257,3 +257,91 @@ def test_features_match_oracle(ctx):
         gpu = U.from_bits(host16(out)).reshape(h, w, 8)
         ref = U.features(pos, nrm, mask, ssao, nz, eye)
         assert (np.abs(gpu - U.bf16(ref)) <= 2.0 ** -7 * np.abs(ref) + 1e-7).all()
+
+
+def _scene_frame(w=64, h=48):
+    import raw_ao
+    vpos, vnrm, idx = raw_ao.host.test_scene_mesh()
+    eye = np.array([4, 4, 6], np.float32)
+    _, pos, nrm, mask = raw_ao.host.rasterize(vpos, vnrm, idx, eye, (0, 1, 0), (0, 1, 0), 0.9,
+                                              float(np.float32(w) / np.float32(h)), 0.1, 100.0, w, h)
+    return raw_ao.host.mesh_tris(vpos, idx), pos, nrm, mask, eye
+
+
+def test_learned_frame_composition(ctx):
+    """learned_ao_dev without a probe == SSAO -> features -> forward, bit for bit; with a
+    forced fallback == the ray-traced frame, bit for bit; the probe's RMSE == the network's
+    error against ray tracing of the probe sub-image, computed here."""
+    w, h = 64, 48
+    tris, pos, nrm, mask, eye = _scene_frame(w, h)
+    ctx.build_ploc(tris)
+    ctx.net_set_params(random_params(70))
+    P, N, M = dev(pos), dev(nrm), dev(mask)
+    out = torch.zeros((h, w), dtype=torch.float32, device="cuda")
+    rep = ctx.learned_ao_dev(P, N, M, None, eye, w, h, out, ssao_samples=24, ssao_radius=2.0)
+    torch.cuda.synchronize()
+    learned = out.cpu().numpy().copy()
+    ss = torch.zeros((h, w), dtype=torch.float32, device="cuda")
+    x = torch.zeros(h * w * 8, dtype=torch.int16, device="cuda")
+    ao = torch.zeros((h, w), dtype=torch.float32, device="cuda")
+    ctx.compute_ssao_dev(P, N, M, w, h, 24, 2.0, ss)
+    ctx.net_features_dev(P, N, M, ss, None, eye, w, h, x)
+    ctx.net_forward_dev(x, M, 1, w, h, ao)
+    torch.cuda.synchronize()
+    assert np.array_equal(learned.view(np.uint32), ao.cpu().numpy().view(np.uint32))
+    assert rep["probe_pixels"] == 0 and not rep["fell_back"]
+    # the probe: every 4th pixel from 2, traced as its own image
+    st = 4
+    rep = ctx.learned_ao_dev(P, N, M, None, eye, w, h, out, ssao_samples=24, ssao_radius=2.0, rt_spp=64, rt_radius=2.0,
+                             probe_stride=st, fallback_rmse=1e9)
+    sub = (slice(st // 2, h, st), slice(st // 2, w, st))
+    prt = ctx.compute_rtao(np.ascontiguousarray(pos[sub]), np.ascontiguousarray(nrm[sub]), np.ascontiguousarray(mask[sub]),
+                           64, 2.0)
+    pm = mask[sub] != 0
+    d = (learned[sub] - prt)[pm].astype(np.float64)
+    assert rep["probe_pixels"] == pm.sum() > 0
+    assert abs(rep["probe_rmse"] - np.sqrt((d * d).mean())) <= 1e-6 * max(1e-3, rep["probe_rmse"])
+    assert rep["probe_max_error"] == np.float32(np.abs(d).max())
+    assert not rep["fell_back"]
+    # forced fallback: the full ray-traced frame
+    rep = ctx.learned_ao_dev(P, N, M, None, eye, w, h, out, ssao_samples=24, ssao_radius=2.0, rt_spp=64, rt_radius=2.0,
+                             probe_stride=st, fallback_rmse=-1.0)
+    torch.cuda.synchronize()
+    rt = ctx.compute_rtao(pos, nrm, mask, 64, 2.0)
+    assert rep["fell_back"]
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), rt.view(np.uint32))
+    # the host-pointer form gives the same frame
+    hao, hrep = ctx.learned_ao(pos, nrm, mask, eye, ssao_samples=24, ssao_radius=2.0)
+    assert np.array_equal(hao.view(np.uint32), learned.view(np.uint32))
+
+
+def test_weights_file_loads_through_the_abi(ctx, tmp_path):
+    import raw_ao.learned as L
+    p = random_params(80)
+    f = str(tmp_path / "w.raow")
+    L.save_weights(f, p, {"mode": "test"})
+    ctx.net_load(f)
+    assert np.array_equal(ctx.net_get_params(), p)
+    bad = tmp_path / "bad.raow"
+    bad.write_bytes(b"RAOX" + bytes(40))
+    with pytest.raises(Exception):
+        ctx.net_load(str(bad))
+
+
+def test_conv_kernels_agree_bit_for_bit(ctx):
+    """The tap-major LDS kernel and the general kernel sum in the same order: every layer's
+    activations of a forward pass are identical (odd sizes: pixel tiles cross rows and images)."""
+    import raw_ao
+    n, h, w = 3, 24, 40
+    ctx.net_set_params(random_params(90))
+    xb, mask, _ = net_case(91, n, h, w)
+    keep = [dev(xb), dev(mask)]
+    ao = torch.zeros(n * h * w, dtype=torch.float32, device="cuda")
+    acts = {}
+    for kern in (1, 0):
+        ctx.set_option(raw_ao.OPT_CONV_KERNEL, kern)
+        ctx.net_forward_dev(keep[0], keep[1], n, w, h, ao)
+        acts[kern] = [ctx.net_read_act(i, n * (h >> lv) * (w >> lv) * co) for i, (_, _, co, lv) in enumerate(U.LAYERS)]
+    ctx.set_option(raw_ao.OPT_CONV_KERNEL, 0)
+    for a, b in zip(acts[0], acts[1]):
+        assert np.array_equal(a, b)
