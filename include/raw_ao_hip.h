@@ -60,9 +60,11 @@ enum {
      * many 64ths of its live lanes wait on leaf triangles (or none can take a node step).
      * Results are the same for every value; only the speed differs. */
     RAW_AO_OPT_TRI_PHASE = 3,
-    /* The network's convolution kernel: 0 (default) the tap-major kernel with LDS-staged
-     * weights wherever a layer's input channels are a multiple of 32, 1 the general
-     * kernel everywhere. The sums run in the same order: results are identical. */
+    /* The network's convolution kernel: 0 (default) the LDS-tiled kernel wherever a
+     * layer's input channels are a multiple of 32 (its float32 sums run chunk-major, so
+     * outputs can differ from the others' by one bf16 rounding), 1 the general kernel
+     * everywhere, 2 the tap-major kernel where it applies (the general kernel's sum order:
+     * identical outputs to 1). */
     RAW_AO_OPT_CONV_KERNEL = 4
 };
 enum {

@@ -121,7 +121,7 @@ int raw_ao_set_option(raw_ao_ctx* x, int option, int value) {
         return RAW_AO_OK;
     }
     if (option == RAW_AO_OPT_CONV_KERNEL) {
-        if (value != 0 && value != 1) return fail(c, RAW_AO_ERR_ARG, "convolution kernel must be 0 or 1");
+        if (value < 0 || value > 2) return fail(c, RAW_AO_ERR_ARG, "convolution kernel must be 0, 1 or 2");
         c.conv_kernel = value;
         return RAW_AO_OK;
     }

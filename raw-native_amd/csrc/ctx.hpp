@@ -78,7 +78,7 @@ struct Ctx {
     float ms_build{0}, ms_trace{0}, ms_ssao{0}, ms_error{0};
 
     UNet* net{nullptr};          // the learned AO surrogate, made on first use
-    int conv_kernel{0};          // 0: tap-major LDS kernel where it applies, 1: the general one only
+    int conv_kernel{0};          // 0: LDS-tiled where it applies, 1: general, 2: tap-major where it applies
 
     // grids of the last eager build, replayed by the capturable build (ploc_build_fixed,
     // wide_build_fixed)

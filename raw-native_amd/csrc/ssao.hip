@@ -4,6 +4,7 @@
 // stay in the tile's cache lines.
 #include "ao_math.hpp"
 #include "ctx.hpp"
+#include <algorithm>
 
 namespace raw_ao {
 namespace {
@@ -21,44 +22,72 @@ __global__ __launch_bounds__(TX* TY) void k_ssao(const float* __restrict__ pos, 
     const V3 n = v3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
     const int R = 6;
     int occ = 0, used = 0;
-    for (int s = 0; s < samples; ++s) {
-        const uint32_t ka = hash24(uint32_t(x), uint32_t(y), uint32_t(s));
-        const float a = (float(ka) / float(0x1000000)) * 6.2831853f;
-        const float rad = (0.3f + 0.7f * hash01(uint32_t(x), uint32_t(y), uint32_t(s + 97))) * float(R);
-        float sa, ca;
-        sincos_base(a, sa, ca);
-        if (trig) trig_fix(trig, ka, sa, ca);
-        const int sx = x + lround_f(ca * rad);
-        const int sy = y + lround_f(sa * rad);
-        if (sx < 0 || sy < 0 || sx >= w || sy >= h) continue;
-        const size_t j = size_t(sy) * size_t(w) + size_t(sx);
-        if (!mask[j]) continue;
-        ++used;
-        const V3 d = sub(v3(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]), p);
-        const float dist = length(d);
-        if (dist < 1e-4f || dist > radius) continue;
-        const float ndl = dot(n, normalize(d));
-        if (ndl > 0.15f) ++occ;
+    // Taps in groups of U: the group's coordinates first, then all its loads (issued
+    // together, from a clamped index where a tap leaves the image), then the tests. Each
+    // tap's float operations are the ones of the one-tap-at-a-time loop.
+    constexpr int U = 8;
+    for (int s0 = 0; s0 < samples; s0 += U) {
+        size_t jj[U];
+        bool inb[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int s = s0 + u;
+            const uint32_t ka = hash24(uint32_t(x), uint32_t(y), uint32_t(s));
+            const float a = (float(ka) / float(0x1000000)) * 6.2831853f;
+            const float rad = (0.3f + 0.7f * hash01(uint32_t(x), uint32_t(y), uint32_t(s + 97))) * float(R);
+            float sa, ca;
+            sincos_base(a, sa, ca);
+            if (trig) trig_fix(trig, ka, sa, ca);
+            const int sx = x + lround_f(ca * rad);
+            const int sy = y + lround_f(sa * rad);
+            inb[u] = s < samples && sx >= 0 && sy >= 0 && sx < w && sy < h;
+            jj[u] = inb[u] ? size_t(sy) * size_t(w) + size_t(sx) : i;
+        }
+        uint8_t mk[U];
+        float qx[U], qy[U], qz[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            mk[u] = mask[jj[u]];
+            qx[u] = pos[3 * jj[u]];
+            qy[u] = pos[3 * jj[u] + 1];
+            qz[u] = pos[3 * jj[u] + 2];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (!inb[u] || !mk[u]) continue;
+            ++used;
+            const V3 d = sub(v3(qx[u], qy[u], qz[u]), p);
+            const float dist = length(d);
+            if (dist < 1e-4f || dist > radius) continue;
+            const float ndl = dot(n, normalize(d));
+            if (ndl > 0.15f) ++occ;
+        }
     }
     const float a = used > 0 ? 1.0f - float(occ) / float(used) : 1.0f;
     ao[i] = a < 0.0f ? 0.0f : a;
 }
 
-// |approx - truth| where covered, 0 elsewhere; the maximum through an unsigned
-// atomic on the bits (every value is >= 0, and a NaN never beats the running
-// maximum, as `e > maxError` in the reference).
+// |approx - truth| where covered, 0 elsewhere; the maximum through an unsigned atomic on
+// the bits (every value is >= 0, and a NaN never beats the running maximum, as
+// `e > maxError` in the reference), one atomic a block: grid-stride over the pixels, the
+// block's maximum reduced in LDS first.
 __global__ __launch_bounds__(256) void k_error_map(const float* __restrict__ a, const float* __restrict__ t,
                                                    const uint8_t* __restrict__ mask, uint32_t npix, float* __restrict__ err,
                                                    uint32_t* __restrict__ maxbits) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    float e = 0.0f;
-    if (i < npix) {
-        e = mask[i] ? fabsf(a[i] - t[i]) : 0.0f;
+    __shared__ float wmax[4];
+    float m = 0.0f;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < npix; i += gridDim.x * 256) {
+        const float e = mask[i] ? fabsf(a[i] - t[i]) : 0.0f;
         err[i] = e;
-        if (!(e > 0.0f)) e = 0.0f;
+        if (e > m) m = e;
     }
-    for (int off = 32; off > 0; off >>= 1) e = fmaxf(e, __shfl_xor(e, off));
-    if ((threadIdx.x & 63) == 0 && e > 0.0f) atomicMax(maxbits, __float_as_uint(e));
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float b = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+        if (b > 0.0f) atomicMax(maxbits, __float_as_uint(b));
+    }
 }
 
 }  // namespace
@@ -82,7 +111,8 @@ int error_map(Ctx& c, const float* d_a, const float* d_t, const uint8_t* d_mask,
     const uint32_t npix = uint32_t(w) * uint32_t(h);
     if (!hip_ok(c, hipMemsetAsync(d_max, 0, 4, s), "max clear")) return -2;
     (void)hipEventRecord(c.ev[6], s);
-    k_error_map<<<(npix + 255) / 256, 256, 0, s>>>(d_a, d_t, d_mask, npix, d_err, reinterpret_cast<uint32_t*>(d_max));
+    k_error_map<<<std::min(2048u, (npix + 255) / 256), 256, 0, s>>>(d_a, d_t, d_mask, npix, d_err,
+                                                                    reinterpret_cast<uint32_t*>(d_max));
     (void)hipEventRecord(c.ev[7], s);
     return hip_ok(c, hipGetLastError(), "error map launch") ? 0 : -2;
 }

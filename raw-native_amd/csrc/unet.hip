@@ -203,6 +203,83 @@ __global__ __launch_bounds__(256) void k_conv_t(ConvArgs a) {
     }
 }
 
+// The convolution from LDS tiles, for cin a multiple of 32: a block computes an 8 x 32
+// pixel tile of one image (a wave: 2 rows as 4 tiles of 16 pixels) for up to 64 output
+// channels. Per 32-channel chunk it stages the tile's input with a 1-pixel border
+// (10 x 34 pixels, the concatenation's upsampled half read in place) and the chunk's
+// weights for all 9 taps in LDS, then runs the 9 taps from LDS: every input value is read
+// from global memory once per chunk instead of once per tap. LDS rows are padded so the 16
+// lanes of a fragment row hit distinct banks. The sums run chunk-major (taps inside a
+// chunk), so the results may differ from k_conv's by the rounding of the float32 sums.
+constexpr int LT_H = 8, LT_W = 32, LH_H = LT_H + 2, LH_W = LT_W + 2;
+constexpr int LH_PS = 40;            // bf16 a staged pixel: 32 channels + 8 pad
+constexpr int LW_RS = 9 * 32 + 8;    // bf16 a staged weight row: 9 taps x 32 channels + 8 pad
+template <int CT>
+__global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
+    __shared__ __attribute__((aligned(16))) bf16_t hal[LH_H * LH_W * LH_PS];
+    __shared__ __attribute__((aligned(16))) bf16_t wls[CT * 16 * LW_RS];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int g = lane >> 4, r = lane & 15;
+    const int tiles_x = (a.wd + LT_W - 1) / LT_W, tiles_y = (a.h + LT_H - 1) / LT_H;
+    const int bx = int(blockIdx.x) % tiles_x, q0 = int(blockIdx.x) / tiles_x;
+    const int by = q0 % tiles_y, img = q0 / tiles_y;
+    const int oy = by * LT_H, ox = bx * LT_W;
+    const int cin = a.c0 + a.c1, hh = a.h >> 1, hw = a.wd >> 1;
+    f32x4 acc[CT][4];
+    for (int i = 0; i < CT; ++i)
+        for (int t = 0; t < 4; ++t) acc[i][t] = f32x4{0, 0, 0, 0};
+    for (int c = 0; c < cin; c += 32) {
+        __syncthreads();
+        for (int e = tid; e < LH_H * LH_W * 4; e += 256) {
+            const int hp = e >> 2, q = e & 3;
+            const int gy = oy + hp / LH_W - 1, gx = ox + hp % LH_W - 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (gy >= 0 && gy < a.h && gx >= 0 && gx < a.wd) {
+                const int ch = c + 8 * q;
+                const bf16_t* src = ch < a.c0 ? a.in0 + (size_t(img * a.h + gy) * a.wd + gx) * a.c0 + ch
+                                              : a.in1 + (size_t(img * hh + (gy >> 1)) * hw + (gx >> 1)) * a.c1 + (ch - a.c0);
+                v = *reinterpret_cast<const uint4*>(src);
+            }
+            *reinterpret_cast<uint4*>(&hal[hp * LH_PS + 8 * q]) = v;
+        }
+        for (int e = tid; e < CT * 16 * 36; e += 256) {
+            const int co = e / 36, rem = e - co * 36, tap = rem >> 2, q = rem & 3;
+            *reinterpret_cast<uint4*>(&wls[co * LW_RS + tap * 32 + 8 * q]) =
+                *reinterpret_cast<const uint4*>(a.w + size_t(a.co0 + co) * a.kpad + tap * cin + c + 8 * q);
+        }
+        __syncthreads();
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap % 3;
+            V8 bv[4];
+            for (int t = 0; t < 4; ++t) {
+                const int hp = (2 * wv + (t >> 1) + dy) * LH_W + 16 * (t & 1) + r + dx;
+                bv[t].u = *reinterpret_cast<const uint4*>(&hal[hp * LH_PS + 8 * g]);
+            }
+            for (int i = 0; i < CT; ++i) {
+                V8 av;
+                av.u = *reinterpret_cast<const uint4*>(&wls[(i * 16 + r) * LW_RS + tap * 32 + 8 * g]);
+                for (int t = 0; t < 4; ++t) acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av.v, bv[t].v, acc[i][t], 0, 0, 0);
+            }
+        }
+    }
+    for (int t = 0; t < 4; ++t) {
+        const int y = oy + 2 * wv + (t >> 1), x = ox + 16 * (t & 1) + r;
+        if (y >= a.h || x >= a.wd) continue;
+        const size_t p = size_t(img * a.h + y) * a.wd + x;
+        for (int i = 0; i < CT; ++i) {
+            const int co = a.co0 + i * 16 + 4 * g;
+            float v[4];
+            for (int j = 0; j < 4; ++j) {
+                v[j] = acc[i][t][j] + (a.bias ? a.bias[co + j] : 0.0f);
+                if (a.relu && v[j] < 0.0f) v[j] = 0.0f;
+            }
+            const uint32_t lo = uint32_t(f2bf(v[0])) | (uint32_t(f2bf(v[1])) << 16);
+            const uint32_t hi = uint32_t(f2bf(v[2])) | (uint32_t(f2bf(v[3])) << 16);
+            *reinterpret_cast<uint2*>(a.out + p * a.cout + co) = make_uint2(lo, hi);
+        }
+    }
+}
+
 // ---- weight gradient ----------------------------------------------------------------
 // dW[co][k] = sum_p dy[p][co] * X[p][k], X the forward's input patches (k < K) and 1 at
 // k == K (the bias column). A block: one 64-wide k tile, all cout (<= 128), a chunk of
@@ -518,7 +595,21 @@ bool conv_shape_ok(int c0, int c1, int cout) {
 bool conv3x3(Ctx& c, const bf16_t* in0, const bf16_t* in1, int c0, int c1, const bf16_t* w, int kpad, const float* bias,
              int cout, int relu, int n, int h, int wd, bf16_t* out, hipStream_t s) {
     ConvArgs a{in0, in1, w, bias, out, n, h, wd, c0, c1, cout, kpad, relu, 0};
-    const bool tapwise = c.conv_kernel != 1 && (c0 + c1) % 32 == 0 && c0 % 32 == 0 && kpad == 9 * (c0 + c1);
+    const bool chunked = (c0 + c1) % 32 == 0 && c0 % 32 == 0;
+    if (chunked && c.conv_kernel == 0 && (c1 == 0 || (h % 2 == 0 && wd % 2 == 0))) {
+        const unsigned blocks = unsigned(size_t(n) * ((h + LT_H - 1) / LT_H) * ((wd + LT_W - 1) / LT_W));
+        for (int co0 = 0; co0 < cout; co0 += 64) {
+            a.co0 = co0;
+            switch (std::min(4, (cout - co0) / 16)) {
+                case 1: k_conv_lds<1><<<blocks, 256, 0, s>>>(a); break;
+                case 2: k_conv_lds<2><<<blocks, 256, 0, s>>>(a); break;
+                case 3: k_conv_lds<3><<<blocks, 256, 0, s>>>(a); break;
+                default: k_conv_lds<4><<<blocks, 256, 0, s>>>(a); break;
+            }
+        }
+        return hip_ok(c, hipGetLastError(), "conv3x3");
+    }
+    const bool tapwise = c.conv_kernel != 1 && chunked && kpad == 9 * (c0 + c1);
     for (int co0 = 0; co0 < cout; co0 += 128) {
         a.co0 = co0;
         if (tapwise) {

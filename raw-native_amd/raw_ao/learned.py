@@ -73,10 +73,11 @@ def load_weights(path):
     return params, meta
 
 
-def training_scene(seed, rng):
-    """A seeded scene of the 1M-triangle family with 600-3000 objects."""
+def training_scene(seed, rng, objects=(600, 3000)):
+    """A seeded scene of the 1M-triangle family with a random object count in `objects`."""
     s = host.SCENE_1M
-    return host.SceneSpec(seed, int(rng.integers(600, 3001)), s.p_sphere, s.subdiv, s.smin, s.smax, s.ground, s.height)
+    return host.SceneSpec(seed, int(rng.integers(objects[0], objects[1] + 1)), s.p_sphere, s.subdiv, s.smin, s.smax,
+                          s.ground, s.height)
 
 
 def random_camera(rng):
@@ -110,7 +111,7 @@ class ViewSet:
 
 
 def render_views(ctx, n_scenes, views_per_scene, w, h, seed=1000, spp=64, radius=1.0, ssao_samples=24, ssao_radius=1.0,
-                 noisy_spp=0, threads=16, log=print, stream=None):
+                 noisy_spp=0, threads=16, log=print, stream=None, objects=(600, 3000)):
     """Renders n_scenes x views_per_scene views into a ViewSet. The scene seeds are
     seed + i; the 1M-triangle workload's scene (seed 0x5EED) is never among them."""
     import torch
@@ -124,7 +125,7 @@ def render_views(ctx, n_scenes, views_per_scene, w, h, seed=1000, spp=64, radius
     t0 = time.time()
     with cf.ThreadPoolExecutor(max_workers=threads) as pool:
         for si in range(n_scenes):
-            spec = training_scene(seed + si, rng)
+            spec = training_scene(seed + si, rng, objects)
             vpos, vnrm, idx, _ = host.scene_seeded(spec)
             tris = torch.from_numpy(host.mesh_tris(vpos, idx)).to(dev)
             ctx.build_ploc_dev(tris, 16, st)
@@ -166,7 +167,8 @@ def masked_rmse(a, b, mask):
 
 
 def evaluate(ctx, vs, idx, stream=None):
-    """Mean over views of the network's RMSE against the target, and of SSAO's."""
+    """Mean over views of the network's RMSE against the target, and of its input's (the
+    noisy AO for the denoiser, SSAO otherwise)."""
     import torch
     st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
     ao = torch.empty((vs.h, vs.w), dtype=torch.float32, device="cuda:0")
@@ -174,7 +176,8 @@ def evaluate(ctx, vs, idx, stream=None):
     for v in idx:
         ctx.net_forward_dev(vs.x[v], vs.mask[v], 1, vs.w, vs.h, ao, st)
         net.append(masked_rmse(ao, vs.target[v], vs.mask[v])[0])
-        ss.append(masked_rmse(vs.ssao[v], vs.target[v], vs.mask[v])[0])
+        base = vs.noisy[v] if vs.noisy is not None else vs.ssao[v]
+        ss.append(masked_rmse(base, vs.target[v], vs.mask[v])[0])
     return float(np.mean(net)), float(np.mean(ss))
 
 
@@ -224,11 +227,11 @@ def train(ctx, vs, minutes, batch=8, crop=256, lr_max=1e-3, seed=0, val_views=8,
             rec = dict(step=step, seconds=round(time.time() - t0, 1), loss=float(np.mean(losses[-10:])), val_rmse=vr,
                        val_rmse_ssao=sr)
             hist.append(rec)
-            log("step %d  %.0f s  loss %.5f  val RMSE %.4f (SSAO %.4f)" % (step, rec["seconds"], rec["loss"], vr, sr))
+            log("step %d  %.0f s  loss %.5f  val RMSE %.4f (input %.4f)" % (step, rec["seconds"], rec["loss"], vr, sr))
     torch.cuda.synchronize()
     vr, sr = evaluate(ctx, vs, val, st)
     hist.append(dict(step=step, seconds=round(time.time() - t0, 1), val_rmse=vr, val_rmse_ssao=sr))
-    log("done: %d steps, val RMSE %.4f (SSAO %.4f)" % (step, vr, sr))
+    log("done: %d steps, val RMSE %.4f (input %.4f)" % (step, vr, sr))
     return ctx.net_get_params(), hist
 
 
@@ -248,6 +251,7 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--eval-every", type=int, default=500)
     ap.add_argument("--init", default=None, help="start from this weights file")
+    ap.add_argument("--objects", type=int, nargs=2, default=(600, 3000), help="object count range of the scenes")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
 
@@ -260,13 +264,15 @@ def main(argv=None):
     ctx = raw_ao.Context(0)
     noisy = 4 if a.mode == "denoiser" else 0
     t0 = time.time()
-    vs = render_views(ctx, a.scenes, a.views, a.width, a.height, seed=a.seed, noisy_spp=noisy, log=log)
+    vs = render_views(ctx, a.scenes, a.views, a.width, a.height, seed=a.seed, noisy_spp=noisy, log=log,
+                      objects=tuple(a.objects))
     gen_s = time.time() - t0
     init = load_weights(a.init)[0] if a.init else None
     params, hist = train(ctx, vs, a.minutes, batch=a.batch, crop=a.crop, lr_max=a.lr, seed=a.seed, log=log,
                          eval_every=a.eval_every, params=init)
     meta = dict(mode=a.mode, scenes=a.scenes, views_per_scene=a.views, width=a.width, height=a.height, rt_spp=64,
                 rt_radius=1.0, ssao_samples=24, ssao_radius=1.0, noisy_spp=noisy, scene_seeds=[a.seed, a.seed + a.scenes - 1],
+                objects=list(a.objects),
                 batch=a.batch, crop=a.crop, lr=a.lr, minutes=a.minutes, render_seconds=round(gen_s, 1),
                 history=hist, init=os.path.basename(a.init) if a.init else None)
     save_weights(a.out, params, meta)
