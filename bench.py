@@ -69,6 +69,22 @@ def cpu_baseline(tris, pos, nrm, mask, spp, radius, row_step):
 
 
 WEIGHTS = os.path.join(ROOT, "raw-native_amd", "weights")
+HBM_COUNTERS = os.path.join(ROOT, "profiles", "ao_trace_1m_1080p_hbm_counters.md")
+
+
+def hbm_traffic_per_launch():
+    """HBM bytes per trace launch from the committed FETCH_SIZE / WRITE_SIZE counter runs,
+    scaled by their copy calibration (None without that file)."""
+    import re
+    try:
+        text = open(HBM_COUNTERS).read()
+    except OSError:
+        return None
+    v = {k: float(m.group(1)) for k in ("trace_fetch_kib_per_dispatch", "trace_write_kib_per_dispatch", "fetch_scale",
+                                        "write_scale") for m in [re.search(r"^%s: ([0-9.]+)$" % k, text, re.M)] if m}
+    if len(v) != 4:
+        return None
+    return 1024.0 * (v["trace_fetch_kib_per_dispatch"] * v["fetch_scale"] + v["trace_write_kib_per_dispatch"] * v["write_scale"])
 
 
 def _timed(torch, fn, steps, warmup=2):
@@ -370,8 +386,10 @@ def main():
                            "host_reconcile_sum": float(np.mean(host_ms))},
         "rmse_ssao_vs_rt": float(rmse), "reconcile_pixels": px,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_trace_ao", "algorithmic_bytes_per_launch": alg_bytes,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": hbm_traffic_per_launch(),
+                     "traffic_source": os.path.relpath(HBM_COUNTERS, ROOT),
+                     "kernel": "k_trace_ao4" if a.kernel == "wide" else "k_trace_ao",
+                     "algorithmic_bytes_per_launch": alg_bytes,
                      "node_visits": visits, "triangle_tests": tests},
     }
     if learned is not None:

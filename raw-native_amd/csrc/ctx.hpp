@@ -79,6 +79,7 @@ struct Ctx {
 
     UNet* net{nullptr};          // the learned AO surrogate, made on first use
     int conv_kernel{0};          // 0: LDS-tiled where it applies, 1: general, 2: tap-major where it applies
+    int cus{0};                  // compute units of the device (read on first use)
 
     // grids of the last eager build, replayed by the capturable build (ploc_build_fixed,
     // wide_build_fixed)

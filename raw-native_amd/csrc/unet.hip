@@ -203,25 +203,28 @@ __global__ __launch_bounds__(256) void k_conv_t(ConvArgs a) {
     }
 }
 
-// The convolution from LDS tiles, for cin a multiple of 32: a block computes an 8 x 32
-// pixel tile of one image (a wave: 2 rows as 4 tiles of 16 pixels) for up to 64 output
-// channels. Per 32-channel chunk it stages the tile's input with a 1-pixel border
-// (10 x 34 pixels, the concatenation's upsampled half read in place) and the chunk's
-// weights for all 9 taps in LDS, then runs the 9 taps from LDS: every input value is read
-// from global memory once per chunk instead of once per tap. LDS rows are padded so the 16
-// lanes of a fragment row hit distinct banks. The sums run chunk-major (taps inside a
-// chunk), so the results may differ from k_conv's by the rounding of the float32 sums.
+// The convolution from LDS tiles, for cin a multiple of 32. A block computes one work
+// item: an 8 x 32 pixel tile of one image (a wave: 2 rows as 4 tiles of 16 pixels) for one
+// group of CT*16 output channels (blockIdx.x = group * tiles + tile). Per 32-channel chunk it
+// stages the tile's input with a 1-pixel border (10 x 34 pixels, the concatenation's
+// upsampled half read in place) and the chunk's weights for all 9 taps in LDS, then runs the
+// 9 taps from LDS: every input value is read from global memory once per chunk instead of
+// once per tap. Two or more blocks share a CU, so one block's staging overlaps another's
+// MFMAs. LDS rows are padded so the 16 lanes of a fragment row hit distinct banks. The sums
+// run chunk-major (taps inside a chunk), so the results may differ from k_conv's by the
+// rounding of the float32 sums.
 constexpr int LT_H = 8, LT_W = 32, LH_H = LT_H + 2, LH_W = LT_W + 2;
 constexpr int LH_PS = 40;            // bf16 a staged pixel: 32 channels + 8 pad
 constexpr int LW_RS = 9 * 32 + 8;    // bf16 a staged weight row: 9 taps x 32 channels + 8 pad
 template <int CT>
-__global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
+__global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, int ntiles) {
     __shared__ __attribute__((aligned(16))) bf16_t hal[LH_H * LH_W * LH_PS];
     __shared__ __attribute__((aligned(16))) bf16_t wls[CT * 16 * LW_RS];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int g = lane >> 4, r = lane & 15;
     const int tiles_x = (a.wd + LT_W - 1) / LT_W, tiles_y = (a.h + LT_H - 1) / LT_H;
-    const int bx = int(blockIdx.x) % tiles_x, q0 = int(blockIdx.x) / tiles_x;
+    const int tile = int(blockIdx.x) % ntiles, co0 = a.co0 + (int(blockIdx.x) / ntiles) * CT * 16;
+    const int bx = tile % tiles_x, q0 = tile / tiles_x;
     const int by = q0 % tiles_y, img = q0 / tiles_y;
     const int oy = by * LT_H, ox = bx * LT_W;
     const int cin = a.c0 + a.c1, hh = a.h >> 1, hw = a.wd >> 1;
@@ -230,22 +233,30 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
         for (int t = 0; t < 4; ++t) acc[i][t] = f32x4{0, 0, 0, 0};
     for (int c = 0; c < cin; c += 32) {
         __syncthreads();
-        for (int e = tid; e < LH_H * LH_W * 4; e += 256) {
-            const int hp = e >> 2, q = e & 3;
-            const int gy = oy + hp / LH_W - 1, gx = ox + hp % LH_W - 1;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (gy >= 0 && gy < a.h && gx >= 0 && gx < a.wd) {
-                const int ch = c + 8 * q;
-                const bf16_t* src = ch < a.c0 ? a.in0 + (size_t(img * a.h + gy) * a.wd + gx) * a.c0 + ch
-                                              : a.in1 + (size_t(img * hh + (gy >> 1)) * hw + (gx >> 1)) * a.c1 + (ch - a.c0);
-                v = *reinterpret_cast<const uint4*>(src);
+#pragma unroll
+        for (int k = 0; k < (LH_H * LH_W * 4 + 255) / 256; ++k) {
+            const int e = tid + 256 * k;
+            if (e < LH_H * LH_W * 4) {
+                const int hp = e >> 2, q = e & 3;
+                const int gy = oy + hp / LH_W - 1, gx = ox + hp % LH_W - 1;
+                uint4 v = make_uint4(0, 0, 0, 0);
+                if (gy >= 0 && gy < a.h && gx >= 0 && gx < a.wd) {
+                    const int ch = c + 8 * q;
+                    const bf16_t* src = ch < a.c0 ? a.in0 + (size_t(img * a.h + gy) * a.wd + gx) * a.c0 + ch
+                                                  : a.in1 + (size_t(img * hh + (gy >> 1)) * hw + (gx >> 1)) * a.c1 + (ch - a.c0);
+                    v = *reinterpret_cast<const uint4*>(src);
+                }
+                *reinterpret_cast<uint4*>(&hal[hp * LH_PS + 8 * q]) = v;
             }
-            *reinterpret_cast<uint4*>(&hal[hp * LH_PS + 8 * q]) = v;
         }
-        for (int e = tid; e < CT * 16 * 36; e += 256) {
-            const int co = e / 36, rem = e - co * 36, tap = rem >> 2, q = rem & 3;
-            *reinterpret_cast<uint4*>(&wls[co * LW_RS + tap * 32 + 8 * q]) =
-                *reinterpret_cast<const uint4*>(a.w + size_t(a.co0 + co) * a.kpad + tap * cin + c + 8 * q);
+#pragma unroll
+        for (int k = 0; k < (CT * 16 * 36 + 255) / 256; ++k) {
+            const int e = tid + 256 * k;
+            if (e < CT * 16 * 36) {
+                const int co = e / 36, rem = e - co * 36, tap = rem >> 2, q = rem & 3;
+                *reinterpret_cast<uint4*>(&wls[co * LW_RS + tap * 32 + 8 * q]) =
+                    *reinterpret_cast<const uint4*>(a.w + size_t(co0 + co) * a.kpad + tap * cin + c + 8 * q);
+            }
         }
         __syncthreads();
         for (int tap = 0; tap < 9; ++tap) {
@@ -267,7 +278,7 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
         if (y >= a.h || x >= a.wd) continue;
         const size_t p = size_t(img * a.h + y) * a.wd + x;
         for (int i = 0; i < CT; ++i) {
-            const int co = a.co0 + i * 16 + 4 * g;
+            const int co = co0 + i * 16 + 4 * g;
             float v[4];
             for (int j = 0; j < 4; ++j) {
                 v[j] = acc[i][t][j] + (a.bias ? a.bias[co + j] : 0.0f);
@@ -277,6 +288,92 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
             const uint32_t hi = uint32_t(f2bf(v[2])) | (uint32_t(f2bf(v[3])) << 16);
             *reinterpret_cast<uint2*>(a.out + p * a.cout + co) = make_uint2(lo, hi);
         }
+    }
+}
+
+// The first layer's convolution (8 input channels, K = 72 padded to 96): one 32-wide k step
+// is 4 taps x 8 channels, so lane group g of a fragment reads tap 4*step + g, shifted, from
+// the staged 10 x 34 pixel tile (16 B a pixel). The 3 steps sum k in k_conv's order, so the
+// outputs equal k_conv's. Persistent over tiles, the next tile staged through registers
+// while the current one runs; the weights (cout x 96) stay in LDS.
+constexpr int C8_PS = 8;             // bf16 a staged pixel
+template <int CT>
+__global__ __launch_bounds__(256) void k_conv_c8(ConvArgs a, int ntiles) {
+    __shared__ __attribute__((aligned(16))) bf16_t hal[LH_H * LH_W * C8_PS];
+    __shared__ __attribute__((aligned(16))) bf16_t wls[CT * 16 * 104];
+    constexpr int HV = (LH_H * LH_W + 255) / 256;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int g = lane >> 4, r = lane & 15;
+    const int tiles_x = (a.wd + LT_W - 1) / LT_W, tiles_y = (a.h + LT_H - 1) / LT_H;
+    int tile = int(blockIdx.x);
+    if (tile >= ntiles) return;
+    for (int e = tid; e < CT * 16 * 12; e += 256) {   // 12 pieces of 8 k a row (96 k)
+        const int co = e / 12, q = e - co * 12;
+        *reinterpret_cast<uint4*>(&wls[co * 104 + 8 * q]) = *reinterpret_cast<const uint4*>(a.w + size_t(a.co0 + co) * a.kpad + 8 * q);
+    }
+    uint4 hv[HV];
+    const auto load_halo = [&](int tl) {
+        const int bx = tl % tiles_x, q0 = tl / tiles_x, by = q0 % tiles_y, img = q0 / tiles_y;
+        for (int k = 0; k < HV; ++k) {
+            const int hp = tid + 256 * k;
+            hv[k] = make_uint4(0, 0, 0, 0);
+            if (hp >= LH_H * LH_W) continue;
+            const int gy = by * LT_H + hp / LH_W - 1, gx = bx * LT_W + hp % LH_W - 1;
+            if (gy < 0 || gy >= a.h || gx < 0 || gx >= a.wd) continue;
+            hv[k] = *reinterpret_cast<const uint4*>(a.in0 + (size_t(img * a.h + gy) * a.wd + gx) * 8);
+        }
+    };
+    const auto store_halo = [&]() {
+        for (int k = 0; k < HV; ++k) {
+            const int hp = tid + 256 * k;
+            if (hp < LH_H * LH_W) *reinterpret_cast<uint4*>(&hal[hp * C8_PS]) = hv[k];
+        }
+    };
+    load_halo(tile);
+    store_halo();
+    __syncthreads();
+    for (;;) {
+        const int ntile = tile + int(gridDim.x);
+        if (ntile < ntiles) load_halo(ntile);
+        f32x4 acc[CT][4];
+        for (int i = 0; i < CT; ++i)
+            for (int t = 0; t < 4; ++t) acc[i][t] = f32x4{0, 0, 0, 0};
+        for (int step = 0; step < 3; ++step) {
+            const int tap = 4 * step + g;
+            const int dy = tap < 9 ? tap / 3 : 0, dx = tap < 9 ? tap % 3 : 0;
+            V8 bv[4];
+            for (int t = 0; t < 4; ++t) {
+                const int hp = (2 * wv + (t >> 1) + dy) * LH_W + 16 * (t & 1) + r + dx;
+                bv[t].u = tap < 9 ? *reinterpret_cast<const uint4*>(&hal[hp * C8_PS]) : make_uint4(0, 0, 0, 0);
+            }
+            for (int i = 0; i < CT; ++i) {
+                V8 av;
+                av.u = *reinterpret_cast<const uint4*>(&wls[(i * 16 + r) * 104 + step * 32 + 8 * g]);
+                for (int t = 0; t < 4; ++t) acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av.v, bv[t].v, acc[i][t], 0, 0, 0);
+            }
+        }
+        const int bx = tile % tiles_x, q0 = tile / tiles_x, by = q0 % tiles_y, img = q0 / tiles_y;
+        for (int t = 0; t < 4; ++t) {
+            const int y = by * LT_H + 2 * wv + (t >> 1), x = bx * LT_W + 16 * (t & 1) + r;
+            if (y >= a.h || x >= a.wd) continue;
+            const size_t p = size_t(img * a.h + y) * a.wd + x;
+            for (int i = 0; i < CT; ++i) {
+                const int co = a.co0 + i * 16 + 4 * g;
+                float v[4];
+                for (int j = 0; j < 4; ++j) {
+                    v[j] = acc[i][t][j] + (a.bias ? a.bias[co + j] : 0.0f);
+                    if (a.relu && v[j] < 0.0f) v[j] = 0.0f;
+                }
+                const uint32_t lo = uint32_t(f2bf(v[0])) | (uint32_t(f2bf(v[1])) << 16);
+                const uint32_t hi = uint32_t(f2bf(v[2])) | (uint32_t(f2bf(v[3])) << 16);
+                *reinterpret_cast<uint2*>(a.out + p * a.cout + co) = make_uint2(lo, hi);
+            }
+        }
+        if (ntile >= ntiles) break;
+        __syncthreads();
+        store_halo();
+        __syncthreads();
+        tile = ntile;
     }
 }
 
@@ -522,13 +619,24 @@ __global__ __launch_bounds__(256) void k_head_bwd(const bf16_t* __restrict__ x, 
     }
     if (threadIdx.x < 34) part[size_t(blockIdx.x) * 34 + threadIdx.x] = red[threadIdx.x][0];
 }
-__global__ void k_head_reduce(const float* __restrict__ part, int nblk, float* __restrict__ ghead, float* __restrict__ loss) {
-    const int c = threadIdx.x;
-    if (c >= 34) return;
+// One block a column (32 head weights, the bias, the loss): strided partial sums, then an
+// LDS tree, in a fixed order.
+__global__ __launch_bounds__(256) void k_head_reduce(const float* __restrict__ part, int nblk, float* __restrict__ ghead,
+                                                     float* __restrict__ loss) {
+    __shared__ float red[256];
+    const int c = blockIdx.x;
     float s = 0.0f;
-    for (int b = 0; b < nblk; ++b) s += part[size_t(b) * 34 + c];
-    if (c < 33) ghead[c] = s;
-    else *loss = s;
+    for (int b = threadIdx.x; b < nblk; b += 256) s += part[size_t(b) * 34 + c];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (int(threadIdx.x) < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (c < 33) ghead[c] = red[0];
+        else *loss = red[0];
+    }
 }
 
 // Adam (Kingma and Ba, 2015) on the fp32 master parameters; c1, c2 the bias corrections.
@@ -596,16 +704,45 @@ bool conv3x3(Ctx& c, const bf16_t* in0, const bf16_t* in1, int c0, int c1, const
              int cout, int relu, int n, int h, int wd, bf16_t* out, hipStream_t s) {
     ConvArgs a{in0, in1, w, bias, out, n, h, wd, c0, c1, cout, kpad, relu, 0};
     const bool chunked = (c0 + c1) % 32 == 0 && c0 % 32 == 0;
+    if (c0 == 8 && c1 == 0 && kpad == 96 && c.conv_kernel == 0 && (cout == 16 || cout == 32 || cout == 64)) {
+        if (c.cus == 0) {
+            int cus = 0;
+            c.cus = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) == hipSuccess && cus > 0 ? cus
+                                                                                                                     : 256;
+        }
+        const int ntiles = int(size_t(n) * ((h + LT_H - 1) / LT_H) * ((wd + LT_W - 1) / LT_W));
+        const unsigned blocks = unsigned(std::min(ntiles, c.cus * 4));
+        a.co0 = 0;
+        switch (cout / 16) {
+            case 1: k_conv_c8<1><<<blocks, 256, 0, s>>>(a, ntiles); break;
+            case 2: k_conv_c8<2><<<blocks, 256, 0, s>>>(a, ntiles); break;
+            default: k_conv_c8<4><<<blocks, 256, 0, s>>>(a, ntiles); break;
+        }
+        return hip_ok(c, hipGetLastError(), "conv3x3");
+    }
     if (chunked && c.conv_kernel == 0 && (c1 == 0 || (h % 2 == 0 && wd % 2 == 0))) {
-        const unsigned blocks = unsigned(size_t(n) * ((h + LT_H - 1) / LT_H) * ((wd + LT_W - 1) / LT_W));
-        for (int co0 = 0; co0 < cout; co0 += 64) {
+        if (c.cus == 0) {
+            int cus = 0;
+            c.cus = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) == hipSuccess && cus > 0 ? cus
+                                                                                                                     : 256;
+        }
+        const int ntiles = int(size_t(n) * ((h + LT_H - 1) / LT_H) * ((wd + LT_W - 1) / LT_W));
+        // the widest channel group (<= 64) that still gives every CU two work items;
+        // channel groups of one width run in one launch, a narrower remainder in another
+        int ct = 4;
+        while (ct > 1 && (cout / (16 * ct)) * ntiles < 2 * c.cus) --ct;
+        for (int co0 = 0; co0 < cout;) {
+            const int w16 = std::min(ct, (cout - co0) / 16);
+            const int groups = (cout - co0) / (16 * w16);
+            const unsigned blocks = unsigned(ntiles * groups);
             a.co0 = co0;
-            switch (std::min(4, (cout - co0) / 16)) {
-                case 1: k_conv_lds<1><<<blocks, 256, 0, s>>>(a); break;
-                case 2: k_conv_lds<2><<<blocks, 256, 0, s>>>(a); break;
-                case 3: k_conv_lds<3><<<blocks, 256, 0, s>>>(a); break;
-                default: k_conv_lds<4><<<blocks, 256, 0, s>>>(a); break;
+            switch (w16) {
+                case 1: k_conv_lds<1><<<blocks, 256, 0, s>>>(a, ntiles); break;
+                case 2: k_conv_lds<2><<<blocks, 256, 0, s>>>(a, ntiles); break;
+                case 3: k_conv_lds<3><<<blocks, 256, 0, s>>>(a, ntiles); break;
+                default: k_conv_lds<4><<<blocks, 256, 0, s>>>(a, ntiles); break;
             }
+            co0 += groups * 16 * w16;
         }
         return hip_ok(c, hipGetLastError(), "conv3x3");
     }
@@ -793,7 +930,7 @@ bool UNet::backward(const bf16_t* x, const uint8_t* mask, const float* target, i
     k_mask_count<<<std::min(1024u, hb), 256, 0, s>>>(mask, np, cnt.as<uint32_t>());
     k_head_bwd<<<hb, 256, 0, s>>>(act[kNetLayers - 1].as<bf16_t>(), params.as<float>() + head_off, pred.as<float>(), target,
                                   mask, np, cnt.as<uint32_t>(), lvbuf[0][0].as<bf16_t>(), headpart.as<float>());
-    k_head_reduce<<<1, 64, 0, s>>>(headpart.as<float>(), int(hb), gr + head_off, lossbuf.as<float>());
+    k_head_reduce<<<34, 256, 0, s>>>(headpart.as<float>(), int(hb), gr + head_off, lossbuf.as<float>());
     // Three rotating buffers a level: the grad arriving at a layer's output (cur), its
     // pre-activation grad (cur+1) and the grad it passes to its input (cur+2).
     int cur[4] = {0, 0, 0, 0};

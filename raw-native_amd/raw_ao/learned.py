@@ -235,10 +235,55 @@ def train(ctx, vs, minutes, batch=8, crop=256, lr_max=1e-3, seed=0, val_views=8,
     return ctx.net_get_params(), hist
 
 
+def time_network(w=1920, h=1080, reps=20, batch=8, crop=256, log=print):
+    """Milliseconds of the network's forward pass on a w x h frame and of one training step
+    (batch crops of crop x crop), random weights and inputs, on the current stream."""
+    import raw_ao
+    import torch
+    ctx = raw_ao.Context(0)
+    ctx.net_set_params(init_params(0))
+    r = np.random.default_rng(0)
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream().cuda_stream
+    x = torch.from_numpy(r.integers(0, 1 << 14, (h, w, FEATURES)).astype(np.int16)).to(dev)
+    m = torch.ones((h, w), dtype=torch.uint8, device=dev)
+    ao = torch.empty((h, w), dtype=torch.float32, device=dev)
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / reps
+
+    fwd = timed(lambda: ctx.net_forward_dev(x, m, 1, w, h, ao, st))
+    xb = torch.from_numpy(r.integers(0, 1 << 14, (batch, crop, crop, FEATURES)).astype(np.int16)).to(dev)
+    mb = torch.ones((batch, crop, crop), dtype=torch.uint8, device=dev)
+    tb = torch.rand((batch, crop, crop), device=dev)
+    step = timed(lambda: ctx.net_train_step_dev(xb, mb, tb, batch, crop, crop, 1e-4, st))
+    flops = 2.0 * sum(co * 9 * (c0 + c1) * (w >> lv) * (h >> lv) for (c0, c1, co), lv in
+                      zip(LAYERS, (0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0)))
+    res = dict(width=w, height=h, forward_ms=fwd, forward_tflops=flops / (fwd * 1e-3) / 1e12, forward_gflop=flops / 1e9,
+               train_step_ms=step, train_batch="%dx%dx%d" % (batch, crop, crop))
+    log(json.dumps(res))
+    ctx.close()
+    return res
+
+
 def main(argv=None):
     import argparse
     import sys
-    ap = argparse.ArgumentParser(description="render training views with the method and train the learned AO network")
+    if argv is None:
+        argv = sys.argv[1:]
+    if argv and argv[0] == "--time":
+        rest = [int(v) for v in argv[1:4]]
+        time_network(*rest)
+        return 0
+    ap = argparse.ArgumentParser(description="render training views with the method and train the learned AO network "
+                                             "(or --time [W H REPS]: time the network's forward pass and a training step)")
     ap.add_argument("--mode", choices=["surrogate", "denoiser"], default="surrogate")
     ap.add_argument("--scenes", type=int, default=40)
     ap.add_argument("--views", type=int, default=24, help="views per scene")

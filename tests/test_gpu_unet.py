@@ -50,6 +50,9 @@ CONV_CASES = [  # n, h, w, c0, c1, cout, relu
     (1, 6, 10, 96, 0, 224, 0),
     (3, 5, 7, 16, 0, 48, 0),
     (1, 16, 16, 128, 0, 128, 1),
+    (2, 11, 37, 8, 0, 32, 1),
+    (1, 24, 40, 8, 0, 64, 0),
+    (3, 20, 72, 64, 0, 128, 1),
 ]
 
 
@@ -363,5 +366,17 @@ def test_conv_kernels_agree(ctx):
             gap = np.abs(outs[0] - outs[1])
             assert (gap <= 2.0 ** -7 * np.abs(outs[1]) + 1e-4).all()
             assert (gap == 0).mean() > 0.9
+        # the first layer's kernel (8 channels) sums in the general kernel's order
+        x0 = bf16_case(r, (2, 13, 35, 8))
+        wt = np.zeros((32, 96), np.uint16)
+        wt[:, :72] = bf16_case(r, (32, 72), 0.2)
+        outs = {}
+        for kern in (0, 1):
+            ctx.set_option(raw_ao.OPT_CONV_KERNEL, kern)
+            out = torch.zeros(2 * 13 * 35 * 32, dtype=torch.int16, device="cuda")
+            kp = [dev(x0), dev(wt)]
+            ctx.conv3x3_dev(kp[0], None, 8, 0, kp[1], 96, None, 32, 1, 2, 35, 13, out)
+            outs[kern] = host16(out).copy()
+        assert np.array_equal(outs[0], outs[1])
     finally:
         ctx.set_option(raw_ao.OPT_CONV_KERNEL, 0)
