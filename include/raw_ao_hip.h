@@ -101,6 +101,20 @@ int raw_ao_trace_ao(raw_ao_ctx* ctx, const float* position, const float* normal,
 int raw_ao_trace_ao_dev(raw_ao_ctx* ctx, const float* d_position, const float* d_normal, const uint8_t* d_mask,
                         int w, int h, int spp, float radius, float* d_ao, void* stream);
 
+/* ---- lighting model 2's visibility ---------------------------------------------------
+ * Replaces computeRTVisibility (raw/renderer/ray_ao.hpp:12-14, src/renderer/ray_ao.cpp:18-63),
+ * the pass the default render runs (src/renderer/render.cpp:31-36): ao exactly as
+ * raw_ao_trace_ao; sky = the fraction of the samples open within radius that also hit
+ * nothing with tMax 1e30; sun = 1 when the ray from the AO origin towards
+ * normalize(to_light) hits nothing with tMax 1e30, or when sun_forced (the no-sun control);
+ * all three 1.0 off the mask. Needs the wide traversal form. */
+int raw_ao_trace_visibility(raw_ao_ctx* ctx, const float* position, const float* normal, const uint8_t* mask,
+                            int w, int h, int spp, float radius, const float to_light[3], int sun_forced,
+                            float* ao, float* sky, float* sun);
+int raw_ao_trace_visibility_dev(raw_ao_ctx* ctx, const float* d_position, const float* d_normal, const uint8_t* d_mask,
+                                int w, int h, int spp, float radius, const float to_light[3], int sun_forced,
+                                float* d_ao, float* d_sky, float* d_sun, void* stream);
+
 /* ---- screen-space AO ---------------------------------------------------------------
  * Replaces computeSSAO (raw/renderer/ssao.hpp:5-6, src/renderer/ssao.cpp:11-39). */
 int raw_ao_ssao(raw_ao_ctx* ctx, const float* position, const float* normal, const uint8_t* mask,

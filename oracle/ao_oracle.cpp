@@ -747,6 +747,40 @@ void orc_rtao(void* bvh, const float* tris9, size_t ntri, int mode,
     });
 }
 
+void orc_rtvis(void* bvh, const float* tris9, size_t ntri, int mode, const float* position, const float* normal,
+               const uint8_t* mask, int w, int h, int samples, float radius, const float to_light[3], int sun_forced,
+               int threads, float* ao, float* sky, float* sun) {
+    // src/renderer/ray_ao.cpp:20-50 (the shared loop with sky and sun set), :59-63 (toLight normalized)
+    const Bvh* B = static_cast<const Bvh*>(bvh);
+    constexpr float kFar = 1e30f;
+    const V3 L = nrm(ld3(to_light));
+    const auto occluded = [&](V3 o, V3 d, float tmax) {
+        if (mode == 1) return scanOccluded(tris9, ntri, o, d, tmax);
+        return B->occluded(o, d, mode == 2 ? tmax * 0.99f : tmax);
+    };
+    rows(h, threads, [&](int y) {
+        for (int x = 0; x < w; ++x) {
+            const size_t i = size_t(y) * size_t(w) + size_t(x);
+            if (!mask[i]) { ao[i] = 1.0f; sky[i] = 1.0f; sun[i] = 1.0f; continue; }
+            const V3 p = ld3(position + 3 * i), n = ld3(normal + 3 * i);
+            V3 t, b;
+            basis(n, t, b);
+            const V3 origin = add(p, scl(n, 1e-3f));
+            int open = 0, free = 0;
+            for (int s = 0; s < samples; ++s) {
+                const V3 dir = aoDir(uint32_t(x), uint32_t(y), uint32_t(s), n, t, b);
+                if (!occluded(origin, dir, radius)) {
+                    ++open;
+                    if (!occluded(origin, dir, kFar)) ++free;
+                }
+            }
+            ao[i] = (float)open / (float)samples;
+            sky[i] = (float)free / (float)samples;
+            sun[i] = (sun_forced || !occluded(origin, L, kFar)) ? 1.0f : 0.0f;
+        }
+    });
+}
+
 void orc_ssao(const float* position, const float* normal, const uint8_t* mask,
               int w, int h, int samples, float radius, int threads, float* ao) {
     const int R = 6;

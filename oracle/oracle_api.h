@@ -53,6 +53,16 @@ void orc_rtao(void* bvh, const float* tris9, size_t ntri, int mode,
               int w, int h, int y0, int y1, int samples, float radius, int threads,
               float* ao);
 
+/* computeRTVisibility (src/renderer/ray_ao.cpp:18-63), lighting model 2's visibility: ao
+ * exactly as computeRTAO; sky = the fraction of the samples whose ray is open within radius
+ * and also hits nothing with tMax 1e30; sun = 1 when the ray from the AO origin towards
+ * normalize(to_light) hits nothing with tMax 1e30, or when sun_forced; all 1 off the mask.
+ * mode as orc_rtao. */
+void orc_rtvis(void* bvh, const float* tris9, size_t ntri, int mode,
+               const float* position, const float* normal, const uint8_t* mask,
+               int w, int h, int samples, float radius, const float to_light[3], int sun_forced,
+               int threads, float* ao, float* sky, float* sun);
+
 /* computeSSAO (src/renderer/ssao.cpp:11-39). */
 void orc_ssao(const float* position, const float* normal, const uint8_t* mask,
               int w, int h, int samples, float radius, int threads, float* ao);

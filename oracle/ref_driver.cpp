@@ -102,6 +102,23 @@ void ref_rtao(const float* tris9, size_t ntri, int mode, const float* position, 
     std::memcpy(ao, out.px.data(), sizeof(float) * size_t(w) * size_t(h));
 }
 
+// computeRTVisibility (raw/renderer/ray_ao.hpp:12-14) through LinearAccel, BVH mode.
+void ref_rtvis(const float* tris9, size_t ntri, const float* position, const float* normal, const uint8_t* mask, int w,
+               int h, int samples, float radius, const float to_light[3], int sun_forced, int threads, float* ao,
+               float* sky, float* sun) {
+    unsetenv("RAW_NATIVE_AO_ORACLE");
+    raw::LinearAccel accel;
+    accel.build(triScene(tris9, ntri));
+    const raw::GBuffer g = toG(position, normal, mask, w, h);
+    raw::Buffer<float> a, s, u;
+    raw::computeRTVisibility(g, accel, samples, radius, raw::Vec3{to_light[0], to_light[1], to_light[2]}, a, s, u, threads,
+                             sun_forced != 0);
+    const size_t np = size_t(w) * size_t(h);
+    std::memcpy(ao, a.px.data(), sizeof(float) * np);
+    std::memcpy(sky, s.px.data(), sizeof(float) * np);
+    std::memcpy(sun, u.px.data(), sizeof(float) * np);
+}
+
 // The same, with the accelerator built once (LinearAccel::build, accel.cpp:13-25) and kept,
 // so a caller can time the build and the trace apart (BVH mode only).
 void* ref_accel_build(const float* tris9, size_t ntri) {

@@ -485,6 +485,77 @@ __global__ __launch_bounds__(BT) void k_trace_rays4(const float* __restrict__ ra
 uint32_t cdiv(uint64_t a, uint32_t b) { return uint32_t((a + b - 1) / b); }
 
 
+// Lighting model 2's visibility (computeRTVisibility, src/renderer/ray_ao.cpp:20-63): the AO
+// rays of k_trace_ao4, and each open one traced again with tMax 1e30 for the sky.
+template <bool SPP64>
+__global__ __launch_bounds__(BT) void k_trace_vis4(TraceArgs4 a, float* __restrict__ sky, uint32_t* __restrict__ cnt_free) {
+    __shared__ uint32_t stk[STACK_LDS * BT];
+    constexpr float kFarT = 1e30f;
+    const uint64_t gid = uint64_t(blockIdx.x) * BT + threadIdx.x;
+    const uint32_t pix = SPP64 ? uint32_t(gid >> 6) : uint32_t(gid / a.spp);
+    if (pix >= a.npix) return;
+    const uint32_t s = uint32_t(gid - uint64_t(pix) * a.spp);
+    if (!a.mask[pix]) {
+        if (s == 0) {
+            a.ao[pix] = 1.0f;
+            sky[pix] = 1.0f;
+        }
+        return;
+    }
+    const V3 p = v3(a.pos[3 * size_t(pix)], a.pos[3 * size_t(pix) + 1], a.pos[3 * size_t(pix) + 2]);
+    const V3 n = v3(a.nrm[3 * size_t(pix)], a.nrm[3 * size_t(pix) + 1], a.nrm[3 * size_t(pix) + 2]);
+    const uint32_t y = pix / a.w, x = pix - y * a.w;
+    V3 t, b;
+    basis(n, t, b);
+    const V3 origin = add(p, scl(n, 1e-3f));
+    const uint32_t k1 = hash24(x, y, 2 * s), k2 = hash24(x, y, 2 * s + 1);
+    const float u1 = float(k1) / float(0x1000000), u2 = float(k2) / float(0x1000000);
+    float sphi, cphi;
+    sincos_base(6.2831853f * u2, sphi, cphi);
+    if (a.trig) trig_fix(a.trig, k2, sphi, cphi);
+    const V3 dir = ao_dir(u1, sphi, cphi, n, t, b);
+    bool occ = false, far_occ = false;
+    if (a.ntri) {
+        const Ray r = make_ray(origin, dir);
+        uint32_t visits = 0, tests = 0;
+        const int res = occluded4<false>(r, a.radius, a.nodes, a.tris, a.ntri, stk + threadIdx.x, a.tri_q, visits, tests);
+        occ = res < 0 ? scan_occluded(r, a.radius, a.tris, a.ntri) : res == 1;
+        // blocked within radius: blocked for the sky too
+        const int res2 = occluded4<false>(r, kFarT, a.nodes, a.tris, a.ntri, stk + threadIdx.x, a.tri_q, visits, tests, !occ);
+        far_occ = occ || (res2 < 0 ? scan_occluded(r, kFarT, a.tris, a.ntri) : res2 == 1);
+    }
+    if (SPP64) {
+        const unsigned long long open = __ballot(!occ), free = __ballot(!far_occ);
+        if (s == 0) {
+            a.ao[pix] = float(__popcll(open)) / 64.0f;
+            sky[pix] = float(__popcll(free)) / 64.0f;
+        }
+    } else {
+        if (!occ) atomicAdd(&a.cnt[pix], 1u);
+        if (!far_occ) atomicAdd(&cnt_free[pix], 1u);
+    }
+}
+
+// The sun's visibility: one ray a covered pixel from the AO origin towards
+// normalize(to_light) with tMax 1e30 (src/renderer/ray_ao.cpp:49, :62); 1 off the mask.
+__global__ __launch_bounds__(BT) void k_sun4(TraceArgs4 a, float lx, float ly, float lz, int forced, float* __restrict__ sun) {
+    __shared__ uint32_t stk[STACK_LDS * BT];
+    const uint32_t pix = blockIdx.x * BT + threadIdx.x;
+    if (pix >= a.npix) return;
+    if (!a.mask[pix] || forced || a.ntri == 0) {
+        sun[pix] = 1.0f;
+        return;
+    }
+    const V3 p = v3(a.pos[3 * size_t(pix)], a.pos[3 * size_t(pix) + 1], a.pos[3 * size_t(pix) + 2]);
+    const V3 n = v3(a.nrm[3 * size_t(pix)], a.nrm[3 * size_t(pix) + 1], a.nrm[3 * size_t(pix) + 2]);
+    const V3 origin = add(p, scl(n, 1e-3f));
+    const Ray r = make_ray(origin, normalize(v3(lx, ly, lz)));
+    uint32_t visits = 0, tests = 0;
+    const int res = occluded4<false>(r, 1e30f, a.nodes, a.tris, a.ntri, stk + threadIdx.x, a.tri_q, visits, tests);
+    const bool occ = res < 0 ? scan_occluded(r, 1e30f, a.tris, a.ntri) : res == 1;
+    sun[pix] = occ ? 0.0f : 1.0f;
+}
+
 // ---- batched AO: one persistent launch over many views ------------------------------------
 __global__ __launch_bounds__(BT) void k_batch_clear(BatchDev d) {
     const uint32_t stride = gridDim.x * BT;
@@ -713,6 +784,36 @@ int batch_trace_enqueue(const BatchDev& d, uint32_t persistent_blocks, hipStream
     k_batch_trace<<<persistent_blocks, BT, 0, s>>>(d);
     k_batch_finish<<<dim3(256, d.nviews, 1), BT, 0, s>>>(d);
     return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int trace_vis4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int spp, float radius,
+               const float to_light[3], int sun_forced, float* d_ao, float* d_sky, float* d_sun, hipStream_t s) {
+    const uint32_t npix = uint32_t(w) * uint32_t(h);
+    if (!ensure(c, c.counters, sizeof(TraceCounters))) return -2;
+    if (!hip_ok(c, hipMemsetAsync(c.counters.p, 0, sizeof(TraceCounters), s), "counter clear")) return -2;
+    TraceArgs4 a{d_pos, d_nrm, d_mask, d_ao, nullptr, c.nodes4.as<Node4>(), c.tris_sorted.as<float4>(),
+                 c.host_trig ? c.d_trig : nullptr, c.counters.as<TraceCounters>(), uint32_t(w), npix, uint32_t(spp),
+                 c.ntri, uint32_t(c.tri_q), radius};
+    const bool spp64 = spp == 64;
+    uint32_t* cnt_free = nullptr;
+    if (!spp64) {
+        if (!ensure(c, c.st_cnt, 8 * size_t(npix))) return -2;
+        a.cnt = c.st_cnt.as<uint32_t>();
+        cnt_free = a.cnt + npix;
+        if (!hip_ok(c, hipMemsetAsync(a.cnt, 0, 8 * size_t(npix), s), "count clear")) return -2;
+    }
+    const uint32_t blocks = cdiv(uint64_t(npix) * uint64_t(spp), BT);
+    (void)hipEventRecord(c.ev[2], s);
+    if (spp64) k_trace_vis4<true><<<blocks, BT, 0, s>>>(a, d_sky, nullptr);
+    else k_trace_vis4<false><<<blocks, BT, 0, s>>>(a, d_sky, cnt_free);
+    k_sun4<<<cdiv(npix, BT), BT, 0, s>>>(a, to_light[0], to_light[1], to_light[2], sun_forced, d_sun);
+    (void)hipEventRecord(c.ev[3], s);
+    if (!spp64) {
+        if (ao_finish(c, d_mask, a.cnt, npix, uint32_t(spp), d_ao, s) || ao_finish(c, d_mask, cnt_free, npix, uint32_t(spp), d_sky, s))
+            return -2;
+    }
+    c.have_counts = false;
+    return hip_ok(c, hipGetLastError(), "visibility launch") ? 0 : -2;
 }
 
 }  // namespace raw_ao

@@ -265,6 +265,55 @@ int raw_ao_trace_ao(raw_ao_ctx* x, const float* pos, const float* nrm, const uin
     return RAW_AO_OK;
 }
 
+int raw_ao_trace_visibility_dev(raw_ao_ctx* x, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h,
+                                int spp, float radius, const float to_light[3], int sun_forced, float* d_ao, float* d_sky,
+                                float* d_sun, void* stream) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    if (!image_ok(w, h) || !d_pos || !d_nrm || !d_mask || !d_ao || !d_sky || !d_sun || !to_light)
+        return fail(c, RAW_AO_ERR_ARG, "bad image arguments");
+    if (spp < 1 || spp > 4096) return fail(c, RAW_AO_ERR_ARG, "spp must be in 1..4096");
+    if (!(radius > 0.0f)) return fail(c, RAW_AO_ERR_ARG, "radius must be > 0");
+    if (c.ntri && c.root < 0) return fail(c, RAW_AO_ERR_ARG, "no BVH: call raw_ao_build_ploc first");
+    if (c.ntri && c.built_kernel != RAW_AO_TRACE_WIDE)
+        return fail(c, RAW_AO_ERR_ARG, "the visibility pass needs the wide traversal form: build with RAW_AO_TRACE_WIDE");
+    if (c.host_trig) {
+        const int r = raw_ao::trig_table_upload(c);
+        if (r) return r;
+    }
+    return code_of(raw_ao::trace_vis4(c, d_pos, d_nrm, d_mask, w, h, spp, radius, to_light, sun_forced, d_ao, d_sky, d_sun,
+                                      pick(c, stream)));
+}
+
+int raw_ao_trace_visibility(raw_ao_ctx* x, const float* pos, const float* nrm, const uint8_t* mask, int w, int h, int spp,
+                            float radius, const float to_light[3], int sun_forced, float* ao, float* sky, float* sun) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    if (!image_ok(w, h) || !pos || !nrm || !mask || !ao || !sky || !sun) return fail(c, RAW_AO_ERR_ARG, "bad image arguments");
+    const size_t np = size_t(w) * size_t(h);
+    if (!ensure(c, c.st_pos, 12 * np) || !ensure(c, c.st_nrm, 12 * np) || !ensure(c, c.st_mask, np) ||
+        !ensure(c, c.st_out, 12 * np))
+        return RAW_AO_ERR_DEVICE;
+    hipStream_t s = c.stream;
+    if (!hip_ok(c, hipMemcpyAsync(c.st_pos.p, pos, 12 * np, hipMemcpyHostToDevice, s), "upload") ||
+        !hip_ok(c, hipMemcpyAsync(c.st_nrm.p, nrm, 12 * np, hipMemcpyHostToDevice, s), "upload") ||
+        !hip_ok(c, hipMemcpyAsync(c.st_mask.p, mask, np, hipMemcpyHostToDevice, s), "upload"))
+        return RAW_AO_ERR_DEVICE;
+    float* d = c.st_out.as<float>();
+    const int r = raw_ao_trace_visibility_dev(x, c.st_pos.as<float>(), c.st_nrm.as<float>(), c.st_mask.as<uint8_t>(), w, h, spp,
+                                              radius, to_light, sun_forced, d, d + np, d + 2 * np, nullptr);
+    if (r) return r;
+    if (!hip_ok(c, hipMemcpyAsync(ao, d, 4 * np, hipMemcpyDeviceToHost, s), "download") ||
+        !hip_ok(c, hipMemcpyAsync(sky, d + np, 4 * np, hipMemcpyDeviceToHost, s), "download") ||
+        !hip_ok(c, hipMemcpyAsync(sun, d + 2 * np, 4 * np, hipMemcpyDeviceToHost, s), "download") ||
+        !hip_ok(c, hipStreamSynchronize(s), "visibility"))
+        return RAW_AO_ERR_DEVICE;
+    c.ms_trace = elapsed(c, 2, 3);
+    return RAW_AO_OK;
+}
+
 int raw_ao_ssao_dev(raw_ao_ctx* x, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int samples,
                     float radius, float* d_ao, void* stream) {
     if (!x) return RAW_AO_ERR_ARG;

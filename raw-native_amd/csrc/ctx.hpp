@@ -105,6 +105,8 @@ int ploc_fixed_check(Ctx& c);
 int wide_build_fixed(Ctx& c, hipStream_t s);
 int trace_ao4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int spp,
               float radius, float* d_ao, hipStream_t s);
+int trace_vis4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int spp, float radius,
+               const float to_light[3], int sun_forced, float* d_ao, float* d_sky, float* d_sun, hipStream_t s);
 int trace_rays4(Ctx& c, const float* d_rays, uint32_t n, int any, float far_scale, float* d_out, hipStream_t s);
 int trace_rays(Ctx& c, const float* d_rays, uint32_t n, int any, float far_scale, float* d_out, hipStream_t s);
 int ssao(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int samples,

@@ -55,6 +55,17 @@ Buffer<float> computeRTAO(const GBuffer& g, const LinearAccel& accel, int sample
     return ao;
 }
 
+void computeRTVisibility(const GBuffer& g, const LinearAccel& accel, int samples, float radius, Vec3 toLight, Buffer<float>& ao,
+                         Buffer<float>& sky, Buffer<float>& sun, int, bool sunForced) {
+    if (!accel.built) throw std::runtime_error("raw_ao_hip: LinearAccel not built");
+    ao.resize(g.w, g.h);
+    sky.resize(g.w, g.h);
+    sun.resize(g.w, g.h);
+    const float L[3] = {toLight.x, toLight.y, toLight.z};   // normalized on the device, as the reference does
+    check(raw_ao_trace_visibility(device(), fl(g.position.px), fl(g.normal.px), g.mask.px.data(), g.w, g.h, samples, radius, L,
+                                  sunForced ? 1 : 0, ao.px.data(), sky.px.data(), sun.px.data()));
+}
+
 Buffer<float> computeSSAO(const GBuffer& g, int samples, float radius, Arena*, int) {
     Buffer<float> ao;
     ao.resize(g.w, g.h);

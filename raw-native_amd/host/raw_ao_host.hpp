@@ -1,6 +1,7 @@
 // The C++ host side of raw_ao_hip: the reference's AO/reconcile interface over the C ABI
 // of include/raw_ao_hip.h, with the reference's types and semantics, so a caller of
 //   raw::computeRTAO   (raw/renderer/ray_ao.hpp:6-8)
+//   raw::computeRTVisibility (raw/renderer/ray_ao.hpp:12-14)
 //   raw::computeSSAO   (raw/renderer/ssao.hpp:5-6)
 //   raw::reconcile     (raw/cert/reconcile.hpp:12-13)
 //   raw::LinearAccel   (raw/renderer/accel.hpp:15-23)
@@ -48,6 +49,11 @@ struct LinearAccel {
 
 Buffer<float> computeRTAO(const GBuffer& g, const LinearAccel& accel, int samples, float radius, Arena* arena = nullptr,
                           int threads = 1);
+// Lighting model 2's visibility from the same rays (raw/renderer/ray_ao.hpp:12-14): ao as
+// computeRTAO, sky the fraction of samples with no hit at any distance, sun 1 when a ray
+// towards toLight hits nothing (forced to 1 when sunForced); 1 off the mask.
+void computeRTVisibility(const GBuffer& g, const LinearAccel& accel, int samples, float radius, Vec3 toLight,
+                         Buffer<float>& ao, Buffer<float>& sky, Buffer<float>& sun, int threads = 1, bool sunForced = false);
 Buffer<float> computeSSAO(const GBuffer& g, int samples, float radius, Arena* arena = nullptr, int threads = 1);
 
 struct ReconcileResult {

@@ -48,6 +48,10 @@ SIGNATURES = [
     ("raw_ao_read_keys", C.c_int, [_vp, _u32p, _u32p]),
     ("raw_ao_trace_ao", C.c_int, [_vp, _f32p, _f32p, _u8p, C.c_int, C.c_int, C.c_int, C.c_float, _f32p]),
     ("raw_ao_trace_ao_dev", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp]),
+    ("raw_ao_trace_visibility", C.c_int, [_vp, _f32p, _f32p, _u8p, C.c_int, C.c_int, C.c_int, C.c_float, _f32p, C.c_int,
+                                          _f32p, _f32p, _f32p]),
+    ("raw_ao_trace_visibility_dev", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _f32p, C.c_int,
+                                              _vp, _vp, _vp, _vp]),
     ("raw_ao_ssao", C.c_int, [_vp, _f32p, _f32p, _u8p, C.c_int, C.c_int, C.c_int, C.c_float, _f32p]),
     ("raw_ao_ssao_dev", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp]),
     ("raw_ao_error_map", C.c_int, [_vp, _f32p, _f32p, _u8p, C.c_int, C.c_int, _f32p, C.POINTER(C.c_float)]),
@@ -209,6 +213,22 @@ class Context:
     def compute_rtao_dev(self, position, normal, mask, w, h, samples, radius, out, stream=None):
         self._check(lib().raw_ao_trace_ao_dev(self._h, _ptr(position), _ptr(normal), _ptr(mask), w, h, samples, radius,
                                               _ptr(out), _vp(stream)))
+
+    def compute_rt_visibility(self, position, normal, mask, to_light, samples=64, radius=2.0, sun_forced=False):
+        """computeRTVisibility: (ao, sky, sun)."""
+        mask = np.ascontiguousarray(mask, np.uint8)
+        h, w = mask.shape
+        out = [np.zeros((h, w), np.float32) for _ in range(3)]
+        self._check(lib().raw_ao_trace_visibility(self._h, np.ascontiguousarray(position, np.float32),
+                                                  np.ascontiguousarray(normal, np.float32), mask, w, h, samples, radius,
+                                                  np.asarray(to_light, np.float32), int(bool(sun_forced)), *out))
+        return tuple(out)
+
+    def compute_rt_visibility_dev(self, position, normal, mask, w, h, samples, radius, to_light, ao, sky, sun,
+                                  sun_forced=False, stream=None):
+        self._check(lib().raw_ao_trace_visibility_dev(self._h, _ptr(position), _ptr(normal), _ptr(mask), w, h, samples,
+                                                      radius, np.asarray(to_light, np.float32), int(bool(sun_forced)),
+                                                      _ptr(ao), _ptr(sky), _ptr(sun), _vp(stream)))
 
     def compute_ssao(self, position, normal, mask, samples=24, radius=2.0):
         mask = np.ascontiguousarray(mask, np.uint8)

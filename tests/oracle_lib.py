@@ -45,6 +45,8 @@ def lib():
         L.orc_rtao.argtypes = [C.c_void_p, f32p, SZ, C.c_int, f32p, f32p, u8p, C.c_int, C.c_int, C.c_int, C.c_int,
                                C.c_int, C.c_float, C.c_int, f32p]
         L.orc_ssao.argtypes = [f32p, f32p, u8p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, f32p]
+        L.orc_rtvis.argtypes = [C.c_void_p, f32p, SZ, C.c_int, f32p, f32p, u8p, C.c_int, C.c_int, C.c_int, C.c_float,
+                                f32p, C.c_int, C.c_int, f32p, f32p, f32p]
         L.orc_reconcile.argtypes = [f32p, f32p, u8p, C.c_int, C.c_int, C.c_float, f32p, C.POINTER(C.c_float),
                                     C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         L.orc_ploc_build.argtypes = [f32p, SZ, C.c_int, f32p, i32p, C.POINTER(C.c_uint32)]
@@ -87,6 +89,8 @@ def ref():
         L.ref_accel_build.restype = C.c_void_p
         L.ref_accel_free.argtypes = [C.c_void_p]
         L.ref_rtao_accel.argtypes = [C.c_void_p, f32p, f32p, u8p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, f32p]
+        L.ref_rtvis.argtypes = [f32p, SZ, f32p, f32p, u8p, C.c_int, C.c_int, C.c_int, C.c_float, f32p, C.c_int, C.c_int,
+                                f32p, f32p, f32p]
         _ref = L
     return _ref
 
@@ -148,6 +152,21 @@ class OracleBvh:
                        np.ascontiguousarray(nrm, np.float32), np.ascontiguousarray(mask, np.uint8), w, h, y0, y1,
                        samples, radius, threads or os.cpu_count() or 1, ao)
         return ao
+
+
+def rtvis(tris9, pos, nrm, mask, to_light, samples=64, radius=2.0, sun_forced=False, mode=0, threads=None, impl=None):
+    """computeRTVisibility: (ao, sky, sun); impl=ref() runs the reference's own code (BVH mode)."""
+    tris9 = np.ascontiguousarray(tris9, np.float32).reshape(-1, 9)
+    h, w = mask.shape
+    out = [np.zeros((h, w), np.float32) for _ in range(3)]
+    args = (np.ascontiguousarray(pos, np.float32), np.ascontiguousarray(nrm, np.float32), np.ascontiguousarray(mask, np.uint8),
+            w, h, samples, radius, np.asarray(to_light, np.float32), int(bool(sun_forced)), threads or os.cpu_count() or 1)
+    if impl is not None:
+        impl.ref_rtvis(tris9, tris9.shape[0], *args, *out)
+    else:
+        b = OracleBvh(tris9)
+        lib().orc_rtvis(b.h, b.tris, b.tris.shape[0], mode, *args, *out)
+    return tuple(out)
 
 
 def rtao(tris9, pos, nrm, mask, samples=64, radius=2.0, mode=0, threads=None):

@@ -203,3 +203,93 @@ def test_threads_do_not_change_bytes():
     _, _, _, a1, s1 = render("default", threads=1)
     _, _, _, a5, s5 = render("default", threads=5)
     assert np.array_equal(a1, a5) and np.array_equal(s1, s5)
+
+
+# ---- lighting model 2's visibility (computeRTVisibility, src/renderer/ray_ao.cpp:18-63) ----
+
+def _default_frame(w=96, h=96):
+    _, pos, nrm, mask = O.cli_gbuffer(w, h)
+    vpos, _, idx = O.test_scene_mesh()
+    return O.mesh_tris(vpos, idx), pos, nrm, mask
+
+
+@pytest.mark.parametrize("to_light,forced", [((0.3, 1.0, 0.2), False), ((-1.0, 0.4, -0.5), False),
+                                             ((0.0, 1.0, 0.0), True)])
+def test_rtvis_ao_equals_rtao_and_scan(to_light, forced):
+    """The visibility pass's AO is computeRTAO's (the same rays); BVH == scan for all three planes."""
+    tris, pos, nrm, mask = _default_frame()
+    ao, sky, sun = O.rtvis(tris, pos, nrm, mask, to_light, 64, 2.0, forced)
+    assert np.array_equal(ao, O.rtao(tris, pos, nrm, mask, 64, 2.0))
+    ao1, sky1, sun1 = O.rtvis(tris, pos, nrm, mask, to_light, 64, 2.0, forced, mode=1)
+    assert np.array_equal(ao, ao1) and np.array_equal(sky, sky1) and np.array_equal(sun, sun1)
+    assert (sky <= ao).all()                       # blocked within radius: blocked for the sky too
+    assert (sun[mask == 0] == 1).all() and (sky[mask == 0] == 1).all()
+    if forced:
+        assert (sun == 1).all()
+    else:
+        assert set(np.unique(sun)) <= {0.0, 1.0} and (sun[mask != 0] == 0).any()
+
+
+def test_rtvis_closed_forms():
+    """Under a plane at height 0.5 every sky ray is blocked (sky 0) and a sun straight up is
+    shadowed; with nothing above, sky = AO = 1 and the sun is lit."""
+    h = w = 32
+    pos = np.zeros((h, w, 3), np.float32)
+    pos[..., 1] = np.float32(-1e-3)
+    nrm = np.zeros_like(pos)
+    nrm[..., 1] = 1
+    mask = np.ones((h, w), np.uint8)
+    roof = np.array([[-1e3, 0.5, -1e3, 1e3, 0.5, 1e3, 1e3, 0.5, -1e3], [-1e3, 0.5, -1e3, -1e3, 0.5, 1e3, 1e3, 0.5, 1e3]],
+                    np.float32)
+    ao, sky, sun = O.rtvis(roof, pos, nrm, mask, (0, 1, 0), 64, 1.0)
+    assert (sky == 0).all() and (sun == 0).all() and 0 < ao.mean() < 1
+    floor = np.array([[-1e3, -1.0, -1e3, 1e3, -1.0, -1e3, 1e3, -1.0, 1e3]], np.float32)
+    ao, sky, sun = O.rtvis(floor, pos, nrm, mask, (0.2, 1, 0.1), 64, 1.0)
+    assert (ao == 1).all() and (sky == 1).all() and (sun == 1).all()
+
+
+# ---- the oracle against the reference's own code (oracle/_ref, built from its sources) ----
+
+needs_ref = pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (needs the reference sources)")
+
+
+@needs_ref
+@pytest.mark.parametrize("to_light,forced", [((0.3, 1.0, 0.2), False), ((-1.0, 0.4, -0.5), False),
+                                             ((0.0, 1.0, 0.0), True)])
+def test_rtvis_matches_reference_code(to_light, forced):
+    tris, pos, nrm, mask = _default_frame()
+    mine = O.rtvis(tris, pos, nrm, mask, to_light, 64, 2.0, forced)
+    ref = O.rtvis(tris, pos, nrm, mask, to_light, 64, 2.0, forced, impl=O.ref())
+    for a, b in zip(mine, ref):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@needs_ref
+def test_rtvis_matches_reference_code_seeded_scene():
+    tris = O.scene_tris("dense_block", 3000, 7)
+    lo, hi = tris.reshape(-1, 3).min(0), tris.reshape(-1, 3).max(0)
+    c = (lo + hi) / 2
+    eye = c + (hi - lo) * np.float32(1.2)
+    vpos = tris.reshape(-1, 3)
+    idx = np.arange(vpos.shape[0], dtype=np.int32).reshape(-1, 3)
+    e0, e1 = vpos[idx[:, 1]] - vpos[idx[:, 0]], vpos[idx[:, 2]] - vpos[idx[:, 0]]
+    fn = np.cross(e0, e1)
+    fn /= np.maximum(np.linalg.norm(fn, axis=1, keepdims=True), 1e-20)
+    vnrm = np.repeat(fn, 3, axis=0).astype(np.float32)
+    _, pos, nrm, mask = O.rasterize(vpos, vnrm, idx, eye, c, (0, 1, 0), 0.9, 1.0, 0.1, 1000.0, 64, 64)
+    mine = O.rtvis(tris, pos, nrm, mask, (0.4, 1, 0.3), 32, 1.0)
+    ref = O.rtvis(tris, pos, nrm, mask, (0.4, 1, 0.3), 32, 1.0, impl=O.ref())
+    for a, b in zip(mine, ref):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@needs_ref
+def test_rtao_ssao_match_reference_code():
+    tris, pos, nrm, mask = _default_frame(128, 96)
+    R = O.ref()
+    ref_ao = np.zeros(mask.shape, np.float32)
+    R.ref_rtao(tris, tris.shape[0], 0, pos, nrm, mask, 128, 96, 64, 2.0, 4, ref_ao)
+    assert np.array_equal(O.rtao(tris, pos, nrm, mask, 64, 2.0).view(np.uint32), ref_ao.view(np.uint32))
+    ref_ss = np.zeros(mask.shape, np.float32)
+    R.ref_ssao(pos, nrm, mask, 128, 96, 24, 2.0, 4, ref_ss)
+    assert np.array_equal(O.ssao(pos, nrm, mask, 24, 2.0).view(np.uint32), ref_ss.view(np.uint32))
