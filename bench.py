@@ -5,7 +5,8 @@ Workload (a step is one frame): the seeded ~1M-triangle scene (2,000 boxes and
 icospheres over a 40 x 40 ground quad), G-buffer at 1920 x 1080 from the host rasterizer
 (uploaded once, resident in HBM before timing), then per frame on the GPU: PLOC BVH
 build, ray-traced AO at 64 rays/pixel with radius 1.0, SSAO at 24 samples, the
-reconcile's error map, and the reconcile's float64 sum on the host.
+reconcile's error map, and the reconcile's float64 sum on the host (which overlaps the next
+frame's GPU work; every frame is complete inside the timed region).
 
 value = AO rays per second of whole frames (covered pixels x spp x steps / timed wall
 time). Also reported: the trace kernel's own ray rate, per-kernel milliseconds, the trace
@@ -281,6 +282,8 @@ def main():
                     "(the known-size kernel that scales the HBM counters)")
     ap.add_argument("--kernel", choices=["wide", "binary"], default="wide", help="traversal form")
     ap.add_argument("--tri-q", type=int, default=0, help="wide kernel's triangle-phase threshold (1..64; 0: default)")
+    ap.add_argument("--refill", type=int, default=-1, help="64-spp kernel: refill lanes at this many idle (0 off; -1 default)")
+    ap.add_argument("--ploc-tail", type=int, default=0, help="PLOC: clusters at which one workgroup takes over (1..4096)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned AO surrogate at 1080p")
     ap.add_argument("--no-denoise", action="store_true", help="skip the 4-spp + learned denoise workload at 4K")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 8-scene, 64-view workload")
@@ -322,6 +325,10 @@ def main():
     ctx.set_option(raw_ao.OPT_TRACE_KERNEL, raw_ao.TRACE_BINARY if a.kernel == "binary" else raw_ao.TRACE_WIDE)
     if a.tri_q:
         ctx.set_option(raw_ao.OPT_TRI_PHASE, a.tri_q)
+    if a.ploc_tail:
+        ctx.set_option(raw_ao.OPT_PLOC_TAIL, a.ploc_tail)
+    if a.refill >= 0:
+        ctx.set_option(raw_ao.OPT_REFILL, a.refill)
 
     host_ms = []
 
@@ -350,14 +357,40 @@ def main():
                                                                      tests / max(1, rays_counted)))
     ms = {"build": [], "trace": [], "ssao": [], "error": []}
     host_ms.clear()
+    # Frames back to back, each complete inside the timed region: frame i's error map goes to
+    # pinned host memory asynchronously and its float64 row-major sum runs on the host while
+    # the GPU traces frame i+1 (the PLOC build synchronizes its stream, so the previous frame's
+    # copy has landed when the build returns).
+    hbuf = [torch.empty((H, W), dtype=torch.float32, pin_memory=True) for _ in range(2)]
+
+    def host_sum(i):
+        t = time.perf_counter()
+        r = raw_ao.host.rmse_of(hbuf[i % 2].numpy(), mask)
+        host_ms.append((time.perf_counter() - t) * 1e3)
+        return r
+
+    def gpu_parts(m):
+        for k in ("trace", "ssao", "error"):
+            ms[k].append(m[k])
+
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    rmse = px = None
     for i in range(a.steps):
-        rmse, px = frame()
+        ctx.build_ploc_dev(d_tris, 16, stream)
         m = ctx.last_ms()
-        for k in ms:
-            ms[k].append(m[k])
+        ms["build"].append(m["build"])
+        if i > 0:
+            gpu_parts(m)   # the previous frame's trace, SSAO and error map have finished
+        ctx.compute_rtao_dev(d_pos, d_nrm, d_mask, W, H, spp, radius, d_ao, stream)
+        ctx.compute_ssao_dev(d_pos, d_nrm, d_mask, W, H, 24, radius, d_ss, stream)
+        ctx.error_map_dev(d_ss, d_ao, d_mask, W, H, d_err, d_max, stream)
+        hbuf[i % 2].copy_(d_err, non_blocking=True)
+        if i > 0:
+            rmse, px = host_sum(i - 1)
     torch.cuda.synchronize()
+    gpu_parts(ctx.last_ms())
+    rmse, px = host_sum(a.steps - 1)
     wall = time.perf_counter() - t0
     log("timed %d frames in %.3f s" % (a.steps, wall))
 

@@ -65,7 +65,14 @@ enum {
      * outputs can differ from the others' by one bf16 rounding), 1 the general kernel
      * everywhere, 2 the tap-major kernel where it applies (the general kernel's sum order:
      * identical outputs to 1). */
-    RAW_AO_OPT_CONV_KERNEL = 4
+    RAW_AO_OPT_CONV_KERNEL = 4,
+    /* PLOC tuning, 1..4096 (default 4096): below this many clusters one workgroup runs every
+     * remaining round. The tree is the same for every value; only the speed differs. */
+    RAW_AO_OPT_PLOC_TAIL = 5,
+    /* 64-spp wide kernel, 0..64: with a value n > 0 a wave owns 4 pixels' 256 rays and a lane
+     * that finishes takes the next ray once n lanes are idle; 0 = one pixel's 64 rays a wave.
+     * Results are the same for every value; only the speed differs. */
+    RAW_AO_OPT_REFILL = 6
 };
 enum {
     RAW_AO_TRACE_WIDE = 0,     /* 4-wide nodes, LDS stack, triangle tests batched per wave */

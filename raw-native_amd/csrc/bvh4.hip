@@ -485,6 +485,141 @@ __global__ __launch_bounds__(BT) void k_trace_rays4(const float* __restrict__ ra
 uint32_t cdiv(uint64_t a, uint32_t b) { return uint32_t((a + b - 1) / b); }
 
 
+// 64 spp with refill: a wave owns PW pixels (PW x 64 rays) and a lane that finishes its ray
+// takes the wave's next one once refill_min lanes are idle (or none is live), so the lanes of
+// a wave stay busy while rays of very different lengths finish. Each lane keeps its open
+// counts per pixel; a pixel's AO is their wave sum / 64, as k_trace_ao4 computes it.
+template <int PW>
+__global__ __launch_bounds__(BT) void k_trace_ao4r(TraceArgs4 a, uint32_t refill_min) {
+    __shared__ uint32_t stk[STACK_LDS * BT];
+    uint32_t* const lds_col = stk + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t pix0 = (blockIdx.x * (BT / 64) + (threadIdx.x >> 6)) * PW;
+    if (pix0 >= a.npix) return;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t cnt[PW];
+    for (int k = 0; k < PW; ++k) cnt[k] = 0;
+    const auto finish = [&]() {
+        for (int k = 0; k < PW; ++k) {
+            uint32_t v = cnt[k];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            const uint32_t pix = pix0 + uint32_t(k);
+            if (lane == 0 && pix < a.npix) a.ao[pix] = a.mask[pix] ? float(v) / 64.0f : 1.0f;
+        }
+    };
+    if (a.ntri == 0) {   // nothing to hit: every covered pixel's rays are open
+        for (int k = 0; k < PW; ++k) cnt[k] = 1;
+        finish();
+        return;
+    }
+    const char* const base = reinterpret_cast<const char*>(a.nodes);
+    const float reach = a.radius * kReach;
+    Ray r;
+    uint32_t ox = 0, oy = 0, oz = 0;
+    uint32_t spill[STACK_SPILL];
+    int sp = 0, pk = 0;
+    int32_t node = -1;
+    uint32_t pend = 0, work = 0;
+    int4 leafref = make_int4(0, 0, 0, 0);
+    // ray ri of the wave: pixel pix0 + ri / 64, sample ri % 64; false for an uncovered pixel
+    const auto setup = [&](uint32_t ri) -> bool {
+        pk = int(ri >> 6);
+        const uint32_t pix = pix0 + (ri >> 6), s = ri & 63u;
+        if (pix >= a.npix || !a.mask[pix]) return false;
+        const V3 p = v3(a.pos[3 * size_t(pix)], a.pos[3 * size_t(pix) + 1], a.pos[3 * size_t(pix) + 2]);
+        const V3 n = v3(a.nrm[3 * size_t(pix)], a.nrm[3 * size_t(pix) + 1], a.nrm[3 * size_t(pix) + 2]);
+        const uint32_t y = pix / a.w, x = pix - y * a.w;
+        V3 t, b;
+        basis(n, t, b);
+        const V3 origin = add(p, scl(n, 1e-3f));
+        const uint32_t k1 = hash24(x, y, 2 * s), k2 = hash24(x, y, 2 * s + 1);
+        const float u1 = float(k1) / float(0x1000000), u2 = float(k2) / float(0x1000000);
+        float sphi, cphi;
+        sincos_base(6.2831853f * u2, sphi, cphi);
+        if (a.trig) trig_fix(a.trig, k2, sphi, cphi);
+        r = make_ray(origin, ao_dir(u1, sphi, cphi, n, t, b));
+        ox = r.inv.x >= 0.0f ? 0u : 48u;
+        oy = r.inv.y >= 0.0f ? 0u : 48u;
+        oz = r.inv.z >= 0.0f ? 0u : 48u;
+        sp = 0;
+        node = 0;
+        pend = 0;
+        work = 0;
+        return true;
+    };
+    const auto count_open = [&]() {
+        for (int k = 0; k < PW; ++k) cnt[k] += pk == k ? 1u : 0u;
+    };
+    const uint32_t total = PW * 64u;
+    bool live = setup(lane);
+    uint32_t next = 64;
+    while (true) {
+        if (live && pend == 0 && node >= 0) {
+            const uint32_t off = uint32_t(node) * uint32_t(sizeof(Node4));
+            const uint32_t hit = slab4s(r, base, off, ox, oy, oz, reach, kExit);
+            const int4 ref = *reinterpret_cast<const int4*>(base + off + 96);
+            const uint32_t masks = *reinterpret_cast<const uint32_t*>(base + off + 112);
+            pend = hit & masks & 15u;
+            leafref = ref;
+            bool overflow = false;
+            uint32_t ih = hit & (masks >> 4);
+            if (ih) {
+                node = ref_of(ref, __builtin_ctz(ih));
+                ih &= ih - 1u;
+                while (ih) {
+                    const uint32_t v = uint32_t(ref_of(ref, __builtin_ctz(ih)));
+                    ih &= ih - 1u;
+                    if (sp < STACK_LDS) lds_col[sp * BT] = v;
+                    else if (sp < STACK_LDS + STACK_SPILL) spill[sp - STACK_LDS] = v;
+                    else overflow = true;
+                    ++sp;
+                }
+            } else if (sp > 0) {
+                --sp;
+                node = int32_t(sp < STACK_LDS ? lds_col[sp * BT] : spill[sp - STACK_LDS]);
+            } else {
+                node = -1;
+            }
+            if (++work > a.ntri + 4) overflow = true;
+            if (overflow) {   // answered by the scan of every triangle
+                if (!scan_occluded(r, a.radius, a.tris, a.ntri)) count_open();
+                live = false;
+                pend = 0;
+            }
+        }
+        const bool waiting = live && pend != 0;
+        const bool can_step = live && pend == 0 && node >= 0;
+        const unsigned long long wb = __ballot(waiting), sb = __ballot(can_step), lb = __ballot(live);
+        if (lb != 0 && (sb == 0 || 64u * uint32_t(__popcll(wb)) >= a.tri_q * uint32_t(__popcll(lb)))) {
+            if (waiting) {
+                const int k = __builtin_ctz(pend);
+                pend &= pend - 1u;
+                const uint32_t slot = uint32_t(~ref_of(leafref, k));
+                float t;
+                if (slot < a.ntri && tri_hit(r, a.tris, slot, t) && t < a.radius) {
+                    live = false;   // occluded
+                    pend = 0;
+                }
+            }
+        }
+        if (live && pend == 0 && node < 0) {   // finished: open
+            count_open();
+            live = false;
+        }
+        const unsigned long long idle = __ballot(!live);
+        const uint32_t nidle = uint32_t(__popcll(idle));
+        if (next < total && (nidle >= refill_min || nidle == 64u)) {
+            if (!live) {
+                const uint32_t ri = next + uint32_t(__popcll(idle & below));
+                if (ri < total) live = setup(ri);
+            }
+            next += nidle;
+        }
+        if (__ballot(live) == 0 && next >= total) break;
+    }
+    finish();
+}
+
 // Lighting model 2's visibility (computeRTVisibility, src/renderer/ray_ao.cpp:20-63): the AO
 // rays of k_trace_ao4, and each open one traced again with tMax 1e30 for the sky.
 template <bool SPP64>
@@ -756,7 +891,10 @@ int trace_ao4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_m
     const uint32_t blocks = cdiv(uint64_t(npix) * uint64_t(spp), BT);
     const bool count = c.count_next;
     (void)hipEventRecord(c.ev[2], s);
-    if (spp64) {
+    if (spp64 && !count && c.refill > 0) {
+        constexpr int PW = 4;
+        k_trace_ao4r<PW><<<cdiv(npix, uint32_t(PW * (BT / 64))), BT, 0, s>>>(a, uint32_t(c.refill));
+    } else if (spp64) {
         if (count) k_trace_ao4<true, true><<<blocks, BT, 0, s>>>(a);
         else k_trace_ao4<true, false><<<blocks, BT, 0, s>>>(a);
     } else {

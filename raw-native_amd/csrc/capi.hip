@@ -120,6 +120,16 @@ int raw_ao_set_option(raw_ao_ctx* x, int option, int value) {
         c.trace_kernel = value;
         return RAW_AO_OK;
     }
+    if (option == RAW_AO_OPT_REFILL) {
+        if (value < 0 || value > 64) return fail(c, RAW_AO_ERR_ARG, "refill threshold must be in 0..64");
+        c.refill = value;
+        return RAW_AO_OK;
+    }
+    if (option == RAW_AO_OPT_PLOC_TAIL) {
+        if (value < 1 || value > 4096) return fail(c, RAW_AO_ERR_ARG, "PLOC tail threshold must be in 1..4096");
+        c.ploc_tail = value;
+        return RAW_AO_OK;
+    }
     if (option == RAW_AO_OPT_CONV_KERNEL) {
         if (value < 0 || value > 2) return fail(c, RAW_AO_ERR_ARG, "convolution kernel must be 0, 1 or 2");
         c.conv_kernel = value;

@@ -520,7 +520,8 @@ int ploc_build(Ctx& c, const float* d_tris, uint32_t n, int radius, hipStream_t 
     int parity = 0;
     uint32_t *cin = c.clus_a.as<uint32_t>(), *cout = c.clus_b.as<uint32_t>();
     c.plan_ploc_nb.clear();
-    while (m > uint32_t(TAIL_M)) {
+    const uint32_t tail_at = uint32_t(std::max(1, std::min(c.ploc_tail, TAIL_M)));
+    while (m > tail_at) {
         const uint32_t nb = cdiv(m, BT);
         for (int r = 0; r < kBatch; ++r) {
             c.plan_ploc_nb.push_back(nb);

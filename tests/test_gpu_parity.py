@@ -271,3 +271,44 @@ def test_repeatable(ctx):
     a2 = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
     assert np.array_equal(b1[1], b2[1]) and np.array_equal(bits(b1[0]), bits(b2[0]))
     assert np.array_equal(bits(a1), bits(a2))
+
+
+def test_ploc_tail_threshold_does_not_change_the_tree(ctx):
+    """The single-workgroup tail and the grid-wide rounds run the same rounds: the tree is the
+    same whichever takes over where."""
+    tris = O.scene_tris("dense_block", 60000, 5)
+    trees = []
+    try:
+        for tail in (4096, 1000, 64, 1):
+            ctx.set_option(raw_ao.OPT_PLOC_TAIL, tail)
+            ctx.build_ploc(tris)
+            trees.append(ctx.read_tree())
+    finally:
+        ctx.set_option(raw_ao.OPT_PLOC_TAIL, 4096)
+    for b, l, root, rounds in trees[1:]:
+        assert np.array_equal(b.view(np.uint32), trees[0][0].view(np.uint32)) and np.array_equal(l, trees[0][1])
+        assert (root, rounds) == trees[0][2:]
+
+
+@pytest.mark.parametrize("refill", [1, 24, 64])
+def test_refill_kernel_matches(ctx, refill):
+    """The 64-spp refill kernel (a wave owns 4 pixels' rays and refills idle lanes) gives the
+    one-pixel-a-wave kernel's AO bit for bit: on the 1M-triangle scene (odd width, so a wave's
+    pixels cross rows) and on the golden default frame."""
+    tris, pos, nrm, mask = big_frame(243, 131, seed=3)
+    ctx.build_ploc(tris, 16)
+    base = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
+    ctx.set_option(raw_ao.OPT_REFILL, refill)
+    try:
+        got = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
+        assert np.array_equal(bits(got), bits(base))
+        p = params("default")
+        _, pos2, nrm2, mask2 = raw_ao.host.cli_gbuffer(p["w"], p["h"], p["eye"], p["target"], p["fovy"])
+        vpos, _, idx = raw_ao.host.test_scene_mesh()
+        ctx.build_ploc(raw_ao.host.mesh_tris(vpos, idx), 16)
+        ao = ctx.compute_rtao(pos2, nrm2, mask2, 64, 2.0)
+        assert sha(O.pfm1_bytes(ao)) == CASES["default"]["files"]["ao_rt.pfm"]
+        ctx.build_ploc(np.zeros((0, 9), np.float32), 16)
+        assert np.all(ctx.compute_rtao(pos2, nrm2, mask2, 64, 2.0) == 1.0)
+    finally:
+        ctx.set_option(raw_ao.OPT_REFILL, 0)
