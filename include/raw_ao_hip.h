@@ -122,6 +122,24 @@ int raw_ao_trace_visibility_dev(raw_ao_ctx* ctx, const float* d_position, const 
                                 int w, int h, int spp, float radius, const float to_light[3], int sun_forced,
                                 float* d_ao, float* d_sky, float* d_sun, void* stream);
 
+/* ---- the G-buffer -------------------------------------------------------------------
+ * Replaces raw::rasterize (raw/renderer/raster.hpp:9, src/renderer/raster.cpp:92-126) with
+ * default RasterOptions, for one indexed mesh (vertex positions and normals, 3 floats each;
+ * 3 indices a triangle) and the camera of src/renderer/raster.cpp:94-96 (perspective(fovy,
+ * aspect, near, far) * lookAt(eye, center, up), raw/math/mat.hpp:21-38): depth = the
+ * screen-linear view distance (+inf where nothing covers the pixel), position and normal
+ * (3 floats a pixel, zero where uncovered), mask = 1 where covered. Bit for bit the
+ * reference's result: a pixel keeps the nearest fragment, the earliest drawn among equals.
+ * The _dev form reads indices as given: each must be below the vertex count. */
+int raw_ao_rasterize(raw_ao_ctx* ctx, const float* vpos, const float* vnrm, uint32_t nverts, const int32_t* idx,
+                     uint32_t ntri, const float eye[3], const float center[3], const float up[3], float fovy,
+                     float aspect, float near_z, float far_z, int w, int h, float* depth, float* position,
+                     float* normal, uint8_t* mask);
+int raw_ao_rasterize_dev(raw_ao_ctx* ctx, const float* d_vpos, const float* d_vnrm, const int32_t* d_idx, uint32_t ntri,
+                         const float eye[3], const float center[3], const float up[3], float fovy, float aspect,
+                         float near_z, float far_z, int w, int h, float* d_depth, float* d_position, float* d_normal,
+                         uint8_t* d_mask, void* stream);
+
 /* ---- screen-space AO ---------------------------------------------------------------
  * Replaces computeSSAO (raw/renderer/ssao.hpp:5-6, src/renderer/ssao.cpp:11-39). */
 int raw_ao_ssao(raw_ao_ctx* ctx, const float* position, const float* normal, const uint8_t* mask,

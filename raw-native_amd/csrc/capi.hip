@@ -96,7 +96,8 @@ void raw_ao_close(raw_ao_ctx* x) {
         raw_ao::DevBuf* bufs[] = {&c.tris_in, &c.keys, &c.vals, &c.keys2, &c.vals2, &c.hist, &c.rs_tot, &c.boxes, &c.links,
                                   &c.clus_a, &c.clus_b, &c.nn, &c.flags, &c.blk, &c.state, &c.g4, &c.bounds_part,
                                   &c.nodes, &c.tris_sorted, &c.nodes4, &c.wfront_a, &c.wfront_b, &c.wblk, &c.wstate, &c.st_pos, &c.st_nrm, &c.st_mask, &c.st_out, &c.st_out2,
-                                  &c.st_cnt, &c.st_rays, &c.st_max, &c.counters};
+                                  &c.st_cnt, &c.st_rays, &c.st_max, &c.counters, &c.r_vp, &c.r_keys, &c.r_tris, &c.r_counts,
+                                  &c.r_offs, &c.r_scan_tmp};
         for (auto* b : bufs)
             if (b->p) (void)hipFree(b->p);
         if (c.d_trig) (void)hipFree(c.d_trig);
@@ -321,6 +322,68 @@ int raw_ao_trace_visibility(raw_ao_ctx* x, const float* pos, const float* nrm, c
         !hip_ok(c, hipStreamSynchronize(s), "visibility"))
         return RAW_AO_ERR_DEVICE;
     c.ms_trace = elapsed(c, 2, 3);
+    return RAW_AO_OK;
+}
+
+namespace {
+int raster_args(Ctx& c, const void* vpos, const void* vnrm, const void* idx, uint32_t ntri, const float* eye, const float* center,
+                const float* up, float fovy, float aspect, float nearz, float farz, int w, int h) {
+    if (!image_ok(w, h)) return fail(c, RAW_AO_ERR_ARG, "bad image size");
+    if (ntri && (!vpos || !vnrm || !idx)) return fail(c, RAW_AO_ERR_ARG, "null mesh");
+    if (ntri > (1u << 30)) return fail(c, RAW_AO_ERR_ARG, "at most 2^30 triangles");
+    if (!eye || !center || !up) return fail(c, RAW_AO_ERR_ARG, "null camera");
+    if (!(fovy > 0.0f && fovy < 3.14159f) || !(aspect > 0.0f) || !(nearz > 0.0f) || !(farz > nearz))
+        return fail(c, RAW_AO_ERR_ARG, "bad camera: need 0 < fovy < pi, aspect > 0, 0 < near < far");
+    return RAW_AO_OK;
+}
+}  // namespace
+
+int raw_ao_rasterize_dev(raw_ao_ctx* x, const float* d_vpos, const float* d_vnrm, const int32_t* d_idx, uint32_t ntri,
+                         const float eye[3], const float center[3], const float up[3], float fovy, float aspect, float near_z,
+                         float far_z, int w, int h, float* d_depth, float* d_pos, float* d_nrm, uint8_t* d_mask, void* stream) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    const int a = raster_args(c, d_vpos, d_vnrm, d_idx, ntri, eye, center, up, fovy, aspect, near_z, far_z, w, h);
+    if (a) return a;
+    if (!d_depth || !d_pos || !d_nrm || !d_mask) return fail(c, RAW_AO_ERR_ARG, "null output");
+    return code_of(raw_ao::rasterize(c, d_vpos, d_vnrm, d_idx, ntri, eye, center, up, fovy, aspect, near_z, far_z, w, h, d_depth,
+                                     d_pos, d_nrm, d_mask, pick(c, stream)));
+}
+
+int raw_ao_rasterize(raw_ao_ctx* x, const float* vpos, const float* vnrm, uint32_t nverts, const int32_t* idx, uint32_t ntri,
+                     const float eye[3], const float center[3], const float up[3], float fovy, float aspect, float near_z,
+                     float far_z, int w, int h, float* depth, float* pos, float* nrm, uint8_t* mask) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    const int a = raster_args(c, vpos, vnrm, idx, ntri, eye, center, up, fovy, aspect, near_z, far_z, w, h);
+    if (a) return a;
+    if (!depth || !pos || !nrm || !mask) return fail(c, RAW_AO_ERR_ARG, "null output");
+    for (size_t i = 0; i < 3 * size_t(ntri); ++i)
+        if (idx[i] < 0 || uint32_t(idx[i]) >= nverts) return fail(c, RAW_AO_ERR_ARG, "vertex index out of range");
+    const size_t np = size_t(w) * size_t(h);
+    const size_t vb = 12 * size_t(nverts), ib = 12 * size_t(ntri);
+    // staging: vertices (positions, normals) in st_rays, indices in st_cnt; outputs in st_out, st_mask
+    if (!ensure(c, c.st_rays, 2 * vb) || !ensure(c, c.st_cnt, ib) || !ensure(c, c.st_out, 28 * np) || !ensure(c, c.st_mask, np))
+        return RAW_AO_ERR_DEVICE;
+    hipStream_t s = c.stream;
+    char* v = c.st_rays.as<char>();
+    if ((vb && (!hip_ok(c, hipMemcpyAsync(v, vpos, vb, hipMemcpyHostToDevice, s), "upload") ||
+                !hip_ok(c, hipMemcpyAsync(v + vb, vnrm, vb, hipMemcpyHostToDevice, s), "upload"))) ||
+        (ib && !hip_ok(c, hipMemcpyAsync(c.st_cnt.p, idx, ib, hipMemcpyHostToDevice, s), "upload")))
+        return RAW_AO_ERR_DEVICE;
+    float* o = c.st_out.as<float>();
+    const int r = raw_ao_rasterize_dev(x, reinterpret_cast<const float*>(v), reinterpret_cast<const float*>(v + vb),
+                                       c.st_cnt.as<int32_t>(), ntri, eye, center, up, fovy, aspect, near_z, far_z, w, h, o, o + np,
+                                       o + 4 * np, c.st_mask.as<uint8_t>(), nullptr);
+    if (r) return r;
+    if (!hip_ok(c, hipMemcpyAsync(depth, o, 4 * np, hipMemcpyDeviceToHost, s), "download") ||
+        !hip_ok(c, hipMemcpyAsync(pos, o + np, 12 * np, hipMemcpyDeviceToHost, s), "download") ||
+        !hip_ok(c, hipMemcpyAsync(nrm, o + 4 * np, 12 * np, hipMemcpyDeviceToHost, s), "download") ||
+        !hip_ok(c, hipMemcpyAsync(mask, c.st_mask.p, np, hipMemcpyDeviceToHost, s), "download") ||
+        !hip_ok(c, hipStreamSynchronize(s), "rasterize"))
+        return RAW_AO_ERR_DEVICE;
     return RAW_AO_OK;
 }
 

@@ -88,6 +88,10 @@ struct Ctx {
     std::vector<uint32_t> plan_ploc_nb, plan_wide_nb;
     uint32_t plan_n{0};
     int plan_radius{0};
+
+    // the G-buffer rasterizer: camera matrix, per-pixel (depth, draw id) keys, drawn triangles,
+    // their box pixel counts and the counts' exclusive scan
+    DevBuf r_vp, r_keys, r_tris, r_counts, r_offs, r_scan_tmp;
 };
 
 bool ensure(Ctx& c, DevBuf& b, size_t bytes);
@@ -118,6 +122,10 @@ void ssao_enqueue(const uint8_t* trig, const float* d_pos, const float* d_nrm, c
                   float radius, float* d_ao, hipStream_t s);
 int error_map(Ctx& c, const float* d_a, const float* d_t, const uint8_t* d_mask, int w, int h, float* d_err,
               float* d_max, hipStream_t s);
+
+int rasterize(Ctx& c, const float* d_vpos, const float* d_vnrm, const int32_t* d_idx, uint32_t ntri, const float eye[3],
+              const float center[3], const float up[3], float fovy, float aspect, float nearz, float farz, int w, int h,
+              float* d_depth, float* d_pos, float* d_nrm, uint8_t* d_mask, hipStream_t s);
 
 void net_release(Ctx& c);
 bool conv_shape_ok(int c0, int c1, int cout);

@@ -52,6 +52,10 @@ SIGNATURES = [
                                           _f32p, _f32p, _f32p]),
     ("raw_ao_trace_visibility_dev", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _f32p, C.c_int,
                                               _vp, _vp, _vp, _vp]),
+    ("raw_ao_rasterize", C.c_int, [_vp, _f32p, _f32p, C.c_uint32, _i32p, C.c_uint32, _f32p, _f32p, _f32p, C.c_float,
+                                   C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, _f32p, _f32p, _f32p, _u8p]),
+    ("raw_ao_rasterize_dev", C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _f32p, _f32p, _f32p, C.c_float, C.c_float, C.c_float,
+                                       C.c_float, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]),
     ("raw_ao_ssao", C.c_int, [_vp, _f32p, _f32p, _u8p, C.c_int, C.c_int, C.c_int, C.c_float, _f32p]),
     ("raw_ao_ssao_dev", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp]),
     ("raw_ao_error_map", C.c_int, [_vp, _f32p, _f32p, _u8p, C.c_int, C.c_int, _f32p, C.POINTER(C.c_float)]),
@@ -229,6 +233,28 @@ class Context:
         self._check(lib().raw_ao_trace_visibility_dev(self._h, _ptr(position), _ptr(normal), _ptr(mask), w, h, samples,
                                                       radius, np.asarray(to_light, np.float32), int(bool(sun_forced)),
                                                       _ptr(ao), _ptr(sky), _ptr(sun), _vp(stream)))
+
+    # ---- G-buffer ------------------------------------------------------------------
+    def rasterize(self, vpos, vnrm, idx, eye, center, up, fovy, aspect, near, far, w, h):
+        """raw::rasterize of one indexed mesh: (depth, position, normal, mask)."""
+        vpos = np.ascontiguousarray(vpos, np.float32).reshape(-1, 3)
+        vnrm = np.ascontiguousarray(vnrm, np.float32).reshape(-1, 3)
+        idx = np.ascontiguousarray(idx, np.int32).reshape(-1, 3)
+        depth = np.zeros((h, w), np.float32)
+        pos = np.zeros((h, w, 3), np.float32)
+        nrm = np.zeros((h, w, 3), np.float32)
+        mask = np.zeros((h, w), np.uint8)
+        f3 = lambda v: np.ascontiguousarray(v, np.float32).reshape(3)
+        self._check(lib().raw_ao_rasterize(self._h, vpos, vnrm, vpos.shape[0], idx, idx.shape[0], f3(eye), f3(center), f3(up),
+                                           fovy, aspect, near, far, w, h, depth, pos, nrm, mask))
+        return depth, pos, nrm, mask
+
+    def rasterize_dev(self, vpos, vnrm, idx, ntri, eye, center, up, fovy, aspect, near, far, w, h, depth, pos, nrm, mask,
+                      stream=None):
+        f3 = lambda v: np.ascontiguousarray(v, np.float32).reshape(3)
+        self._check(lib().raw_ao_rasterize_dev(self._h, _ptr(vpos), _ptr(vnrm), _ptr(idx), ntri, f3(eye), f3(center), f3(up),
+                                               fovy, aspect, near, far, w, h, _ptr(depth), _ptr(pos), _ptr(nrm), _ptr(mask),
+                                               _vp(stream)))
 
     def compute_ssao(self, position, normal, mask, samples=24, radius=2.0):
         mask = np.ascontiguousarray(mask, np.uint8)

@@ -15,7 +15,9 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def ctx():
-    """One GPU context for the session (the HIP library must be built and a GPU present)."""
+    """One GPU context for the session (the HIP library must be built and a GPU present).
+    torch loads its HIP runtime before the library does, so tests can share device memory."""
+    import torch  # noqa: F401
     import raw_ao
     c = raw_ao.Context(0)
     yield c
