@@ -3,7 +3,8 @@
 
 Workload (a step is one frame): the seeded ~1M-triangle scene (2,000 boxes and
 icospheres over a 40 x 40 ground quad), G-buffer at 1920 x 1080 from the host rasterizer
-(uploaded once, resident in HBM before timing), then per frame on the GPU: PLOC BVH
+(uploaded once, resident in HBM before timing; the same G-buffer rasterized on the GPU is
+timed beside the frame and checked bit for bit), then per frame on the GPU: PLOC BVH
 build, ray-traced AO at 64 rays/pixel with radius 1.0, SSAO at 24 samples, the
 reconcile's error map, and the reconcile's float64 sum on the host (which overlaps the next
 frame's GPU work; every frame is complete inside the timed region).
@@ -98,6 +99,29 @@ def _timed(torch, fn, steps, warmup=2):
         r = fn()
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) * 1e3 / steps, r
+
+
+def gbuffer_gpu(ctx, torch, vpos, vnrm, idx, eye, W, H, gbuf_host, host_ms, steps, stream):
+    """The frame's G-buffer rasterized on the GPU (raw_ao_rasterize_dev) from the resident mesh:
+    ms per call, the host rasterizer's ms for the same call, and whether the two G-buffers are
+    bit-identical."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    tv, tn, ti = (torch.from_numpy(np.ascontiguousarray(x)).to(dev) for x in (vpos, vnrm, idx))
+    out = [torch.empty((H, W), dtype=torch.float32, device=dev), torch.empty((H, W, 3), dtype=torch.float32, device=dev),
+           torch.empty((H, W, 3), dtype=torch.float32, device=dev), torch.empty((H, W), dtype=torch.uint8, device=dev)]
+    aspect = float(np.float32(W) / np.float32(H))
+
+    def run():
+        ctx.rasterize_dev(tv, tn, ti, idx.shape[0], eye, (0, 2, 0), (0, 1, 0), 0.9, aspect, 0.1, 200.0, W, H, *out,
+                          stream=stream)
+
+    ms, _ = _timed(torch, run, steps)
+    same = all(np.array_equal(np.ascontiguousarray(g.cpu().numpy()).view(np.uint8), np.ascontiguousarray(h).view(np.uint8))
+               for g, h in zip(out, gbuf_host))
+    log("G-buffer on the GPU: %.3f ms (host %.1f ms), identical %s" % (ms, host_ms, same))
+    return {"gpu_ms": ms, "host_ms": host_ms, "speedup": host_ms / ms, "identical_to_host": bool(same),
+            "triangles": int(idx.shape[0])}
 
 
 def _rmse(ctx, raw_ao, torch, approx, truth, mask_t, mask_np, W, H, stream):
@@ -306,8 +330,11 @@ def main():
     vpos, vnrm, idx, nspheres = raw_ao.host.scene_seeded(spec)
     tris = raw_ao.host.mesh_tris(vpos, idx)
     eye = raw_ao.host.orbit_camera(1)
-    _, pos, nrm, mask = raw_ao.host.rasterize(vpos, vnrm, idx, eye, (0, 2, 0), (0, 1, 0), 0.9,
-                                              float(np.float32(W) / np.float32(H)), 0.1, 200.0, W, H)
+    t_r = time.perf_counter()
+    gbuf_host = raw_ao.host.rasterize(vpos, vnrm, idx, eye, (0, 2, 0), (0, 1, 0), 0.9,
+                                      float(np.float32(W) / np.float32(H)), 0.1, 200.0, W, H)
+    host_raster_ms = (time.perf_counter() - t_r) * 1e3
+    _, pos, nrm, mask = gbuf_host
     covered = int(mask.sum())
     log("%d triangles, %dx%d, %d covered pixels" % (tris.shape[0], W, H, covered))
 
@@ -427,6 +454,7 @@ def main():
                      "algorithmic_bytes_per_launch": alg_bytes,
                      "node_visits": visits, "triangle_tests": tests},
     }
+    out["gbuffer"] = gbuffer_gpu(ctx, torch, vpos, vnrm, idx, eye, W, H, gbuf_host, host_raster_ms, a.steps, stream)
     if learned is not None:
         out["learned_ao"] = learned
     if not a.no_denoise:

@@ -96,7 +96,7 @@ void raw_ao_close(raw_ao_ctx* x) {
         raw_ao::DevBuf* bufs[] = {&c.tris_in, &c.keys, &c.vals, &c.keys2, &c.vals2, &c.hist, &c.rs_tot, &c.boxes, &c.links,
                                   &c.clus_a, &c.clus_b, &c.nn, &c.flags, &c.blk, &c.state, &c.g4, &c.bounds_part,
                                   &c.nodes, &c.tris_sorted, &c.nodes4, &c.wfront_a, &c.wfront_b, &c.wblk, &c.wstate, &c.st_pos, &c.st_nrm, &c.st_mask, &c.st_out, &c.st_out2,
-                                  &c.st_cnt, &c.st_rays, &c.st_max, &c.counters, &c.r_vp, &c.r_keys, &c.r_tris, &c.r_counts,
+                                  &c.st_cnt, &c.st_rays, &c.st_max, &c.counters, &c.r_keys, &c.r_tris, &c.r_counts,
                                   &c.r_offs, &c.r_scan_tmp};
         for (auto* b : bufs)
             if (b->p) (void)hipFree(b->p);

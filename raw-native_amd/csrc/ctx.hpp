@@ -89,9 +89,9 @@ struct Ctx {
     uint32_t plan_n{0};
     int plan_radius{0};
 
-    // the G-buffer rasterizer: camera matrix, per-pixel (depth, draw id) keys, drawn triangles,
-    // their box pixel counts and the counts' exclusive scan
-    DevBuf r_vp, r_keys, r_tris, r_counts, r_offs, r_scan_tmp;
+    // the G-buffer rasterizer: per-pixel (depth, draw id) keys, drawn triangles, their box
+    // pixel counts and the counts' exclusive scan
+    DevBuf r_keys, r_tris, r_counts, r_offs, r_scan_tmp;
 };
 
 bool ensure(Ctx& c, DevBuf& b, size_t bytes);

@@ -30,6 +30,10 @@ namespace {
 
 constexpr int RBT = 256;
 
+struct Cam {   // the view-projection matrix, row-major, passed by value
+    float m[16];
+};
+
 struct RTri {
     float sx[3], sy[3], sz[3], invw[3];
     float area;
@@ -83,8 +87,7 @@ __device__ uint32_t setup_tri(const CV v[3], int w, int h, RTri& t) {
 }
 
 __global__ __launch_bounds__(RBT) void k_rast_setup(const float* __restrict__ vpos, const float* __restrict__ vnrm,
-                                                    const int32_t* __restrict__ idx, uint32_t ntri, const float* __restrict__ vp,
-                                                    float nearz, int w, int h, RTri* __restrict__ tris,
+                                                    const int32_t* __restrict__ idx, uint32_t ntri, const Cam vp, float nearz, int w, int h, RTri* __restrict__ tris,
                                                     uint64_t* __restrict__ counts) {
     const uint32_t i = blockIdx.x * RBT + threadIdx.x;
     if (i >= ntri) return;
@@ -94,10 +97,10 @@ __global__ __launch_bounds__(RBT) void k_rast_setup(const float* __restrict__ vp
         const size_t vi = size_t(idx[3 * size_t(i) + k]);
         const V3 p = v3(vpos[3 * vi], vpos[3 * vi + 1], vpos[3 * vi + 2]);
         // mul(Mat4, Vec4) (raw/math/mat.hpp), left to right
-        v[k].cs = make_float4(vp[0] * p.x + vp[1] * p.y + vp[2] * p.z + vp[3] * 1.0f,
-                              vp[4] * p.x + vp[5] * p.y + vp[6] * p.z + vp[7] * 1.0f,
-                              vp[8] * p.x + vp[9] * p.y + vp[10] * p.z + vp[11] * 1.0f,
-                              vp[12] * p.x + vp[13] * p.y + vp[14] * p.z + vp[15] * 1.0f);
+        const float* m = vp.m;
+        v[k].cs = make_float4(m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3] * 1.0f, m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7] * 1.0f,
+                              m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11] * 1.0f,
+                              m[12] * p.x + m[13] * p.y + m[14] * p.z + m[15] * 1.0f);
         v[k].wp = p;
         v[k].wn = v3(vnrm[3 * vi], vnrm[3 * vi + 1], vnrm[3 * vi + 2]);
         behind = behind || v[k].cs.w <= 1e-6f;
@@ -258,13 +261,14 @@ int rasterize(Ctx& c, const float* d_vpos, const float* d_vnrm, const int32_t* d
     const M4 vp = mul(perspective(fovy, aspect, nearz, farz),
                       look_at(v3(eye[0], eye[1], eye[2]), v3(center[0], center[1], center[2]), v3(up[0], up[1], up[2])));
     const uint32_t nsub = 2 * ntri;
-    if (!ensure(c, c.r_vp, 64) || !ensure(c, c.r_keys, 8 * size_t(npix)) || !ensure(c, c.r_tris, sizeof(RTri) * std::max<size_t>(1, nsub)) ||
+    if (!ensure(c, c.r_keys, 8 * size_t(npix)) || !ensure(c, c.r_tris, sizeof(RTri) * std::max<size_t>(1, nsub)) ||
         !ensure(c, c.r_counts, 8 * (size_t(nsub) + 1)) || !ensure(c, c.r_offs, 8 * (size_t(nsub) + 1)))
         return -2;
-    if (!hip_ok(c, hipMemcpyAsync(c.r_vp.p, vp.m, 64, hipMemcpyHostToDevice, s), "camera upload")) return -2;
+    Cam cam;
+    for (int k = 0; k < 16; ++k) cam.m[k] = vp.m[k];
     k_fill_keys<<<std::min<uint32_t>(4096, (npix + RBT - 1) / RBT), RBT, 0, s>>>(c.r_keys.as<unsigned long long>(), npix);
     if (ntri) {
-        k_rast_setup<<<(ntri + RBT - 1) / RBT, RBT, 0, s>>>(d_vpos, d_vnrm, d_idx, ntri, c.r_vp.as<float>(), nearz, w, h,
+        k_rast_setup<<<(ntri + RBT - 1) / RBT, RBT, 0, s>>>(d_vpos, d_vnrm, d_idx, ntri, cam, nearz, w, h,
                                                            c.r_tris.as<RTri>(), c.r_counts.as<uint64_t>());
         size_t tmp = 0;
         if (!hip_ok(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, c.r_counts.as<uint64_t>(), c.r_offs.as<uint64_t>(), nsub, s),
