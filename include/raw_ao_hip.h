@@ -140,6 +140,86 @@ int raw_ao_rasterize_dev(raw_ao_ctx* ctx, const float* d_vpos, const float* d_vn
                          float near_z, float far_z, int w, int h, float* d_depth, float* d_position, float* d_normal,
                          uint8_t* d_mask, void* stream);
 
+/* ---- the default frame on the GPU ----------------------------------------------------
+ * Replaces renderGpu (raw/renderer/gpu.hpp:32, src/renderer/gpu/gpu_renderer.cpp:129-182) with
+ * the passes of the CPU reference frame, renderWithParams (raw/renderer/render.hpp:50-52,
+ * src/renderer/render.cpp:71-105), bit for bit: raster (G-buffer with albedo), the AO
+ * accelerator over the scene's triangles, lighting model 2's visibility (or model 1's AO),
+ * SSAO, the reconcile, shade and shade_hdr (src/renderer/composite.cpp), motion
+ * (src/renderer/motion.cpp). A scene is its meshes concatenated in scene order: vertices,
+ * indices, and per mesh the end of its triangle range and its albedo. */
+typedef struct raw_ao_scene {
+    const float* vpos;          /* nverts x 3 */
+    const float* vnrm;          /* nverts x 3 */
+    uint32_t nverts;
+    const int32_t* idx;         /* ntri x 3 */
+    uint32_t ntri;
+    const uint32_t* mesh_end;   /* host, nmesh: one past each mesh's last triangle (the last = ntri) */
+    const float* albedo;        /* host, nmesh x 3 */
+    uint32_t nmesh;
+    float eye[3], center[3], up[3];
+    float fovy, aspect, near_z, far_z;
+    int32_t has_light;          /* 0: the reference's default light, direction (0,-1,0), intensity 1 */
+    float light_dir[3];
+    float light_intensity;
+} raw_ao_scene;
+
+/* RenderOptions (raw/renderer/render.hpp:40-45) with lightingModel() resolved by the caller:
+ * lighting 2 shades with sky and sun visibility, 1 with (0.2 + sun) x AO; rtao 0 shades model 1
+ * with SSAO and leaves the reconcile empty (pixels 0). */
+typedef struct raw_ao_render_opts {
+    float tolerance;            /* 0.12 */
+    int32_t rtao;               /* 1 */
+    int32_t lighting;           /* 2 */
+    int32_t no_sun;             /* the v2-nosun control: sun visibility forced to 1 */
+    int32_t rt_samples;         /* 64 (kRtSamples) */
+    int32_t ss_samples;         /* 24 (kSsSamples) */
+    float radius;               /* 2.0 (kAoRadius) */
+    int32_t bvh_radius;         /* PLOC search radius, 16 */
+    const float* prev_viewproj; /* host, 16 floats row-major; NULL: this frame's (zero motion) */
+} raw_ao_render_opts;
+
+/* The frame's planes, w x h each (position, normal, albedo, frame, hdr: 3 floats a pixel;
+ * motion: 2). raw_ao_render_frame takes host pointers and skips NULL ones;
+ * raw_ao_render_frame_dev takes device pointers, all required (sky and sun are written with
+ * model 2 only). */
+typedef struct raw_ao_frame {
+    float* depth;
+    float* position;
+    float* normal;
+    float* albedo;
+    uint8_t* mask;
+    float* motion;
+    float* ao_rt;
+    float* ao_ss;
+    float* sky;
+    float* sun;
+    float* error;
+    float* frame;
+    float* hdr;
+} raw_ao_frame;
+
+/* The frame's scalars: the reconcile (rmse from the float64 row-major sum), the motion counts,
+ * maxRadiance of the HDR plane (src/renderer/composite.cpp:82-88), the view-projection, and the
+ * frame's GPU time. */
+typedef struct raw_ao_frame_stats {
+    float rmse, max_error;
+    int32_t pixels, within_tolerance;
+    int32_t motion_valid, motion_total;
+    float max_radiance;
+    float viewproj[16];
+    float gpu_ms;
+} raw_ao_frame_stats;
+
+int raw_ao_render_frame(raw_ao_ctx* ctx, const raw_ao_scene* scene, const raw_ao_render_opts* opts, int w, int h,
+                        raw_ao_frame* out, raw_ao_frame_stats* stats);
+/* Scene vertices and indices in device memory (mesh_end and albedo stay host arrays). */
+int raw_ao_render_frame_dev(raw_ao_ctx* ctx, const raw_ao_scene* scene, const raw_ao_render_opts* opts, int w, int h,
+                            raw_ao_frame* d_out, raw_ao_frame_stats* stats, void* stream);
+/* The reference's defaults: tolerance 0.12, rtao on, lighting 2, 64/24 samples, radius 2, PLOC
+ * radius 16, static camera. */
+void raw_ao_render_opts_default(raw_ao_render_opts* opts);
+
 /* ---- screen-space AO ---------------------------------------------------------------
  * Replaces computeSSAO (raw/renderer/ssao.hpp:5-6, src/renderer/ssao.cpp:11-39). */
 int raw_ao_ssao(raw_ao_ctx* ctx, const float* position, const float* normal, const uint8_t* mask,

@@ -301,7 +301,7 @@ int clipNear(const CV in[3], float wNear, CV out[4]) {   // raster.cpp:17-27
     }
     return n;
 }
-struct GB { int w, h; float* depth; float* pos; float* nrm; uint8_t* mask; };
+struct GB { int w, h; float* depth; float* pos; float* nrm; uint8_t* mask; float* alb{nullptr}; V3 albedo{}; };
 void drawTri(GB& g, const CV v[3]) {   // raster.cpp:30-90 with default options
     const int w = g.w, h = g.h;
     V3 wp[3] = {v[0].wp, v[1].wp, v[2].wp}, wn[3] = {v[0].wn, v[1].wn, v[2].wn};
@@ -340,6 +340,7 @@ void drawTri(GB& g, const CV v[3]) {   // raster.cpp:30-90 with default options
             g.depth[i] = depth;
             st3(g.pos + 3 * i, pc(wp[0], wp[1], wp[2]));
             st3(g.nrm + 3 * i, nrm(pc(wn[0], wn[1], wn[2])));
+            if (g.alb) st3(g.alb + 3 * i, g.albedo);   // raster.cpp:82: the mesh material's albedo
             g.mask[i] = 1;
         }
 }
@@ -697,6 +698,110 @@ void orc_rasterize(const float* vpos, const float* vnrm, const int32_t* idx, siz
             drawTri(g, t);
         }
     }
+}
+
+void orc_rasterize_scene(const float* vpos, const float* vnrm, const int32_t* idx, size_t ntri, const uint32_t* mesh_end,
+                         const float* albedo, size_t nmesh, const float eye[3], const float center[3], const float up[3],
+                         float fovy, float aspect, float nearZ, float farZ, int w, int h, float* depth, float* position,
+                         float* normal, float* albedo_out, uint8_t* mask) {
+    const size_t np = size_t(w) * size_t(h);
+    for (size_t i = 0; i < np; ++i) {
+        depth[i] = std::numeric_limits<float>::infinity();
+        mask[i] = 0;
+    }
+    std::memset(position, 0, np * 3 * sizeof(float));
+    std::memset(normal, 0, np * 3 * sizeof(float));
+    std::memset(albedo_out, 0, np * 3 * sizeof(float));
+    GB g{w, h, depth, position, normal, mask};
+    g.alb = albedo_out;
+    const M4 vp = mul(perspective(fovy, aspect, nearZ, farZ), lookAt(ld3(eye), ld3(center), ld3(up)));
+    size_t i = 0;
+    for (size_t m = 0; m < nmesh; ++m) {   // raster.cpp:107-124: meshes in order, triangles in order
+        g.albedo = ld3(albedo + 3 * m);
+        for (; i < mesh_end[m] && i < ntri; ++i) {
+            CV v[3];
+            bool behind = false;
+            for (int k = 0; k < 3; ++k) {
+                const int32_t vi = idx[3 * i + size_t(k)];
+                const V3 p = ld3(vpos + 3 * size_t(vi));
+                v[k] = {mul(vp, V4{p.x, p.y, p.z, 1}), p, ld3(vnrm + 3 * size_t(vi))};
+                behind = behind || v[k].cs.w <= 1e-6f;
+            }
+            if (!behind) { drawTri(g, v); continue; }
+            CV c[4];
+            const int n = clipNear(v, nearZ, c);
+            for (int k = 1; k + 1 < n; ++k) {
+                const CV t[3] = {c[0], c[k], c[k + 1]};
+                drawTri(g, t);
+            }
+        }
+    }
+}
+
+void orc_view_proj(const float eye[3], const float center[3], const float up[3], float fovy, float aspect, float nearZ,
+                   float farZ, float out16[16]) {
+    const M4 vp = mul(perspective(fovy, aspect, nearZ, farZ), lookAt(ld3(eye), ld3(center), ld3(up)));
+    std::memcpy(out16, vp.m, 64);
+}
+
+void orc_shade(const float* normal, const float* albedo, const uint8_t* mask, int w, int h, int model, const float* plane_a,
+               const float* plane_sun, const float ldir[3], float li, float* frame, float* hdr) {
+    const V3 l = ld3(ldir);
+    const size_t np = size_t(w) * size_t(h);
+    for (size_t i = 0; i < np; ++i) {
+        V3 r{0, 0, 0};
+        if (mask[i]) {
+            const V3 n = ld3(normal + 3 * i);
+            const float ndl = std::max(0.0f, dot(n, V3{l.x * -1.0f, l.y * -1.0f, l.z * -1.0f})) * li;   // composite.cpp:11,21
+            float lit;
+            if (model == 2) lit = 0.2f * plane_a[i] + ndl * plane_sun[i];   // composite.cpp:22
+            else {
+                const float ambient = 0.2f;   // composite.cpp:12-14
+                lit = (ambient + ndl) * plane_a[i];
+            }
+            const V3 a = ld3(albedo + 3 * i);
+            r = V3{a.x * lit, a.y * lit, a.z * lit};
+        }
+        st3(hdr + 3 * i, r);
+        st3(frame + 3 * i, V3{std::clamp(r.x, 0.0f, 1.0f), std::clamp(r.y, 0.0f, 1.0f), std::clamp(r.z, 0.0f, 1.0f)});
+    }
+}
+
+float orc_max_radiance(const float* hdr, size_t npix) {   // composite.cpp:82-88
+    float m = 0.0f;
+    for (size_t i = 0; i < npix; ++i) m = std::max(m, std::max(hdr[3 * i], std::max(hdr[3 * i + 1], hdr[3 * i + 2])));
+    return m;
+}
+
+int32_t orc_motion(const float* position, const uint8_t* mask, int w, int h, const float cur16[16], const float prev16[16],
+                   float* motion) {
+    M4 cur, prev;
+    std::memcpy(cur.m, cur16, 64);
+    std::memcpy(prev.m, prev16, 64);
+    const auto project = [](const M4& vp, V3 p, float& u, float& v) {   // motion.cpp:7-17
+        const V4 cs = mul(vp, V4{p.x, p.y, p.z, 1.0f});
+        if (cs.w <= 1e-6f) return false;
+        const float invw = 1.0f / cs.w;
+        const float ndcx = cs.x * invw, ndcy = cs.y * invw;
+        u = ndcx * 0.5f + 0.5f;
+        v = 1.0f - (ndcy * 0.5f + 0.5f);
+        return true;
+    };
+    int32_t valid = 0;
+    for (size_t i = 0; i < size_t(w) * size_t(h); ++i) {   // motion.cpp:18-31
+        motion[2 * i] = 0.0f;
+        motion[2 * i + 1] = 0.0f;
+        if (!mask[i]) continue;
+        const V3 p = ld3(position + 3 * i);
+        float cu, cv, pu, pv;
+        if (!project(cur, p, cu, cv) || !project(prev, p, pu, pv)) continue;
+        const float mx = cu - pu, my = cv - pv;
+        if (!std::isfinite(mx) || !std::isfinite(my)) continue;
+        motion[2 * i] = mx;
+        motion[2 * i + 1] = my;
+        ++valid;
+    }
+    return valid;
 }
 
 size_t orc_test_scene(float* vpos, float* vnrm, int32_t* idx) {

@@ -33,6 +33,29 @@ void orc_rasterize(const float* vpos, const float* vnrm, const int32_t* idx, siz
                    float fovy, float aspect, float nearZ, float farZ, int w, int h,
                    float* depth, float* position, float* normal, uint8_t* mask);
 
+/* The same over a scene of meshes concatenated in order (src/renderer/raster.cpp:107-124):
+ * mesh m holds triangles below mesh_end[m]; albedo_out (w*h*3) gets the albedo of the mesh
+ * whose fragment a pixel keeps (raster.cpp:82), 0 where nothing covers it. */
+void orc_rasterize_scene(const float* vpos, const float* vnrm, const int32_t* idx, size_t ntri,
+                         const uint32_t* mesh_end, const float* albedo, size_t nmesh,
+                         const float eye[3], const float center[3], const float up[3],
+                         float fovy, float aspect, float nearZ, float farZ, int w, int h,
+                         float* depth, float* position, float* normal, float* albedo_out, uint8_t* mask);
+/* perspective * lookAt (raw/math/mat.hpp:7-38), row-major. */
+void orc_view_proj(const float eye[3], const float center[3], const float up[3], float fovy,
+                   float aspect, float nearZ, float farZ, float out16[16]);
+/* shade/shadeHDR (model 1, plane_a = AO) and shadeV2/shadeHDRV2 (model 2, plane_a = sky,
+ * plane_sun = sun) (src/renderer/composite.cpp:8-65): frame clamped to [0,1], hdr not;
+ * ldir = the light's direction (not negated), li its intensity. */
+void orc_shade(const float* normal, const float* albedo, const uint8_t* mask, int w, int h, int model,
+               const float* plane_a, const float* plane_sun, const float ldir[3], float li,
+               float* frame, float* hdr);
+/* maxRadiance (src/renderer/composite.cpp:82-88). */
+float orc_max_radiance(const float* hdr, size_t npix);
+/* computeMotion (src/renderer/motion.cpp:7-31): returns the valid count. */
+int32_t orc_motion(const float* position, const uint8_t* mask, int w, int h, const float cur16[16],
+                   const float prev16[16], float* motion);
+
 /* The 14-triangle test scene (src/scene/scene.cpp:18-32) as one vertex/index list
  * (plane mesh then box mesh). Returns triangle count; pass NULL to size. */
 size_t orc_test_scene(float* vpos, float* vnrm, int32_t* idx);

@@ -7,7 +7,9 @@
 // Signatures mirror oracle_api.h with a ref_ prefix.
 #include "raw/cert/reconcile.hpp"
 #include "raw/renderer/accel.hpp"
+#include "raw/renderer/composite.hpp"
 #include "raw/renderer/raster.hpp"
+#include "raw/renderer/render.hpp"
 #include "raw/renderer/ray_ao.hpp"
 #include "raw/renderer/rt_bvh.hpp"
 #include "raw/renderer/rt_scenes.hpp"
@@ -153,6 +155,65 @@ void ref_reconcile(const float* approx, const float* truth, const uint8_t* mask,
     const raw::ReconcileResult r = raw::reconcile(a, t, m, tolerance);
     std::memcpy(error_map, r.errorMap.px.data(), sizeof(float) * size_t(w) * size_t(h));
     *rmse = r.rmse; *max_error = r.maxError; *pixels = r.pixels; *within = r.withinTolerance ? 1 : 0;
+}
+
+// renderWithParams (src/renderer/render.cpp:71-105) of a scene of meshes concatenated in order
+// (every mesh sees the whole vertex array; its indices select its own vertices): every plane of
+// the FrameResult, the reconcile, the motion counts, the view-projection and maxRadiance(hdr).
+// lighting 1/2 as RenderOptions::lighting; no_sun sets RAW_NATIVE_LIGHTING=v2-nosun for the call.
+// prev16 NULL: this frame's view-projection (zero motion).
+void ref_render_frame(const float* vpos, const float* vnrm, size_t nverts, const int32_t* idx, size_t ntri,
+                      const uint32_t* mesh_end, const float* albedo, size_t nmesh, const float eye[3], const float center[3],
+                      const float up[3], float fovy, float aspect, float nearZ, float farZ, int has_light,
+                      const float light_dir[3], float light_intensity, int w, int h, const float* prev16, int lighting,
+                      int no_sun, int rtao, float tolerance, int threads, float* depth, float* position, float* normal,
+                      float* albedo_out, uint8_t* mask, float* motion, float* ao_rt, float* ao_ss, float* error_map,
+                      float* frame, float* hdr, float* stats8, float* viewproj16) {
+    raw::Scene s;
+    size_t t0 = 0;
+    for (size_t m = 0; m < nmesh; ++m) {
+        raw::Mesh mesh;
+        for (size_t v = 0; v < nverts; ++v) {
+            mesh.positions.push_back({vpos[3 * v], vpos[3 * v + 1], vpos[3 * v + 2]});
+            mesh.normals.push_back({vnrm[3 * v], vnrm[3 * v + 1], vnrm[3 * v + 2]});
+        }
+        for (size_t i = 3 * t0; i < 3 * size_t(mesh_end[m]) && i < 3 * ntri; ++i) mesh.indices.push_back(idx[i]);
+        t0 = mesh_end[m];
+        mesh.material.albedo = {albedo[3 * m], albedo[3 * m + 1], albedo[3 * m + 2]};
+        s.meshes.push_back(std::move(mesh));
+    }
+    if (has_light) s.lights.push_back(raw::Light{{light_dir[0], light_dir[1], light_dir[2]}, light_intensity});
+    s.camera.eye = {eye[0], eye[1], eye[2]};
+    s.camera.center = {center[0], center[1], center[2]};
+    s.camera.up = {up[0], up[1], up[2]};
+    s.camera.fovy = fovy; s.camera.aspect = aspect; s.camera.nearZ = nearZ; s.camera.farZ = farZ;
+    raw::Mat4 prev = raw::mul(s.camera.proj(), s.camera.view());
+    if (prev16) std::memcpy(prev.m, prev16, 64);
+    raw::RenderOptions o;
+    o.tolerance = tolerance; o.rtao = rtao != 0; o.threads = threads; o.lighting = lighting;
+    unsetenv("RAW_NATIVE_AO_ORACLE");
+    if (no_sun) setenv("RAW_NATIVE_LIGHTING", "v2-nosun", 1);
+    else unsetenv("RAW_NATIVE_LIGHTING");
+    const raw::FrameResult r = raw::renderWithParams(s, w, h, prev, nullptr, o);
+    unsetenv("RAW_NATIVE_LIGHTING");
+    const size_t np = size_t(w) * size_t(h);
+    const auto v3 = [np](float* out, const raw::Buffer<raw::Vec3>& b) {
+        for (size_t i = 0; i < np; ++i) { out[3 * i] = b.px[i].x; out[3 * i + 1] = b.px[i].y; out[3 * i + 2] = b.px[i].z; }
+    };
+    std::memcpy(depth, r.g.depth.px.data(), 4 * np);
+    v3(position, r.g.position);
+    v3(normal, r.g.normal);
+    v3(albedo_out, r.g.albedo);
+    std::memcpy(mask, r.g.mask.px.data(), np);
+    for (size_t i = 0; i < np; ++i) { motion[2 * i] = r.g.motion.px[i].x; motion[2 * i + 1] = r.g.motion.px[i].y; }
+    if (rtao) std::memcpy(ao_rt, r.aoRT.px.data(), 4 * np);
+    std::memcpy(ao_ss, r.aoSS.px.data(), 4 * np);
+    std::memcpy(error_map, r.rec.errorMap.px.data(), 4 * np);
+    v3(frame, r.frame);
+    v3(hdr, r.hdr);
+    stats8[0] = r.rec.rmse; stats8[1] = r.rec.maxError; stats8[2] = float(r.rec.pixels); stats8[3] = r.rec.withinTolerance ? 1.0f : 0.0f;
+    stats8[4] = float(r.motionValid); stats8[5] = float(r.motionTotal); stats8[6] = raw::maxRadiance(r.hdr); stats8[7] = 0.0f;
+    std::memcpy(viewproj16, r.viewProj.m, 64);
 }
 
 int32_t ref_ploc_build(const float* tris9, size_t ntri, int radius, float* boxes6, int32_t* links3, uint32_t* rounds) {

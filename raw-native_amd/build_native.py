@@ -12,7 +12,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 BUILD = os.path.join(HERE, "build")
-HIP_SOURCES = ["csrc/capi.hip", "csrc/ploc.hip", "csrc/trace.hip", "csrc/bvh4.hip", "csrc/ssao.hip", "csrc/unet.hip", "csrc/batch.hip", "csrc/raster.hip",
+HIP_SOURCES = ["csrc/capi.hip", "csrc/ploc.hip", "csrc/trace.hip", "csrc/bvh4.hip", "csrc/ssao.hip", "csrc/unet.hip", "csrc/batch.hip", "csrc/raster.hip", "csrc/frame.hip",
                "csrc/trig_table.cpp"]
 ARCH = os.environ.get("RAW_AO_ARCH", "gfx950")
 

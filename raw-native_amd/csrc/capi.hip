@@ -97,11 +97,12 @@ void raw_ao_close(raw_ao_ctx* x) {
                                   &c.clus_a, &c.clus_b, &c.nn, &c.flags, &c.blk, &c.state, &c.g4, &c.bounds_part,
                                   &c.nodes, &c.tris_sorted, &c.nodes4, &c.wfront_a, &c.wfront_b, &c.wblk, &c.wstate, &c.st_pos, &c.st_nrm, &c.st_mask, &c.st_out, &c.st_out2,
                                   &c.st_cnt, &c.st_rays, &c.st_max, &c.counters, &c.r_keys, &c.r_tris, &c.r_counts,
-                                  &c.r_offs, &c.r_scan_tmp};
+                                  &c.r_offs, &c.r_scan_tmp, &c.fr_mesh, &c.fr_scalars, &c.fr_planes};
         for (auto* b : bufs)
             if (b->p) (void)hipFree(b->p);
         if (c.d_trig) (void)hipFree(c.d_trig);
         if (c.h_state) (void)hipHostFree(c.h_state);
+        if (c.fr_host) (void)hipHostFree(c.fr_host);
         for (auto& e : c.ev)
             if (e) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(c.stream);
@@ -384,6 +385,122 @@ int raw_ao_rasterize(raw_ao_ctx* x, const float* vpos, const float* vnrm, uint32
         !hip_ok(c, hipMemcpyAsync(mask, c.st_mask.p, np, hipMemcpyDeviceToHost, s), "download") ||
         !hip_ok(c, hipStreamSynchronize(s), "rasterize"))
         return RAW_AO_ERR_DEVICE;
+    return RAW_AO_OK;
+}
+
+void raw_ao_render_opts_default(raw_ao_render_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->tolerance = 0.12f;
+    o->rtao = 1;
+    o->lighting = 2;
+    o->rt_samples = 64;
+    o->ss_samples = 24;
+    o->radius = 2.0f;
+    o->bvh_radius = 16;
+}
+
+namespace {
+int frame_args(Ctx& c, const raw_ao_scene* sc, const raw_ao_render_opts& o, int w, int h) {
+    if (!sc) return fail(c, RAW_AO_ERR_ARG, "null scene");
+    const int a = raster_args(c, sc->vpos, sc->vnrm, sc->idx, sc->ntri, sc->eye, sc->center, sc->up, sc->fovy, sc->aspect,
+                              sc->near_z, sc->far_z, w, h);
+    if (a) return a;
+    if (sc->ntri > RAW_AO_MAX_TRIANGLES) return fail(c, RAW_AO_ERR_ARG, "more than RAW_AO_MAX_TRIANGLES triangles");
+    if (sc->nmesh < 1 || !sc->mesh_end || !sc->albedo) return fail(c, RAW_AO_ERR_ARG, "a scene needs at least one mesh");
+    for (uint32_t m = 0; m < sc->nmesh; ++m)
+        if ((m && sc->mesh_end[m] < sc->mesh_end[m - 1]) || sc->mesh_end[m] > sc->ntri)
+            return fail(c, RAW_AO_ERR_ARG, "mesh ends must be non-decreasing and at most ntri");
+    if (sc->mesh_end[sc->nmesh - 1] != sc->ntri) return fail(c, RAW_AO_ERR_ARG, "the last mesh must end at ntri");
+    if (o.lighting != 1 && o.lighting != 2) return fail(c, RAW_AO_ERR_ARG, "lighting must be 1 or 2");
+    if (o.rt_samples < 1 || o.rt_samples > 4096) return fail(c, RAW_AO_ERR_ARG, "rt_samples must be in 1..4096");
+    if (o.ss_samples < 0 || o.ss_samples > 4096) return fail(c, RAW_AO_ERR_ARG, "ss_samples must be in 0..4096");
+    if (!(o.radius > 0.0f)) return fail(c, RAW_AO_ERR_ARG, "radius must be > 0");
+    if (o.bvh_radius < 1 || o.bvh_radius > 32) return fail(c, RAW_AO_ERR_ARG, "PLOC radius must be in 1..32");
+    if (o.rtao && o.lighting == 2 && c.trace_kernel == RAW_AO_TRACE_BINARY)
+        return fail(c, RAW_AO_ERR_ARG, "lighting model 2 needs the wide traversal form (RAW_AO_TRACE_WIDE)");
+    return RAW_AO_OK;
+}
+}  // namespace
+
+int raw_ao_render_frame_dev(raw_ao_ctx* x, const raw_ao_scene* sc, const raw_ao_render_opts* opts, int w, int h,
+                            raw_ao_frame* d, raw_ao_frame_stats* stats, void* stream) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    raw_ao_render_opts o;
+    raw_ao_render_opts_default(&o);
+    if (opts) o = *opts;
+    const int a = frame_args(c, sc, o, w, h);
+    if (a) return a;
+    if (!d || !d->depth || !d->position || !d->normal || !d->albedo || !d->mask || !d->motion || !d->ao_rt || !d->ao_ss ||
+        !d->error || !d->frame || !d->hdr || (o.rtao && o.lighting == 2 && (!d->sky || !d->sun)))
+        return fail(c, RAW_AO_ERR_ARG, "null output plane");
+    if (c.host_trig) {
+        const int r = raw_ao::trig_table_upload(c);
+        if (r) return r;
+    }
+    return code_of(raw_ao::render_frame(c, *sc, o, w, h, *d, stats, pick(c, stream)));
+}
+
+int raw_ao_render_frame(raw_ao_ctx* x, const raw_ao_scene* sc, const raw_ao_render_opts* opts, int w, int h, raw_ao_frame* out,
+                        raw_ao_frame_stats* stats) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    raw_ao_render_opts o;
+    raw_ao_render_opts_default(&o);
+    if (opts) o = *opts;
+    const int a = frame_args(c, sc, o, w, h);
+    if (a) return a;
+    if (!out) return fail(c, RAW_AO_ERR_ARG, "null output");
+    for (size_t i = 0; i < 3 * size_t(sc->ntri); ++i)
+        if (sc->idx[i] < 0 || uint32_t(sc->idx[i]) >= sc->nverts) return fail(c, RAW_AO_ERR_ARG, "vertex index out of range");
+    const size_t np = size_t(w) * size_t(h);
+    const size_t vb = 12 * size_t(sc->nverts), ib = 12 * size_t(sc->ntri);
+    // planes in one allocation: 4 B depth, 12 position, 12 normal, 12 albedo, 8 motion, 4 each of
+    // ao_rt, ao_ss, sky, sun, error, 12 frame, 12 hdr, then 1 B mask
+    const size_t floats_px = 1 + 3 + 3 + 3 + 2 + 1 + 1 + 1 + 1 + 1 + 3 + 3;
+    if (!ensure(c, c.st_rays, 2 * vb) || !ensure(c, c.st_cnt, ib) || !ensure(c, c.fr_planes, 4 * floats_px * np + np))
+        return RAW_AO_ERR_DEVICE;
+    hipStream_t s = c.stream;
+    char* v = c.st_rays.as<char>();
+    if ((vb && (!hip_ok(c, hipMemcpyAsync(v, sc->vpos, vb, hipMemcpyHostToDevice, s), "upload") ||
+                !hip_ok(c, hipMemcpyAsync(v + vb, sc->vnrm, vb, hipMemcpyHostToDevice, s), "upload"))) ||
+        (ib && !hip_ok(c, hipMemcpyAsync(c.st_cnt.p, sc->idx, ib, hipMemcpyHostToDevice, s), "upload")))
+        return RAW_AO_ERR_DEVICE;
+    float* p = c.fr_planes.as<float>();
+    raw_ao_frame d;
+    d.depth = p; p += np;
+    d.position = p; p += 3 * np;
+    d.normal = p; p += 3 * np;
+    d.albedo = p; p += 3 * np;
+    d.motion = p; p += 2 * np;
+    d.ao_rt = p; p += np;
+    d.ao_ss = p; p += np;
+    d.sky = p; p += np;
+    d.sun = p; p += np;
+    d.error = p; p += np;
+    d.frame = p; p += 3 * np;
+    d.hdr = p; p += 3 * np;
+    d.mask = reinterpret_cast<uint8_t*>(p);
+    raw_ao_scene dsc = *sc;
+    dsc.vpos = reinterpret_cast<const float*>(v);
+    dsc.vnrm = reinterpret_cast<const float*>(v + vb);
+    dsc.idx = c.st_cnt.as<int32_t>();
+    const int r = raw_ao_render_frame_dev(x, &dsc, &o, w, h, &d, stats, nullptr);
+    if (r) return r;
+    const bool model2 = o.rtao && o.lighting == 2;
+    struct Plane { void* host; const void* dev; size_t bytes; bool ok; } planes[] = {
+        {out->depth, d.depth, 4, true}, {out->position, d.position, 12, true}, {out->normal, d.normal, 12, true},
+        {out->albedo, d.albedo, 12, true}, {out->mask, d.mask, 1, true}, {out->motion, d.motion, 8, true},
+        {out->ao_rt, d.ao_rt, 4, o.rtao != 0}, {out->ao_ss, d.ao_ss, 4, true}, {out->sky, d.sky, 4, model2},
+        {out->sun, d.sun, 4, model2}, {out->error, d.error, 4, true}, {out->frame, d.frame, 12, true},
+        {out->hdr, d.hdr, 12, true}};
+    for (const Plane& q : planes)
+        if (q.host && q.ok && !hip_ok(c, hipMemcpyAsync(q.host, q.dev, q.bytes * np, hipMemcpyDeviceToHost, s), "download"))
+            return RAW_AO_ERR_DEVICE;
+    if (!hip_ok(c, hipStreamSynchronize(s), "frame download")) return RAW_AO_ERR_DEVICE;
     return RAW_AO_OK;
 }
 

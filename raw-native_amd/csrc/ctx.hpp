@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include "../../include/raw_ao_hip.h"
 #include <string>
 #include <vector>
 
@@ -76,7 +77,7 @@ struct Ctx {
     TraceCounters last_counts{};
     bool have_counts{false};
 
-    hipEvent_t ev[8]{};
+    hipEvent_t ev[10]{};         // 0/1 build, 2/3 trace, 4/5 SSAO, 6/7 error map, 8/9 a whole frame
     float ms_build{0}, ms_trace{0}, ms_ssao{0}, ms_error{0};
 
     UNet* net{nullptr};          // the learned AO surrogate, made on first use
@@ -92,6 +93,12 @@ struct Ctx {
     // the G-buffer rasterizer: per-pixel (depth, draw id) keys, drawn triangles, their box
     // pixel counts and the counts' exclusive scan
     DevBuf r_keys, r_tris, r_counts, r_offs, r_scan_tmp;
+
+    // the frame: mesh table, device scalars, planes for the host-pointer form, and the pinned
+    // read-back of the error map (+ the scalars after it)
+    DevBuf fr_mesh, fr_scalars, fr_planes;
+    float* fr_host{nullptr};
+    size_t fr_host_cap{0};
 };
 
 bool ensure(Ctx& c, DevBuf& b, size_t bytes);
@@ -125,7 +132,15 @@ int error_map(Ctx& c, const float* d_a, const float* d_t, const uint8_t* d_mask,
 
 int rasterize(Ctx& c, const float* d_vpos, const float* d_vnrm, const int32_t* d_idx, uint32_t ntri, const float eye[3],
               const float center[3], const float up[3], float fovy, float aspect, float nearz, float farz, int w, int h,
-              float* d_depth, float* d_pos, float* d_nrm, uint8_t* d_mask, hipStream_t s);
+              float* d_depth, float* d_pos, float* d_nrm, uint8_t* d_mask, hipStream_t s, const uint32_t* d_mesh_end = nullptr,
+              const float* d_albedo = nullptr, uint32_t nmesh = 0, float* d_albedo_out = nullptr);
+// perspective(fovy, aspect, near, far) * lookAt(eye, center, up) as the reference computes it
+// (raw/math/mat.hpp:7-38), row-major.
+void view_proj(const float eye[3], const float center[3], const float up[3], float fovy, float aspect, float nearz, float farz,
+               float out[16]);
+
+int render_frame(Ctx& c, const raw_ao_scene& sc, const raw_ao_render_opts& o, int w, int h, const raw_ao_frame& d,
+                 raw_ao_frame_stats* stats, hipStream_t s);
 
 void net_release(Ctx& c);
 bool conv_shape_ok(int c0, int c1, int cout);

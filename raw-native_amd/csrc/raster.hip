@@ -169,9 +169,13 @@ __global__ __launch_bounds__(RBT) void k_rast_cover(const RTri* __restrict__ tri
     }
 }
 
+// alb_out (optional): the albedo of the winning triangle's mesh, mesh m holding the triangles
+// below mesh_end[m] (and at or above mesh_end[m-1]).
 __global__ __launch_bounds__(RBT) void k_rast_resolve(const RTri* __restrict__ tris, const unsigned long long* __restrict__ keys,
                                                       uint32_t npix, int w, float* __restrict__ depth, float* __restrict__ pos,
-                                                      float* __restrict__ nrm, uint8_t* __restrict__ mask) {
+                                                      float* __restrict__ nrm, uint8_t* __restrict__ mask,
+                                                      const uint32_t* __restrict__ mesh_end, const float* __restrict__ albedo,
+                                                      uint32_t nmesh, float* __restrict__ alb_out) {
     const uint32_t i = blockIdx.x * RBT + threadIdx.x;
     if (i >= npix) return;
     const unsigned long long key = keys[i];
@@ -180,11 +184,23 @@ __global__ __launch_bounds__(RBT) void k_rast_resolve(const RTri* __restrict__ t
         for (int k = 0; k < 3; ++k) {
             pos[3 * size_t(i) + k] = 0.0f;
             nrm[3 * size_t(i) + k] = 0.0f;
+            if (alb_out) alb_out[3 * size_t(i) + k] = 0.0f;
         }
         mask[i] = 0;
         return;
     }
-    const RTri& t = tris[uint32_t(key & 0xffffffffull)];
+    const uint32_t sub = uint32_t(key & 0xffffffffull);
+    if (alb_out) {
+        const uint32_t tri = sub >> 1;
+        uint32_t lo = 0, hi = nmesh - 1;   // the first mesh whose end is past tri
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (mesh_end[mid] > tri) hi = mid;
+            else lo = mid + 1;
+        }
+        for (int k = 0; k < 3; ++k) alb_out[3 * size_t(i) + k] = albedo[3 * size_t(lo) + k];
+    }
+    const RTri& t = tris[sub];
     float w0, w1, w2;
     weights(t, int(i % uint32_t(w)), int(i / uint32_t(w)), w0, w1, w2);
     const float iw = w0 * t.invw[0] + w1 * t.invw[1] + w2 * t.invw[2];
@@ -254,12 +270,20 @@ M4 look_at(V3 eye, V3 center, V3 up) {
 
 }  // namespace
 
-int rasterize(Ctx& c, const float* d_vpos, const float* d_vnrm, const int32_t* d_idx, uint32_t ntri, const float eye[3],
-              const float center[3], const float up[3], float fovy, float aspect, float nearz, float farz, int w, int h,
-              float* d_depth, float* d_pos, float* d_nrm, uint8_t* d_mask, hipStream_t s) {
-    const uint32_t npix = uint32_t(w) * uint32_t(h);
+void view_proj(const float eye[3], const float center[3], const float up[3], float fovy, float aspect, float nearz, float farz,
+               float out[16]) {
     const M4 vp = mul(perspective(fovy, aspect, nearz, farz),
                       look_at(v3(eye[0], eye[1], eye[2]), v3(center[0], center[1], center[2]), v3(up[0], up[1], up[2])));
+    for (int k = 0; k < 16; ++k) out[k] = vp.m[k];
+}
+
+int rasterize(Ctx& c, const float* d_vpos, const float* d_vnrm, const int32_t* d_idx, uint32_t ntri, const float eye[3],
+              const float center[3], const float up[3], float fovy, float aspect, float nearz, float farz, int w, int h,
+              float* d_depth, float* d_pos, float* d_nrm, uint8_t* d_mask, hipStream_t s, const uint32_t* d_mesh_end,
+              const float* d_albedo, uint32_t nmesh, float* d_albedo_out) {
+    const uint32_t npix = uint32_t(w) * uint32_t(h);
+    M4 vp;
+    view_proj(eye, center, up, fovy, aspect, nearz, farz, vp.m);
     const uint32_t nsub = 2 * ntri;
     if (!ensure(c, c.r_keys, 8 * size_t(npix)) || !ensure(c, c.r_tris, sizeof(RTri) * std::max<size_t>(1, nsub)) ||
         !ensure(c, c.r_counts, 8 * (size_t(nsub) + 1)) || !ensure(c, c.r_offs, 8 * (size_t(nsub) + 1)))
@@ -284,7 +308,8 @@ int rasterize(Ctx& c, const float* d_vpos, const float* d_vnrm, const int32_t* d
                                          c.r_keys.as<unsigned long long>());
     }
     k_rast_resolve<<<(npix + RBT - 1) / RBT, RBT, 0, s>>>(c.r_tris.as<RTri>(), c.r_keys.as<unsigned long long>(), npix, w, d_depth,
-                                                         d_pos, d_nrm, d_mask);
+                                                         d_pos, d_nrm, d_mask, d_mesh_end, d_albedo, nmesh,
+                                                         nmesh ? d_albedo_out : nullptr);
     return hip_ok(c, hipGetLastError(), "rasterize launch") ? 0 : -2;
 }
 

@@ -56,6 +56,9 @@ SIGNATURES = [
                                    C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, _f32p, _f32p, _f32p, _u8p]),
     ("raw_ao_rasterize_dev", C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _f32p, _f32p, _f32p, C.c_float, C.c_float, C.c_float,
                                        C.c_float, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]),
+    ("raw_ao_render_frame", C.c_int, [_vp, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    ("raw_ao_render_frame_dev", C.c_int, [_vp, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, _vp]),
+    ("raw_ao_render_opts_default", None, [C.c_void_p]),
     ("raw_ao_ssao", C.c_int, [_vp, _f32p, _f32p, _u8p, C.c_int, C.c_int, C.c_int, C.c_float, _f32p]),
     ("raw_ao_ssao_dev", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp]),
     ("raw_ao_error_map", C.c_int, [_vp, _f32p, _f32p, _u8p, C.c_int, C.c_int, _f32p, C.POINTER(C.c_float)]),
@@ -110,6 +113,46 @@ class LearnedReport(C.Structure):
     def as_dict(self):
         return dict(probe_pixels=self.probe_pixels, probe_rmse=self.probe_rmse, probe_max_error=self.probe_max_error,
                     fell_back=bool(self.fell_back))
+
+
+_F3 = C.c_float * 3
+
+
+class SceneDesc(C.Structure):
+    """raw_ao_scene"""
+    _fields_ = [("vpos", C.c_void_p), ("vnrm", C.c_void_p), ("nverts", C.c_uint32), ("idx", C.c_void_p),
+                ("ntri", C.c_uint32), ("mesh_end", C.c_void_p), ("albedo", C.c_void_p), ("nmesh", C.c_uint32),
+                ("eye", _F3), ("center", _F3), ("up", _F3), ("fovy", C.c_float), ("aspect", C.c_float),
+                ("near_z", C.c_float), ("far_z", C.c_float), ("has_light", C.c_int32), ("light_dir", _F3),
+                ("light_intensity", C.c_float)]
+
+
+class RenderOpts(C.Structure):
+    """raw_ao_render_opts"""
+    _fields_ = [("tolerance", C.c_float), ("rtao", C.c_int32), ("lighting", C.c_int32), ("no_sun", C.c_int32),
+                ("rt_samples", C.c_int32), ("ss_samples", C.c_int32), ("radius", C.c_float), ("bvh_radius", C.c_int32),
+                ("prev_viewproj", C.c_void_p)]
+
+
+FRAME_PLANES = ("depth", "position", "normal", "albedo", "mask", "motion", "ao_rt", "ao_ss", "sky", "sun", "error",
+                "frame", "hdr")
+
+
+class FramePlanes(C.Structure):
+    """raw_ao_frame"""
+    _fields_ = [(k, C.c_void_p) for k in FRAME_PLANES]
+
+
+class FrameStats(C.Structure):
+    """raw_ao_frame_stats"""
+    _fields_ = [("rmse", C.c_float), ("max_error", C.c_float), ("pixels", C.c_int32), ("within_tolerance", C.c_int32),
+                ("motion_valid", C.c_int32), ("motion_total", C.c_int32), ("max_radiance", C.c_float),
+                ("viewproj", C.c_float * 16), ("gpu_ms", C.c_float)]
+
+    def as_dict(self):
+        return dict(rmse=self.rmse, max_error=self.max_error, pixels=self.pixels, within=bool(self.within_tolerance),
+                    motion_valid=self.motion_valid, motion_total=self.motion_total, max_radiance=self.max_radiance,
+                    viewproj=np.array(self.viewproj[:], np.float32), gpu_ms=self.gpu_ms)
 
 
 class RawAoError(RuntimeError):
@@ -233,6 +276,78 @@ class Context:
         self._check(lib().raw_ao_trace_visibility_dev(self._h, _ptr(position), _ptr(normal), _ptr(mask), w, h, samples,
                                                       radius, np.asarray(to_light, np.float32), int(bool(sun_forced)),
                                                       _ptr(ao), _ptr(sky), _ptr(sun), _vp(stream)))
+
+    # ---- the default frame ---------------------------------------------------------
+    @staticmethod
+    def _scene_desc(sc, keep, ptr):
+        """raw_ao_scene from a dict (vpos, vnrm, idx, mesh_end, albedo, eye, center, up, fovy, aspect,
+        near, far, has_light, light_dir, light_intensity); ptr maps the mesh arrays to addresses."""
+        me = np.ascontiguousarray(sc["mesh_end"], np.uint32)
+        al = np.ascontiguousarray(sc["albedo"], np.float32).reshape(-1, 3)
+        keep += [me, al]
+        d = SceneDesc()
+        d.vpos, d.vnrm, d.idx = ptr(sc["vpos"]), ptr(sc["vnrm"]), ptr(sc["idx"])
+        d.nverts = int(np.prod(sc["vpos"].shape)) // 3
+        d.ntri = int(np.prod(sc["idx"].shape)) // 3
+        d.mesh_end, d.albedo, d.nmesh = me.ctypes.data, al.ctypes.data, len(me)
+        d.eye, d.center, d.up = (_F3(*[float(v) for v in sc[k]]) for k in ("eye", "center", "up"))
+        d.fovy, d.aspect, d.near_z, d.far_z = sc["fovy"], sc["aspect"], sc["near"], sc["far"]
+        d.has_light = int(bool(sc.get("has_light", False)))
+        d.light_dir = _F3(*[float(v) for v in sc.get("light_dir", (0.0, -1.0, 0.0))])
+        d.light_intensity = float(sc.get("light_intensity", 1.0))
+        return d
+
+    @staticmethod
+    def _render_opts(keep, prev, lighting, no_sun, rtao, tolerance, rt_samples, ss_samples, radius):
+        o = RenderOpts()
+        lib().raw_ao_render_opts_default(C.byref(o))
+        o.tolerance, o.rtao, o.lighting, o.no_sun = tolerance, int(bool(rtao)), lighting, int(bool(no_sun))
+        o.rt_samples, o.ss_samples, o.radius = rt_samples, ss_samples, radius
+        if prev is not None:
+            pv = np.ascontiguousarray(prev, np.float32).reshape(16)
+            keep.append(pv)
+            o.prev_viewproj = pv.ctypes.data
+        return o
+
+    def render_frame(self, sc, w, h, prev=None, lighting=2, no_sun=False, rtao=True, tolerance=0.12, rt_samples=64,
+                     ss_samples=24, radius=2.0):
+        """renderWithParams on the GPU: a dict of host planes and the frame's scalars."""
+        keep = []
+
+        def host(a, t):
+            a = np.ascontiguousarray(a, t)
+            keep.append(a)
+            return a.ctypes.data
+
+        sc = dict(sc, vpos=np.ascontiguousarray(sc["vpos"], np.float32), vnrm=np.ascontiguousarray(sc["vnrm"], np.float32),
+                  idx=np.ascontiguousarray(sc["idx"], np.int32))
+        d = self._scene_desc(sc, keep, lambda a: host(a, a.dtype))
+        o = self._render_opts(keep, prev, lighting, no_sun, rtao, tolerance, rt_samples, ss_samples, radius)
+        shapes = dict(depth=(h, w), position=(h, w, 3), normal=(h, w, 3), albedo=(h, w, 3), mask=(h, w), motion=(h, w, 2),
+                      ao_rt=(h, w), ao_ss=(h, w), sky=(h, w), sun=(h, w), error=(h, w), frame=(h, w, 3), hdr=(h, w, 3))
+        out = {k: np.zeros(shapes[k], np.uint8 if k == "mask" else np.float32) for k in FRAME_PLANES}
+        planes = FramePlanes(*[out[k].ctypes.data for k in FRAME_PLANES])
+        st = FrameStats()
+        self._check(lib().raw_ao_render_frame(self._h, C.byref(d), C.byref(o), w, h, C.byref(planes), C.byref(st)))
+        if not rtao:
+            del out["ao_rt"]
+        if not (rtao and lighting == 2):
+            del out["sky"], out["sun"]
+        out.update(st.as_dict())
+        return out
+
+    def render_frame_dev(self, sc, w, h, planes, prev=None, lighting=2, no_sun=False, rtao=True, tolerance=0.12,
+                         rt_samples=64, ss_samples=24, radius=2.0, stream=None):
+        """The same with the mesh (vpos, vnrm, idx) and every plane in device memory (torch tensors);
+        returns the frame's scalars."""
+        keep = []
+        d = self._scene_desc(sc, keep, lambda t: t.data_ptr())
+        o = self._render_opts(keep, prev, lighting, no_sun, rtao, tolerance, rt_samples, ss_samples, radius)
+        fp = FramePlanes(*[planes[k].data_ptr() if planes.get(k) is not None else None for k in FRAME_PLANES])
+        st = FrameStats()
+        self._check(lib().raw_ao_render_frame_dev(self._h, C.byref(d), C.byref(o), w, h, C.byref(fp), C.byref(st),
+                                                  _vp(stream)))
+        return st.as_dict()
 
     # ---- G-buffer ------------------------------------------------------------------
     def rasterize(self, vpos, vnrm, idx, eye, center, up, fovy, aspect, near, far, w, h):

@@ -62,6 +62,14 @@ def lib():
         L.orc_scene_tris.argtypes = [C.c_char_p, SZ, C.c_uint64, C.c_void_p]
         L.orc_scene_tris.restype = SZ
         L.orc_libm_angles.argtypes = [f32p, f32p]
+        L.orc_rasterize_scene.argtypes = [f32p, f32p, i32p, SZ, u32p, f32p, SZ, f32p, f32p, f32p, C.c_float, C.c_float,
+                                          C.c_float, C.c_float, C.c_int, C.c_int, f32p, f32p, f32p, f32p, u8p]
+        L.orc_view_proj.argtypes = [f32p, f32p, f32p, C.c_float, C.c_float, C.c_float, C.c_float, f32p]
+        L.orc_shade.argtypes = [f32p, f32p, u8p, C.c_int, C.c_int, C.c_int, f32p, f32p, f32p, C.c_float, f32p, f32p]
+        L.orc_max_radiance.argtypes = [f32p, SZ]
+        L.orc_max_radiance.restype = C.c_float
+        L.orc_motion.argtypes = [f32p, u8p, C.c_int, C.c_int, f32p, f32p, f32p]
+        L.orc_motion.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -91,6 +99,10 @@ def ref():
         L.ref_rtao_accel.argtypes = [C.c_void_p, f32p, f32p, u8p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, f32p]
         L.ref_rtvis.argtypes = [f32p, SZ, f32p, f32p, u8p, C.c_int, C.c_int, C.c_int, C.c_float, f32p, C.c_int, C.c_int,
                                 f32p, f32p, f32p]
+        L.ref_render_frame.argtypes = [f32p, f32p, SZ, i32p, SZ, u32p, f32p, SZ, f32p, f32p, f32p, C.c_float, C.c_float,
+                                       C.c_float, C.c_float, C.c_int, f32p, C.c_float, C.c_int, C.c_int, C.c_void_p,
+                                       C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, f32p, f32p, f32p, f32p, u8p, f32p,
+                                       f32p, f32p, f32p, f32p, f32p, f32p, f32p]
         _ref = L
     return _ref
 
@@ -263,6 +275,118 @@ def pgm_bytes(img):
     return b"P5\n%d %d\n255\n" % (w, h) + b.tobytes()
 
 
+def rgb8(img):
+    """to8 per channel (src/core/image.cpp:9-10): (unsigned char)(clamp(v, 0, 1) * 255 + 0.5)."""
+    v = np.clip(np.asarray(img, np.float32), np.float32(0), np.float32(1))
+    return np.floor(v * np.float32(255.0) + np.float32(0.5)).astype(np.uint8)
+
+
+def ppm_bytes(img):
+    """writePPM (src/core/image.cpp:11-18)."""
+    h, w, _ = img.shape
+    return b"P6\n%d %d\n255\n" % (w, h) + rgb8(img).tobytes()
+
+
+def pfm3_bytes(img):
+    """writePFM (src/core/image.cpp:40-52): rows bottom to top, little-endian float32."""
+    h, w, _ = img.shape
+    return b"PF\n%d %d\n-1.0\n" % (w, h) + np.ascontiguousarray(img[::-1], dtype="<f4").tobytes()
+
+
 def mask_pgm_bytes(mask):
     h, w = mask.shape
     return b"P5\n%d %d\n255\n" % (w, h) + np.where(mask != 0, 255, 0).astype(np.uint8).tobytes()
+
+
+# ---- the default frame (renderWithParams, src/renderer/render.cpp:71-105) ------------
+
+def f32_normalize(v):
+    """normalize (raw/math/vec.hpp) in float32: v * (1 / sqrt(dot(v, v)))."""
+    v = np.asarray(v, np.float32)
+    length = np.sqrt(np.float32(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]), dtype=np.float32)
+    return (v * (np.float32(1.0) / length)).astype(np.float32)
+
+
+def frame_test_scene(w, h, eye=(4, 4, 6), target=(0, 1, 0), fovy=0.9):
+    """buildTestScene (src/scene/scene.cpp:18-32) as a frame scene: the plane mesh (albedo 0.7)
+    and the box mesh (0.8, 0.3, 0.2), one light; the camera as cameraFromParams makes it
+    (src/tools/cli_params.cpp:12-18), aspect (float)w / (float)h."""
+    vpos, vnrm, idx = test_scene_mesh()
+    return dict(vpos=vpos, vnrm=vnrm, idx=idx, mesh_end=np.array([2, idx.shape[0]], np.uint32),
+                albedo=np.array([[0.7, 0.7, 0.7], [0.8, 0.3, 0.2]], np.float32),
+                eye=np.array(eye, np.float32), center=np.array(target, np.float32),
+                up=np.array([0, 1, 0], np.float32), fovy=float(np.float32(fovy)), aspect=float(np.float32(w) / np.float32(h)),
+                near=0.1, far=100.0, has_light=True, light_dir=f32_normalize((-0.4, -1.0, -0.3)), light_intensity=1.0)
+
+
+def view_proj(sc):
+    out = np.zeros(16, np.float32)
+    lib().orc_view_proj(sc["eye"], sc["center"], sc["up"], sc["fovy"], sc["aspect"], sc["near"], sc["far"], out)
+    return out
+
+
+def frame(sc, w, h, prev=None, lighting=2, no_sun=False, rtao_on=True, tol=0.12, threads=None):
+    """The oracle's frame: renderWithParams restated from the oracle's passes, in its order."""
+    L = lib()
+    vpos, vnrm, idx = (np.ascontiguousarray(sc[k], t) for k, t in (("vpos", np.float32), ("vnrm", np.float32), ("idx", np.int32)))
+    ntri = idx.shape[0]
+    depth = np.zeros((h, w), np.float32)
+    pos, nrm, alb = (np.zeros((h, w, 3), np.float32) for _ in range(3))
+    mask = np.zeros((h, w), np.uint8)
+    L.orc_rasterize_scene(vpos, vnrm, idx, ntri, np.ascontiguousarray(sc["mesh_end"], np.uint32),
+                          np.ascontiguousarray(sc["albedo"], np.float32), len(sc["mesh_end"]), sc["eye"], sc["center"],
+                          sc["up"], sc["fovy"], sc["aspect"], sc["near"], sc["far"], w, h, depth, pos, nrm, alb, mask)
+    vp = view_proj(sc)
+    prev = vp if prev is None else np.ascontiguousarray(prev, np.float32)
+    motion = np.zeros((h, w, 2), np.float32)
+    valid = L.orc_motion(pos, mask, w, h, vp, prev, motion)
+    ldir = np.asarray(sc["light_dir"], np.float32) if sc["has_light"] else np.array([0, -1, 0], np.float32)
+    li = float(sc["light_intensity"]) if sc["has_light"] else 1.0
+    model = lighting if rtao_on else 1
+    out = dict(depth=depth, position=pos, normal=nrm, albedo=alb, mask=mask, motion=motion, viewproj=vp,
+               motion_valid=int(valid), motion_total=int(mask.sum()))
+    if rtao_on:
+        tris = mesh_tris(vpos, idx)
+        to_light = (ldir * np.float32(-1.0)).astype(np.float32)
+        if model == 2:
+            out["ao_rt"], out["sky"], out["sun"] = rtvis(tris, pos, nrm, mask, to_light, 64, 2.0, no_sun, threads=threads)
+        else:
+            out["ao_rt"] = rtao(tris, pos, nrm, mask, 64, 2.0, threads=threads)
+    out["ao_ss"] = ssao(pos, nrm, mask, 24, 2.0, threads=threads)
+    if rtao_on:
+        out["error"], out["rmse"], out["max_error"], out["pixels"], out["within"] = reconcile(out["ao_ss"], out["ao_rt"], mask,
+                                                                                            tol)
+    else:
+        out["error"], out["rmse"], out["max_error"], out["pixels"], out["within"] = (np.zeros((h, w), np.float32), 0.0,
+                                                                                    0.0, 0, False)
+    plane_a = out["sky"] if model == 2 else (out["ao_rt"] if rtao_on else out["ao_ss"])
+    plane_sun = out["sun"] if model == 2 else plane_a
+    fr, hdr = np.zeros((h, w, 3), np.float32), np.zeros((h, w, 3), np.float32)
+    L.orc_shade(nrm, alb, mask, w, h, model, np.ascontiguousarray(plane_a), np.ascontiguousarray(plane_sun), ldir, li, fr, hdr)
+    out["frame"], out["hdr"] = fr, hdr
+    out["max_radiance"] = float(L.orc_max_radiance(hdr, w * h))
+    return out
+
+
+def ref_frame(sc, w, h, prev=None, lighting=2, no_sun=False, rtao_on=True, tol=0.12, threads=None):
+    """renderWithParams itself (the reference's sources, oracle/_ref), the same planes."""
+    vpos, vnrm, idx = (np.ascontiguousarray(sc[k], t) for k, t in (("vpos", np.float32), ("vnrm", np.float32), ("idx", np.int32)))
+    depth = np.zeros((h, w), np.float32)
+    pos, nrm, alb, fr, hdr = (np.zeros((h, w, 3), np.float32) for _ in range(5))
+    mask = np.zeros((h, w), np.uint8)
+    motion = np.zeros((h, w, 2), np.float32)
+    ao_rt, ao_ss, em = (np.zeros((h, w), np.float32) for _ in range(3))
+    st, vp = np.zeros(8, np.float32), np.zeros(16, np.float32)
+    prev_p = None if prev is None else np.ascontiguousarray(prev, np.float32).ctypes.data
+    ref().ref_render_frame(vpos, vnrm, vpos.shape[0], idx, idx.shape[0], np.ascontiguousarray(sc["mesh_end"], np.uint32),
+                           np.ascontiguousarray(sc["albedo"], np.float32), len(sc["mesh_end"]), sc["eye"], sc["center"],
+                           sc["up"], sc["fovy"], sc["aspect"], sc["near"], sc["far"], int(bool(sc["has_light"])),
+                           np.asarray(sc["light_dir"], np.float32), float(sc["light_intensity"]), w, h, prev_p, lighting,
+                           int(bool(no_sun)), int(bool(rtao_on)), tol, threads or os.cpu_count() or 1, depth, pos, nrm,
+                           alb, mask, motion, ao_rt, ao_ss, em, fr, hdr, st, vp)
+    out = dict(depth=depth, position=pos, normal=nrm, albedo=alb, mask=mask, motion=motion, ao_ss=ao_ss, error=em,
+               frame=fr, hdr=hdr, rmse=float(st[0]), max_error=float(st[1]), pixels=int(st[2]), within=bool(st[3]),
+               motion_valid=int(st[4]), motion_total=int(st[5]), max_radiance=float(st[6]), viewproj=vp)
+    if rtao_on:
+        out["ao_rt"] = ao_rt
+    return out

@@ -293,3 +293,110 @@ def test_rtao_ssao_match_reference_code():
     ref_ss = np.zeros(mask.shape, np.float32)
     R.ref_ssao(pos, nrm, mask, 128, 96, 24, 2.0, 4, ref_ss)
     assert np.array_equal(O.ssao(pos, nrm, mask, 24, 2.0).view(np.uint32), ref_ss.view(np.uint32))
+
+
+# ---- the default frame: shade, shade_hdr and motion on top of the passes above --------
+
+def _frame_case(name, lighting):
+    from cases import params
+    p = params(name)
+    sc = O.frame_test_scene(p["w"], p["h"], p["eye"], p["target"], p["fovy"])
+    prev = None
+    if p["prev_eye"]:
+        ps = dict(sc)
+        ps["eye"] = np.array(p["prev_eye"], np.float32)
+        prev = O.view_proj(ps)
+    return O.frame(sc, p["w"], p["h"], prev=prev, lighting=lighting, rtao_on=p["rt"])
+
+
+@pytest.mark.parametrize("lighting,key", [(2, "files"), (1, "files_lighting_v1")])
+@pytest.mark.parametrize("name", ["default", "motion-256", "nort-256", "close-512", "wide-512"])
+def test_frame_golden_hashes(name, lighting, key):
+    """frame.ppm and frame_hdr.pfm of the reference's identity renders (evidence/identity-golden.json
+    for lighting model 2, identity-golden-v1.json for model 1) from the oracle's frame."""
+    from cases import FRAMES
+    f = _frame_case(name, lighting)
+    g = FRAMES[name][key]
+    assert sha(O.ppm_bytes(f["frame"])) == g["frame.ppm"]
+    assert sha(O.pfm3_bytes(f["hdr"])) == g["frame_hdr.pfm"]
+    if key == "files":
+        assert sha(O.mask_pgm_bytes(f["mask"])) == g["mask.pgm"]
+        assert sha(O.pfm1_bytes(f["ao_ss"])) == g["ao_ss.pfm"]
+
+
+def test_motion_closed_forms():
+    """A static camera gives zero motion on every covered pixel (all valid); a previous eye
+    moves every covered pixel; projecting the current view-projection of a pixel's position
+    lands inside that pixel (u*w, v*h within the pixel's square)."""
+    w, h = 96, 64
+    sc = O.frame_test_scene(w, h)
+    f = O.frame(sc, w, h, rtao_on=False)
+    assert (f["motion"] == 0).all() and f["motion_valid"] == f["motion_total"] == int(f["mask"].sum())
+    ps = dict(sc)
+    ps["eye"] = np.array([4.3, 4.0, 5.7], np.float32)
+    g = O.frame(sc, w, h, prev=O.view_proj(ps), rtao_on=False)
+    m = g["mask"] != 0
+    assert g["motion_valid"] == int(m.sum())
+    assert (np.abs(g["motion"][m]).sum(axis=1) > 0).mean() > 0.99
+    vp = O.view_proj(sc).reshape(4, 4).astype(np.float64)
+    ys, xs = np.nonzero(m)
+    p = np.concatenate([g["position"][m].astype(np.float64), np.ones((len(xs), 1))], axis=1) @ vp.T
+    u = (p[:, 0] / p[:, 3] * 0.5 + 0.5) * w
+    v = (1 - (p[:, 1] / p[:, 3] * 0.5 + 0.5)) * h
+    assert (np.abs(u - (xs + 0.5)) <= 0.5 + 1e-3).all() and (np.abs(v - (ys + 0.5)) <= 0.5 + 1e-3).all()
+
+
+def test_shade_closed_forms():
+    """Model 2 with sky = sun = 1 equals albedo x (0.2 + max(0, n.-l) li) computed in float64
+    within float32 rounding; model 1 with ao = 0 is black; frame = clip(hdr, 0, 1)."""
+    w, h = 80, 60
+    sc = O.frame_test_scene(w, h)
+    f = O.frame(sc, w, h)
+    L = O.lib()
+    ones = np.ones((h, w), np.float32)
+    fr, hdr = np.zeros((h, w, 3), np.float32), np.zeros((h, w, 3), np.float32)
+    L.orc_shade(f["normal"], f["albedo"], f["mask"], w, h, 2, ones, ones, sc["light_dir"], 1.5, fr, hdr)
+    ndl = np.maximum(0, -(f["normal"].astype(np.float64) @ sc["light_dir"].astype(np.float64))) * 1.5
+    want = f["albedo"] * (0.2 + ndl)[..., None] * (f["mask"] != 0)[..., None]
+    assert np.allclose(hdr, want, rtol=1e-6, atol=1e-7)
+    assert np.array_equal(fr, np.clip(hdr, 0, 1))
+    L.orc_shade(f["normal"], f["albedo"], f["mask"], w, h, 1, np.zeros((h, w), np.float32), ones, sc["light_dir"], 1.0, fr, hdr)
+    assert (hdr == 0).all() and (fr == 0).all()
+    assert f["max_radiance"] == float(f["hdr"].max())
+
+
+def _multi_mesh_scene(w, h):
+    """A seeded scene split into meshes of uneven size, each with its own albedo, a light of
+    intensity 1.7 and a camera low among the objects."""
+    import raw_ao.host as H
+    vpos, vnrm, idx, _ = H.scene_seeded(H.SceneSpec(0xF4A3, 60, 0.4, 2, 0.1, 1.5, 14.0, 4.0))
+    rng = np.random.default_rng(3)
+    ends = np.sort(rng.choice(np.arange(1, idx.shape[0]), 6, replace=False)).astype(np.uint32)
+    ends = np.append(ends, np.uint32(idx.shape[0]))
+    return dict(vpos=vpos, vnrm=vnrm, idx=idx, mesh_end=ends, albedo=rng.uniform(0.1, 1.0, (len(ends), 3)).astype(np.float32),
+                eye=np.array([6.0, 2.5, 7.0], np.float32), center=np.array([0.0, 0.8, 0.0], np.float32),
+                up=np.array([0, 1, 0], np.float32), fovy=1.0, aspect=float(np.float32(w) / np.float32(h)), near=0.1,
+                far=100.0, has_light=True, light_dir=O.f32_normalize((0.3, -1.0, 0.5)), light_intensity=1.7)
+
+
+def _same_frame(a, b):
+    for k, v in b.items():
+        if isinstance(v, np.ndarray):
+            assert np.array_equal(np.ascontiguousarray(a[k]).view(np.uint8), np.ascontiguousarray(v).view(np.uint8)), k
+        else:
+            assert a[k] == v, k
+
+
+@needs_ref
+@pytest.mark.parametrize("opts", [dict(), dict(lighting=1), dict(no_sun=True), dict(rtao_on=False)])
+def test_frame_matches_reference_code(opts):
+    """Every plane and scalar of renderWithParams (the reference's code) from the oracle's frame:
+    the test scene with a moving camera, and a seeded multi-mesh scene."""
+    w, h = 96, 72
+    sc = O.frame_test_scene(w, h)
+    ps = dict(sc)
+    ps["eye"] = np.array([4.3, 4.0, 5.7], np.float32)
+    prev = O.view_proj(ps)
+    _same_frame(O.frame(sc, w, h, prev=prev, **opts), O.ref_frame(sc, w, h, prev=prev, **opts))
+    sc = _multi_mesh_scene(80, 60)
+    _same_frame(O.frame(sc, 80, 60, **opts), O.ref_frame(sc, 80, 60, **opts))
