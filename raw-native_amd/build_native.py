@@ -38,7 +38,7 @@ def build(verbose=False):
     os.replace(tmp, hip_out)
     host_out = os.path.join(BUILD, "libraw_ao_host.so")
     tmp = host_out + ".tmp"
-    out += run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-Wall", "host/raw_ao_host.cpp",
+    out += run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-Wall", "host/raw_ao_host.cpp", "host/raw_ao_outputs.cpp",
                 "-L" + BUILD, "-lraw_ao_hip", "-Wl,-rpath,$ORIGIN", "-o", tmp], HERE)
     os.replace(tmp, host_out)
     # the C++ host interface's frame test (tests/cpp), linked to the host library

@@ -45,8 +45,78 @@ def lib():
         L.rah_rasterize.restype = None
         L.rah_rasterize.argtypes = [_f32p, _f32p, _i32p, C.c_size_t, _f32p, _f32p, _f32p, C.c_float, C.c_float, C.c_float,
                                     C.c_float, C.c_int, C.c_int, _f32p, _f32p, _f32p, _u8p]
+        L.rah_emit_outputs.restype = C.c_int
+        L.rah_emit_outputs.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_char_p, C.c_size_t]
+        L.rah_verify.restype = C.c_int
+        L.rah_verify.argtypes = [C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
+        L.rah_sha256.restype = None
+        L.rah_sha256.argtypes = [C.c_void_p, C.c_size_t, C.c_char_p]
         _lib = L
     return _lib
+
+
+_F3 = C.c_float * 3
+
+
+class CliParams(C.Structure):
+    """rah_params: the CLI parameters a render's files record (raw/tools/cli_params.hpp)."""
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("eye", _F3), ("center", _F3), ("up", _F3),
+                ("fovy", C.c_float), ("tolerance", C.c_float), ("rtao", C.c_int32), ("gpu", C.c_int32),
+                ("gpu_backend", C.c_char_p), ("has_prev_eye", C.c_int32), ("has_prev_center", C.c_int32),
+                ("has_prev_up", C.c_int32), ("prev_eye", _F3), ("prev_center", _F3), ("prev_up", _F3)]
+
+
+class _Frame(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("depth", "position", "normal", "albedo", "mask", "motion", "ao_rt", "ao_ss",
+                                          "error", "frame", "hdr")] + \
+               [("rmse", C.c_float), ("max_error", C.c_float), ("pixels", C.c_int32), ("within", C.c_int32),
+                ("motion_valid", C.c_int32), ("motion_total", C.c_int32)]
+
+
+def cli_params(w=256, h=256, eye=(4, 4, 6), target=(0, 1, 0), up=(0, 1, 0), fovy=0.9, tolerance=0.12, rt=True,
+               prev_eye=None, prev_target=None, prev_up=None, gpu_backend=None):
+    p = CliParams(width=w, height=h, fovy=fovy, tolerance=tolerance, rtao=int(bool(rt)))
+    p.eye, p.center, p.up = _F3(*eye), _F3(*target), _F3(*up)
+    if gpu_backend is not None:
+        p.gpu, p.gpu_backend = 1, gpu_backend.encode()
+    for name, v in (("eye", prev_eye), ("center", prev_target), ("up", prev_up)):
+        if v is not None:
+            setattr(p, "has_prev_" + name, 1)
+            setattr(p, "prev_" + name, _F3(*v))
+    return p
+
+
+def emit_outputs(out_dir, frame, params, backend="raw-native-cpu"):
+    """Write a frame's files as the reference CLI does (frame.ppm, frame_hdr.pfm, the AO images,
+    mask.pgm, certificate.json, channels.json, receipt.json). frame: a dict of planes and the
+    reconcile / motion scalars (raw_ao.Context.render_frame's result)."""
+    keep = []
+
+    def ptr(k, t):
+        if frame.get(k) is None:
+            return None
+        a = np.ascontiguousarray(frame[k], t)
+        keep.append(a)
+        return a.ctypes.data
+
+    f = _Frame(*[ptr(k, np.uint8 if k == "mask" else np.float32) for k in
+                 ("depth", "position", "normal", "albedo", "mask", "motion", "ao_rt", "ao_ss", "error", "frame", "hdr")])
+    f.rmse, f.max_error = frame["rmse"], frame["max_error"]
+    f.pixels, f.within = frame["pixels"], int(bool(frame["within"]))
+    f.motion_valid, f.motion_total = frame["motion_valid"], frame["motion_total"]
+    err = C.create_string_buffer(512)
+    rc = lib().rah_emit_outputs(str(out_dir).encode(), C.byref(params), C.byref(f), backend.encode(), err, 512)
+    if rc != 0:
+        raise RuntimeError("emit_outputs: %s" % err.value.decode())
+
+
+def verify(out_dir):
+    """raw_native_cli verify DIR: (exit code, report)."""
+    n = C.c_size_t()
+    lib().rah_verify(str(out_dir).encode(), None, 0, C.byref(n))
+    buf = C.create_string_buffer(n.value + 1)
+    rc = lib().rah_verify(str(out_dir).encode(), buf, n.value + 1, C.byref(n))
+    return rc, buf.value.decode()
 
 
 def rmse_of(err, mask):

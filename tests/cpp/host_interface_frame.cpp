@@ -1,8 +1,12 @@
-// Renders the reference's default frame through the C++ host interface (raw::LinearAccel,
-// raw::computeRTAO, raw::computeSSAO, raw::reconcile, raw::certificateJson), the way
-// src/renderer/render.cpp:26-54 calls them, and writes ao_rt.pfm, ao_ss.pfm and
-// certificate.json into argv[1] (PFM as src/core/image.cpp:53-62 writes it).
-// Usage: host_interface_frame OUTDIR [W H]
+// Renders the reference's default frame through the C++ host interface.
+//   host_interface_frame OUTDIR [W H]
+//     raw::LinearAccel, raw::computeRTAO, raw::computeSSAO, raw::reconcile, raw::certificateJson
+//     the way src/renderer/render.cpp:26-54 calls them; writes ao_rt.pfm, ao_ss.pfm and
+//     certificate.json into OUTDIR.
+//   host_interface_frame OUTDIR W H frame [PREV_EYE_X PREV_EYE_Y PREV_EYE_Z]
+//     the CLI's render (src/tools/run.cpp:12-26, app/main.cpp:69-88): raw::renderGpu of the test
+//     scene, raw::emitOutputs (images, certificate.json, channels.json, receipt.json), then
+//     raw::verifyDir on the directory; prints the verify report, exits with its code.
 #include "../../raw-native_amd/host/raw_ao_host.hpp"
 #include <cstdio>
 #include <cstdlib>
@@ -16,17 +20,39 @@ void rah_rasterize(const float* vpos, const float* vnrm, const int32_t* idx, siz
                    float* depth, float* position, float* normal, uint8_t* mask);
 }
 
-static void writePFM1(const raw::Buffer<float>& img, const std::string& path) {
-    FILE* f = std::fopen(path.c_str(), "wb");
-    std::fprintf(f, "Pf\n%d %d\n-1.0\n", img.w, img.h);
-    for (int y = img.h - 1; y >= 0; --y) std::fwrite(&img.px[size_t(y) * size_t(img.w)], 4, size_t(img.w), f);
-    std::fclose(f);
+static int cliFrame(const std::string& out, int w, int h, int argc, char** argv) {
+    raw::CliParams p;
+    p.out = out;
+    p.width = w;
+    p.height = h;
+    if (argc > 7) p.prevEye = raw::Vec3{float(std::atof(argv[5])), float(std::atof(argv[6])), float(std::atof(argv[7]))};
+    raw::Scene s = raw::buildTestScene(w, h);   // renderFromParams (src/tools/run.cpp:12-26)
+    s.camera = raw::cameraFromParams(p);
+    const raw::Mat4 prev = p.hasPrevCamera() ? raw::prevCameraFromParams(p).viewProj() : s.camera.viewProj();
+    raw::RenderOptions opts;
+    opts.tolerance = p.tolerance;
+    opts.rtao = p.rtao;
+    raw::FrameResult o;
+    std::string err;
+    if (!raw::renderGpu(s, w, h, prev, opts, o, err)) {
+        std::fprintf(stderr, "%s\n", err.c_str());
+        return 4;
+    }
+    if (raw::emitOutputs(o, p, "raw-native-cpu", err) != 0) {
+        std::fprintf(stderr, "%s\n", err.c_str());
+        return 2;
+    }
+    std::string report;
+    const int rc = raw::verifyDir(out, report);
+    std::printf("%s", report.c_str());
+    return rc;
 }
 
 int main(int argc, char** argv) {
     if (argc < 2) return 2;
     const std::string out = argv[1];
     const int w = argc > 3 ? std::atoi(argv[2]) : 256, h = argc > 3 ? std::atoi(argv[3]) : 256;
+    if (argc > 4 && std::string(argv[4]) == "frame") return cliFrame(out, w, h, argc, argv);
     const size_t ntri = rah_test_scene(nullptr, nullptr, nullptr);
     std::vector<float> vpos(6 * ntri), vnrm(6 * ntri);
     std::vector<int32_t> idx(3 * ntri);
@@ -49,8 +75,8 @@ int main(int argc, char** argv) {
         const raw::Buffer<float> aoRT = raw::computeRTAO(g, accel, 64, 2.0f);
         const raw::Buffer<float> aoSS = raw::computeSSAO(g, 24, 2.0f);
         const raw::ReconcileResult rec = raw::reconcile(aoSS, aoRT, g.mask, 0.12f);
-        writePFM1(aoRT, out + "/ao_rt.pfm");
-        writePFM1(aoSS, out + "/ao_ss.pfm");
+        raw::writePFM1(aoRT, out + "/ao_rt.pfm");
+        raw::writePFM1(aoSS, out + "/ao_ss.pfm");
         raw::CertInputs in;
         in.renderer = "raw-ao-mi355x";
         const std::string cert = raw::certificateJson(rec, in);

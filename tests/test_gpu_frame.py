@@ -159,3 +159,36 @@ def test_million_triangle_frame_composition(ctx):
     table = {tuple(r) for r in sc["albedo"].view(np.uint32)}
     assert all(tuple(r) in table for r in np.unique(a.view(np.uint32), axis=0))
     assert (got["albedo"][mask == 0] == 0).all()
+
+
+@pytest.mark.parametrize("name", ["default", "motion-256", "nort-256", "wide-1440x900"])
+def test_cli_files_from_the_gpu_frame(ctx, tmp_path, name):
+    """The GPU frame written as the CLI writes it: every file of the reference's identity render
+    (images, certificate.json, channels.json, receipt.json) hashes as the golden records, and
+    verify passes."""
+    import raw_ao.host as H
+    print("frame files", name, flush=True)
+    p, sc, prev = case(name)
+    got = ctx.render_frame(sc, p["w"], p["h"], prev=prev, rtao=p["rt"])
+    H.emit_outputs(tmp_path, got, H.cli_params(p["w"], p["h"], p["eye"], p["target"], fovy=p["fovy"], rt=p["rt"],
+                                               prev_eye=p["prev_eye"]))
+    g = FRAMES[name]["files"]
+    assert {k: sha((tmp_path / k).read_bytes()) for k in g} == g
+    rc, report = H.verify(tmp_path)
+    assert rc == 0, report
+
+
+@pytest.mark.parametrize("name,extra", [("default", []), ("motion-256", ["4.3", "4", "5.7"])])
+def test_cpp_render_gpu_and_cli_files(tmp_path, name, extra):
+    """The C++ host interface end to end: raw::renderGpu of the test scene, raw::emitOutputs,
+    raw::verifyDir (tests/cpp/host_interface_frame.cpp, frame mode)."""
+    import os
+    import subprocess
+    import raw_ao
+    print("cpp frame", name, flush=True)
+    exe = os.path.join(os.path.dirname(raw_ao.LIB_PATH), "host_interface_frame")
+    r = subprocess.run([exe, str(tmp_path), "256", "256", "frame"] + extra, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.endswith("verify: all checks match\n")
+    g = FRAMES[name]["files"]
+    assert {k: sha((tmp_path / k).read_bytes()) for k in g} == g
