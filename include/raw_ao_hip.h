@@ -140,6 +140,16 @@ int raw_ao_rasterize_dev(raw_ao_ctx* ctx, const float* d_vpos, const float* d_vn
                          float near_z, float far_z, int w, int h, float* d_depth, float* d_position, float* d_normal,
                          uint8_t* d_mask, void* stream);
 
+/* ---- the adapter -------------------------------------------------------------------
+ * What the device says about itself, for raw-gpu-cert/1 (rhi::AdapterInfo,
+ * raw/rhi/rhi.hpp:39-41): vendor, architecture (the gfx target), device (marketing name),
+ * description (compute units and memory), backend ("hip"), driver (HIP runtime and driver
+ * versions). Zero-terminated, truncated to the field size. */
+typedef struct raw_ao_adapter {
+    char vendor[64], architecture[64], device[128], description[128], backend[16], driver[64];
+} raw_ao_adapter;
+int raw_ao_adapter_info(raw_ao_ctx* ctx, raw_ao_adapter* out);
+
 /* ---- the default frame on the GPU ----------------------------------------------------
  * Replaces renderGpu (raw/renderer/gpu.hpp:32, src/renderer/gpu/gpu_renderer.cpp:129-182) with
  * the passes of the CPU reference frame, renderWithParams (raw/renderer/render.hpp:50-52,

@@ -10,12 +10,15 @@
 #include "raw/renderer/composite.hpp"
 #include "raw/renderer/raster.hpp"
 #include "raw/renderer/render.hpp"
+#include "raw/renderer/gpu_reconcile.hpp"
+#include "raw/core/version.hpp"
 #include "raw/renderer/ray_ao.hpp"
 #include "raw/renderer/rt_bvh.hpp"
 #include "raw/renderer/rt_scenes.hpp"
 #include "raw/renderer/ssao.hpp"
 #include "raw/renderer/swr_scenes.hpp"
 #include "raw/scene/scene.hpp"
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -214,6 +217,52 @@ void ref_render_frame(const float* vpos, const float* vnrm, size_t nverts, const
     stats8[0] = r.rec.rmse; stats8[1] = r.rec.maxError; stats8[2] = float(r.rec.pixels); stats8[3] = r.rec.withinTolerance ? 1.0f : 0.0f;
     stats8[4] = float(r.motionValid); stats8[5] = float(r.motionTotal); stats8[6] = raw::maxRadiance(r.hdr); stats8[7] = 0.0f;
     std::memcpy(viewproj16, r.viewProj.m, 64);
+}
+
+// A frame's planes as reconcileGpuCpu reads them (FrameResult, raw/renderer/render.hpp:10-21).
+struct ref_frame_planes {
+    const float *depth, *position, *normal;
+    const uint8_t* mask;
+    const float *motion, *ao_rt, *ao_ss, *frame;
+    float rmse;
+    int32_t within;
+};
+static raw::FrameResult refFrame(const ref_frame_planes& f, int w, int h) {
+    raw::FrameResult r;
+    r.g.resize(w, h);
+    r.aoSS.resize(w, h);
+    r.frame.resize(w, h);
+    if (f.ao_rt) r.aoRT.resize(w, h);
+    for (size_t i = 0; i < size_t(w) * size_t(h); ++i) {
+        r.g.depth.px[i] = f.depth[i];
+        r.g.position.px[i] = {f.position[3 * i], f.position[3 * i + 1], f.position[3 * i + 2]};
+        r.g.normal.px[i] = {f.normal[3 * i], f.normal[3 * i + 1], f.normal[3 * i + 2]};
+        r.g.mask.px[i] = f.mask[i];
+        r.g.motion.px[i] = {f.motion[2 * i], f.motion[2 * i + 1]};
+        if (f.ao_rt) r.aoRT.px[i] = f.ao_rt[i];
+        r.aoSS.px[i] = f.ao_ss[i];
+        r.frame.px[i] = {f.frame[3 * i], f.frame[3 * i + 1], f.frame[3 * i + 2]};
+    }
+    r.rec.rmse = f.rmse;
+    r.rec.withinTolerance = f.within != 0;
+    return r;
+}
+
+// reconcileGpuCpu + gpuCertificateJson (src/renderer/gpu_reconcile.cpp:36-134) of two frames
+// (either NULL: no frame, reason says why), with an empty adapter; the text into out.
+size_t ref_gpu_certificate(const ref_frame_planes* gpu, const ref_frame_planes* cpu, int w, int h, int rt, const char* reason,
+                           const char* params_json, double gpu_ms, double cpu_ms, char* out, size_t cap) {
+    raw::GpuReconcile r;
+    if (gpu && cpu) r = raw::reconcileGpuCpu(refFrame(*gpu, w, h), refFrame(*cpu, w, h), rt != 0);
+    r.rt = rt != 0;
+    if (reason) r.reason = reason;
+    const std::string s = raw::gpuCertificateJson(r, raw::GpuAdapterInfo{}, raw::version(), params_json, gpu_ms, cpu_ms);
+    if (out && cap) {
+        const size_t k = std::min(cap - 1, s.size());
+        std::memcpy(out, s.data(), k);
+        out[k] = 0;
+    }
+    return s.size();
 }
 
 int32_t ref_ploc_build(const float* tris9, size_t ntri, int radius, float* boxes6, int32_t* links3, uint32_t* rounds) {

@@ -1,6 +1,7 @@
 // The C ABI of include/raw_ao_hip.h over the gfx950 kernels.
 #include "../../include/raw_ao_hip.h"
 #include "ctx.hpp"
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
@@ -385,6 +386,26 @@ int raw_ao_rasterize(raw_ao_ctx* x, const float* vpos, const float* vnrm, uint32
         !hip_ok(c, hipMemcpyAsync(mask, c.st_mask.p, np, hipMemcpyDeviceToHost, s), "download") ||
         !hip_ok(c, hipStreamSynchronize(s), "rasterize"))
         return RAW_AO_ERR_DEVICE;
+    return RAW_AO_OK;
+}
+
+int raw_ao_adapter_info(raw_ao_ctx* x, raw_ao_adapter* out) {
+    if (!x || !out) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    hipDeviceProp_t p;
+    if (!hip_ok(c, hipGetDeviceProperties(&p, c.device), "hipGetDeviceProperties")) return RAW_AO_ERR_DEVICE;
+    int rt = 0, drv = 0;
+    (void)hipRuntimeGetVersion(&rt);
+    (void)hipDriverGetVersion(&drv);
+    std::memset(out, 0, sizeof *out);
+    std::snprintf(out->vendor, sizeof out->vendor, "AMD");
+    std::snprintf(out->architecture, sizeof out->architecture, "%s", p.gcnArchName);
+    std::snprintf(out->device, sizeof out->device, "%s", p.name);
+    std::snprintf(out->description, sizeof out->description, "%d compute units, %zu MiB", p.multiProcessorCount,
+                  size_t(p.totalGlobalMem >> 20));
+    std::snprintf(out->backend, sizeof out->backend, "hip");
+    std::snprintf(out->driver, sizeof out->driver, "HIP runtime %d, driver %d", rt, drv);
     return RAW_AO_OK;
 }
 

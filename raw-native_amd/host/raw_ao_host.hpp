@@ -157,4 +157,36 @@ int emitOutputs(const FrameResult& o, const CliParams& p, const std::string& bac
 // raw_native_cli verify DIR (src/tools/verify.cpp): 0 all match, 2 missing, 3 mismatch.
 int verifyDir(const std::string& dir, std::string& report);
 
+// ---- the GPU frame against the CPU reference: raw-gpu-cert/1 (raw/renderer/gpu_reconcile.hpp,
+// src/renderer/gpu_reconcile.cpp, raw/cert/gpu_tolerance.hpp) -------------------------------
+struct AdapterInfo { std::string vendor, architecture, device, description, backend, driver; };
+AdapterInfo adapterInfo();   // this device's (raw_ao_adapter_info); empty without a GPU
+struct GpuChannelCheck {
+    std::string name;
+    long long values{0};
+    double rmse{0}, maxError{0}, bound{0};
+    bool pass{false};
+};
+struct GpuReconcile {
+    int width{0}, height{0};
+    long long coveredEither{0}, coveredBoth{0}, maskMismatch{0};
+    double maskMismatchFraction{0};
+    bool maskPass{false};
+    std::vector<GpuChannelCheck> channels;
+    bool rt{true};
+    float cpuAoRmse{0}, gpuAoRmse{0};
+    bool cpuWithin{false}, gpuWithin{false};
+    bool reconcilePass{false}, verdictPass{false};
+    bool sizeMatch{false};
+    std::string reason;
+    std::string gpuFrameSha256, cpuFrameSha256;
+    bool pass() const;
+};
+GpuReconcile reconcileGpuCpu(const FrameResult& gpu, const FrameResult& cpu, bool rt);
+std::string gpuCertificateJson(const GpuReconcile& r, const AdapterInfo& a, const std::string& renderer,
+                               const std::string& paramsJson, double gpuMs, double cpuMs);
+// gpu_receipt.json (src/tools/receipt.cpp:150-196); gpu or cpu may be null (no frame)
+std::string gpuReceiptJson(const FrameResult* gpu, const FrameResult* cpu, const GpuReconcile& r, const CliParams& p,
+                           const FileDigests& outputs);
+
 }  // namespace raw

@@ -22,6 +22,7 @@
 #include <cstring>
 #include <fstream>
 #include <iterator>
+#include <optional>
 #include <sstream>
 #include <stdexcept>
 
@@ -827,6 +828,219 @@ int emitOutputs(const FrameResult& o, const CliParams& p, const std::string& bac
     return 0;
 }
 
+// ==== raw-gpu-cert/1 =====================================================================
+namespace {
+// raw/cert/gpu_tolerance.hpp: the bounds the reference committed before its first GPU run
+constexpr double kMaskMismatchFraction = 0.002, kDepthRelRmse = 1e-4, kPositionRmse = 1e-3, kNormalRmse = 1e-3,
+                 kMotionRmse = 1e-4, kAoSsRmse = 0.01, kAoRtRmse = 0.01, kFrameRmse = 0.01, kReconcileRmseDelta = 0.005;
+constexpr bool kVerdictMustMatch = true;
+
+struct Acc {   // gpu_reconcile.cpp:11-20
+    double sumSq{0}, max{0};
+    long long n{0};
+    void add(double d) {
+        d = std::fabs(d);
+        sumSq += d * d;
+        if (d > max) max = d;
+        ++n;
+    }
+    GpuChannelCheck done(const std::string& name, double bound) const {
+        GpuChannelCheck c;
+        c.name = name;
+        c.values = n;
+        c.bound = bound;
+        c.rmse = n ? std::sqrt(sumSq / (double)n) : 0.0;
+        c.maxError = max;
+        c.pass = n > 0 && std::isfinite(c.rmse) && c.rmse <= bound;
+        return c;
+    }
+};
+std::string gstr(const std::string& s) {   // gpu_reconcile.cpp:21-29
+    std::string o = "\"";
+    for (char ch : s) {
+        if (ch == '"' || ch == '\\') { o += '\\'; o += ch; }
+        else if ((unsigned char)ch < 0x20) { char b[8]; std::snprintf(b, sizeof b, "\\u%04x", (unsigned char)ch); o += b; }
+        else o += ch;
+    }
+    return o + "\"";
+}
+std::string g9d(double v) {
+    if (!std::isfinite(v)) return "null";
+    char b[40];
+    std::snprintf(b, sizeof b, "%.9g", v);
+    return b;
+}
+}  // namespace
+
+AdapterInfo adapterInfo() {
+    AdapterInfo a;
+    try {
+        raw_ao_adapter x{};
+        if (raw_ao_adapter_info(device(), &x) == RAW_AO_OK) {
+            a.vendor = x.vendor;
+            a.architecture = x.architecture;
+            a.device = x.device;
+            a.description = x.description;
+            a.backend = x.backend;
+            a.driver = x.driver;
+        }
+    } catch (const std::exception&) {
+    }
+    return a;
+}
+
+bool GpuReconcile::pass() const {
+    if (!sizeMatch || !maskPass) return false;
+    for (const auto& c : channels)
+        if (!c.pass) return false;
+    if (rt && (!reconcilePass || !verdictPass)) return false;
+    return true;
+}
+
+GpuReconcile reconcileGpuCpu(const FrameResult& gpu, const FrameResult& cpu, bool rt) {
+    GpuReconcile r;
+    r.rt = rt;
+    r.width = cpu.g.w;
+    r.height = cpu.g.h;
+    if (!gpu.frame.px.empty()) r.gpuFrameSha256 = sha256Hex(rgb8Bytes(gpu.frame));
+    if (!cpu.frame.px.empty()) r.cpuFrameSha256 = sha256Hex(rgb8Bytes(cpu.frame));
+    r.sizeMatch = gpu.g.w == cpu.g.w && gpu.g.h == cpu.g.h && gpu.frame.w == cpu.frame.w && gpu.frame.h == cpu.frame.h &&
+                  (!rt || (gpu.aoRT.w == cpu.aoRT.w && gpu.aoRT.h == cpu.aoRT.h));
+    if (!r.sizeMatch) return r;
+    Acc depth, pos, nrm, mot, ss, rta, frame;
+    for (int y = 0; y < r.height; ++y)
+        for (int x = 0; x < r.width; ++x) {
+            const bool a = gpu.g.mask.at(x, y) != 0, b = cpu.g.mask.at(x, y) != 0;
+            if (a || b) ++r.coveredEither;
+            if (a != b) { ++r.maskMismatch; continue; }
+            if (!a) continue;
+            ++r.coveredBoth;
+            const float dc = cpu.g.depth.at(x, y);
+            depth.add((gpu.g.depth.at(x, y) - dc) / (std::fabs(dc) > 1e-6f ? dc : 1.0f));
+            const Vec3 gp = gpu.g.position.at(x, y), cp = cpu.g.position.at(x, y);
+            pos.add(gp.x - cp.x); pos.add(gp.y - cp.y); pos.add(gp.z - cp.z);
+            const Vec3 gn = gpu.g.normal.at(x, y), cn = cpu.g.normal.at(x, y);
+            nrm.add(gn.x - cn.x); nrm.add(gn.y - cn.y); nrm.add(gn.z - cn.z);
+            const Vec2 gm = gpu.g.motion.at(x, y), cm = cpu.g.motion.at(x, y);
+            mot.add(gm.x - cm.x); mot.add(gm.y - cm.y);
+            ss.add(gpu.aoSS.at(x, y) - cpu.aoSS.at(x, y));
+            if (rt) rta.add(gpu.aoRT.at(x, y) - cpu.aoRT.at(x, y));
+            const Vec3 gf = gpu.frame.at(x, y), cf = cpu.frame.at(x, y);
+            frame.add(gf.x - cf.x); frame.add(gf.y - cf.y); frame.add(gf.z - cf.z);
+        }
+    r.maskMismatchFraction = r.coveredEither ? (double)r.maskMismatch / (double)r.coveredEither : 0.0;
+    r.maskPass = r.coveredEither > 0 && r.maskMismatchFraction <= kMaskMismatchFraction;
+    r.channels.push_back(depth.done("depth_rel", kDepthRelRmse));
+    r.channels.push_back(pos.done("position", kPositionRmse));
+    r.channels.push_back(nrm.done("normal", kNormalRmse));
+    r.channels.push_back(mot.done("motion", kMotionRmse));
+    r.channels.push_back(ss.done("ao_ss", kAoSsRmse));
+    if (rt) r.channels.push_back(rta.done("ao_rt", kAoRtRmse));
+    r.channels.push_back(frame.done("frame", kFrameRmse));
+    r.cpuAoRmse = cpu.rec.rmse;
+    r.gpuAoRmse = gpu.rec.rmse;
+    r.cpuWithin = cpu.rec.withinTolerance;
+    r.gpuWithin = gpu.rec.withinTolerance;
+    r.reconcilePass = std::fabs((double)r.gpuAoRmse - (double)r.cpuAoRmse) <= kReconcileRmseDelta;
+    r.verdictPass = !kVerdictMustMatch || r.cpuWithin == r.gpuWithin;
+    return r;
+}
+
+std::string gpuCertificateJson(const GpuReconcile& r, const AdapterInfo& a, const std::string& renderer,
+                               const std::string& paramsJson, double gpuMs, double cpuMs) {   // gpu_reconcile.cpp:85-134
+    const char* verdict = !r.sizeMatch ? "unverifiable" : (r.pass() ? "verified" : "refuted");
+    std::string o = "{\"schema\":\"raw-gpu-cert/1\"";
+    o += ",\"claim\":\"the GPU frame matches the CPU reference within the committed tolerance\"";
+    o += std::string(",\"verdict\":\"") + verdict + "\"";
+    const bool both = !r.gpuFrameSha256.empty() && !r.cpuFrameSha256.empty();
+    o += ",\"identity\":" + (both ? std::string(r.gpuFrameSha256 == r.cpuFrameSha256 ? "\"MATCH\"" : "\"DRIFT\"") : std::string("null"));
+    o += ",\"frame_rgb8_sha256\":" +
+         (both ? "{\"gpu\":" + gstr(r.gpuFrameSha256) + ",\"cpu\":" + gstr(r.cpuFrameSha256) + "}" : std::string("null"));
+    o += ",\"oracle\":\"raw-gpu-cpu-v1\"";
+    if (!r.reason.empty()) o += ",\"reason\":" + gstr(r.reason);
+    o += ",\"renderer\":" + gstr(renderer);
+    o += ",\"params\":" + paramsJson;
+    o += ",\"adapter\":{\"vendor\":" + gstr(a.vendor) + ",\"architecture\":" + gstr(a.architecture) + ",\"device\":" +
+         gstr(a.device) + ",\"description\":" + gstr(a.description) + ",\"backend\":" + gstr(a.backend) +
+         ",\"driver\":" + gstr(a.driver) + "}";
+    o += ",\"tolerance_source\":\"raw/gpu_tolerance.hpp, committed before the first GPU run\"";
+    o += ",\"coverage\":{\"covered_either\":" + std::to_string(r.coveredEither) + ",\"covered_both\":" +
+         std::to_string(r.coveredBoth) + ",\"mismatch\":" + std::to_string(r.maskMismatch) +
+         ",\"mismatch_fraction\":" + g9d(r.maskMismatchFraction) + ",\"bound\":" + g9d(kMaskMismatchFraction) +
+         ",\"pass\":" + (r.maskPass ? "true" : "false") + "}";
+    o += ",\"channels\":[";
+    for (size_t i = 0; i < r.channels.size(); ++i) {
+        const auto& c = r.channels[i];
+        o += std::string(i ? "," : "") + "{\"name\":" + gstr(c.name) + ",\"values\":" + std::to_string(c.values) +
+             ",\"rmse\":" + g9d(c.rmse) + ",\"max_error\":" + g9d(c.maxError) + ",\"bound\":" + g9d(c.bound) +
+             ",\"pass\":" + (c.pass ? "true" : "false") + "}";
+    }
+    o += "]";
+    if (r.rt && r.sizeMatch) {
+        o += ",\"ao_verdict\":{\"cpu_rmse\":" + exactFloat(r.cpuAoRmse) + ",\"gpu_rmse\":" + exactFloat(r.gpuAoRmse) +
+             ",\"rmse_delta_bound\":" + g9d(kReconcileRmseDelta) + ",\"cpu\":\"" + (r.cpuWithin ? "verified" : "refuted") +
+             "\",\"gpu\":\"" + (r.gpuWithin ? "verified" : "refuted") +
+             "\",\"pass\":" + ((r.reconcilePass && r.verdictPass) ? "true" : "false") + "}";
+    } else {
+        o += ",\"ao_verdict\":null";
+    }
+    o += ",\"timing_ms\":{\"gpu\":" + g9d(gpuMs) + ",\"cpu\":" + g9d(cpuMs) + ",\"judged\":false}";
+    o += ",\"does_not_prove\":["
+         "\"Agreement with the CPU path is not agreement with ground truth: the CPU ray-traced AO is itself a 64-sample estimate.\","
+         "\"Checked on the adapter and driver named here only. Another GPU, driver or browser may differ.\","
+         "\"An RMSE bound limits the average difference, not the worst pixel. Maximum errors are reported, not bounded.\","
+         "\"One built-in scene. No claim about other geometry, lights or materials.\","
+         "\"The timings are one run each and are not judged.\"]";
+    return o + "}";
+}
+
+std::string gpuReceiptJson(const FrameResult* gpu, const FrameResult* cpu, const GpuReconcile& r, const CliParams& p,
+                           const FileDigests& outputs) {
+    const std::string subject = gpu ? rgb8Bytes(gpu->frame) : std::string();
+    const std::string subjectSha = sha256Hex(subject);
+    const std::string refSha = cpu ? sha256Hex(rgb8Bytes(cpu->frame)) : std::string();
+    J metrics = J::obj(), bounds = J::obj();
+    bounds.set("coverage_mismatch_frac_max", kMaskMismatchFraction);
+    const std::pair<const char*, double> b[] = {{"depth_rel", kDepthRelRmse}, {"position", kPositionRmse},
+                                                {"normal", kNormalRmse},      {"motion", kMotionRmse},
+                                                {"ao_ss", kAoSsRmse},         {"ao_rt", kAoRtRmse},
+                                                {"frame", kFrameRmse}};
+    for (const auto& [name, bound] : b) bounds.set(std::string(name) + "_rmse_max", bound);
+    bounds.set("ao_rmse_delta_max", kReconcileRmseDelta);
+    std::string verdict = "unverifiable", reason = r.reason;
+    if (r.sizeMatch && gpu && cpu) {
+        metrics.set("coverage_mismatch_frac", r.maskMismatchFraction);
+        for (const auto& c : r.channels) {
+            metrics.set(c.name + "_rmse", c.rmse);
+            metrics.set(c.name + "_max_error", c.maxError);
+        }
+        if (r.rt) metrics.set("ao_rmse_delta", std::fabs((double)r.gpuAoRmse - (double)r.cpuAoRmse));
+        verdict = r.pass() ? "verified" : "refuted";
+    } else if (reason.empty()) {
+        reason = "the GPU frame and the CPU reference differ in size";
+    }
+    const std::string backend = "raw-native-" + (p.gpuBackend.empty() ? std::string("none") : p.gpuBackend);
+    J outs = J::obj();
+    for (const auto& kv : outputs) outs.set(kv.first, kv.second);
+    J media = J::obj();
+    media.set("kind", "image").set("width", p.width).set("height", p.height).set("format", "rgb8").set("transfer", "linear-u8");
+    J ref = J::obj();
+    ref.set("backend", "raw-native-cpu").set("content_sha256", refSha.empty() ? std::string("none") : refSha);
+    J tol = J::obj();
+    tol.set("verdict", verdict).set("metrics", metrics).set("bounds", bounds);
+    if (!reason.empty()) tol.set("reason", reason);
+    J rc = J::obj();
+    rc.set("identity", (refSha.empty() ? std::string("none") : refSha) == subjectSha ? "MATCH" : "DRIFT").set("tolerance", tol);
+    return receipt(backend, sceneValue(p), media, subject, outs, ref, rc,
+                   {"Identity compares the 8-bit frames only; the tolerance verdict covers every channel at float precision.",
+                    "Agreement with the CPU path is not agreement with ground truth: the CPU ray-traced AO is itself a 64-sample "
+                    "estimate.",
+                    "Checked on the adapter and driver named in gpu_certificate.json only. Another GPU, driver or browser may "
+                    "differ.",
+                    "An RMSE bound limits the average difference, not the worst pixel. Maximum errors are reported, not bounded.",
+                    "One built-in scene. No claim about other geometry, lights or materials."});
+}
+
 // ==== verify ============================================================================
 namespace {
 // A JSON reader for the receipt: numbers without a fraction or exponent within +-2^53 are
@@ -1148,6 +1362,29 @@ template <class T, class S> void fillBuf(raw::Buffer<T>& b, int w, int h, const 
     b.resize(w, h);
     if (src) std::memcpy(static_cast<void*>(b.px.data()), src, sizeof(T) * b.px.size());
 }
+raw::FrameResult toFrame(const rah_frame& f, int w, int h) {
+    raw::FrameResult o;
+    o.g.w = w;
+    o.g.h = h;
+    fillBuf(o.g.depth, w, h, f.depth);
+    fillBuf(o.g.position, w, h, f.position);
+    fillBuf(o.g.normal, w, h, f.normal);
+    fillBuf(o.g.albedo, w, h, f.albedo);
+    fillBuf(o.g.mask, w, h, f.mask);
+    fillBuf(o.g.motion, w, h, f.motion);
+    if (f.ao_rt) fillBuf(o.aoRT, w, h, f.ao_rt);
+    fillBuf(o.aoSS, w, h, f.ao_ss);
+    fillBuf(o.rec.errorMap, w, h, f.error);
+    fillBuf(o.frame, w, h, f.frame);
+    fillBuf(o.hdr, w, h, f.hdr);
+    o.rec.rmse = f.rmse;
+    o.rec.maxError = f.max_error;
+    o.rec.pixels = f.pixels;
+    o.rec.withinTolerance = f.within != 0;
+    o.motionValid = f.motion_valid;
+    o.motionTotal = f.motion_total;
+    return o;
+}
 size_t copyOut(const std::string& s, char* out, size_t cap) {
     if (out && cap) {
         const size_t k = std::min(cap - 1, s.size());
@@ -1164,27 +1401,7 @@ extern "C" {
 // certificate.json, channels.json, receipt.json). Returns 0, or 2 with the message in err.
 int rah_emit_outputs(const char* dir, const rah_params* q, const rah_frame* f, const char* backend, char* err, size_t cap) {
     if (!dir || !q || !f) return 2;
-    const int w = q->width, h = q->height;
-    raw::FrameResult o;
-    o.g.w = w;
-    o.g.h = h;
-    fillBuf(o.g.depth, w, h, f->depth);
-    fillBuf(o.g.position, w, h, f->position);
-    fillBuf(o.g.normal, w, h, f->normal);
-    fillBuf(o.g.albedo, w, h, f->albedo);
-    fillBuf(o.g.mask, w, h, f->mask);
-    fillBuf(o.g.motion, w, h, f->motion);
-    if (f->ao_rt) fillBuf(o.aoRT, w, h, f->ao_rt);
-    fillBuf(o.aoSS, w, h, f->ao_ss);
-    fillBuf(o.rec.errorMap, w, h, f->error);
-    fillBuf(o.frame, w, h, f->frame);
-    fillBuf(o.hdr, w, h, f->hdr);
-    o.rec.rmse = f->rmse;
-    o.rec.maxError = f->max_error;
-    o.rec.pixels = f->pixels;
-    o.rec.withinTolerance = f->within != 0;
-    o.motionValid = f->motion_valid;
-    o.motionTotal = f->motion_total;
+    const raw::FrameResult o = toFrame(*f, q->width, q->height);
     std::string e;
     int rc = 2;
     try {
@@ -1209,6 +1426,41 @@ int rah_verify(const char* dir, char* out, size_t cap, size_t* len) {
     const size_t n = copyOut(report, out, cap);
     if (len) *len = n;
     return rc;
+}
+
+// raw-gpu-cert/1 and gpu_receipt.json for a GPU frame against a CPU frame (either may be NULL:
+// no frame, as when no GPU was found; reason then says why). adapter NULL: this device's.
+// The certificate is written to dir/gpu_certificate.json, the receipt (listing the
+// certificate's digest) to dir/gpu_receipt.json. Returns 0, or 2 when a file cannot be written.
+int rah_gpu_certificate(const char* dir, const rah_params* q, const rah_frame* gpu, const rah_frame* cpu, const char* reason,
+                        const raw_ao_adapter* adapter, double gpu_ms, double cpu_ms) {
+    if (!dir || !q) return 2;
+    try {
+        const raw::CliParams p = toParams(*q, dir);
+        std::optional<raw::FrameResult> g, c;
+        if (gpu) g = toFrame(*gpu, q->width, q->height);
+        if (cpu) c = toFrame(*cpu, q->width, q->height);
+        raw::GpuReconcile r;
+        if (g && c) r = raw::reconcileGpuCpu(*g, *c, p.rtao);
+        r.rt = p.rtao;
+        if (reason) r.reason = reason;
+        raw::AdapterInfo a;
+        if (adapter) a = {adapter->vendor, adapter->architecture, adapter->device, adapter->description, adapter->backend,
+                          adapter->driver};
+        else a = raw::adapterInfo();
+        const std::string d = std::string(dir) + "/";
+        {
+            std::ofstream f(d + "gpu_certificate.json", std::ios::binary);
+            f << raw::gpuCertificateJson(r, a, raw::version(), raw::canonicalParamsJson(p), gpu_ms, cpu_ms);
+            if (!f) return 2;
+        }
+        const raw::FileDigests outs{{"gpu_certificate.json", raw::sha256File(d + "gpu_certificate.json")}};
+        std::ofstream f(d + "gpu_receipt.json", std::ios::binary);
+        f << raw::gpuReceiptJson(g ? &*g : nullptr, c ? &*c : nullptr, r, p, outs);
+        return f ? 0 : 2;
+    } catch (const std::exception&) {
+        return 2;
+    }
 }
 
 // SHA-256 of n bytes as 64 hex digits and a terminating zero.

@@ -59,6 +59,7 @@ SIGNATURES = [
     ("raw_ao_render_frame", C.c_int, [_vp, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     ("raw_ao_render_frame_dev", C.c_int, [_vp, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, _vp]),
     ("raw_ao_render_opts_default", None, [C.c_void_p]),
+    ("raw_ao_adapter_info", C.c_int, [_vp, C.c_void_p]),
     ("raw_ao_ssao", C.c_int, [_vp, _f32p, _f32p, _u8p, C.c_int, C.c_int, C.c_int, C.c_float, _f32p]),
     ("raw_ao_ssao_dev", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp]),
     ("raw_ao_error_map", C.c_int, [_vp, _f32p, _f32p, _u8p, C.c_int, C.c_int, _f32p, C.POINTER(C.c_float)]),
@@ -153,6 +154,12 @@ class FrameStats(C.Structure):
         return dict(rmse=self.rmse, max_error=self.max_error, pixels=self.pixels, within=bool(self.within_tolerance),
                     motion_valid=self.motion_valid, motion_total=self.motion_total, max_radiance=self.max_radiance,
                     viewproj=np.array(self.viewproj[:], np.float32), gpu_ms=self.gpu_ms)
+
+
+class Adapter(C.Structure):
+    """raw_ao_adapter"""
+    _fields_ = [("vendor", C.c_char * 64), ("architecture", C.c_char * 64), ("device", C.c_char * 128),
+                ("description", C.c_char * 128), ("backend", C.c_char * 16), ("driver", C.c_char * 64)]
 
 
 class RawAoError(RuntimeError):
@@ -276,6 +283,11 @@ class Context:
         self._check(lib().raw_ao_trace_visibility_dev(self._h, _ptr(position), _ptr(normal), _ptr(mask), w, h, samples,
                                                       radius, np.asarray(to_light, np.float32), int(bool(sun_forced)),
                                                       _ptr(ao), _ptr(sky), _ptr(sun), _vp(stream)))
+
+    def adapter_info(self):
+        a = Adapter()
+        self._check(lib().raw_ao_adapter_info(self._h, C.byref(a)))
+        return {k: getattr(a, k).decode() for k, _ in Adapter._fields_}
 
     # ---- the default frame ---------------------------------------------------------
     @staticmethod

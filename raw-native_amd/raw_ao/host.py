@@ -49,6 +49,9 @@ def lib():
         L.rah_emit_outputs.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_char_p, C.c_size_t]
         L.rah_verify.restype = C.c_int
         L.rah_verify.argtypes = [C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
+        L.rah_gpu_certificate.restype = C.c_int
+        L.rah_gpu_certificate.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_void_p,
+                                          C.c_double, C.c_double]
         L.rah_sha256.restype = None
         L.rah_sha256.argtypes = [C.c_void_p, C.c_size_t, C.c_char_p]
         _lib = L
@@ -91,7 +94,14 @@ def emit_outputs(out_dir, frame, params, backend="raw-native-cpu"):
     mask.pgm, certificate.json, channels.json, receipt.json). frame: a dict of planes and the
     reconcile / motion scalars (raw_ao.Context.render_frame's result)."""
     keep = []
+    f = _frame_struct(frame, keep)
+    err = C.create_string_buffer(512)
+    rc = lib().rah_emit_outputs(str(out_dir).encode(), C.byref(params), C.byref(f), backend.encode(), err, 512)
+    if rc != 0:
+        raise RuntimeError("emit_outputs: %s" % err.value.decode())
 
+
+def _frame_struct(frame, keep):
     def ptr(k, t):
         if frame.get(k) is None:
             return None
@@ -104,10 +114,25 @@ def emit_outputs(out_dir, frame, params, backend="raw-native-cpu"):
     f.rmse, f.max_error = frame["rmse"], frame["max_error"]
     f.pixels, f.within = frame["pixels"], int(bool(frame["within"]))
     f.motion_valid, f.motion_total = frame["motion_valid"], frame["motion_total"]
-    err = C.create_string_buffer(512)
-    rc = lib().rah_emit_outputs(str(out_dir).encode(), C.byref(params), C.byref(f), backend.encode(), err, 512)
+    return f
+
+
+def gpu_certificate(out_dir, params, gpu=None, cpu=None, reason=None, adapter=None, gpu_ms=0.0, cpu_ms=0.0):
+    """Write gpu_certificate.json (raw-gpu-cert/1) and gpu_receipt.json for a GPU frame against a
+    CPU frame (dicts as emit_outputs takes; None: no frame). adapter: a dict of the raw_ao_adapter
+    fields, None for this device's, {} for an empty one."""
+    keep = []
+    g = _frame_struct(gpu, keep) if gpu is not None else None
+    c = _frame_struct(cpu, keep) if cpu is not None else None
+    a = None
+    if adapter is not None:
+        import raw_ao
+        a = raw_ao.Adapter(**{k: v.encode() for k, v in adapter.items()})
+    rc = lib().rah_gpu_certificate(str(out_dir).encode(), C.byref(params), C.byref(g) if g is not None else None,
+                                   C.byref(c) if c is not None else None, reason.encode() if reason else None,
+                                   C.byref(a) if a is not None else None, gpu_ms, cpu_ms)
     if rc != 0:
-        raise RuntimeError("emit_outputs: %s" % err.value.decode())
+        raise RuntimeError("gpu_certificate: could not write %s" % out_dir)
 
 
 def verify(out_dir):

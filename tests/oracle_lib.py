@@ -390,3 +390,8 @@ def ref_frame(sc, w, h, prev=None, lighting=2, no_sun=False, rtao_on=True, tol=0
     if rtao_on:
         out["ao_rt"] = ao_rt
     return out
+
+
+def sha256_file(path):
+    import hashlib
+    return hashlib.sha256(open(path, "rb").read()).hexdigest()

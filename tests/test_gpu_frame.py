@@ -192,3 +192,23 @@ def test_cpp_render_gpu_and_cli_files(tmp_path, name, extra):
     assert r.stdout.endswith("verify: all checks match\n")
     g = FRAMES[name]["files"]
     assert {k: sha((tmp_path / k).read_bytes()) for k in g} == g
+
+
+def test_gpu_run_certificate(ctx, tmp_path):
+    """The --gpu run (src/tools/gpu_run.cpp:70-93): the GPU frame against the CPU reference frame
+    (here the oracle's) as raw-gpu-cert/1 with this device's adapter: identity MATCH, every
+    channel at zero error, verified."""
+    import json
+    import raw_ao.host as H
+    print("gpu certificate", flush=True)
+    p, sc, prev = case("motion-256")
+    g = ctx.render_frame(sc, p["w"], p["h"], prev=prev)
+    c = O.frame(sc, p["w"], p["h"], prev=prev)
+    H.gpu_certificate(tmp_path, H.cli_params(p["w"], p["h"], prev_eye=p["prev_eye"], gpu_backend="hip"), gpu=g, cpu=c,
+                      gpu_ms=g["gpu_ms"], cpu_ms=0.0)
+    cert = json.loads((tmp_path / "gpu_certificate.json").read_text())
+    assert cert["verdict"] == "verified" and cert["identity"] == "MATCH"
+    assert all(ch["rmse"] == 0 and ch["pass"] for ch in cert["channels"])
+    assert cert["adapter"]["backend"] == "hip" and cert["adapter"]["architecture"].startswith("gfx950")
+    rec = json.loads((tmp_path / "gpu_receipt.json").read_text())
+    assert rec["reconcile"]["identity"] == "MATCH" and rec["reconcile"]["tolerance"]["verdict"] == "verified"
