@@ -256,6 +256,16 @@ int raw_ao_error_map_dev(raw_ao_ctx* ctx, const float* d_approx, const float* d_
  * reference's short-far control). */
 int raw_ao_trace_rays(raw_ao_ctx* ctx, const float* rays8, uint32_t nrays, int any, float far_scale, float* out4);
 
+/* ---- closest hits with barycentrics -------------------------------------------------
+ * Replaces rt::closest (raw/renderer/rt_bvh.hpp:55, src/renderer/rt_bvh.cpp:125-152), the
+ * query the path tracer runs on the PLOC tree PtScene builds (src/renderer/rt_pt_scene.cpp:36),
+ * for a batch of rays through the context's BVH: rays8 {origin xyz, direction xyz, tMax, pad};
+ * hits4 per ray {t, u, v, triangle (int32)} as HitOut (raw/rhi/hw.hpp:125): the nearest hit with
+ * t < tMax, ties to the lower triangle index, u and v intersectTri's weights of b and c; a miss
+ * is {-1, 0, 0, -1}. Bit for bit the reference's Hit. Needs the wide traversal form. */
+int raw_ao_closest_hits(raw_ao_ctx* ctx, const float* rays8, uint32_t nrays, void* hits4);
+int raw_ao_closest_hits_dev(raw_ao_ctx* ctx, const float* d_rays8, uint32_t nrays, void* d_hits4, void* stream);
+
 /* ---- measurement -------------------------------------------------------------------
  * Node visits and triangle tests of the last trace_ao over its rays (from a separate
  * counting launch when enabled with raw_ao_count_next_trace), and the milliseconds of the

@@ -1007,6 +1007,112 @@ void orc_scan_rays(const float* tris9, size_t ntri, const float* rays6, const fl
     });
 }
 
+// makeRays (src/renderer/gpu/rt_bvh_parity.cpp:36-48): xorshift64 draws; with a camera every odd
+// ray leaves the eye through a jittered point of the view, the rest start in a 24^3 cube with a
+// random direction. cam_first_u: whether the camera ray's first draw is the s (right) offset,
+// the left-to-right evaluation of f + s * (next() - 0.5) + u * (next() - 0.5).
+void orc_r2_rays(const float* eye, const float* target, const float* up, size_t n, uint64_t seed, int cam_first_u,
+                 float* rays8) {
+    uint64_t st = seed;
+    const auto next = [&st]() {
+        st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+        return float(double(st >> 11) * (1.0 / 9007199254740992.0));
+    };
+    for (size_t k = 0; k < n; ++k) {
+        V3 o, d;
+        if (eye && (k % 2)) {
+            const V3 e = ld3(eye), tg = ld3(target), uu = ld3(up);
+            const V3 f = nrm(sub(tg, e)), sv = nrm(cross(f, uu)), u = cross(sv, f);
+            float a, b;
+            if (cam_first_u) { a = next(); b = next(); }
+            else { b = next(); a = next(); }
+            const V3 js = V3{sv.x * (a - 0.5f), sv.y * (a - 0.5f), sv.z * (a - 0.5f)};
+            const V3 ju = V3{u.x * (b - 0.5f), u.y * (b - 0.5f), u.z * (b - 0.5f)};
+            const V3 fs = V3{f.x + js.x, f.y + js.y, f.z + js.z};
+            o = e;
+            d = nrm(V3{fs.x + ju.x, fs.y + ju.y, fs.z + ju.z});
+        } else {
+            const float x = 24 * next() - 12, y = 24 * next() - 12, z = 24 * next() - 12;
+            o = V3{x, y, z};
+            const float dx = next() - 0.5f, dy = next() - 0.5f, dz = next() - 0.5f;
+            d = nrm(V3{dx, dy, dz});
+        }
+        st3(rays8 + 8 * k, o);
+        st3(rays8 + 8 * k + 3, d);
+        rays8[8 * k + 6] = 1e30f;
+        rays8[8 * k + 7] = 0.0f;
+    }
+}
+
+// bruteForce (rt_bvh_parity.cpp:19-34): the nearest hit of a scan of every triangle (ties to the
+// lower index), its u, v, and the second-nearest t (t2, -1 without a hit), over t < 1e30.
+void orc_scan_hits(const float* tris9, size_t ntri, const float* rays8, size_t nrays, int threads, float* hits4, float* t2) {
+    rows(int(nrays), threads, [&](int k) {
+        const V3 o = ld3(rays8 + 8 * size_t(k)), d = ld3(rays8 + 8 * size_t(k) + 3);
+        float best = 1e30f, second = 1e30f, bu = 0, bv = 0;
+        int tri = -1;
+        for (size_t i = 0; i < ntri; ++i) {
+            float t, u, v;
+            if (!intersect(o, d, triAt(tris9, i), t, u, v)) continue;
+            if (t < best) { second = best; best = t; tri = int(i); bu = u; bv = v; }
+            else if (t < second) second = t;
+        }
+        float* h = hits4 + 4 * size_t(k);
+        h[0] = tri >= 0 ? best : -1.0f;
+        h[1] = tri >= 0 ? bu : 0.0f;
+        h[2] = tri >= 0 ? bv : 0.0f;
+        std::memcpy(&h[3], &tri, 4);
+        if (t2) t2[k] = tri >= 0 ? second : -1.0f;
+    });
+}
+
+// rt::closest (src/renderer/rt_bvh.cpp:125-152) over a binary tree in the reference's node
+// layout (boxes6 {lo, hi}, links3 {left, right, tri}); the ray's own tMax. hits4 as above.
+void orc_tree_closest(const float* boxes6, const int32_t* links3, int32_t root, const float* tris9, const float* rays8,
+                      size_t nrays, int threads, float* hits4) {
+    const float kSlab = 1.0f + 2.0f * (3.0f * 5.96046448e-8f) / (1.0f - 3.0f * 5.96046448e-8f);
+    rows(int(nrays), threads, [&](int k) {
+        const V3 ro = ld3(rays8 + 8 * size_t(k)), rd = ld3(rays8 + 8 * size_t(k) + 3);
+        const float tmax = rays8[8 * size_t(k) + 6];
+        float* h = hits4 + 4 * size_t(k);
+        h[0] = -1.0f; h[1] = 0.0f; h[2] = 0.0f;
+        int32_t none = -1;
+        std::memcpy(&h[3], &none, 4);
+        if (root < 0) return;
+        const auto inv = [](float x) { return x != 0.0f ? 1.0f / x : std::copysign(std::numeric_limits<float>::infinity(), x); };
+        const float o[3] = {ro.x, ro.y, ro.z}, iv[3] = {inv(rd.x), inv(rd.y), inv(rd.z)};
+        int stack[256], sp = 0;
+        stack[sp++] = root;
+        bool any = false;
+        float best = tmax;
+        int32_t htri = -1;
+        float hu = 0, hv = 0;
+        while (sp) {
+            const int32_t ni = stack[--sp];
+            const float* b = boxes6 + 6 * size_t(ni);
+            float t0 = 0.0f, t1 = best;
+            for (int a = 0; a < 3; ++a) {
+                const float x = (b[a] - o[a]) * iv[a], y = (b[3 + a] - o[a]) * iv[a];
+                t0 = std::max(t0, std::min(x, y));
+                t1 = std::min(t1, std::max(x, y));
+            }
+            if (!(t0 <= t1 * kSlab)) continue;
+            const int32_t tri = links3[3 * size_t(ni) + 2];
+            if (tri >= 0) {
+                float tt, u, v;
+                if (intersect(ro, rd, triAt(tris9, size_t(tri)), tt, u, v) && (tt < best || (any && tt == best && tri < htri))) {
+                    best = tt; htri = tri; hu = u; hv = v; any = true;
+                }
+                continue;
+            }
+            if (sp > 253) std::abort();
+            stack[sp++] = links3[3 * size_t(ni) + 1];
+            stack[sp++] = links3[3 * size_t(ni)];
+        }
+        if (any) { h[0] = best; h[1] = hu; h[2] = hv; std::memcpy(&h[3], &htri, 4); }
+    });
+}
+
 size_t orc_scene_tris(const char* name, size_t count, uint64_t seed, float* tris9) {
     const std::string s(name);
     std::vector<Tri> t;

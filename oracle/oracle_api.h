@@ -116,6 +116,19 @@ double orc_binned_sah_cost(const float* tris9, size_t ntri, size_t* node_count);
 void orc_scan_rays(const float* tris9, size_t ntri, const float* rays6, const float* tmax,
                    size_t nrays, int any, int threads, float* t_out, int32_t* tri_out);
 
+/* The R2 check's rays (src/renderer/gpu/rt_bvh_parity.cpp:36-48): rays8 {o, d, tMax = 1e30,
+ * 0}; eye/target/up NULL for the scene without a camera (soup). cam_first_u picks the order of
+ * the two draws of a camera ray's jitter. */
+void orc_r2_rays(const float* eye, const float* target, const float* up, size_t n, uint64_t seed,
+                 int cam_first_u, float* rays8);
+/* bruteForce (rt_bvh_parity.cpp:19-34): hits4 {t, u, v, tri (int32 bits)} of the nearest hit
+ * (ties to the lower index; -1, 0, 0, -1 on a miss) and the second-nearest t. */
+void orc_scan_hits(const float* tris9, size_t ntri, const float* rays8, size_t nrays, int threads,
+                   float* hits4, float* t2);
+/* rt::closest (src/renderer/rt_bvh.cpp:125-152) over a tree of boxes6/links3 from root. */
+void orc_tree_closest(const float* boxes6, const int32_t* links3, int32_t root, const float* tris9,
+                      const float* rays8, size_t nrays, int threads, float* hits4);
+
 /* std::sin and std::cos of every sample angle 6.2831853f * k / 2^24, k < 2^24. */
 void orc_libm_angles(float* s, float* c);
 

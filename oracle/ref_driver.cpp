@@ -265,6 +265,26 @@ size_t ref_gpu_certificate(const ref_frame_planes* gpu, const ref_frame_planes* 
     return s.size();
 }
 
+// rt::closest (src/renderer/rt_bvh.cpp:125-152) on the reference's own PLOC tree of tris9:
+// hits4 {t, u, v, tri bits} per ray of rays8 {o, d, tMax, pad}.
+void ref_tree_closest(const float* tris9, size_t ntri, const float* rays8, size_t nrays, float* hits4) {
+    std::vector<raw::Tri> t(ntri);
+    for (size_t i = 0; i < ntri; ++i)
+        t[i] = {{tris9[9 * i], tris9[9 * i + 1], tris9[9 * i + 2]}, {tris9[9 * i + 3], tris9[9 * i + 4], tris9[9 * i + 5]},
+                {tris9[9 * i + 6], tris9[9 * i + 7], tris9[9 * i + 8]}};
+    const raw::rt::Tree tree = raw::rt::buildPloc(t);
+    for (size_t k = 0; k < nrays; ++k) {
+        const float* q = rays8 + 8 * k;
+        const raw::Ray r{{q[0], q[1], q[2]}, {q[3], q[4], q[5]}};
+        raw::Hit h;
+        float* o = hits4 + 4 * k;
+        int32_t tri = -1;
+        o[0] = -1.0f; o[1] = 0.0f; o[2] = 0.0f;
+        if (raw::rt::closest(tree, t, r, q[6], h)) { o[0] = h.t; o[1] = h.u; o[2] = h.v; tri = h.tri; }
+        std::memcpy(&o[3], &tri, 4);
+    }
+}
+
 int32_t ref_ploc_build(const float* tris9, size_t ntri, int radius, float* boxes6, int32_t* links3, uint32_t* rounds) {
     std::vector<raw::Tri> t(ntri);
     for (size_t i = 0; i < ntri; ++i)

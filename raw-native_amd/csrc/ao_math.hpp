@@ -171,6 +171,38 @@ AO_HD bool intersect_t(V3 ro, V3 rd, const RayFrame& f, V3 ta, V3 tb, V3 tc, flo
     return t > 1e-4f;
 }
 
+// The whole of intersectTri: the hit decision, t and the barycentric weights u, v of b and c.
+AO_HD bool intersect_tuv(V3 ro, V3 rd, const RayFrame& f, V3 ta, V3 tb, V3 tc, float& t, float& u, float& v) {
+    const V3 A = wframe(sub(ta, ro), f.kz, f.sw), B = wframe(sub(tb, ro), f.kz, f.sw), C = wframe(sub(tc, ro), f.kz, f.sw);
+    const float Ax = A.x - f.sx * A.z, Ay = A.y - f.sy * A.z;
+    const float Bx = B.x - f.sx * B.z, By = B.y - f.sy * B.z;
+    const float Cx = C.x - f.sx * C.z, Cy = C.y - f.sy * C.z;
+    const float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const V3 e1 = sub(tb, ta), e2 = sub(tc, ta), p = cross(rd, e2);
+    const float dm = dot(e1, p);
+    if (dm > -1e-7f && dm < 1e-7f) {
+        const float T = U * (f.sz * A.z) + V * (f.sz * B.z) + W * (f.sz * C.z), inv = 1.0f / det;
+        t = T * inv;
+        u = V * inv;
+        v = W * inv;
+        return t > 1e-4f;
+    }
+    const float inv = 1.0f / dm;
+    const V3 tv = sub(ro, ta), q = cross(tv, e1);
+    u = stdmax(0.0f, dot(tv, p) * inv);
+    v = stdmax(0.0f, dot(rd, q) * inv);
+    t = dot(e2, q) * inv;
+    const float s = u + v;
+    if (s > 1.0f) {
+        u /= s;
+        v /= s;
+    }
+    return t > 1e-4f;
+}
+
 // basis (src/renderer/ray_ao.cpp:14-17).
 AO_HD void basis(V3 n, V3& t, V3& b) {
     const V3 a = fabsf(n.x) > 0.9f ? v3(0, 1, 0) : v3(1, 0, 0);

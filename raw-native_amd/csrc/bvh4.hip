@@ -482,6 +482,80 @@ __global__ __launch_bounds__(BT) void k_trace_rays4(const float* __restrict__ ra
     out4[4 * size_t(k) + 3] = float(tests);
 }
 
+// rt::closest (src/renderer/rt_bvh.cpp:125-152) for a batch of rays: the nearest hit with
+// t < tMax, ties to the lower input index (a function of the triangles alone, as the box test
+// only adds visits), then u and v from intersectTri of that triangle in input order. A ray whose
+// stack would overflow is answered by a scan of every triangle. hits: {t, u, v, tri bits};
+// a miss is {-1, 0, 0, -1}.
+__global__ __launch_bounds__(BT) void k_closest4(const float* __restrict__ rays8, uint32_t nrays, const Node4* __restrict__ nodes,
+                                                 const float4* __restrict__ ts, uint32_t ntri, const float* __restrict__ tris_in,
+                                                 float4* __restrict__ hits) {
+    __shared__ uint32_t stk[STACK_LDS * BT];
+    const uint32_t k = blockIdx.x * BT + threadIdx.x;
+    if (k >= nrays) return;
+    const float* q = rays8 + 8 * size_t(k);
+    const Ray r = make_ray(v3(q[0], q[1], q[2]), v3(q[3], q[4], q[5]));
+    const float tmax = q[6];
+    float best = tmax;
+    int32_t bi = -1;
+    bool overflow = false;
+    const auto take = [&](uint32_t slot) {
+        float t;
+        if (!tri_hit(r, ts, slot, t)) return;
+        const int32_t id = __float_as_int(ts[3 * size_t(slot) + 2].y);
+        if (t < best || (bi >= 0 && t == best && id < bi)) {
+            best = t;
+            bi = id;
+        }
+    };
+    if (ntri) {
+        uint32_t* const lds_col = stk + threadIdx.x;
+        uint32_t spill[STACK_SPILL];
+        int sp = 0;
+        int32_t node = 0;
+        uint32_t visits = 0;
+        while (node >= 0) {
+            if (++visits > 4 * ntri + 4) { overflow = true; break; }   // a malformed tree
+            const Node4 nd = nodes[node];
+            const uint32_t hit = slab4(r, nd, best * kReach, kExit);
+            int32_t next = -1;
+            for (int c = 0; c < 4 && !overflow; ++c) {
+                if (!((hit >> c) & 1u)) continue;
+                const int32_t ch = ref_of(nd.ref, c);
+                if (ch == EMPTY) continue;
+                if (ch < 0) {
+                    if (uint32_t(~ch) < ntri) take(uint32_t(~ch));
+                } else if (next < 0) {
+                    next = ch;
+                } else if (sp < STACK_LDS) {
+                    lds_col[sp++ * BT] = uint32_t(ch);
+                } else if (sp < STACK_LDS + STACK_SPILL) {
+                    spill[sp++ - STACK_LDS] = uint32_t(ch);
+                } else {
+                    overflow = true;
+                }
+            }
+            if (overflow) break;
+            if (next >= 0) node = next;
+            else if (sp > 0) { --sp; node = int32_t(sp < STACK_LDS ? lds_col[sp * BT] : spill[sp - STACK_LDS]); }
+            else node = -1;
+        }
+        if (overflow) {   // the scan of every triangle
+            best = tmax;
+            bi = -1;
+            for (uint32_t i = 0; i < ntri; ++i) take(i);
+        }
+    }
+    float4 h = make_float4(-1.0f, 0.0f, 0.0f, __int_as_float(-1));
+    if (bi >= 0) {
+        const float* t9 = tris_in + 9 * size_t(bi);
+        float t, u, v;
+        if (intersect_tuv(r.o, r.d, r.f, v3(t9[0], t9[1], t9[2]), v3(t9[3], t9[4], t9[5]), v3(t9[6], t9[7], t9[8]), t, u, v))
+            h = make_float4(t, u, v, __int_as_float(bi));
+    }
+    hits[k] = h;
+}
+
 uint32_t cdiv(uint64_t a, uint32_t b) { return uint32_t((a + b - 1) / b); }
 
 
@@ -913,6 +987,13 @@ int trace_rays4(Ctx& c, const float* d_rays, uint32_t n, int any, float far_scal
     k_trace_rays4<<<cdiv(n, BT), BT, 0, s>>>(d_rays, n, any, far_scale, c.nodes4.as<Node4>(), c.tris_sorted.as<float4>(),
                                               c.ntri, d_out);
     return hip_ok(c, hipGetLastError(), "trace_rays launch") ? 0 : -2;
+}
+
+int closest_hits4(Ctx& c, const float* d_rays, uint32_t n, float* d_hits, hipStream_t s) {
+    if (n == 0) return 0;
+    k_closest4<<<cdiv(n, BT), BT, 0, s>>>(d_rays, n, c.nodes4.as<Node4>(), c.tris_sorted.as<float4>(), c.ntri,
+                                           c.tris_in.as<float>(), reinterpret_cast<float4*>(d_hits));
+    return hip_ok(c, hipGetLastError(), "closest_hits launch") ? 0 : -2;
 }
 
 int batch_trace_enqueue(const BatchDev& d, uint32_t persistent_blocks, hipStream_t s) {

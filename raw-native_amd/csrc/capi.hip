@@ -623,6 +623,35 @@ int raw_ao_trace_rays(raw_ao_ctx* x, const float* rays8, uint32_t nrays, int any
     return RAW_AO_OK;
 }
 
+int raw_ao_closest_hits_dev(raw_ao_ctx* x, const float* d_rays8, uint32_t nrays, void* d_hits4, void* stream) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    if (nrays && (!d_rays8 || !d_hits4)) return fail(c, RAW_AO_ERR_ARG, "null rays");
+    if (c.ntri && c.root < 0) return fail(c, RAW_AO_ERR_ARG, "no BVH: call raw_ao_build_ploc first");
+    if (c.ntri && c.built_kernel != RAW_AO_TRACE_WIDE)
+        return fail(c, RAW_AO_ERR_ARG, "closest hits need the wide traversal form: build with RAW_AO_TRACE_WIDE");
+    return code_of(raw_ao::closest_hits4(c, d_rays8, nrays, static_cast<float*>(d_hits4), pick(c, stream)));
+}
+
+int raw_ao_closest_hits(raw_ao_ctx* x, const float* rays8, uint32_t nrays, void* hits4) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    if (!begin(c)) return RAW_AO_ERR_DEVICE;
+    if (nrays && (!rays8 || !hits4)) return fail(c, RAW_AO_ERR_ARG, "null rays");
+    if (!nrays) return RAW_AO_OK;
+    if (!ensure(c, c.st_rays, 32 * size_t(nrays)) || !ensure(c, c.st_out2, 16 * size_t(nrays))) return RAW_AO_ERR_DEVICE;
+    hipStream_t s = c.stream;
+    if (!hip_ok(c, hipMemcpyAsync(c.st_rays.p, rays8, 32 * size_t(nrays), hipMemcpyHostToDevice, s), "upload"))
+        return RAW_AO_ERR_DEVICE;
+    const int r = raw_ao_closest_hits_dev(x, c.st_rays.as<float>(), nrays, c.st_out2.p, nullptr);
+    if (r) return r;
+    if (!hip_ok(c, hipMemcpyAsync(hits4, c.st_out2.p, 16 * size_t(nrays), hipMemcpyDeviceToHost, s), "download") ||
+        !hip_ok(c, hipStreamSynchronize(s), "closest_hits"))
+        return RAW_AO_ERR_DEVICE;
+    return RAW_AO_OK;
+}
+
 int raw_ao_count_next_trace(raw_ao_ctx* x, int enable) {
     if (!x) return RAW_AO_ERR_ARG;
     x->c.err.clear();

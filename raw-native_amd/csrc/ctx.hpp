@@ -122,6 +122,7 @@ int trace_vis4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_
                const float to_light[3], int sun_forced, float* d_ao, float* d_sky, float* d_sun, hipStream_t s);
 int trace_rays4(Ctx& c, const float* d_rays, uint32_t n, int any, float far_scale, float* d_out, hipStream_t s);
 int trace_rays(Ctx& c, const float* d_rays, uint32_t n, int any, float far_scale, float* d_out, hipStream_t s);
+int closest_hits4(Ctx& c, const float* d_rays, uint32_t n, float* d_hits, hipStream_t s);
 int ssao(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int samples,
          float radius, float* d_ao, hipStream_t s);
 // k_ssao without event records (stream-capture safe); trig may be null (correctly rounded).

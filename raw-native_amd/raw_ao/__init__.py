@@ -65,6 +65,8 @@ SIGNATURES = [
     ("raw_ao_error_map", C.c_int, [_vp, _f32p, _f32p, _u8p, C.c_int, C.c_int, _f32p, C.POINTER(C.c_float)]),
     ("raw_ao_error_map_dev", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp]),
     ("raw_ao_trace_rays", C.c_int, [_vp, _f32p, C.c_uint32, C.c_int, C.c_float, _f32p]),
+    ("raw_ao_closest_hits", C.c_int, [_vp, _f32p, C.c_uint32, C.c_void_p]),
+    ("raw_ao_closest_hits_dev", C.c_int, [_vp, _vp, C.c_uint32, _vp, _vp]),
     ("raw_ao_count_next_trace", C.c_int, [_vp, C.c_int]),
     ("raw_ao_trace_counts", C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("raw_ao_last_ms", C.c_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
@@ -421,6 +423,18 @@ class Context:
         fs = host.ROBUST_FAR_SCALE if far_scale is None else far_scale
         self._check(lib().raw_ao_trace_rays(self._h, r, r.shape[0], int(bool(any_hit)), fs, out))
         return out
+
+    HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("tri", "<i4")])
+
+    def closest_hits(self, rays8):
+        """rt::closest for every ray (rows of {o, d, tMax, pad}): a structured array of (t, u, v, tri)."""
+        r = np.ascontiguousarray(rays8, np.float32).reshape(-1, 8)
+        out = np.zeros(r.shape[0], self.HIT_DTYPE)
+        self._check(lib().raw_ao_closest_hits(self._h, r, r.shape[0], out.ctypes.data))
+        return out
+
+    def closest_hits_dev(self, rays8, nrays, hits, stream=None):
+        self._check(lib().raw_ao_closest_hits_dev(self._h, _ptr(rays8), nrays, _ptr(hits), _vp(stream)))
 
     # ---- measurement ---------------------------------------------------------------
     def count_next_trace(self, enable=True):

@@ -62,6 +62,9 @@ def lib():
         L.orc_scene_tris.argtypes = [C.c_char_p, SZ, C.c_uint64, C.c_void_p]
         L.orc_scene_tris.restype = SZ
         L.orc_libm_angles.argtypes = [f32p, f32p]
+        L.orc_r2_rays.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, SZ, C.c_uint64, C.c_int, f32p]
+        L.orc_scan_hits.argtypes = [f32p, SZ, f32p, SZ, C.c_int, f32p, C.c_void_p]
+        L.orc_tree_closest.argtypes = [f32p, i32p, C.c_int32, f32p, f32p, SZ, C.c_int, f32p]
         L.orc_rasterize_scene.argtypes = [f32p, f32p, i32p, SZ, u32p, f32p, SZ, f32p, f32p, f32p, C.c_float, C.c_float,
                                           C.c_float, C.c_float, C.c_int, C.c_int, f32p, f32p, f32p, f32p, u8p]
         L.orc_view_proj.argtypes = [f32p, f32p, f32p, C.c_float, C.c_float, C.c_float, C.c_float, f32p]
@@ -99,6 +102,7 @@ def ref():
         L.ref_rtao_accel.argtypes = [C.c_void_p, f32p, f32p, u8p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, f32p]
         L.ref_rtvis.argtypes = [f32p, SZ, f32p, f32p, u8p, C.c_int, C.c_int, C.c_int, C.c_float, f32p, C.c_int, C.c_int,
                                 f32p, f32p, f32p]
+        L.ref_tree_closest.argtypes = [f32p, SZ, f32p, SZ, f32p]
         L.ref_render_frame.argtypes = [f32p, f32p, SZ, i32p, SZ, u32p, f32p, SZ, f32p, f32p, f32p, C.c_float, C.c_float,
                                        C.c_float, C.c_float, C.c_int, f32p, C.c_float, C.c_int, C.c_int, C.c_void_p,
                                        C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, f32p, f32p, f32p, f32p, u8p, f32p,
@@ -395,3 +399,50 @@ def ref_frame(sc, w, h, prev=None, lighting=2, no_sun=False, rtao_on=True, tol=0
 def sha256_file(path):
     import hashlib
     return hashlib.sha256(open(path, "rb").read()).hexdigest()
+
+
+# ---- the R2 check (src/renderer/gpu/rt_bvh_parity.cpp) -------------------------------
+
+# The scenes' cameras (eye, target; up (0, 1, 0)): src/renderer/swr_scenes.cpp:41,65,80 and
+# src/renderer/rt_scenes.cpp:35. The soup has none.
+R2_CAMERAS = {"retro_room": ((0.0, 1.6, 1.0), (0.3, 1.2, -8.0)), "iso_street": ((24.0, 20.0, 24.0), (0.0, 0.5, 0.0)),
+              "clip_stress": ((0.0, 0.25, 0.0), (0.2, 0.12, -3.0)), "dense_block": ((58.0, 40.0, 62.0), (0.0, 2.0, 0.0)),
+              "soup": None}
+HIT = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("tri", "<i4")])
+
+
+def r2_rays(name, ntri, n, cam_first_u=False):
+    """makeRays (rt_bvh_parity.cpp:36-48) with seed 0xb2b2 + the triangle count; the camera ray's
+    two draws in the order the reference's R2 evidence was made with (the up offset's first)."""
+    rays = np.zeros((n, 8), np.float32)
+    cam = R2_CAMERAS[name]
+    if cam is None:
+        lib().orc_r2_rays(None, None, None, n, 0xB2B2 + ntri, int(cam_first_u), rays)
+    else:
+        e, t, u = (np.array(v, np.float32) for v in (cam[0], cam[1], (0.0, 1.0, 0.0)))
+        lib().orc_r2_rays(e.ctypes.data, t.ctypes.data, u.ctypes.data, n, 0xB2B2 + ntri, int(cam_first_u), rays)
+    return rays
+
+
+def scan_hits(tris9, rays8, threads=None):
+    """bruteForce: (hits as HIT records, second-nearest t)."""
+    tris9 = np.ascontiguousarray(tris9, np.float32).reshape(-1, 9)
+    rays8 = np.ascontiguousarray(rays8, np.float32).reshape(-1, 8)
+    n = rays8.shape[0]
+    hits = np.zeros(n, HIT)
+    t2 = np.zeros(n, np.float32)
+    lib().orc_scan_hits(tris9, tris9.shape[0], rays8, n, threads or os.cpu_count() or 1, hits.view(np.float32), t2.ctypes.data)
+    return hits, t2
+
+
+def tree_closest(boxes, links, root, tris9, rays8, threads=None, impl=None):
+    """rt::closest over a given tree (impl=ref(): the reference's own, on its own PLOC tree)."""
+    tris9 = np.ascontiguousarray(tris9, np.float32).reshape(-1, 9)
+    rays8 = np.ascontiguousarray(rays8, np.float32).reshape(-1, 8)
+    hits = np.zeros(rays8.shape[0], HIT)
+    if impl is not None:
+        impl.ref_tree_closest(tris9, tris9.shape[0], rays8, rays8.shape[0], hits.view(np.float32))
+    else:
+        lib().orc_tree_closest(np.ascontiguousarray(boxes, np.float32), np.ascontiguousarray(links, np.int32), root, tris9,
+                               rays8, rays8.shape[0], threads or os.cpu_count() or 1, hits.view(np.float32))
+    return hits

@@ -400,3 +400,43 @@ def test_frame_matches_reference_code(opts):
     _same_frame(O.frame(sc, w, h, prev=prev, **opts), O.ref_frame(sc, w, h, prev=prev, **opts))
     sc = _multi_mesh_scene(80, 60)
     _same_frame(O.frame(sc, 80, 60, **opts), O.ref_frame(sc, 80, 60, **opts))
+
+
+# ---- the R2 check's rays, scan and CPU traversal ---------------------------------------
+
+@pytest.mark.parametrize("name", ["retro_room", "iso_street", "clip_stress", "dense_block", "soup"])
+def test_r2_rays_and_scan_match_the_evidence(name):
+    """makeRays + bruteForce reproduce the hit counts of the reference's R2 record
+    (evidence/rt-r2-bvh-gpu-d3d12-rtx4090.json, 100,000 rays a scene); the CPU rt::closest over
+    the oracle's PLOC tree finds the scan's nearest hit (t, u, v, triangle) on every ray."""
+    from cases import TREES
+    tris = O.scene_tris(name, 20000, 7)
+    g = TREES[name]
+    rays = O.r2_rays(name, tris.shape[0], g["r2_rays"])
+    scan, t2 = O.scan_hits(tris, rays)
+    assert int((scan["tri"] >= 0).sum()) == g["r2_hits"]
+    assert ((t2 >= scan["t"]) | (scan["tri"] < 0)).all()
+    b, l, root, _ = O.ploc(tris)
+    n = 20000
+    cpu = O.tree_closest(b, l, root, tris, rays[:n])
+    assert np.array_equal(cpu.view(np.uint32), scan[:n].view(np.uint32))
+
+
+def test_r2_camera_draw_order_is_pinned():
+    """The other order of the camera ray's two draws misses the evidence's count."""
+    from cases import TREES
+    tris = O.scene_tris("iso_street")
+    rays = O.r2_rays("iso_street", tris.shape[0], 100000, cam_first_u=True)
+    assert int((O.scan_hits(tris, rays)[0]["tri"] >= 0).sum()) != TREES["iso_street"]["r2_hits"]
+
+
+@needs_ref
+def test_tree_closest_matches_reference_code():
+    """orc_tree_closest against the reference's own rt::closest on its own PLOC tree."""
+    tris = O.scene_tris("retro_room")
+    rays = O.r2_rays("retro_room", tris.shape[0], 20000)
+    rays[::3, 6] = 2.5   # some finite tMax
+    b, l, root, _ = O.ploc(tris)
+    mine = O.tree_closest(b, l, root, tris, rays)
+    ref = O.tree_closest(None, None, None, tris, rays, impl=O.ref())
+    assert np.array_equal(mine.view(np.uint32), ref.view(np.uint32))
