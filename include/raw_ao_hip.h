@@ -72,7 +72,11 @@ enum {
     /* 64-spp wide kernel, 0..64: with a value n > 0 a wave owns 4 pixels' 256 rays and a lane
      * that finishes takes the next ray once n lanes are idle; 0 = one pixel's 64 rays a wave.
      * Results are the same for every value; only the speed differs. */
-    RAW_AO_OPT_REFILL = 6
+    RAW_AO_OPT_REFILL = 6,
+    /* 64-spp wide kernel, 0 or 1 (default 1): descend into the hit child box with the smallest
+     * entry distance first (0: the first in slot order). Results are the same; only the speed
+     * differs. */
+    RAW_AO_OPT_NEAR_FIRST = 7
 };
 enum {
     RAW_AO_TRACE_WIDE = 0,     /* 4-wide nodes, LDS stack, triangle tests batched per wave */

@@ -263,6 +263,20 @@ __device__ __forceinline__ uint32_t slab4s(const Ray& r, const char* __restrict_
     return (one(nx.x, ny.x, nz.x, fx.x, fy.x, fz.x) ? 1u : 0u) | (one(nx.y, ny.y, nz.y, fx.y, fy.y, fz.y) ? 2u : 0u) |
            (one(nx.z, ny.z, nz.z, fx.z, fy.z, fz.z) ? 4u : 0u) | (one(nx.w, ny.w, nz.w, fx.w, fy.w, fz.w) ? 8u : 0u);
 }
+// slab4s that also returns each slot's entry distance.
+__device__ __forceinline__ uint32_t slab4s_t(const Ray& r, const char* __restrict__ base, uint32_t off, uint32_t ox, uint32_t oy,
+                                             uint32_t oz, float reach, float exit_scale, float (&t0s)[4]) {
+    const float4 nx = fma4(ld4(base, off + ox), r.inv.x, -r.oinv.x), fx = fma4(ld4(base, off + 48 - ox), r.inv.x, -r.oinv.x);
+    const float4 ny = fma4(ld4(base, off + 16 + oy), r.inv.y, -r.oinv.y), fy = fma4(ld4(base, off + 64 - oy), r.inv.y, -r.oinv.y);
+    const float4 nz = fma4(ld4(base, off + 32 + oz), r.inv.z, -r.oinv.z), fz = fma4(ld4(base, off + 80 - oz), r.inv.z, -r.oinv.z);
+    const auto one = [&](float a, float b, float c, float d, float e, float f, float& t0) {
+        t0 = fmaxf(fmaxf(a, b), fmaxf(c, 0.0f));
+        const float t1 = fminf(fminf(d, e), fminf(f, reach));
+        return t0 <= t1 * exit_scale;
+    };
+    return (one(nx.x, ny.x, nz.x, fx.x, fy.x, fz.x, t0s[0]) ? 1u : 0u) | (one(nx.y, ny.y, nz.y, fx.y, fy.y, fz.y, t0s[1]) ? 2u : 0u) |
+           (one(nx.z, ny.z, nz.z, fx.z, fy.z, fz.z, t0s[2]) ? 4u : 0u) | (one(nx.w, ny.w, nz.w, fx.w, fy.w, fz.w, t0s[3]) ? 8u : 0u);
+}
 __device__ __forceinline__ int32_t ref_of(const int4& r, int k) {
     const int32_t lo = (k & 1) ? r.y : r.x, hi = (k & 1) ? r.w : r.z;
     return (k & 2) ? hi : lo;
@@ -277,7 +291,10 @@ __device__ __forceinline__ bool tri_hit(const Ray& r, const float4* __restrict__
 // register.
 // Any hit with t < tmax through the wide tree. Returns 1 occluded, 0 open, -1 the stack
 // would overflow (the caller answers that ray by the scan).
-template <bool COUNT>
+// NEAR: descend into the hit inner slot with the smallest entry distance first and stack the
+// others farthest first; otherwise the first hit slot, the rest in slot order. The answer is
+// the same either way.
+template <bool COUNT, bool NEAR = true>
 __device__ int occluded4(const Ray& r, float tmax, const Node4* __restrict__ nodes, const float4* __restrict__ ts,
                          uint32_t ntri, uint32_t* lds_col, uint32_t tri_q, uint32_t& visits, uint32_t& tests,
                          bool active = true) {
@@ -300,15 +317,41 @@ __device__ int occluded4(const Ray& r, float tmax, const Node4* __restrict__ nod
         if (live && pend == 0 && node >= 0) {
             const uint32_t off = uint32_t(node) * uint32_t(sizeof(Node4));
             if (COUNT) ++visits;
-            const uint32_t hit = slab4s(r, base, off, ox, oy, oz, reach, kExit);
+            float t0s[4];
+            const uint32_t hit = NEAR ? slab4s_t(r, base, off, ox, oy, oz, reach, kExit, t0s) : slab4s(r, base, off, ox, oy, oz, reach, kExit);
             const int4 ref = *reinterpret_cast<const int4*>(base + off + 96);
             const uint32_t masks = *reinterpret_cast<const uint32_t*>(base + off + 112);   // leaf slots bits 0-3, inner 4-7
             pend = hit & masks & 15u;
             leafref = ref;
             uint32_t ih = hit & (masks >> 4);
             if (ih) {
-                node = ref_of(ref, __builtin_ctz(ih));
-                ih &= ih - 1u;
+                int first = __builtin_ctz(ih);
+                if (NEAR) {
+                    float tb = t0s[first];
+                    for (int c = first + 1; c < 4; ++c)
+                        if (((ih >> c) & 1u) && t0s[c] < tb) { tb = t0s[c]; first = c; }
+                }
+                node = ref_of(ref, first);
+                ih &= ~(1u << first);
+                if (NEAR && (ih & (ih - 1u))) {   // two or three more: push the farthest first, so the nearest pops first
+                    uint32_t order[3];
+                    float dist[3];
+                    int m = 0;
+                    for (int c = 0; c < 4; ++c)
+                        if ((ih >> c) & 1u) { order[m] = uint32_t(ref_of(ref, c)); dist[m] = t0s[c]; ++m; }
+                    for (int i = 1; i < m; ++i)   // descending distance
+                        for (int j = i; j > 0 && dist[j] > dist[j - 1]; --j) {
+                            const float td = dist[j]; dist[j] = dist[j - 1]; dist[j - 1] = td;
+                            const uint32_t to = order[j]; order[j] = order[j - 1]; order[j - 1] = to;
+                        }
+                    for (int i = 0; i < m; ++i) {
+                        if (sp < STACK_LDS) lds_col[sp * BT] = order[i];
+                        else if (sp < STACK_LDS + STACK_SPILL) spill[sp - STACK_LDS] = order[i];
+                        else { result = -1; live = false; }
+                        ++sp;
+                    }
+                    ih = 0;
+                }
                 while (ih) {
                     const uint32_t v = uint32_t(ref_of(ref, __builtin_ctz(ih)));
                     ih &= ih - 1u;
@@ -367,7 +410,7 @@ struct TraceArgs4 {
     float radius;
 };
 
-template <bool SPP64, bool COUNT>
+template <bool SPP64, bool COUNT, bool NEAR = true>
 __global__ __launch_bounds__(BT) void k_trace_ao4(TraceArgs4 a) {
     __shared__ uint32_t stk[STACK_LDS * BT];
     const uint64_t gid = uint64_t(blockIdx.x) * BT + threadIdx.x;
@@ -394,7 +437,7 @@ __global__ __launch_bounds__(BT) void k_trace_ao4(TraceArgs4 a) {
     uint32_t visits = 0, tests = 0;
     if (a.ntri) {
         const Ray r = make_ray(origin, dir);
-        const int res = occluded4<COUNT>(r, a.radius, a.nodes, a.tris, a.ntri, stk + threadIdx.x, a.tri_q, visits, tests);
+        const int res = occluded4<COUNT, NEAR>(r, a.radius, a.nodes, a.tris, a.ntri, stk + threadIdx.x, a.tri_q, visits, tests);
         if (res < 0) {
             occ = scan_occluded(r, a.radius, a.tris, a.ntri);
             atomicAdd(&a.counters->overflows, 1ull);
@@ -969,8 +1012,14 @@ int trace_ao4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_m
         constexpr int PW = 4;
         k_trace_ao4r<PW><<<cdiv(npix, uint32_t(PW * (BT / 64))), BT, 0, s>>>(a, uint32_t(c.refill));
     } else if (spp64) {
-        if (count) k_trace_ao4<true, true><<<blocks, BT, 0, s>>>(a);
-        else k_trace_ao4<true, false><<<blocks, BT, 0, s>>>(a);
+        if (count) {
+            if (c.near_first) k_trace_ao4<true, true><<<blocks, BT, 0, s>>>(a);
+            else k_trace_ao4<true, true, false><<<blocks, BT, 0, s>>>(a);
+        } else if (c.near_first) {
+            k_trace_ao4<true, false><<<blocks, BT, 0, s>>>(a);
+        } else {
+            k_trace_ao4<true, false, false><<<blocks, BT, 0, s>>>(a);
+        }
     } else {
         if (count) k_trace_ao4<false, true><<<blocks, BT, 0, s>>>(a);
         else k_trace_ao4<false, false><<<blocks, BT, 0, s>>>(a);

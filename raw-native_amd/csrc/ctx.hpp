@@ -44,6 +44,7 @@ struct Ctx {
     int tri_q{16};               // wide kernel: triangle phase when tri_q/64 of live lanes wait
     int ploc_tail{4096};         // the single-workgroup PLOC tail takes over at this many clusters
     int refill{0};               // 64-spp wide kernel: refill lanes once this many are idle (0: off)
+    bool near_first{true};       // 64-spp wide kernel: descend into the nearest hit child first
     bool count_next{false};
 
     // host-libm sin/cos correction table: a 2 MB bitmap of the corrected angle indices, then one nibble an index

@@ -312,3 +312,18 @@ def test_refill_kernel_matches(ctx, refill):
         assert np.all(ctx.compute_rtao(pos2, nrm2, mask2, 64, 2.0) == 1.0)
     finally:
         ctx.set_option(raw_ao.OPT_REFILL, 0)
+
+
+def test_near_first_order_does_not_change_ao(ctx):
+    """The 64-spp kernel's child order (nearest first, or slot order) leaves every pixel's AO
+    bit for bit the same."""
+    tris, pos, nrm, mask = big_frame(160, 90, seed=11)
+    ctx.build_ploc(tris, 16)
+    outs = []
+    try:
+        for nf in (1, 0):
+            ctx.set_option(raw_ao.OPT_NEAR_FIRST, nf)
+            outs.append(ctx.compute_rtao(pos, nrm, mask, 64, 1.0))
+    finally:
+        ctx.set_option(raw_ao.OPT_NEAR_FIRST, 1)
+    assert np.array_equal(bits(outs[0]), bits(outs[1]))
