@@ -59,9 +59,15 @@ bool begin(Ctx& c) {
     return hip_ok(c, hipSetDevice(c.device), "hipSetDevice");
 }
 bool image_ok(int w, int h) { return w > 0 && h > 0 && int64_t(w) * int64_t(h) <= (int64_t(1) << 30); }
+// Milliseconds between two of the context's events, 0 when either has not been recorded yet.
+// The failed query's error is taken off the thread's last-error slot, so the next launch check
+// does not report it.
 float elapsed(Ctx& c, int a, int b) {
     float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, c.ev[a], c.ev[b]) != hipSuccess) return 0.0f;
+    if (hipEventElapsedTime(&ms, c.ev[a], c.ev[b]) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.0f;
+    }
     return ms;
 }
 }  // namespace

@@ -238,6 +238,7 @@ int render_frame(Ctx& c, const raw_ao_scene& sc, const raw_ao_render_opts& o, in
         std::memcpy(stats->viewproj, cur.m, 64);
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, c.ev[8], c.ev[9]) == hipSuccess) stats->gpu_ms = ms;
+        else (void)hipGetLastError();
     }
     return 0;
 }

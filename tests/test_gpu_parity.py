@@ -327,3 +327,16 @@ def test_near_first_order_does_not_change_ao(ctx):
     finally:
         ctx.set_option(raw_ao.OPT_NEAR_FIRST, 1)
     assert np.array_equal(bits(outs[0]), bits(outs[1]))
+
+
+def test_timings_before_every_kernel_ran_leave_no_error():
+    """raw_ao_last_ms before the SSAO and error-map kernels have ever run reports 0 for them,
+    and the next launches do not see that query's error."""
+    import raw_ao
+    tris, pos, nrm, mask = big_frame(64, 48, seed=2)
+    with raw_ao.Context(0) as c:
+        c.build_ploc(tris, 16)
+        m = c.last_ms()
+        assert m["ssao"] == 0.0 and m["error"] == 0.0
+        ao = c.compute_rtao(pos, nrm, mask, 64, 1.0)
+        assert ao.shape == mask.shape
