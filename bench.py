@@ -308,6 +308,7 @@ def main():
     ap.add_argument("--tri-q", type=int, default=0, help="wide kernel's triangle-phase threshold (1..64; 0: default)")
     ap.add_argument("--refill", type=int, default=-1, help="64-spp kernel: refill lanes at this many idle (0 off; -1 default)")
     ap.add_argument("--near-first", type=int, default=-1, help="64-spp kernel: nearest child first (0/1; -1 default)")
+    ap.add_argument("--wave-cut", type=int, default=-1, help="64-spp kernel: start rays from the wave's cut (0..64; -1 default)")
     ap.add_argument("--ploc-tail", type=int, default=0, help="PLOC: clusters at which one workgroup takes over (1..4096)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned AO surrogate at 1080p")
     ap.add_argument("--no-denoise", action="store_true", help="skip the 4-spp + learned denoise workload at 4K")
@@ -359,6 +360,8 @@ def main():
         ctx.set_option(raw_ao.OPT_REFILL, a.refill)
     if a.near_first >= 0:
         ctx.set_option(raw_ao.OPT_NEAR_FIRST, a.near_first)
+    if a.wave_cut >= 0:
+        ctx.set_option(raw_ao.OPT_WAVE_CUT, a.wave_cut)
 
     host_ms = []
 

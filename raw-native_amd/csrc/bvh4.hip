@@ -286,6 +286,29 @@ __device__ __forceinline__ bool tri_hit(const Ray& r, const float4* __restrict__
     return intersect_t(r.o, r.d, r.f, v3(p0.x, p0.y, p0.z), v3(p0.w, p1.x, p1.y), v3(p1.z, p1.w, p2.x), t);
 }
 
+constexpr int CUT_N = 32, CUT_LEAF = 16;
+constexpr int CUT_WAVE = 7 * (CUT_N + CUT_LEAF);   // LDS words a wave: node ids and boxes, leaf slots and boxes
+// A wave's cut of the tree (wave_cut) in LDS.
+struct CutView {
+    uint32_t* id;      // CUT_N node indices
+    float* box;        // 6 rows of CUT_N: lo x, y, z, hi x, y, z
+    uint32_t* leaf;    // CUT_LEAF sorted triangle positions
+    float* lbox;       // 6 rows of CUT_LEAF
+    int n, nleaf;      // entries; n < 0: no cut (start from the root)
+    __device__ static bool enters(const Ray& r, const float* bx, int stride, int k, float reach, float exit_scale) {
+        const float lx = bx[k], ly = bx[stride + k], lz = bx[2 * stride + k];
+        const float hx = bx[3 * stride + k], hy = bx[4 * stride + k], hz = bx[5 * stride + k];
+        const bool px = r.inv.x >= 0.0f, py = r.inv.y >= 0.0f, pz = r.inv.z >= 0.0f;
+        // the same operations as slab4s on one slot
+        const float nx = fmaf(px ? lx : hx, r.inv.x, -r.oinv.x), fx = fmaf(px ? hx : lx, r.inv.x, -r.oinv.x);
+        const float ny = fmaf(py ? ly : hy, r.inv.y, -r.oinv.y), fy = fmaf(py ? hy : ly, r.inv.y, -r.oinv.y);
+        const float nz = fmaf(pz ? lz : hz, r.inv.z, -r.oinv.z), fz = fmaf(pz ? hz : lz, r.inv.z, -r.oinv.z);
+        const float t0 = fmaxf(fmaxf(nx, ny), fmaxf(nz, 0.0f));
+        const float t1 = fminf(fminf(fx, fy), fminf(fz, reach));
+        return t0 <= t1 * exit_scale;
+    }
+};
+
 // The per-lane stack: the top STACK_LDS entries live in LDS (a column per lane, entry e
 // at lds_col[e * BT]), deeper ones in a private spill array; the stack pointer stays in a
 // register.
@@ -294,16 +317,28 @@ __device__ __forceinline__ bool tri_hit(const Ray& r, const float4* __restrict__
 // NEAR: descend into the hit inner slot with the smallest entry distance first and stack the
 // others farthest first; otherwise the first hit slot, the rest in slot order. The answer is
 // the same either way.
+// cut (optional, the wave's shared cut of the tree, see wave_cut): instead of the root, the
+// traversal starts from the first cut node the ray enters and takes the next one whenever its
+// stack runs empty.
 template <bool COUNT, bool NEAR = true>
 __device__ int occluded4(const Ray& r, float tmax, const Node4* __restrict__ nodes, const float4* __restrict__ ts,
                          uint32_t ntri, uint32_t* lds_col, uint32_t tri_q, uint32_t& visits, uint32_t& tests,
-                         bool active = true) {
+                         bool active = true, const CutView* cut = nullptr) {
     const float reach = tmax * kReach;
+    const int ncut = cut ? cut->n : -1;
+    int ci = 0;
+    const auto next_cut = [&]() -> int32_t {
+        while (ci < ncut) {
+            const int k = ci++;
+            if (CutView::enters(r, cut->box, CUT_N, k, reach, kExit)) return int32_t(cut->id[k]);
+        }
+        return -1;
+    };
     const uint32_t ox = r.inv.x >= 0.0f ? 0u : 48u, oy = r.inv.y >= 0.0f ? 0u : 48u, oz = r.inv.z >= 0.0f ? 0u : 48u;
     const char* const base = reinterpret_cast<const char*>(nodes);
     uint32_t spill[STACK_SPILL];
     int sp = 0;
-    int32_t node = 0;                 // the wide root
+    int32_t node = ncut < 0 ? 0 : (active ? next_cut() : -1);   // the wide root, or the first cut node entered
     uint32_t pend = 0;                // pending leaf slots of `leafref`
     int4 leafref = make_int4(0, 0, 0, 0);
     int result = 0;
@@ -364,7 +399,7 @@ __device__ int occluded4(const Ray& r, float tmax, const Node4* __restrict__ nod
                 --sp;
                 node = int32_t(sp < STACK_LDS ? lds_col[sp * BT] : spill[sp - STACK_LDS]);
             } else {
-                node = -1;
+                node = next_cut();
             }
             if (++work > ntri + 4) { result = -1; live = false; }
         }
@@ -396,6 +431,194 @@ __device__ bool scan_occluded(const Ray& r, float tmax, const float4* __restrict
     return false;
 }
 
+// The wave's cut of the tree for rays that all lie in the box [lo, hi] (the 64 rays of one
+// pixel: one origin, length <= reach). The wave walks the top of the tree together, one
+// frontier node a lane: a slot whose box misses [lo, hi] is dropped, a leaf slot that overlaps
+// goes to the cut's leaves (its sorted triangle position and box), an inner slot that overlaps
+// goes to the cut's nodes once its box is no wider than `small` on every axis and to the next
+// frontier otherwise. When a level would overflow a list, the level is dropped and its frontier
+// becomes the rest of the cut's nodes. Every triangle a ray can hit overlaps [lo, hi], and so
+// do all its ancestors, so the cut covers every triangle the traversal from the root can reach.
+// Each entry keeps its box as the parent slot holds it, so a ray enters an entry exactly when
+// the root traversal's test of that slot would pass (CutView::enters). Called by all 64 lanes
+// of a wave together; the frontiers live in the wave's rows of the traversal stack (rows 0-6
+// and 7-13: node, then lo x, y, z, hi x, y, z), which is free until the rays start.
+// Orders one wave's LDS writes before its lanes read each other's entries.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// The tree's slots down to depth TOP_DEPTH (the root's children are depth 1): every slot at that
+// depth and every leaf slot above it, with the box its parent holds, in breadth-first order.
+// They partition the triangles, so a walk may start from them instead of the root.
+constexpr int TOP_DEPTH = 4, TOP_MAX = 256;   // 4^TOP_DEPTH
+struct alignas(16) TopEntry {
+    float lx, ly, lz, hx, hy, hz;
+    int32_t ref;   // >= 0 a node, < 0 ~ a leaf's sorted triangle position
+    uint32_t pad;
+};
+// One wave; four lanes a node, one a slot. Level L holds at most 4^L nodes, so the next
+// frontier (at most 64 before the last level) and the table (at most TOP_MAX) never overflow.
+__global__ __launch_bounds__(64) void k_top_table(const Node4* __restrict__ nodes, TopEntry* __restrict__ out,
+                                                  uint32_t* __restrict__ count) {
+    __shared__ uint32_t ids[2][64];
+    const uint32_t lane = threadIdx.x;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const char* const base = reinterpret_cast<const char*>(nodes);
+    int nf = 1, cur = 0, total = 0;
+    if (lane == 0) ids[0][0] = 0u;
+    wave_sync();
+    for (int level = 0; level < TOP_DEPTH && nf > 0; ++level) {
+        int nn = 0;
+        for (int i0 = 0; i0 < nf; i0 += 16) {
+            const int i = i0 + int(lane >> 2);
+            const uint32_t c = lane & 3u;
+            bool to_table = false, to_next = false;
+            TopEntry e{0, 0, 0, 0, 0, 0, 0, 0};
+            if (i < nf) {
+                const uint32_t off = ids[cur][i] * uint32_t(sizeof(Node4));
+                const uint32_t masks = *reinterpret_cast<const uint32_t*>(base + off + 112);
+                if ((masks >> c) & 0x11u) {
+                    const float* f = reinterpret_cast<const float*>(base + off);
+                    e.lx = f[c];
+                    e.ly = f[4 + c];
+                    e.lz = f[8 + c];
+                    e.hx = f[12 + c];
+                    e.hy = f[16 + c];
+                    e.hz = f[20 + c];
+                    e.ref = reinterpret_cast<const int32_t*>(base + off + 96)[c];
+                    to_table = ((masks >> c) & 1u) || level == TOP_DEPTH - 1;
+                    to_next = !to_table;
+                }
+            }
+            const unsigned long long bt = __ballot(to_table), bn = __ballot(to_next);
+            if (to_table) out[total + __popcll(bt & below)] = e;
+            if (to_next) ids[cur ^ 1][nn + __popcll(bn & below)] = uint32_t(e.ref);
+            total += __popcll(bt);
+            nn += __popcll(bn);
+        }
+        wave_sync();
+        cur ^= 1;
+        nf = nn;
+    }
+    if (lane == 0) *count = uint32_t(total);
+}
+
+// The walk starts from the top table when it has entries (else from the root).
+__device__ void wave_cut(const Node4* __restrict__ nodes, const TopEntry* __restrict__ top, uint32_t ntop, V3 lo, V3 hi,
+                         float small, uint32_t* rows, CutView& cv, uint32_t& steps) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const char* const base = reinterpret_cast<const char*>(nodes);
+    // frontier f's row k, entry i: rows[(7 f + k) BT + i]
+    const auto fr = [&](int f, int k) { return rows + (7 * f + k) * BT; };
+    int cur = 0, nn = 0;
+    cv.n = 0;
+    cv.nleaf = 0;
+    // files one slot (box b, ref) of a lane: dropped, a cut leaf, a cut node or the next frontier
+    const auto place = [&](bool valid, const float (&b)[6], int32_t ref, bool is_leaf) {
+        const bool ov = valid && b[0] <= hi.x && b[3] >= lo.x && b[1] <= hi.y && b[4] >= lo.y && b[2] <= hi.z && b[5] >= lo.z;
+        const bool small_box = b[3] - b[0] <= small && b[4] - b[1] <= small && b[5] - b[2] <= small;
+        const bool to_leaf = ov && is_leaf, to_cut = ov && !is_leaf && small_box, to_front = ov && !is_leaf && !small_box;
+        const unsigned long long bl = __ballot(to_leaf), bc = __ballot(to_cut), bf = __ballot(to_front);
+        if (to_leaf) {
+            const int pos = cv.nleaf + __popcll(bl & below);
+            if (pos < CUT_LEAF) {
+                cv.leaf[pos] = uint32_t(~ref);
+                for (int k = 0; k < 6; ++k) cv.lbox[k * CUT_LEAF + pos] = b[k];
+            }
+        }
+        if (to_cut) {
+            const int pos = cv.n + __popcll(bc & below);
+            if (pos < CUT_N) {
+                cv.id[pos] = uint32_t(ref);
+                for (int k = 0; k < 6; ++k) cv.box[k * CUT_N + pos] = b[k];
+            }
+        }
+        if (to_front) {
+            const int pos = nn + __popcll(bf & below);
+            if (pos < 64) {
+                fr(cur ^ 1, 0)[pos] = uint32_t(ref);
+                for (int k = 0; k < 6; ++k) reinterpret_cast<float*>(fr(cur ^ 1, k + 1))[pos] = b[k];
+            }
+        }
+        cv.nleaf += __popcll(bl);
+        cv.n += __popcll(bc);
+        nn += __popcll(bf);
+    };
+    const auto overflow = [&]() { return cv.nleaf > CUT_LEAF || cv.n > CUT_N || nn > 64; };
+    int nf;
+    bool boxed;   // the frontier's entries carry their boxes (all but the root)
+    if (ntop > 0) {
+        for (uint32_t e0 = 0; e0 < ntop; e0 += 64) {
+            const uint32_t e = e0 + lane;
+            float b[6] = {0, 0, 0, 0, 0, 0};
+            int32_t ref = 0;
+            if (e < ntop) {
+                const float4 p = *reinterpret_cast<const float4*>(top + e);
+                const float4 q = *(reinterpret_cast<const float4*>(top + e) + 1);
+                b[0] = p.x; b[1] = p.y; b[2] = p.z; b[3] = p.w; b[4] = q.x; b[5] = q.y;
+                ref = __float_as_int(q.z);
+                if ((e & 3u) == 0) ++steps;   // four 32-byte entries: one node record's bytes
+            }
+            place(e < ntop, b, ref, ref < 0);
+        }
+        wave_sync();
+        if (overflow()) { cv.n = -1; cv.nleaf = 0; return; }
+        cur = 1;
+        nf = nn;
+        boxed = true;
+    } else {
+        if (lane == 0) fr(0, 0)[0] = 0u;
+        wave_sync();
+        nf = 1;
+        boxed = false;
+    }
+    for (int level = 0; nf > 0; ++level) {
+        if (level == 64) { cv.n = -1; cv.nleaf = 0; return; }   // deeper than any tree this library builds: a malformed tree
+        const int n0 = cv.n, l0 = cv.nleaf;
+        nn = 0;
+        // four lanes a frontier node, one a slot: 16 nodes a pass
+        for (int i0 = 0; i0 < nf; i0 += 16) {
+            const int i = i0 + int(lane >> 2);
+            const uint32_t c = lane & 3u;
+            bool valid = false, is_leaf = false;
+            int32_t ref = 0;
+            float b[6] = {0, 0, 0, 0, 0, 0};
+            if (i < nf) {
+                const uint32_t off = fr(cur, 0)[i] * uint32_t(sizeof(Node4));
+                const uint32_t masks = *reinterpret_cast<const uint32_t*>(base + off + 112);   // leaf slots bits 0-3, inner 4-7
+                if (c == 0) ++steps;
+                if ((masks >> c) & 0x11u) {
+                    for (int k = 0; k < 6; ++k) b[k] = *reinterpret_cast<const float*>(base + off + 16 * k + 4 * c);
+                    ref = *reinterpret_cast<const int32_t*>(base + off + 96 + 4 * c);
+                    is_leaf = (masks >> c) & 1u;
+                    valid = true;
+                }
+            }
+            place(valid, b, ref, is_leaf);
+        }
+        wave_sync();
+        if (overflow()) {
+            // drop this level: its frontier (with the boxes its parents gave it) ends the cut
+            cv.n = n0;
+            cv.nleaf = l0;
+            if (!boxed || n0 + nf > CUT_N) { cv.n = -1; cv.nleaf = 0; return; }
+            if (lane < uint32_t(nf)) {
+                cv.id[n0 + lane] = fr(cur, 0)[lane];
+                for (int k = 0; k < 6; ++k) cv.box[k * CUT_N + n0 + lane] = reinterpret_cast<const float*>(fr(cur, k + 1))[lane];
+            }
+            cv.n = n0 + nf;
+            wave_sync();
+            return;
+        }
+        cur ^= 1;
+        nf = nn;
+        boxed = true;
+    }
+}
+
 struct TraceArgs4 {
     const float* pos;
     const float* nrm;
@@ -408,11 +631,16 @@ struct TraceArgs4 {
     TraceCounters* counters;
     uint32_t w, npix, spp, ntri, tri_q;
     float radius;
+    float cut_small;   // CUT: the wave's cut stops at boxes no wider than this
+    const TopEntry* top;   // CUT: the tree's top table and its entry count
+    const uint32_t* ntop;
 };
 
-template <bool SPP64, bool COUNT, bool NEAR = true>
-__global__ __launch_bounds__(BT) void k_trace_ao4(TraceArgs4 a) {
-    __shared__ uint32_t stk[STACK_LDS * BT];
+// CUT (64 spp only): the wave first walks the top of the tree together (wave_cut) and each
+// ray starts from the cut instead of the root. stk: STACK_LDS x BT words; cutmem: CUT_WAVE
+// words a wave.
+template <bool SPP64, bool COUNT, bool NEAR, bool CUT>
+__device__ __forceinline__ void trace_ao4(const TraceArgs4& a, uint32_t* stk, uint32_t* cutmem) {
     const uint64_t gid = uint64_t(blockIdx.x) * BT + threadIdx.x;
     const uint32_t pix = SPP64 ? uint32_t(gid >> 6) : uint32_t(gid / a.spp);
     if (pix >= a.npix) return;
@@ -427,6 +655,23 @@ __global__ __launch_bounds__(BT) void k_trace_ao4(TraceArgs4 a) {
     V3 t, b;
     basis(n, t, b);
     const V3 origin = add(p, scl(n, 1e-3f));
+    uint32_t visits = 0, tests = 0;
+    CutView cv;
+    if (CUT && a.ntri) {   // before the ray's own set-up, which need not stay live across the walk
+        // Every ray of the pixel stays inside origin +- radius on each axis; the margin covers
+        // the slab test's reach (radius x 1.0001 x 1.00001) and float rounding near the origin.
+        const float big = fmaxf(fabsf(origin.x), fmaxf(fabsf(origin.y), fabsf(origin.z)));
+        const float half = a.radius * 1.001f + (big + a.radius) * 1e-5f;
+        uint32_t* mine = cutmem + (threadIdx.x >> 6) * CUT_WAVE;
+        cv.id = mine;
+        cv.box = reinterpret_cast<float*>(mine + CUT_N);
+        cv.leaf = mine + 7 * CUT_N;
+        cv.lbox = reinterpret_cast<float*>(mine + 7 * CUT_N + CUT_LEAF);
+        uint32_t steps = 0;
+        wave_cut(a.nodes, a.top, *a.ntop, v3(origin.x - half, origin.y - half, origin.z - half),
+                 v3(origin.x + half, origin.y + half, origin.z + half), a.cut_small, stk + (threadIdx.x & ~63u), cv, steps);
+        if (COUNT) visits += steps;
+    }
     const uint32_t k1 = hash24(x, y, 2 * s), k2 = hash24(x, y, 2 * s + 1);
     const float u1 = float(k1) / float(0x1000000), u2 = float(k2) / float(0x1000000);
     float sphi, cphi;
@@ -434,10 +679,24 @@ __global__ __launch_bounds__(BT) void k_trace_ao4(TraceArgs4 a) {
     if (a.trig) trig_fix(a.trig, k2, sphi, cphi);
     const V3 dir = ao_dir(u1, sphi, cphi, n, t, b);
     bool occ = false;
-    uint32_t visits = 0, tests = 0;
     if (a.ntri) {
         const Ray r = make_ray(origin, dir);
-        const int res = occluded4<COUNT, NEAR>(r, a.radius, a.nodes, a.tris, a.ntri, stk + threadIdx.x, a.tri_q, visits, tests);
+        int res;
+        if (CUT) {
+            bool hit = false;
+            const float reach = a.radius * kReach;
+            for (int i = 0; i < cv.nleaf && !hit; ++i) {   // nleaf is 0 when the cut overflowed
+                float t;
+                if (!CutView::enters(r, cv.lbox, CUT_LEAF, i, reach, kExit)) continue;
+                if (COUNT) ++tests;
+                if (cv.leaf[i] < a.ntri && tri_hit(r, a.tris, cv.leaf[i], t) && t < a.radius) hit = true;
+            }
+            res = occluded4<COUNT, NEAR>(r, a.radius, a.nodes, a.tris, a.ntri, stk + threadIdx.x, a.tri_q, visits, tests,
+                                         !hit, &cv);
+            if (hit) res = 1;
+        } else {
+            res = occluded4<COUNT, NEAR>(r, a.radius, a.nodes, a.tris, a.ntri, stk + threadIdx.x, a.tri_q, visits, tests);
+        }
         if (res < 0) {
             occ = scan_occluded(r, a.radius, a.tris, a.ntri);
             atomicAdd(&a.counters->overflows, 1ull);
@@ -456,6 +715,19 @@ __global__ __launch_bounds__(BT) void k_trace_ao4(TraceArgs4 a) {
     } else if (!occ) {
         atomicAdd(&a.cnt[pix], 1u);
     }
+}
+
+template <bool SPP64, bool COUNT, bool NEAR = true>
+__global__ __launch_bounds__(BT) void k_trace_ao4(TraceArgs4 a) {
+    __shared__ uint32_t stk[STACK_LDS * BT];
+    trace_ao4<SPP64, COUNT, NEAR, false>(a, stk, nullptr);
+}
+// 64 spp from the wave's cut; held to 7 waves a SIMD (72 registers), which its LDS allows.
+template <bool COUNT>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(7, 7))) void k_trace_ao4c(TraceArgs4 a) {
+    __shared__ uint32_t stk[STACK_LDS * BT];
+    __shared__ uint32_t cutmem[CUT_WAVE * (BT / 64)];
+    trace_ao4<true, COUNT, true, true>(a, stk, cutmem);
 }
 
 // Nearest (ties to the lower triangle index) or any hit of general rays.
@@ -913,6 +1185,13 @@ __global__ __launch_bounds__(BT) void k_batch_finish(BatchDev d) {
 }
 }  // namespace
 
+// The top table of the wide tree in c.nodes4 (TOP_MAX entries, then the count).
+static int top_table(Ctx& c, hipStream_t s) {
+    if (!ensure(c, c.top, sizeof(TopEntry) * TOP_MAX + 16)) return -2;
+    k_top_table<<<1, 64, 0, s>>>(c.nodes4.as<Node4>(), c.top.as<TopEntry>(), c.top.as<uint32_t>() + TOP_MAX * (sizeof(TopEntry) / 4));
+    return hip_ok(c, hipGetLastError(), "wide build") ? 0 : -2;
+}
+
 int wide_build(Ctx& c, hipStream_t s) {
     const uint32_t n = c.ntri;
     if (n == 0) return 0;
@@ -923,7 +1202,7 @@ int wide_build(Ctx& c, hipStream_t s) {
     if (n == 1) {
         k_wide_single<<<1, 64, 0, s>>>(c.boxes.as<float>(), c.nodes4.as<Node4>());
         c.wide_nodes = 1;
-        return hip_ok(c, hipGetLastError(), "wide launch") ? 0 : -2;
+        return hip_ok(c, hipGetLastError(), "wide launch") ? top_table(c, s) : -2;
     }
     WideState init{1, 0, 1, 0};
     const uint32_t root = uint32_t(c.root);
@@ -956,7 +1235,7 @@ int wide_build(Ctx& c, hipStream_t s) {
         }
         if (level > 4 * 64) { c.err = "wide collapse did not finish"; return -2; }
     }
-    return hip_ok(c, hipGetLastError(), "wide build") ? 0 : -2;
+    return top_table(c, s);
 }
 
 namespace {
@@ -973,7 +1252,7 @@ int wide_build_fixed(Ctx& c, hipStream_t s) {
     if (n == 0) return 0;
     if (n == 1) {
         k_wide_single<<<1, 64, 0, s>>>(c.boxes.as<float>(), c.nodes4.as<Node4>());
-        return hip_ok(c, hipGetLastError(), "wide launch") ? 0 : -2;
+        return hip_ok(c, hipGetLastError(), "wide launch") ? top_table(c, s) : -2;
     }
     if (c.plan_wide_nb.empty()) { c.err = "no recorded wide collapse plan"; return -1; }
     WideState* d_st = c.wstate.as<WideState>();
@@ -988,7 +1267,7 @@ int wide_build_fixed(Ctx& c, hipStream_t s) {
         parity ^= 1;
         std::swap(fin, fout);
     }
-    return hip_ok(c, hipGetLastError(), "wide build") ? 0 : -2;
+    return top_table(c, s);
 }
 
 int trace_ao4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int spp, float radius,
@@ -998,7 +1277,8 @@ int trace_ao4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_m
     if (!hip_ok(c, hipMemsetAsync(c.counters.p, 0, sizeof(TraceCounters), s), "counter clear")) return -2;
     TraceArgs4 a{d_pos, d_nrm, d_mask, d_ao, nullptr, c.nodes4.as<Node4>(), c.tris_sorted.as<float4>(),
                  c.host_trig ? c.d_trig : nullptr, c.counters.as<TraceCounters>(), uint32_t(w), npix, uint32_t(spp),
-                 c.ntri, uint32_t(c.tri_q), radius};
+                 c.ntri, uint32_t(c.tri_q), radius, float(c.wave_cut) * radius, c.top.as<TopEntry>(),
+                 c.top.p ? c.top.as<uint32_t>() + TOP_MAX * (sizeof(TopEntry) / 4) : nullptr};
     const bool spp64 = spp == 64;
     if (!spp64) {
         if (!ensure(c, c.st_cnt, 4 * size_t(npix))) return -2;
@@ -1013,8 +1293,11 @@ int trace_ao4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_m
         k_trace_ao4r<PW><<<cdiv(npix, uint32_t(PW * (BT / 64))), BT, 0, s>>>(a, uint32_t(c.refill));
     } else if (spp64) {
         if (count) {
-            if (c.near_first) k_trace_ao4<true, true><<<blocks, BT, 0, s>>>(a);
+            if (c.wave_cut) k_trace_ao4c<true><<<blocks, BT, 0, s>>>(a);
+            else if (c.near_first) k_trace_ao4<true, true><<<blocks, BT, 0, s>>>(a);
             else k_trace_ao4<true, true, false><<<blocks, BT, 0, s>>>(a);
+        } else if (c.wave_cut) {
+            k_trace_ao4c<false><<<blocks, BT, 0, s>>>(a);
         } else if (c.near_first) {
             k_trace_ao4<true, false><<<blocks, BT, 0, s>>>(a);
         } else {

@@ -130,6 +130,11 @@ int raw_ao_set_option(raw_ao_ctx* x, int option, int value) {
         return RAW_AO_OK;
     }
     if (option == RAW_AO_OPT_NEAR_FIRST) { c.near_first = value != 0; return RAW_AO_OK; }
+    if (option == RAW_AO_OPT_WAVE_CUT) {
+        if (value < 0 || value > 64) return fail(c, RAW_AO_ERR_ARG, "wave cut must be in 0..64");
+        c.wave_cut = value;
+        return RAW_AO_OK;
+    }
     if (option == RAW_AO_OPT_REFILL) {
         if (value < 0 || value > 64) return fail(c, RAW_AO_ERR_ARG, "refill threshold must be in 0..64");
         c.refill = value;

@@ -329,6 +329,46 @@ def test_near_first_order_does_not_change_ao(ctx):
     assert np.array_equal(bits(outs[0]), bits(outs[1]))
 
 
+@pytest.mark.parametrize("k", [1, 2, 8, 64])
+def test_wave_cut_does_not_change_ao(ctx, k):
+    """Starting each pixel's rays from the wave's shared cut of the tree (cut at k x radius;
+    k = 1 overflows the cut lists on many pixels, which then start from the root) leaves every
+    pixel's AO bit for bit the same, counting launch included."""
+    print("wave cut", k, flush=True)
+    tris, pos, nrm, mask = big_frame(160, 90, seed=11)
+    ctx.build_ploc(tris, 16)
+    want = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
+    try:
+        ctx.set_option(raw_ao.OPT_WAVE_CUT, k)
+        got = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
+        ctx.count_next_trace()
+        counted = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
+        visits, _, rays = ctx.trace_counts()
+    finally:
+        ctx.set_option(raw_ao.OPT_WAVE_CUT, 0)
+    assert np.array_equal(bits(got), bits(want))
+    assert np.array_equal(bits(counted), bits(want))
+    assert rays == 64 * int(mask.sum()) and visits > 0
+
+
+def test_wave_cut_small_scenes(ctx):
+    """The cut on the 12-triangle test scene (radius 2, the tree's leaves reached in the wave's
+    walk) and on an empty tree."""
+    _, pos, nrm, mask = raw_ao.host.cli_gbuffer(96, 96)
+    vpos, _, idx = raw_ao.host.test_scene_mesh()
+    tris = raw_ao.host.mesh_tris(vpos, idx)
+    try:
+        ctx.set_option(raw_ao.OPT_WAVE_CUT, 4)
+        for t in (tris, tris[:1], np.zeros((0, 9), np.float32)):
+            ctx.build_ploc(t, 16)
+            ao = ctx.compute_rtao(pos, nrm, mask, 64, 2.0)
+            assert np.array_equal(bits(ao), bits(O.rtao(t, pos, nrm, mask, 64, 2.0)))
+    finally:
+        ctx.set_option(raw_ao.OPT_WAVE_CUT, 0)
+    with pytest.raises(raw_ao.RawAoError):
+        ctx.set_option(raw_ao.OPT_WAVE_CUT, 65)
+
+
 def test_timings_before_every_kernel_ran_leave_no_error():
     """raw_ao_last_ms before the SSAO and error-map kernels have ever run reports 0 for them,
     and the next launches do not see that query's error."""
