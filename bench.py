@@ -290,6 +290,15 @@ def batched(raw_ao, torch, steps):
 ARGS = None
 
 
+def trace_kernel_name(a):
+    """The kernel that traces the 64-spp frame under these options."""
+    if a.kernel == "binary":
+        return "k_trace_ao"
+    if a.refill > 0:
+        return "k_trace_ao4r"
+    return "k_trace_ao4" if a.wave_cut == 0 else "k_trace_ao4c"
+
+
 def main():
     global ARGS
     ap = argparse.ArgumentParser()
@@ -456,7 +465,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": hbm_traffic_per_launch(),
                      "traffic_source": os.path.relpath(HBM_COUNTERS, ROOT),
-                     "kernel": "k_trace_ao4" if a.kernel == "wide" else "k_trace_ao",
+                     "kernel": trace_kernel_name(a),
                      "algorithmic_bytes_per_launch": alg_bytes,
                      "node_visits": visits, "triangle_tests": tests},
     }

@@ -77,10 +77,11 @@ enum {
      * entry distance first (0: the first in slot order). Results are the same; only the speed
      * differs. */
     RAW_AO_OPT_NEAR_FIRST = 7,
-    /* 64-spp wide kernel, 0..64 (default 0): with n > 0 the 64 lanes of a pixel first walk the
+    /* 64-spp wide kernel, 0..64 (default 2): with n > 0 the 64 lanes of a pixel first walk the
      * top of the tree together, keeping the subtrees whose boxes meet the box the pixel's rays
-     * can reach and are no wider than n x radius, and every ray starts from those. Results are
-     * the same for every value; only the speed differs. */
+     * can reach and are no wider than n x radius, and every ray starts from those (n = 0: every
+     * ray starts from the root; RAW_AO_OPT_NEAR_FIRST applies to that traversal only). Results
+     * are the same for every value; only the speed differs. */
     RAW_AO_OPT_WAVE_CUT = 8
 };
 enum {

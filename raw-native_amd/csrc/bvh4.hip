@@ -452,17 +452,18 @@ __device__ __forceinline__ void wave_sync() {
 // The tree's slots down to depth TOP_DEPTH (the root's children are depth 1): every slot at that
 // depth and every leaf slot above it, with the box its parent holds, in breadth-first order.
 // They partition the triangles, so a walk may start from them instead of the root.
-constexpr int TOP_DEPTH = 4, TOP_MAX = 256;   // 4^TOP_DEPTH
+constexpr int TOP_DEPTH = 3, TOP_MAX = 64;   // 4^TOP_DEPTH
 struct alignas(16) TopEntry {
     float lx, ly, lz, hx, hy, hz;
     int32_t ref;   // >= 0 a node, < 0 ~ a leaf's sorted triangle position
     uint32_t pad;
 };
 // One wave; four lanes a node, one a slot. Level L holds at most 4^L nodes, so the next
-// frontier (at most 64 before the last level) and the table (at most TOP_MAX) never overflow.
+// frontier (at most TOP_MAX / 4 before the last level) and the table (at most TOP_MAX) never
+// overflow.
 __global__ __launch_bounds__(64) void k_top_table(const Node4* __restrict__ nodes, TopEntry* __restrict__ out,
                                                   uint32_t* __restrict__ count) {
-    __shared__ uint32_t ids[2][64];
+    __shared__ uint32_t ids[2][TOP_MAX / 4];
     const uint32_t lane = threadIdx.x;
     const unsigned long long below = (1ull << lane) - 1ull;
     const char* const base = reinterpret_cast<const char*>(nodes);
@@ -587,15 +588,15 @@ __device__ void wave_cut(const Node4* __restrict__ nodes, const TopEntry* __rest
             int32_t ref = 0;
             float b[6] = {0, 0, 0, 0, 0, 0};
             if (i < nf) {
+                // the slot's words load together with the masks (one round trip; an unused slot's
+                // words are read but not used)
                 const uint32_t off = fr(cur, 0)[i] * uint32_t(sizeof(Node4));
                 const uint32_t masks = *reinterpret_cast<const uint32_t*>(base + off + 112);   // leaf slots bits 0-3, inner 4-7
+                for (int k = 0; k < 6; ++k) b[k] = *reinterpret_cast<const float*>(base + off + 16 * k + 4 * c);
+                ref = *reinterpret_cast<const int32_t*>(base + off + 96 + 4 * c);
                 if (c == 0) ++steps;
-                if ((masks >> c) & 0x11u) {
-                    for (int k = 0; k < 6; ++k) b[k] = *reinterpret_cast<const float*>(base + off + 16 * k + 4 * c);
-                    ref = *reinterpret_cast<const int32_t*>(base + off + 96 + 4 * c);
-                    is_leaf = (masks >> c) & 1u;
-                    valid = true;
-                }
+                is_leaf = (masks >> c) & 1u;
+                valid = (masks >> c) & 0x11u;
             }
             place(valid, b, ref, is_leaf);
         }

@@ -44,7 +44,7 @@ struct Ctx {
     int tri_q{16};               // wide kernel: triangle phase when tri_q/64 of live lanes wait
     int ploc_tail{4096};         // the single-workgroup PLOC tail takes over at this many clusters
     int refill{0};               // 64-spp wide kernel: refill lanes once this many are idle (0: off)
-    int wave_cut{0};             // 64-spp wide kernel: > 0 starts rays from the wave's cut at wave_cut x radius
+    int wave_cut{2};             // 64-spp wide kernel: > 0 starts rays from the wave's cut at wave_cut x radius
     bool near_first{true};       // 64-spp wide kernel: descend into the nearest hit child first
     bool count_next{false};
 
@@ -70,7 +70,7 @@ struct Ctx {
     DevBuf tris_sorted;          // 48 B each, sorted (leaf) order
     uint32_t trav_root{0};
     DevBuf nodes4;               // 4-wide records, 128 B each, root at 0
-    DevBuf top;                  // the wide tree's top table (slots to depth 4) and its entry count
+    DevBuf top;                  // the wide tree's top table (slots to depth 3) and its entry count
     DevBuf wfront_a, wfront_b, wblk, wstate;
     uint32_t wide_nodes{0};
 

@@ -321,31 +321,38 @@ def test_near_first_order_does_not_change_ao(ctx):
     ctx.build_ploc(tris, 16)
     outs = []
     try:
+        ctx.set_option(raw_ao.OPT_WAVE_CUT, 0)   # the child order applies to the traversal from the root
         for nf in (1, 0):
             ctx.set_option(raw_ao.OPT_NEAR_FIRST, nf)
             outs.append(ctx.compute_rtao(pos, nrm, mask, 64, 1.0))
     finally:
         ctx.set_option(raw_ao.OPT_NEAR_FIRST, 1)
+        ctx.set_option(raw_ao.OPT_WAVE_CUT, WAVE_CUT_DEFAULT)
     assert np.array_equal(bits(outs[0]), bits(outs[1]))
+
+
+WAVE_CUT_DEFAULT = 2
 
 
 @pytest.mark.parametrize("k", [1, 2, 8, 64])
 def test_wave_cut_does_not_change_ao(ctx, k):
     """Starting each pixel's rays from the wave's shared cut of the tree (cut at k x radius;
-    k = 1 overflows the cut lists on many pixels, which then start from the root) leaves every
-    pixel's AO bit for bit the same, counting launch included."""
+    k = 1 fills the cut lists on many pixels, which then end the cut at the frontier of the
+    level that would overflow) leaves every pixel's AO bit for bit the same as the traversal
+    from the root, counting launch included."""
     print("wave cut", k, flush=True)
     tris, pos, nrm, mask = big_frame(160, 90, seed=11)
     ctx.build_ploc(tris, 16)
-    want = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
     try:
+        ctx.set_option(raw_ao.OPT_WAVE_CUT, 0)
+        want = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
         ctx.set_option(raw_ao.OPT_WAVE_CUT, k)
         got = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
         ctx.count_next_trace()
         counted = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
         visits, _, rays = ctx.trace_counts()
     finally:
-        ctx.set_option(raw_ao.OPT_WAVE_CUT, 0)
+        ctx.set_option(raw_ao.OPT_WAVE_CUT, WAVE_CUT_DEFAULT)
     assert np.array_equal(bits(got), bits(want))
     assert np.array_equal(bits(counted), bits(want))
     assert rays == 64 * int(mask.sum()) and visits > 0
@@ -364,7 +371,7 @@ def test_wave_cut_small_scenes(ctx):
             ao = ctx.compute_rtao(pos, nrm, mask, 64, 2.0)
             assert np.array_equal(bits(ao), bits(O.rtao(t, pos, nrm, mask, 64, 2.0)))
     finally:
-        ctx.set_option(raw_ao.OPT_WAVE_CUT, 0)
+        ctx.set_option(raw_ao.OPT_WAVE_CUT, WAVE_CUT_DEFAULT)
     with pytest.raises(raw_ao.RawAoError):
         ctx.set_option(raw_ao.OPT_WAVE_CUT, 65)
 
