@@ -723,12 +723,15 @@ __global__ __launch_bounds__(BT) void k_trace_ao4(TraceArgs4 a) {
     __shared__ uint32_t stk[STACK_LDS * BT];
     trace_ao4<SPP64, COUNT, NEAR, false>(a, stk, nullptr);
 }
-// 64 spp from the wave's cut; held to 7 waves a SIMD (72 registers), which its LDS allows.
+// 64 spp from the wave's cut, children in slot order (measured faster here than nearest first,
+// whose sort costs more than it saves on the cut's short subtrees); held to 7 waves a SIMD
+// (72 registers), which its LDS allows. Eight (64 registers, a smaller stack or cut) measured
+// slower.
 template <bool COUNT>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(7, 7))) void k_trace_ao4c(TraceArgs4 a) {
     __shared__ uint32_t stk[STACK_LDS * BT];
     __shared__ uint32_t cutmem[CUT_WAVE * (BT / 64)];
-    trace_ao4<true, COUNT, true, true>(a, stk, cutmem);
+    trace_ao4<true, COUNT, false, true>(a, stk, cutmem);
 }
 
 // Nearest (ties to the lower triangle index) or any hit of general rays.
