@@ -633,23 +633,40 @@ struct TraceArgs4 {
     uint32_t w, npix, spp, ntri, tri_q;
     float radius;
     float cut_small;   // CUT: the wave's cut stops at boxes no wider than this
+    float cut_span;    // CUT below 64 spp: no walk when the wave's union box is wider than this
     const TopEntry* top;   // CUT: the tree's top table and its entry count
     const uint32_t* ntop;
 };
 
-// CUT (64 spp only): the wave first walks the top of the tree together (wave_cut) and each
-// ray starts from the cut instead of the root. stk: STACK_LDS x BT words; cutmem: CUT_WAVE
-// words a wave.
+__device__ __forceinline__ float wave_min(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// CUT: the wave first walks the top of the tree together (wave_cut) and each ray starts from
+// the cut instead of the root. With 64 spp the wave is one pixel; otherwise its lanes span
+// several pixels, all lanes stay for the walk, and the walk's box is the union of the live
+// lanes' reach boxes (no walk when that union is wider than a.cut_span). stk: STACK_LDS x BT
+// words; cutmem: CUT_WAVE words a wave.
 template <bool SPP64, bool COUNT, bool NEAR, bool CUT>
 __device__ __forceinline__ void trace_ao4(const TraceArgs4& a, uint32_t* stk, uint32_t* cutmem) {
+    constexpr bool STAY = CUT && !SPP64;
     const uint64_t gid = uint64_t(blockIdx.x) * BT + threadIdx.x;
-    const uint32_t pix = SPP64 ? uint32_t(gid >> 6) : uint32_t(gid / a.spp);
-    if (pix >= a.npix) return;
-    const uint32_t s = uint32_t(gid - uint64_t(pix) * a.spp);
-    if (!a.mask[pix]) {
+    const uint32_t pix0 = SPP64 ? uint32_t(gid >> 6) : uint32_t(gid / a.spp);
+    bool live = pix0 < a.npix;
+    if (!STAY && !live) return;
+    const uint32_t pix = live ? pix0 : 0u;   // a lane past the end reads pixel 0 and is not live
+    const uint32_t s = uint32_t(gid - uint64_t(pix0) * a.spp);
+    if (live && !a.mask[pix]) {
         if (s == 0) a.ao[pix] = 1.0f;
-        return;
+        live = false;
     }
+    if (!STAY && !live) return;
+    if (STAY && __ballot(live) == 0ull) return;   // the same in every lane of the wave
     const V3 p = v3(a.pos[3 * size_t(pix)], a.pos[3 * size_t(pix) + 1], a.pos[3 * size_t(pix) + 2]);
     const V3 n = v3(a.nrm[3 * size_t(pix)], a.nrm[3 * size_t(pix) + 1], a.nrm[3 * size_t(pix) + 2]);
     const uint32_t y = pix / a.w, x = pix - y * a.w;
@@ -663,15 +680,30 @@ __device__ __forceinline__ void trace_ao4(const TraceArgs4& a, uint32_t* stk, ui
         // the slab test's reach (radius x 1.0001 x 1.00001) and float rounding near the origin.
         const float big = fmaxf(fabsf(origin.x), fmaxf(fabsf(origin.y), fabsf(origin.z)));
         const float half = a.radius * 1.001f + (big + a.radius) * 1e-5f;
+        V3 lo = v3(origin.x - half, origin.y - half, origin.z - half);
+        V3 hi = v3(origin.x + half, origin.y + half, origin.z + half);
+        bool walk = true;
+        if (!SPP64) {
+            if (!live) {
+                lo = v3(kFar, kFar, kFar);
+                hi = v3(-kFar, -kFar, -kFar);
+            }
+            lo = v3(wave_min(lo.x), wave_min(lo.y), wave_min(lo.z));
+            hi = v3(wave_max(hi.x), wave_max(hi.y), wave_max(hi.z));
+            walk = hi.x - lo.x <= a.cut_span && hi.y - lo.y <= a.cut_span && hi.z - lo.z <= a.cut_span;
+        }
         uint32_t* mine = cutmem + (threadIdx.x >> 6) * CUT_WAVE;
         cv.id = mine;
         cv.box = reinterpret_cast<float*>(mine + CUT_N);
         cv.leaf = mine + 7 * CUT_N;
         cv.lbox = reinterpret_cast<float*>(mine + 7 * CUT_N + CUT_LEAF);
-        uint32_t steps = 0;
-        wave_cut(a.nodes, a.top, *a.ntop, v3(origin.x - half, origin.y - half, origin.z - half),
-                 v3(origin.x + half, origin.y + half, origin.z + half), a.cut_small, stk + (threadIdx.x & ~63u), cv, steps);
-        if (COUNT) visits += steps;
+        cv.n = -1;   // no walk: every ray starts from the root
+        cv.nleaf = 0;
+        if (walk) {
+            uint32_t steps = 0;
+            wave_cut(a.nodes, a.top, *a.ntop, lo, hi, a.cut_small, stk + (threadIdx.x & ~63u), cv, steps);
+            if (COUNT && live) visits += steps;
+        }
     }
     const uint32_t k1 = hash24(x, y, 2 * s), k2 = hash24(x, y, 2 * s + 1);
     const float u1 = float(k1) / float(0x1000000), u2 = float(k2) / float(0x1000000);
@@ -686,14 +718,14 @@ __device__ __forceinline__ void trace_ao4(const TraceArgs4& a, uint32_t* stk, ui
         if (CUT) {
             bool hit = false;
             const float reach = a.radius * kReach;
-            for (int i = 0; i < cv.nleaf && !hit; ++i) {   // nleaf is 0 when the cut overflowed
+            for (int i = 0; i < cv.nleaf && !hit && live; ++i) {   // nleaf is 0 when the cut overflowed
                 float t;
                 if (!CutView::enters(r, cv.lbox, CUT_LEAF, i, reach, kExit)) continue;
                 if (COUNT) ++tests;
                 if (cv.leaf[i] < a.ntri && tri_hit(r, a.tris, cv.leaf[i], t) && t < a.radius) hit = true;
             }
             res = occluded4<COUNT, NEAR>(r, a.radius, a.nodes, a.tris, a.ntri, stk + threadIdx.x, a.tri_q, visits, tests,
-                                         !hit, &cv);
+                                         live && !hit, &cv);
             if (hit) res = 1;
         } else {
             res = occluded4<COUNT, NEAR>(r, a.radius, a.nodes, a.tris, a.ntri, stk + threadIdx.x, a.tri_q, visits, tests);
@@ -705,6 +737,7 @@ __device__ __forceinline__ void trace_ao4(const TraceArgs4& a, uint32_t* stk, ui
             occ = res == 1;
         }
     }
+    if (STAY && !live) return;   // past the wave's collective steps
     if (COUNT) {
         atomicAdd(&a.counters->visits, (unsigned long long)visits);
         atomicAdd(&a.counters->tests, (unsigned long long)tests);
@@ -723,15 +756,15 @@ __global__ __launch_bounds__(BT) void k_trace_ao4(TraceArgs4 a) {
     __shared__ uint32_t stk[STACK_LDS * BT];
     trace_ao4<SPP64, COUNT, NEAR, false>(a, stk, nullptr);
 }
-// 64 spp from the wave's cut, children in slot order (measured faster here than nearest first,
+// From the wave's cut, children in slot order (measured faster here than nearest first,
 // whose sort costs more than it saves on the cut's short subtrees); held to 7 waves a SIMD
 // (72 registers), which its LDS allows. Eight (64 registers, a smaller stack or cut) measured
 // slower.
-template <bool COUNT>
+template <bool SPP64, bool COUNT>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(7, 7))) void k_trace_ao4c(TraceArgs4 a) {
     __shared__ uint32_t stk[STACK_LDS * BT];
     __shared__ uint32_t cutmem[CUT_WAVE * (BT / 64)];
-    trace_ao4<true, COUNT, false, true>(a, stk, cutmem);
+    trace_ao4<SPP64, COUNT, false, true>(a, stk, cutmem);
 }
 
 // Nearest (ties to the lower triangle index) or any hit of general rays.
@@ -1281,7 +1314,7 @@ int trace_ao4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_m
     if (!hip_ok(c, hipMemsetAsync(c.counters.p, 0, sizeof(TraceCounters), s), "counter clear")) return -2;
     TraceArgs4 a{d_pos, d_nrm, d_mask, d_ao, nullptr, c.nodes4.as<Node4>(), c.tris_sorted.as<float4>(),
                  c.host_trig ? c.d_trig : nullptr, c.counters.as<TraceCounters>(), uint32_t(w), npix, uint32_t(spp),
-                 c.ntri, uint32_t(c.tri_q), radius, float(c.wave_cut) * radius, c.top.as<TopEntry>(),
+                 c.ntri, uint32_t(c.tri_q), radius, float(c.wave_cut) * radius, 4.0f * radius, c.top.as<TopEntry>(),
                  c.top.p ? c.top.as<uint32_t>() + TOP_MAX * (sizeof(TopEntry) / 4) : nullptr};
     const bool spp64 = spp == 64;
     if (!spp64) {
@@ -1297,19 +1330,25 @@ int trace_ao4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_m
         k_trace_ao4r<PW><<<cdiv(npix, uint32_t(PW * (BT / 64))), BT, 0, s>>>(a, uint32_t(c.refill));
     } else if (spp64) {
         if (count) {
-            if (c.wave_cut) k_trace_ao4c<true><<<blocks, BT, 0, s>>>(a);
+            if (c.wave_cut) k_trace_ao4c<true, true><<<blocks, BT, 0, s>>>(a);
             else if (c.near_first) k_trace_ao4<true, true><<<blocks, BT, 0, s>>>(a);
             else k_trace_ao4<true, true, false><<<blocks, BT, 0, s>>>(a);
         } else if (c.wave_cut) {
-            k_trace_ao4c<false><<<blocks, BT, 0, s>>>(a);
+            k_trace_ao4c<true, false><<<blocks, BT, 0, s>>>(a);
         } else if (c.near_first) {
             k_trace_ao4<true, false><<<blocks, BT, 0, s>>>(a);
         } else {
             k_trace_ao4<true, false, false><<<blocks, BT, 0, s>>>(a);
         }
     } else {
-        if (count) k_trace_ao4<false, true><<<blocks, BT, 0, s>>>(a);
-        else k_trace_ao4<false, false><<<blocks, BT, 0, s>>>(a);
+        if (c.wave_cut) {
+            if (count) k_trace_ao4c<false, true><<<blocks, BT, 0, s>>>(a);
+            else k_trace_ao4c<false, false><<<blocks, BT, 0, s>>>(a);
+        } else if (count) {
+            k_trace_ao4<false, true><<<blocks, BT, 0, s>>>(a);
+        } else {
+            k_trace_ao4<false, false><<<blocks, BT, 0, s>>>(a);
+        }
     }
     (void)hipEventRecord(c.ev[3], s);
     if (!spp64) ao_finish(c, d_mask, a.cnt, npix, uint32_t(spp), d_ao, s);

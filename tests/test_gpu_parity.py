@@ -334,28 +334,29 @@ def test_near_first_order_does_not_change_ao(ctx):
 WAVE_CUT_DEFAULT = 2
 
 
-@pytest.mark.parametrize("k", [1, 2, 8, 64])
-def test_wave_cut_does_not_change_ao(ctx, k):
+@pytest.mark.parametrize("k,spp", [(1, 64), (2, 64), (8, 64), (64, 64), (2, 4), (2, 33), (8, 1)])
+def test_wave_cut_does_not_change_ao(ctx, k, spp):
     """Starting each pixel's rays from the wave's shared cut of the tree (cut at k x radius;
     k = 1 fills the cut lists on many pixels, which then end the cut at the frontier of the
-    level that would overflow) leaves every pixel's AO bit for bit the same as the traversal
-    from the root, counting launch included."""
-    print("wave cut", k, flush=True)
+    level that would overflow; below 64 spp a wave spans several pixels and walks for their
+    union) leaves every pixel's AO bit for bit the same as the traversal from the root,
+    counting launch included."""
+    print("wave cut", k, spp, flush=True)
     tris, pos, nrm, mask = big_frame(160, 90, seed=11)
     ctx.build_ploc(tris, 16)
     try:
         ctx.set_option(raw_ao.OPT_WAVE_CUT, 0)
-        want = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
+        want = ctx.compute_rtao(pos, nrm, mask, spp, 1.0)
         ctx.set_option(raw_ao.OPT_WAVE_CUT, k)
-        got = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
+        got = ctx.compute_rtao(pos, nrm, mask, spp, 1.0)
         ctx.count_next_trace()
-        counted = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
+        counted = ctx.compute_rtao(pos, nrm, mask, spp, 1.0)
         visits, _, rays = ctx.trace_counts()
     finally:
         ctx.set_option(raw_ao.OPT_WAVE_CUT, WAVE_CUT_DEFAULT)
     assert np.array_equal(bits(got), bits(want))
     assert np.array_equal(bits(counted), bits(want))
-    assert rays == 64 * int(mask.sum()) and visits > 0
+    assert rays == spp * int(mask.sum()) and visits > 0
 
 
 def test_wave_cut_small_scenes(ctx):
