@@ -219,7 +219,7 @@ def denoise_4k(ctx, raw_ao, torch, steps, stream):
             "rmse_vs_rt256": {k: v[0] for k, v in q.items()}, "max_error_vs_rt256": {k: v[1] for k, v in q.items()}}
 
 
-def batched(raw_ao, torch, steps):
+def batched(raw_ao, torch, steps, wave_cut=-1):
     """The batched uneven workload: 8 seeded scenes of 50k-2M triangles (the 1M-triangle
     scene's generator) and 64 views, sides log-uniform in [256, 2048], 16-128 rays a pixel
     (log-uniform), radius 1.0, SSAO 24 and the reconcile each; one captured graph of the 8
@@ -232,6 +232,8 @@ def batched(raw_ao, torch, steps):
     base = raw_ao.host.SCENE_1M
     tris_per_object = 0.385 * 1280 + 0.615 * 12
     b = raw_ao.Batch(0)
+    if wave_cut >= 0:
+        b.set_option(raw_ao.OPT_WAVE_CUT, wave_cut)
     meshes, ntris = [], []
     for k in range(8):
         target = float(np.exp(rng.uniform(np.log(5e4), np.log(2e6))))
@@ -475,7 +477,7 @@ def main():
     if not a.no_denoise:
         out["denoise_4k"] = denoise_4k(ctx, raw_ao, torch, a.steps, stream)
     if not a.no_batch:
-        out["batched"] = batched(raw_ao, torch, max(2, a.steps // 3))
+        out["batched"] = batched(raw_ao, torch, max(2, a.steps // 3), a.wave_cut)
     if a.calib_copy:
         n = a.calib_copy * (1 << 20) // 4
         x = torch.empty(n, dtype=torch.float32, device=dev).fill_(1.0)

@@ -317,6 +317,9 @@ int raw_ao_batch_add_view(raw_ao_batch* batch, int scene, const float* d_pos, co
  * on them. */
 int raw_ao_batch_set_tuning(raw_ao_batch* batch, int chunk, int tri_q, int blocks_per_cu);
 int raw_ao_batch_get_tuning(raw_ao_batch* batch, int* chunk, int* tri_q, int* blocks_per_cu);
+/* RAW_AO_OPT_HOST_LIBM_TRIG or RAW_AO_OPT_WAVE_CUT for every view of the batch (as
+ * raw_ao_set_option; drops a captured graph). Results never depend on the wave cut. */
+int raw_ao_batch_set_option(raw_ao_batch* batch, int option, int value);
 /* Enqueues one batched frame (rebuild != 0: first rebuild every scene from its triangles
  * with the recorded grids) on stream (NULL: the batch's own). */
 int raw_ao_batch_run(raw_ao_batch* batch, int rebuild, void* stream);

@@ -126,7 +126,8 @@ int prepare(Batch& b) {
     std::vector<BatchScene> sc;
     for (auto* s : b.scenes) {
         const Ctx& x = s->c;
-        sc.push_back(BatchScene{x.nodes4.p, x.tris_sorted.as<float4>(), x.ntri, 0});
+        const uint32_t* ntop = x.top.p ? x.top.as<uint32_t>() + top_count_offset() : nullptr;
+        sc.push_back(BatchScene{x.nodes4.p, x.tris_sorted.as<float4>(), x.top.p, ntop, x.ntri, 0});
     }
     const size_t nv = b.views.size();
     if (!ensure(c, b.d_views, sizeof(BatchView) * nv) || !ensure(c, b.d_scenes, sizeof(BatchScene) * std::max<size_t>(1, sc.size())) ||
@@ -146,7 +147,8 @@ BatchDev dev_of(Batch& b) {
     Ctx& c = C(b);
     return BatchDev{b.d_views.as<BatchView>(), b.d_scenes.as<BatchScene>(), uint32_t(b.views.size()), b.total_pix,
                     b.cov.as<uint32_t>(), b.cnt.as<uint32_t>(), b.ncov.as<uint32_t>(), b.gbase.as<uint32_t>(),
-                    b.head.as<uint32_t>(), c.host_trig ? c.d_trig : nullptr, b.tune.tri_q, b.tune.chunk};
+                    b.head.as<uint32_t>(), c.host_trig ? c.d_trig : nullptr, b.tune.tri_q, b.tune.chunk,
+                    float(c.wave_cut)};
 }
 
 // Everything of one batched frame on stream s, with no host copies, waits or memset nodes.
@@ -329,6 +331,15 @@ int raw_ao_batch_set_tuning(raw_ao_batch* x, int chunk, int tri_q, int blocks_pe
     return RAW_AO_OK;
 }
 
+int raw_ao_batch_set_option(raw_ao_batch* x, int option, int value) {
+    if (!x || !x->b.self) return RAW_AO_ERR_ARG;
+    if (option != RAW_AO_OPT_HOST_LIBM_TRIG && option != RAW_AO_OPT_WAVE_CUT)
+        return bfail(x->b, RAW_AO_ERR_ARG, "a batch takes RAW_AO_OPT_HOST_LIBM_TRIG and RAW_AO_OPT_WAVE_CUT only");
+    const int rc = raw_ao_set_option(x->b.self, option, value);
+    if (rc == RAW_AO_OK) raw_ao::drop_graph(x->b);
+    return rc;
+}
+
 int raw_ao_batch_get_tuning(raw_ao_batch* x, int* chunk, int* tri_q, int* blocks_per_cu) {
     if (!x || !x->b.self) return RAW_AO_ERR_ARG;
     if (chunk) *chunk = int(x->b.tune.chunk);
@@ -430,11 +441,11 @@ int raw_ao_batch_tune(raw_ao_batch* x, const char* cache_path, float* best_ms) {
     raw_ao::Ctx& c = b.self->c;
     int r = raw_ao::prepare(b);
     if (r) return raw_ao::code_of(r);
-    // the workload's signature: device, CUs, views (size, spp) and scenes (triangles)
+    // the workload's signature: device, CUs, the wave cut, views (size, spp) and scenes (triangles)
     hipDeviceProp_t prop;
     if (!hip_ok(c, hipGetDeviceProperties(&prop, c.device), "device properties")) return RAW_AO_ERR_DEVICE;
     std::ostringstream sig;
-    sig << prop.gcnArchName << ":" << b.cus;
+    sig << prop.gcnArchName << ":" << b.cus << ":c" << c.wave_cut;
     for (const auto& v : b.views) sig << ":" << v.w << "x" << v.h << "x" << v.spp << "s" << v.scene;
     for (auto* s : b.scenes) sig << ":t" << s->c.ntri;
     const std::string key = sig.str();

@@ -8,6 +8,8 @@ namespace raw_ao {
 struct BatchScene {
     const void* nodes;      // the scene's 4-wide records
     const float4* tris;     // its sorted triangles
+    const void* top;        // its top table (slots to depth 3)
+    const uint32_t* ntop;   // the table's entry count
     uint32_t ntri, pad;
 };
 
@@ -39,6 +41,7 @@ struct BatchDev {
     const uint8_t* trig;
     uint32_t tri_q;         // triangle phase threshold (of 64)
     uint32_t chunk;         // groups a wave takes from the queue at a time
+    float cut_k;            // > 0: a group's rays start from the wave's cut at cut_k x radius
 };
 
 // The tunables of a batched launch.

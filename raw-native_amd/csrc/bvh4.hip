@@ -692,7 +692,7 @@ __device__ __forceinline__ void trace_ao4(const TraceArgs4& a, uint32_t* stk, ui
             hi = v3(wave_max(hi.x), wave_max(hi.y), wave_max(hi.z));
             walk = hi.x - lo.x <= a.cut_span && hi.y - lo.y <= a.cut_span && hi.z - lo.z <= a.cut_span;
         }
-        uint32_t* mine = cutmem + (threadIdx.x >> 6) * CUT_WAVE;
+        uint32_t* mine = cutmem + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * CUT_WAVE;
         cv.id = mine;
         cv.box = reinterpret_cast<float*>(mine + CUT_N);
         cv.leaf = mine + 7 * CUT_N;
@@ -1156,14 +1156,29 @@ __global__ void k_batch_plan(BatchDev d) {
 // Persistent: each wave takes `chunk` groups from the queue until it is empty. A group is
 // 64 consecutive rays of one view (pixel-major, the samples of a pixel adjacent), one ray
 // a lane; a pixel's open rays are summed in the wave and added once.
+// CUT: each 64-ray group walks the tree once for the union of its live lanes' reach boxes
+// (as trace_ao4 below 64 spp; no walk when that union is wider than 4 x radius) and its rays
+// start from the cut.
+template <bool CUT>
 __global__ __launch_bounds__(BT) void k_batch_trace(BatchDev d) {
     __shared__ uint32_t stk[STACK_LDS * BT];
+    __shared__ uint32_t cutmem[CUT ? CUT_WAVE * (BT / 64) : 1];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t total = d.gbase[d.nviews];
+    CutView cv;
+    if (CUT) {
+        uint32_t* mine = cutmem + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * CUT_WAVE;
+        cv.id = mine;
+        cv.box = reinterpret_cast<float*>(mine + CUT_N);
+        cv.leaf = mine + 7 * CUT_N;
+        cv.lbox = reinterpret_cast<float*>(mine + 7 * CUT_N + CUT_LEAF);
+    }
     for (;;) {
         uint32_t g0 = 0;
         if (lane == 0) g0 = atomicAdd(d.head, d.chunk);
-        g0 = __shfl(g0, 0);
+        // the group index, and with it the view and scene, is the same in every lane: kept
+        // in scalar registers
+        g0 = __builtin_amdgcn_readfirstlane(__shfl(g0, 0));
         if (g0 >= total) break;
         const uint32_t g1 = min(total, g0 + d.chunk);
         for (uint32_t g = g0; g < g1; ++g) {
@@ -1173,6 +1188,7 @@ __global__ __launch_bounds__(BT) void k_batch_trace(BatchDev d) {
                 if (d.gbase[mid] <= g) lo = mid;
                 else hi = mid;
             }
+            lo = __builtin_amdgcn_readfirstlane(lo);
             const BatchView& bv = d.views[lo];
             const BatchScene& sc = d.scenes[bv.scene];
             const uint64_t rr = uint64_t(g - d.gbase[lo]) * 64u + lane;
@@ -1184,10 +1200,27 @@ __global__ __launch_bounds__(BT) void k_batch_trace(BatchDev d) {
             if (sc.ntri) {
                 const V3 p = v3(bv.pos[3 * size_t(pix)], bv.pos[3 * size_t(pix) + 1], bv.pos[3 * size_t(pix) + 2]);
                 const V3 n = v3(bv.nrm[3 * size_t(pix)], bv.nrm[3 * size_t(pix) + 1], bv.nrm[3 * size_t(pix) + 2]);
+                const V3 origin = add(p, scl(n, 1e-3f));
+                uint32_t visits = 0, tests = 0;
+                if (CUT) {
+                    const float big = fmaxf(fabsf(origin.x), fmaxf(fabsf(origin.y), fabsf(origin.z)));
+                    const float half = bv.radius * 1.001f + (big + bv.radius) * 1e-5f;
+                    const V3 blo = active ? v3(origin.x - half, origin.y - half, origin.z - half) : v3(kFar, kFar, kFar);
+                    const V3 bhi = active ? v3(origin.x + half, origin.y + half, origin.z + half) : v3(-kFar, -kFar, -kFar);
+                    const V3 ulo = v3(wave_min(blo.x), wave_min(blo.y), wave_min(blo.z));
+                    const V3 uhi = v3(wave_max(bhi.x), wave_max(bhi.y), wave_max(bhi.z));
+                    const float span = 4.0f * bv.radius;
+                    cv.n = -1;
+                    cv.nleaf = 0;
+                    if (uhi.x - ulo.x <= span && uhi.y - ulo.y <= span && uhi.z - ulo.z <= span) {
+                        uint32_t steps = 0;
+                        wave_cut(static_cast<const Node4*>(sc.nodes), static_cast<const TopEntry*>(sc.top), *sc.ntop, ulo, uhi,
+                                 d.cut_k * bv.radius, stk + (threadIdx.x & ~63u), cv, steps);
+                    }
+                }
                 const uint32_t y = pix / bv.w, x = pix - y * bv.w;
                 V3 t, b;
                 basis(n, t, b);
-                const V3 origin = add(p, scl(n, 1e-3f));
                 const uint32_t k1 = hash24(x, y, 2 * s), k2 = hash24(x, y, 2 * s + 1);
                 const float u1 = float(k1) / float(0x1000000), u2 = float(k2) / float(0x1000000);
                 float sphi, cphi;
@@ -1195,10 +1228,19 @@ __global__ __launch_bounds__(BT) void k_batch_trace(BatchDev d) {
                 if (d.trig) trig_fix(d.trig, k2, sphi, cphi);
                 const V3 dir = ao_dir(u1, sphi, cphi, n, t, b);
                 const Ray r = make_ray(origin, dir);
-                uint32_t visits = 0, tests = 0;
-                const int res = occluded4<false>(r, bv.radius, static_cast<const Node4*>(sc.nodes), sc.tris, sc.ntri,
-                                                 stk + threadIdx.x, d.tri_q, visits, tests, active);
-                occ = res < 0 ? scan_occluded(r, bv.radius, sc.tris, sc.ntri) : res == 1;
+                bool hit = false;
+                if (CUT) {
+                    const float reach = bv.radius * kReach;
+                    for (int i = 0; i < cv.nleaf && !hit && active; ++i) {
+                        float th;
+                        if (!CutView::enters(r, cv.lbox, CUT_LEAF, i, reach, kExit)) continue;
+                        if (cv.leaf[i] < sc.ntri && tri_hit(r, sc.tris, cv.leaf[i], th) && th < bv.radius) hit = true;
+                    }
+                }
+                const int res = occluded4<false, !CUT>(r, bv.radius, static_cast<const Node4*>(sc.nodes), sc.tris, sc.ntri,
+                                                       stk + threadIdx.x, d.tri_q, visits, tests, active && !hit,
+                                                       CUT ? &cv : nullptr);
+                occ = hit || (res < 0 ? scan_occluded(r, bv.radius, sc.tris, sc.ntri) : res == 1);
             }
             // segments of equal ci: a lane starts one where ci changes
             const uint32_t prev = __shfl_up(ci, 1);
@@ -1221,6 +1263,8 @@ __global__ __launch_bounds__(BT) void k_batch_finish(BatchDev d) {
         bv.ao[i] = bv.mask[i] ? float(d.cnt[bv.pix_base + i]) / float(bv.spp) : 1.0f;
 }
 }  // namespace
+
+size_t top_count_offset() { return TOP_MAX * (sizeof(TopEntry) / 4); }
 
 // The top table of the wide tree in c.nodes4 (TOP_MAX entries, then the count).
 static int top_table(Ctx& c, hipStream_t s) {
@@ -1375,7 +1419,8 @@ int batch_trace_enqueue(const BatchDev& d, uint32_t persistent_blocks, hipStream
     k_batch_clear<<<1024, BT, 0, s>>>(d);
     k_batch_compact<<<dim3(256, d.nviews, 1), BT, 0, s>>>(d);
     k_batch_plan<<<1, 64, 0, s>>>(d);
-    k_batch_trace<<<persistent_blocks, BT, 0, s>>>(d);
+    if (d.cut_k > 0.0f) k_batch_trace<true><<<persistent_blocks, BT, 0, s>>>(d);
+    else k_batch_trace<false><<<persistent_blocks, BT, 0, s>>>(d);
     k_batch_finish<<<dim3(256, d.nviews, 1), BT, 0, s>>>(d);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }

@@ -116,6 +116,8 @@ int trace_ao(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_ma
 int angle_table(Ctx& c, float* d_sin, float* d_cos, hipStream_t s);
 int ao_finish(Ctx& c, const uint8_t* d_mask, const uint32_t* d_cnt, uint32_t npix, uint32_t spp, float* d_ao, hipStream_t s);
 int wide_build(Ctx& c, hipStream_t s);
+// The word offset of the entry count in Ctx::top.
+size_t top_count_offset();
 int ploc_build_fixed(Ctx& c, const float* d_tris, hipStream_t s);
 int ploc_fixed_check(Ctx& c);
 int wide_build_fixed(Ctx& c, hipStream_t s);

@@ -82,6 +82,7 @@ SIGNATURES = [
     ("raw_ao_batch_add_view", C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int,
                                         C.c_float, _vp, _vp, _vp, C.POINTER(C.c_int)]),
     ("raw_ao_batch_set_tuning", C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    ("raw_ao_batch_set_option", C.c_int, [_vp, C.c_int, C.c_int]),
     ("raw_ao_batch_get_tuning", C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("raw_ao_batch_run", C.c_int, [_vp, C.c_int, _vp]),
     ("raw_ao_batch_capture", C.c_int, [_vp, C.c_int]),
@@ -595,6 +596,9 @@ class Batch:
 
     def set_tuning(self, chunk, tri_q, blocks_per_cu):
         self._check(lib().raw_ao_batch_set_tuning(self._h, chunk, tri_q, blocks_per_cu))
+
+    def set_option(self, opt, value):
+        self._check(lib().raw_ao_batch_set_option(self._h, opt, int(value)))
 
     def tuning(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()

@@ -122,6 +122,24 @@ def test_batch_results_do_not_depend_on_tuning(batch_case):
     b.set_tuning(4, 16, 8)
 
 
+def test_batch_wave_cut_does_not_change_results(batch_case):
+    """Each 64-ray group walking the tree once for its pixels' union box (the default) or
+    every ray from the root: the same AO bit for bit in every view."""
+    b, _, views = batch_case
+    b.run()
+    base = snapshot(views)
+    try:
+        for k in (0, 8):
+            b.set_option(raw_ao.OPT_WAVE_CUT, k)
+            b.run(rebuild=True)
+            for (a0, _, _), (a1, _, _) in zip(base, snapshot(views)):
+                assert np.array_equal(bits(a0), bits(a1))
+        with pytest.raises(raw_ao.RawAoError):
+            b.set_option(raw_ao.OPT_TRI_PHASE, 8)
+    finally:
+        b.set_option(raw_ao.OPT_WAVE_CUT, 2)
+
+
 def test_batch_tuner_and_cache(batch_case, tmp_path):
     b, _, views = batch_case
     cache = str(tmp_path / "tune.txt")
