@@ -72,6 +72,32 @@ def cpu_baseline(tris, pos, nrm, mask, spp, radius, row_step):
 
 WEIGHTS = os.path.join(ROOT, "raw-native_amd", "weights")
 HBM_COUNTERS = os.path.join(ROOT, "profiles", "ao_trace_1m_1080p_hbm_counters.md")
+L2_PEAK_GBS = 8 * 16 * 128 * 2.4   # 8 XCDs x 16 channels x 128 B a clock x 2.4 GHz
+
+
+def trace_counters():
+    """The trace kernel's L2 and VALU figures from the committed counter runs (empty without
+    that file): L2 requests, hit rates, VALU lane utilization and issue rate, kernel cycles."""
+    import re
+    try:
+        text = open(HBM_COUNTERS).read()
+    except OSError:
+        return {}
+    keys = ("l2_requests_per_dispatch", "l2_hit_rate", "l1_hit_rate", "valu_lane_utilization", "valu_issue_per_simd_cycle",
+            "kernel_cycles")
+    return {k: float(m.group(1)) for k in keys for m in [re.search(r"^%s: ([0-9.]+)$" % k, text, re.M)] if m}
+
+
+def l2_figures(trace_ms):
+    """L2 rate (128 B a request) over this run's trace time against the L2 peak (8 XCDs x 16
+    channels x 128 B x 2.4 GHz), with the counter runs' hit rates and VALU figures."""
+    c = trace_counters()
+    if "l2_requests_per_dispatch" not in c:
+        return None
+    gbs = c["l2_requests_per_dispatch"] * 128.0 / (trace_ms * 1e-3) / 1e9
+    out = {"achieved": gbs, "peak": L2_PEAK_GBS, "unit": "GB/s", "frac": gbs / L2_PEAK_GBS}
+    out.update({k: v for k, v in c.items() if k != "l2_requests_per_dispatch"})
+    return out
 
 
 def hbm_traffic_per_launch():
@@ -469,7 +495,8 @@ def main():
                      "traffic_source": os.path.relpath(HBM_COUNTERS, ROOT),
                      "kernel": trace_kernel_name(a),
                      "algorithmic_bytes_per_launch": alg_bytes,
-                     "node_visits": visits, "triangle_tests": tests},
+                     "node_visits": visits, "triangle_tests": tests,
+                     "l2": l2_figures(trace_ms)},
     }
     out["gbuffer"] = gbuffer_gpu(ctx, torch, vpos, vnrm, idx, eye, W, H, gbuf_host, host_raster_ms, a.steps, stream)
     if learned is not None:
