@@ -41,6 +41,11 @@ __device__ __forceinline__ V3 centroid(const float* __restrict__ t) {   // rt_bv
     const float third = 0.333333343f;
     return v3((t[0] + t[3] + t[6]) * third, (t[1] + t[4] + t[7]) * third, (t[2] + t[5] + t[8]) * third);
 }
+// An unsigned key in the order of float `<` for non-NaN values, with -0 and +0 equal.
+__device__ __forceinline__ uint32_t area_order(float d) {
+    const uint32_t u = d == 0.0f ? 0u : __float_as_uint(d);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
 __device__ __forceinline__ float area6(float lx, float ly, float lz, float hx, float hy, float hz) {   // rt_bvh.cpp:46-52
     const float dx = hx - lx, dy = hy - ly, dz = hz - lz;
     return 2.0f * (dy * (dx + dz) + dz * dx);
@@ -369,6 +374,7 @@ __global__ __launch_bounds__(TAIL_T) void k_ploc_tail(const PlocState* __restric
     __shared__ float sb[6][TAIL_M];
     __shared__ uint32_t sid[TAIL_M];
     __shared__ int8_t snn[TAIL_M];
+    __shared__ unsigned long long skey[TAIL_M];
     __shared__ uint32_t wsum[TAIL_T / 64][2];
     PlocState s = *st_in;
     uint32_t m = s.m;
@@ -382,20 +388,39 @@ __global__ __launch_bounds__(TAIL_T) void k_ploc_tail(const PlocState* __restric
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i0 = threadIdx.x * TAIL_PER;
     while (m > 1) {
+        // Each pair (i, i + d), d = 1..radius, is evaluated once, by the owner of i: it is a
+        // right candidate of i (kept in a register) and a left candidate of i + d (an LDS
+        // atomic minimum). Keys are (area order, candidate position), so the minimum is the
+        // smallest area at the lowest position: the reference's ascending scan with a strict
+        // `<` from FLT_MAX (areas that are not below FLT_MAX never count).
+        for (int k = 0; k < TAIL_PER; ++k) {
+            const int i = i0 + k;
+            if (i < int(m)) skey[i] = ~0ull;
+        }
+        __syncthreads();
+        unsigned long long rbest[TAIL_PER];
+        for (int k = 0; k < TAIL_PER; ++k) {
+            const int i = i0 + k;
+            rbest[k] = ~0ull;
+            if (i >= int(m)) continue;
+            const float al0 = sb[0][i], al1 = sb[1][i], al2 = sb[2][i], ah0 = sb[3][i], ah1 = sb[4][i], ah2 = sb[5][i];
+            const int j1 = min(int(m) - 1, i + radius);
+            for (int j = i + 1; j <= j1; ++j) {
+                const float d = area6(stdmin(al0, sb[0][j]), stdmin(al1, sb[1][j]), stdmin(al2, sb[2][j]),
+                                      stdmax(ah0, sb[3][j]), stdmax(ah1, sb[4][j]), stdmax(ah2, sb[5][j]));
+                if (d < FLT_MAX) {
+                    const unsigned long long kd = (unsigned long long)area_order(d) << 32;
+                    rbest[k] = min(rbest[k], kd | uint32_t(j));
+                    atomicMin(&skey[j], kd | uint32_t(i));
+                }
+            }
+        }
+        __syncthreads();
         for (int k = 0; k < TAIL_PER; ++k) {
             const int i = i0 + k;
             if (i >= int(m)) break;
-            const float al0 = sb[0][i], al1 = sb[1][i], al2 = sb[2][i], ah0 = sb[3][i], ah1 = sb[4][i], ah2 = sb[5][i];
-            float best = FLT_MAX;
-            int bj = i;
-            const int j0 = max(0, i - radius), j1 = min(int(m) - 1, i + radius);
-            for (int j = j0; j <= j1; ++j) {
-                if (j == i) continue;
-                const float d = area6(stdmin(al0, sb[0][j]), stdmin(al1, sb[1][j]), stdmin(al2, sb[2][j]),
-                                      stdmax(ah0, sb[3][j]), stdmax(ah1, sb[4][j]), stdmax(ah2, sb[5][j]));
-                if (d < best) { best = d; bj = j; }
-            }
-            snn[i] = int8_t(bj - i);
+            const unsigned long long best = min(skey[i], rbest[k]);
+            snn[i] = best == ~0ull ? int8_t(0) : int8_t(int(uint32_t(best)) - i);
         }
         __syncthreads();
         bool mg[TAIL_PER], sv[TAIL_PER];
