@@ -242,6 +242,7 @@ __global__ __launch_bounds__(BT) void k_ploc_nn(const PlocState* __restrict__ st
                                                 uint8_t* __restrict__ flags, uint32_t* __restrict__ counts, uint32_t nblk) {
     __shared__ float sb[6][BT + 4 * MAX_R];
     __shared__ int32_t snn[BT + 2 * MAX_R];
+    __shared__ unsigned long long skey[BT + 2 * MAX_R];
     __shared__ uint32_t wsum[BT / 64][2];
     const int32_t m = int32_t(st->m);
     const int32_t b0 = int32_t(blockIdx.x) * BT;
@@ -256,22 +257,40 @@ __global__ __launch_bounds__(BT) void k_ploc_nn(const PlocState* __restrict__ st
     }
     __syncthreads();
     const int nspan = BT + 2 * radius;             // neighbours of positions b0-radius ..
-    for (int q = threadIdx.x; q < nspan; q += BT) {
-        const int32_t pos = b0 - radius + q;
-        if (pos < 0 || pos >= m) continue;
-        const int ip = pos - base;
-        const float al0 = sb[0][ip], al1 = sb[1][ip], al2 = sb[2][ip], ah0 = sb[3][ip], ah1 = sb[4][ip], ah2 = sb[5][ip];
-        float best = FLT_MAX;
-        int32_t bj = pos;
-        const int32_t j0 = max(0, pos - radius), j1 = min(m - 1, pos + radius);
-        for (int32_t j = j0; j <= j1; ++j) {
-            if (j == pos) continue;
+    const int32_t q0 = b0 - radius;                // position of snn[0] and skey[0]
+    for (int q = threadIdx.x; q < nspan; q += BT) skey[q] = ~0ull;
+    __syncthreads();
+    // Each pair (p, p + d), d = 1..radius, that some position of the span needs is evaluated
+    // once, by the owner of p (positions b0-2radius .. b0+BT+radius-1): a right candidate of p
+    // and a left candidate of p + d, both by an LDS atomic minimum of (area order, candidate
+    // position). The minimum is the smallest area at the lowest position: the reference's
+    // ascending scan with a strict `<` from FLT_MAX (areas not below FLT_MAX never count).
+    for (int k = threadIdx.x; k < BT + 3 * radius; k += BT) {
+        const int32_t p = base + k;
+        if (p < 0 || p >= m) continue;
+        const bool needs = p >= q0;                // p's own neighbour is wanted
+        const float al0 = sb[0][k], al1 = sb[1][k], al2 = sb[2][k], ah0 = sb[3][k], ah1 = sb[4][k], ah2 = sb[5][k];
+        unsigned long long rb = ~0ull;
+        const int32_t j1 = min(m - 1, p + radius);
+        for (int32_t j = p + 1; j <= j1; ++j) {
             const int jp = j - base;
             const float d = area6(stdmin(al0, sb[0][jp]), stdmin(al1, sb[1][jp]), stdmin(al2, sb[2][jp]),
                                   stdmax(ah0, sb[3][jp]), stdmax(ah1, sb[4][jp]), stdmax(ah2, sb[5][jp]));
-            if (d < best) { best = d; bj = j; }
+            if (d < FLT_MAX) {
+                const unsigned long long kd = (unsigned long long)area_order(d) << 32;
+                rb = min(rb, kd | uint32_t(j));
+                const int qj = j - q0;
+                if (qj >= 0 && qj < nspan) atomicMin(&skey[qj], kd | uint32_t(p));
+            }
         }
-        snn[q] = bj;
+        if (needs && rb != ~0ull) atomicMin(&skey[p - q0], rb);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nspan; q += BT) {
+        const int32_t pos = q0 + q;
+        if (pos < 0 || pos >= m) continue;
+        const unsigned long long best = skey[q];
+        snn[q] = best == ~0ull ? pos : int32_t(uint32_t(best));
     }
     __syncthreads();
     const int32_t pos = b0 + int32_t(threadIdx.x);
