@@ -150,6 +150,31 @@ def gbuffer_gpu(ctx, torch, vpos, vnrm, idx, eye, W, H, gbuf_host, host_ms, step
             "triangles": int(idx.shape[0])}
 
 
+def whole_frame(ctx, torch, vpos, vnrm, idx, eye, W, H, radius, steps, stream):
+    """The reference's whole default frame (renderWithParams: raster, PLOC build, visibility or
+    AO, SSAO, reconcile, shade, motion) on the GPU from the resident mesh (raw_ao_render_frame_dev),
+    lighting models 2 (AO + sky + sun) and 1 (AO only), ms a frame."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    aspect = float(np.float32(W) / np.float32(H))
+    sc = dict(vpos=torch.from_numpy(np.ascontiguousarray(vpos)).to(dev), vnrm=torch.from_numpy(np.ascontiguousarray(vnrm)).to(dev),
+              idx=torch.from_numpy(np.ascontiguousarray(idx)).to(dev), mesh_end=np.array([idx.shape[0]], np.uint32),
+              albedo=np.full((1, 3), 0.8, np.float32), eye=eye, center=np.array([0, 2, 0], np.float32),
+              up=np.array([0, 1, 0], np.float32), fovy=0.9, aspect=aspect, near=0.1, far=200.0, has_light=False)
+    shapes = dict(depth=(H, W), position=(H, W, 3), normal=(H, W, 3), albedo=(H, W, 3), mask=(H, W), motion=(H, W, 2),
+                  ao_rt=(H, W), ao_ss=(H, W), sky=(H, W), sun=(H, W), error=(H, W), frame=(H, W, 3), hdr=(H, W, 3))
+    planes = {k: torch.empty(sh, dtype=torch.uint8 if k == "mask" else torch.float32, device=dev) for k, sh in shapes.items()}
+    out = {"workload": "renderWithParams on the %d-triangle scene at %dx%d, 64 spp, radius %.1f, SSAO 24, every plane on "
+                       "the device" % (idx.shape[0], W, H, radius)}
+    for lighting in (2, 1):
+        ms, st = _timed(torch, lambda: ctx.render_frame_dev(sc, W, H, planes, lighting=lighting, radius=radius,
+                                                           stream=stream), steps)
+        out["lighting_%d_ms" % lighting] = ms
+        out["lighting_%d_rmse" % lighting] = float(st["rmse"])
+        log("whole frame, lighting %d: %.3f ms" % (lighting, ms))
+    return out
+
+
 def _rmse(ctx, raw_ao, torch, approx, truth, mask_t, mask_np, W, H, stream):
     """RMSE (the reconcile's float64 row-major sum) and maximum error over covered pixels."""
     err = torch.empty_like(approx)
@@ -345,6 +370,7 @@ def main():
     ap.add_argument("--tri-q", type=int, default=0, help="wide kernel's triangle-phase threshold (1..64; 0: default)")
     ap.add_argument("--refill", type=int, default=-1, help="64-spp kernel: refill lanes at this many idle (0 off; -1 default)")
     ap.add_argument("--near-first", type=int, default=-1, help="64-spp kernel: nearest child first (0/1; -1 default)")
+    ap.add_argument("--no-whole-frame", action="store_true", help="skip the whole default frame (raw_ao_render_frame_dev)")
     ap.add_argument("--wave-cut", type=int, default=-1, help="64-spp kernel: start rays from the wave's cut (0..64; -1 default)")
     ap.add_argument("--ploc-tail", type=int, default=0, help="PLOC: clusters at which one workgroup takes over (1..4096)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned AO surrogate at 1080p")
@@ -499,6 +525,8 @@ def main():
                      "l2": l2_figures(trace_ms)},
     }
     out["gbuffer"] = gbuffer_gpu(ctx, torch, vpos, vnrm, idx, eye, W, H, gbuf_host, host_raster_ms, a.steps, stream)
+    if not a.no_whole_frame:
+        out["whole_frame"] = whole_frame(ctx, torch, vpos, vnrm, idx, eye, W, H, radius, a.steps, stream)
     if learned is not None:
         out["learned_ao"] = learned
     if not a.no_denoise:

@@ -636,6 +636,8 @@ struct TraceArgs4 {
     float cut_span;    // CUT below 64 spp: no walk when the wave's union box is wider than this
     const TopEntry* top;   // CUT: the tree's top table and its entry count
     const uint32_t* ntop;
+    float* sky;            // SKY: sky out (64 spp) or the unblocked counts (below)
+    uint32_t* cnt_free;
 };
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -652,7 +654,10 @@ __device__ __forceinline__ float wave_max(float v) {
 // several pixels, all lanes stay for the walk, and the walk's box is the union of the live
 // lanes' reach boxes (no walk when that union is wider than a.cut_span). stk: STACK_LDS x BT
 // words; cutmem: CUT_WAVE words a wave.
-template <bool SPP64, bool COUNT, bool NEAR, bool CUT>
+// SKY (the visibility pass): each open AO ray is traced again to tMax 1e30 from the root (in
+// the same lane, right after), sky = the unblocked fraction (a.sky; counts in a.cnt_free below
+// 64 spp), 1 off the mask.
+template <bool SPP64, bool COUNT, bool NEAR, bool CUT, bool SKY = false>
 __device__ __forceinline__ void trace_ao4(const TraceArgs4& a, uint32_t* stk, uint32_t* cutmem) {
     constexpr bool STAY = CUT && !SPP64;
     const uint64_t gid = uint64_t(blockIdx.x) * BT + threadIdx.x;
@@ -662,7 +667,10 @@ __device__ __forceinline__ void trace_ao4(const TraceArgs4& a, uint32_t* stk, ui
     const uint32_t pix = live ? pix0 : 0u;   // a lane past the end reads pixel 0 and is not live
     const uint32_t s = uint32_t(gid - uint64_t(pix0) * a.spp);
     if (live && !a.mask[pix]) {
-        if (s == 0) a.ao[pix] = 1.0f;
+        if (s == 0) {
+            a.ao[pix] = 1.0f;
+            if (SKY) a.sky[pix] = 1.0f;
+        }
         live = false;
     }
     if (!STAY && !live) return;
@@ -737,6 +745,14 @@ __device__ __forceinline__ void trace_ao4(const TraceArgs4& a, uint32_t* stk, ui
             occ = res == 1;
         }
     }
+    bool far_occ = occ;
+    if (SKY && a.ntri) {   // blocked within the radius: blocked for the sky too
+        constexpr float kFarT = 1e30f;
+        const Ray r = make_ray(origin, dir);
+        const int res2 = occluded4<false>(r, kFarT, a.nodes, a.tris, a.ntri, stk + threadIdx.x, a.tri_q, visits, tests,
+                                          live && !occ);
+        far_occ = occ || (res2 < 0 ? (live && scan_occluded(r, kFarT, a.tris, a.ntri)) : res2 == 1);
+    }
     if (STAY && !live) return;   // past the wave's collective steps
     if (COUNT) {
         atomicAdd(&a.counters->visits, (unsigned long long)visits);
@@ -746,8 +762,13 @@ __device__ __forceinline__ void trace_ao4(const TraceArgs4& a, uint32_t* stk, ui
     if (SPP64) {
         const unsigned long long open = __ballot(!occ);
         if (s == 0) a.ao[pix] = float(__popcll(open)) / 64.0f;
-    } else if (!occ) {
-        atomicAdd(&a.cnt[pix], 1u);
+        if (SKY) {
+            const unsigned long long free = __ballot(!far_occ);
+            if (s == 0) a.sky[pix] = float(__popcll(free)) / 64.0f;
+        }
+    } else {
+        if (!occ) atomicAdd(&a.cnt[pix], 1u);
+        if (SKY && !far_occ) atomicAdd(&a.cnt_free[pix], 1u);
     }
 }
 
@@ -765,6 +786,14 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(7, 7))) void
     __shared__ uint32_t stk[STACK_LDS * BT];
     __shared__ uint32_t cutmem[CUT_WAVE * (BT / 64)];
     trace_ao4<SPP64, COUNT, false, true>(a, stk, cutmem);
+}
+
+// The visibility pass with the AO rays from the wave's cut (the sky rays from the root).
+template <bool SPP64>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(7, 7))) void k_vis4c(TraceArgs4 a) {
+    __shared__ uint32_t stk[STACK_LDS * BT];
+    __shared__ uint32_t cutmem[CUT_WAVE * (BT / 64)];
+    trace_ao4<SPP64, false, false, true, true>(a, stk, cutmem);
 }
 
 // Nearest (ties to the lower triangle index) or any hit of general rays.
@@ -1442,8 +1471,19 @@ int trace_vis4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_
         if (!hip_ok(c, hipMemsetAsync(a.cnt, 0, 8 * size_t(npix), s), "count clear")) return -2;
     }
     const uint32_t blocks = cdiv(uint64_t(npix) * uint64_t(spp), BT);
+    const bool cut = c.wave_cut > 0 && c.ntri > 0;
+    if (cut) {
+        a.sky = d_sky;
+        a.cnt_free = cnt_free;
+        a.cut_small = float(c.wave_cut) * radius;
+        a.cut_span = 4.0f * radius;
+        a.top = c.top.as<TopEntry>();
+        a.ntop = c.top.as<uint32_t>() + top_count_offset();
+    }
     (void)hipEventRecord(c.ev[2], s);
-    if (spp64) k_trace_vis4<true><<<blocks, BT, 0, s>>>(a, d_sky, nullptr);
+    if (cut && spp64) k_vis4c<true><<<blocks, BT, 0, s>>>(a);
+    else if (cut) k_vis4c<false><<<blocks, BT, 0, s>>>(a);
+    else if (spp64) k_trace_vis4<true><<<blocks, BT, 0, s>>>(a, d_sky, nullptr);
     else k_trace_vis4<false><<<blocks, BT, 0, s>>>(a, d_sky, cnt_free);
     k_sun4<<<cdiv(npix, BT), BT, 0, s>>>(a, to_light[0], to_light[1], to_light[2], sun_forced, d_sun);
     (void)hipEventRecord(c.ev[3], s);

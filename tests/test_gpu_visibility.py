@@ -57,6 +57,29 @@ def test_visibility_seeded_scene(ctx):
         assert np.array_equal(bits(g), bits(o))
 
 
+@pytest.mark.parametrize("spp", [64, 16])
+def test_visibility_wave_cut(ctx, spp):
+    """The 1M-triangle scene at radius 1: the AO rays from the wave's cut (the sky rays from the
+    root), at 64 spp and below, against every ray from the root: ao, sky and sun bit for bit."""
+    import raw_ao
+    import raw_ao.host as H
+    print("visibility wave cut", spp, flush=True)
+    vpos, vnrm, idx, _ = H.scene_seeded()
+    _, pos, nrm, mask = H.rasterize(vpos, vnrm, idx, H.orbit_camera(3), (0, 2, 0), (0, 1, 0), 0.9, 160 / 90, 0.1, 200.0,
+                                    160, 90)
+    ctx.build_ploc(H.mesh_tris(vpos, idx))
+    try:
+        ctx.set_option(raw_ao.OPT_WAVE_CUT, 0)
+        want = ctx.compute_rt_visibility(pos, nrm, mask, DEFAULT_TO_LIGHT, spp, 1.0)
+        ctx.set_option(raw_ao.OPT_WAVE_CUT, 2)
+        got = ctx.compute_rt_visibility(pos, nrm, mask, DEFAULT_TO_LIGHT, spp, 1.0)
+    finally:
+        ctx.set_option(raw_ao.OPT_WAVE_CUT, 2)
+    for g, o in zip(got, want):
+        assert np.array_equal(bits(g), bits(o))
+    assert 0.0 < float(np.mean(got[1][mask != 0])) < 1.0
+
+
 def test_visibility_empty_scene(ctx):
     _, pos, nrm, mask = O.cli_gbuffer(32, 32)
     ctx.build_ploc(np.zeros((0, 9), np.float32))
