@@ -371,6 +371,7 @@ def main():
     ap.add_argument("--refill", type=int, default=-1, help="64-spp kernel: refill lanes at this many idle (0 off; -1 default)")
     ap.add_argument("--near-first", type=int, default=-1, help="64-spp kernel: nearest child first (0/1; -1 default)")
     ap.add_argument("--no-whole-frame", action="store_true", help="skip the whole default frame (raw_ao_render_frame_dev)")
+    ap.add_argument("--in-flight", type=int, default=3, help="frames in flight (contexts and streams; 1: one frame at a time)")
     ap.add_argument("--wave-cut", type=int, default=-1, help="64-spp kernel: start rays from the wave's cut (0..64; -1 default)")
     ap.add_argument("--ploc-tail", type=int, default=0, help="PLOC: clusters at which one workgroup takes over (1..4096)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned AO surrogate at 1080p")
@@ -413,34 +414,45 @@ def main():
     d_err = torch.empty_like(d_ao)
     d_max = torch.zeros(4, dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
-    ctx = raw_ao.Context(0)
-    ctx.set_option(raw_ao.OPT_TRACE_KERNEL, raw_ao.TRACE_BINARY if a.kernel == "binary" else raw_ao.TRACE_WIDE)
-    if a.tri_q:
-        ctx.set_option(raw_ao.OPT_TRI_PHASE, a.tri_q)
-    if a.ploc_tail:
-        ctx.set_option(raw_ao.OPT_PLOC_TAIL, a.ploc_tail)
-    if a.refill >= 0:
-        ctx.set_option(raw_ao.OPT_REFILL, a.refill)
-    if a.near_first >= 0:
-        ctx.set_option(raw_ao.OPT_NEAR_FIRST, a.near_first)
-    if a.wave_cut >= 0:
-        ctx.set_option(raw_ao.OPT_WAVE_CUT, a.wave_cut)
+    def open_ctx():
+        c = raw_ao.Context(0)
+        c.set_option(raw_ao.OPT_TRACE_KERNEL, raw_ao.TRACE_BINARY if a.kernel == "binary" else raw_ao.TRACE_WIDE)
+        if a.tri_q:
+            c.set_option(raw_ao.OPT_TRI_PHASE, a.tri_q)
+        if a.ploc_tail:
+            c.set_option(raw_ao.OPT_PLOC_TAIL, a.ploc_tail)
+        if a.refill >= 0:
+            c.set_option(raw_ao.OPT_REFILL, a.refill)
+        if a.near_first >= 0:
+            c.set_option(raw_ao.OPT_NEAR_FIRST, a.near_first)
+        if a.wave_cut >= 0:
+            c.set_option(raw_ao.OPT_WAVE_CUT, a.wave_cut)
+        return c
 
+    # Frames in flight: frame i runs on context and stream i % n (its own tree and planes), so
+    # frame i+1's PLOC build (whose host-side waits see only its own stream) overlaps frame i's
+    # trace. Every frame does all its work: build, trace, SSAO, error map, host float64 sum.
+    nfl = max(1, a.in_flight)
+    ctxs = [open_ctx() for _ in range(nfl)]
+    ctx = ctxs[0]
+    tstreams = [torch.cuda.Stream() for _ in range(nfl)]
+    streams = [t.cuda_stream for t in tstreams]
+    planes = [(d_ao, d_ss, d_err, d_max)] + [(torch.empty_like(d_ao), torch.empty_like(d_ao), torch.empty_like(d_ao),
+                                             torch.zeros(4, dtype=torch.float32, device=dev)) for _ in range(nfl - 1)]
     host_ms = []
 
-    def frame():
-        ctx.build_ploc_dev(d_tris, 16, stream)
-        ctx.compute_rtao_dev(d_pos, d_nrm, d_mask, W, H, spp, radius, d_ao, stream)
-        ctx.compute_ssao_dev(d_pos, d_nrm, d_mask, W, H, 24, radius, d_ss, stream)
-        ctx.error_map_dev(d_ss, d_ao, d_mask, W, H, d_err, d_max, stream)
-        err = d_err.cpu()   # synchronizes the stream
-        t0 = time.perf_counter()
-        rmse, px = raw_ao.host.rmse_of(err.numpy(), mask)
-        host_ms.append((time.perf_counter() - t0) * 1e3)
-        return rmse, px
+    def enqueue(k):
+        c, st = ctxs[k], streams[k]
+        p_ao, p_ss, p_err, p_max = planes[k]
+        c.compute_rtao_dev(d_pos, d_nrm, d_mask, W, H, spp, radius, p_ao, st)
+        c.compute_ssao_dev(d_pos, d_nrm, d_mask, W, H, 24, radius, p_ss, st)
+        c.error_map_dev(p_ss, p_ao, d_mask, W, H, p_err, p_max, st)
 
-    for i in range(a.warmup):
-        frame()
+    for i in range(max(a.warmup, nfl)):
+        k = i % nfl
+        ctxs[k].build_ploc_dev(d_tris, 16, streams[k])
+        enqueue(k)
+    torch.cuda.synchronize()
     log("warmup done")
     # counting launch (outside the timed region): node visits and triangle tests per frame
     ctx.count_next_trace(True)
@@ -451,44 +463,57 @@ def main():
     visits, tests, rays_counted = _counts(ctx, raw_ao)
     log("counting run: %d rays, %.2f visits/ray, %.2f tests/ray" % (rays_counted, visits / max(1, rays_counted),
                                                                      tests / max(1, rays_counted)))
-    ms = {"build": [], "trace": [], "ssao": [], "error": []}
-    host_ms.clear()
-    # Frames back to back, each complete inside the timed region: frame i's error map goes to
-    # pinned host memory asynchronously and its float64 row-major sum runs on the host while
-    # the GPU traces frame i+1 (the PLOC build synchronizes its stream, so the previous frame's
-    # copy has landed when the build returns).
-    hbuf = [torch.empty((H, W), dtype=torch.float32, pin_memory=True) for _ in range(2)]
+    hbuf = [torch.empty((H, W), dtype=torch.float32, pin_memory=True) for _ in range(nfl + 1)]
 
-    def host_sum(i):
-        t = time.perf_counter()
-        r = raw_ao.host.rmse_of(hbuf[i % 2].numpy(), mask)
-        host_ms.append((time.perf_counter() - t) * 1e3)
-        return r
+    def run_frames(n, steps):
+        """steps frames with n in flight: (wall seconds, per-part GPU ms, last rmse, pixels).
+        Frame i's error map goes to pinned host buffer i % (n+1) on its stream; its float64
+        row-major sum runs on the host once frame i+n's build has waited on that stream, after
+        frame i+n's work is enqueued, so the GPU is busy meanwhile."""
+        ms = {"build": [], "trace": [], "ssao": [], "error": []}
+        host_ms.clear()
 
-    def gpu_parts(m):
-        for k in ("trace", "ssao", "error"):
-            ms[k].append(m[k])
+        def host_sum(j):
+            t = time.perf_counter()
+            r = raw_ao.host.rmse_of(hbuf[j % (n + 1)].numpy(), mask)
+            host_ms.append((time.perf_counter() - t) * 1e3)
+            return r
 
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    rmse = px = None
-    for i in range(a.steps):
-        ctx.build_ploc_dev(d_tris, 16, stream)
-        m = ctx.last_ms()
-        ms["build"].append(m["build"])
-        if i > 0:
-            gpu_parts(m)   # the previous frame's trace, SSAO and error map have finished
-        ctx.compute_rtao_dev(d_pos, d_nrm, d_mask, W, H, spp, radius, d_ao, stream)
-        ctx.compute_ssao_dev(d_pos, d_nrm, d_mask, W, H, 24, radius, d_ss, stream)
-        ctx.error_map_dev(d_ss, d_ao, d_mask, W, H, d_err, d_max, stream)
-        hbuf[i % 2].copy_(d_err, non_blocking=True)
-        if i > 0:
-            rmse, px = host_sum(i - 1)
-    torch.cuda.synchronize()
-    gpu_parts(ctx.last_ms())
-    rmse, px = host_sum(a.steps - 1)
-    wall = time.perf_counter() - t0
-    log("timed %d frames in %.3f s" % (a.steps, wall))
+        def gpu_parts(m):
+            for key in ("trace", "ssao", "error"):
+                ms[key].append(m[key])
+
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rmse = px = None
+        for i in range(steps):
+            k = i % n
+            ctxs[k].build_ploc_dev(d_tris, 16, streams[k])   # waits on stream k: frame i-n has finished
+            m = ctxs[k].last_ms()
+            ms["build"].append(m["build"])
+            if i >= n:
+                gpu_parts(m)
+            enqueue(k)
+            with torch.cuda.stream(tstreams[k]):
+                hbuf[i % (n + 1)].copy_(planes[k][2], non_blocking=True)
+            if i >= n:
+                rmse, px = host_sum(i - n)   # frame i-n finished before frame i's build returned
+        torch.cuda.synchronize()
+        for i in range(max(0, steps - n), steps):
+            gpu_parts(ctxs[i % n].last_ms())
+            rmse, px = host_sum(i)
+        return time.perf_counter() - t0, ms, rmse, px
+
+    # One frame at a time first: its per-kernel times (HIP events on each launch's stream) are
+    # the kernels' own, which the roofline uses; then the timed frames with nfl in flight, whose
+    # kernels overlap (their event times stretch) and whose wall time gives `value`.
+    w1, ms, rmse, px = run_frames(1, a.steps)
+    latency_ms = w1 * 1e3 / a.steps
+    log("one frame in flight: %.3f ms/frame" % latency_ms)
+    wall = w1
+    if nfl > 1:
+        wall, ms_fl, rmse, px = run_frames(nfl, a.steps)
+        log("timed %d frames (%d in flight) in %.3f s" % (a.steps, nfl, wall))
 
     frame_ms = wall * 1e3 / a.steps
     learned = None
@@ -512,6 +537,7 @@ def main():
                    "triangles": int(tris.shape[0]), "width": W, "height": H, "spp": spp, "radius": radius,
                    "covered_pixels": covered, "rays_per_frame": rays, "parallelism": "single GPU"},
         "trace_rays_per_s": rays / (trace_ms * 1e-3),
+        "frames_in_flight": nfl, "ms_per_frame_one_in_flight": latency_ms,
         "ms_frame_parts": {"ploc_build": float(np.mean(ms["build"])), "ao_trace": trace_ms,
                            "ssao": float(np.mean(ms["ssao"])), "error_map": float(np.mean(ms["error"])),
                            "host_reconcile_sum": float(np.mean(host_ms))},
@@ -547,7 +573,8 @@ def main():
         cb = cpu_baseline(tris, pos, nrm, mask, spp, radius, a.cpu_row_step)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
         out["vs_cpu_trace"] = out["trace_rays_per_s"] / cb["value"]
-    ctx.close()
+    for c in ctxs:
+        c.close()
     print(json.dumps(out))
 
 
