@@ -726,11 +726,27 @@ __device__ __forceinline__ void trace_ao4(const TraceArgs4& a, uint32_t* stk, ui
         if (CUT) {
             bool hit = false;
             const float reach = a.radius * kReach;
-            for (int i = 0; i < cv.nleaf && !hit && live; ++i) {   // nleaf is 0 when the cut overflowed
-                float t;
-                if (!CutView::enters(r, cv.lbox, CUT_LEAF, i, reach, kExit)) continue;
-                if (COUNT) ++tests;
-                if (cv.leaf[i] < a.ntri && tri_hit(r, a.tris, cv.leaf[i], t) && t < a.radius) hit = true;
+            // the cut leaves this ray enters (nleaf is 0 when the cut overflowed), then their
+            // triangles two at a time, both loads in flight together
+            uint32_t lp = 0;
+            if (live)
+                for (int i = 0; i < cv.nleaf; ++i)
+                    if (cv.leaf[i] < a.ntri && CutView::enters(r, cv.lbox, CUT_LEAF, i, reach, kExit)) lp |= 1u << i;
+            while (lp && !hit) {
+                const uint32_t s0 = cv.leaf[__builtin_ctz(lp)];
+                lp &= lp - 1u;
+                const bool two = lp != 0;
+                const uint32_t s1 = two ? cv.leaf[__builtin_ctz(lp)] : s0;
+                if (two) lp &= lp - 1u;
+                const float4* q0 = a.tris + 3 * size_t(s0);
+                const float4* q1 = a.tris + 3 * size_t(s1);
+                const float4 a0 = q0[0], a1 = q0[1], a2 = q0[2], b0 = q1[0], b1 = q1[1], b2 = q1[2];
+                if (COUNT) tests += two ? 2u : 1u;
+                float t0, t1;
+                hit = (intersect_t(r.o, r.d, r.f, v3(a0.x, a0.y, a0.z), v3(a0.w, a1.x, a1.y), v3(a1.z, a1.w, a2.x), t0) &&
+                       t0 < a.radius) ||
+                      (two && intersect_t(r.o, r.d, r.f, v3(b0.x, b0.y, b0.z), v3(b0.w, b1.x, b1.y), v3(b1.z, b1.w, b2.x), t1) &&
+                       t1 < a.radius);
             }
             res = occluded4<COUNT, NEAR>(r, a.radius, a.nodes, a.tris, a.ntri, stk + threadIdx.x, a.tri_q, visits, tests,
                                          live && !hit, &cv);
