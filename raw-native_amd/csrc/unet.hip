@@ -291,6 +291,92 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, int ntiles) {
     }
 }
 
+// The persistent form of k_conv_lds for layers of one 32-channel chunk in and one channel
+// group out (cin = 32, cout = CT x 16): the weights are staged once; block b takes tiles b,
+// b + gridDim.x, ..., the next tile's halo loaded into registers while the current tile's
+// MFMAs run. The staging layout, the fragments and the sum order are k_conv_lds's, so the
+// outputs are the same bit for bit. (Measured: the same form for layers of several chunks,
+// with each chunk's weights prefetched too, and for 64 outputs ran slower than k_conv_lds.)
+constexpr int PF_BLOCKS = 3;   // k_conv_pf<2> blocks a CU: 46 KB of LDS each
+template <int CT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_conv_pf(ConvArgs a, int ntiles) {
+    __shared__ __attribute__((aligned(16))) bf16_t hal[LH_H * LH_W * LH_PS];
+    __shared__ __attribute__((aligned(16))) bf16_t wls[CT * 16 * LW_RS];
+    constexpr int HV = (LH_H * LH_W * 4 + 255) / 256;   // halo pieces (8 channels) a thread
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int g = lane >> 4, r = lane & 15;
+    const int tiles_x = (a.wd + LT_W - 1) / LT_W, tiles_y = (a.h + LT_H - 1) / LT_H;
+    int tile = int(blockIdx.x);
+    if (tile >= ntiles) return;
+    for (int e = tid; e < CT * 16 * 36; e += 256) {
+        const int co = e / 36, rem = e - co * 36, tap = rem >> 2, q = rem & 3;
+        *reinterpret_cast<uint4*>(&wls[co * LW_RS + tap * 32 + 8 * q]) =
+            *reinterpret_cast<const uint4*>(a.w + size_t(a.co0 + co) * a.kpad + tap * 32 + 8 * q);
+    }
+    uint4 hv[HV];
+    const auto load = [&](int tl) {
+        const int bx = tl % tiles_x, q0 = tl / tiles_x, by = q0 % tiles_y, img = q0 / tiles_y;
+#pragma unroll
+        for (int k = 0; k < HV; ++k) {
+            const int e = tid + 256 * k;
+            hv[k] = make_uint4(0, 0, 0, 0);
+            if (e >= LH_H * LH_W * 4) continue;
+            const int hp = e >> 2, q = e & 3;
+            const int gy = by * LT_H + hp / LH_W - 1, gx = bx * LT_W + hp % LH_W - 1;
+            if (gy < 0 || gy >= a.h || gx < 0 || gx >= a.wd) continue;
+            hv[k] = *reinterpret_cast<const uint4*>(a.in0 + (size_t(img * a.h + gy) * a.wd + gx) * 32 + 8 * q);
+        }
+    };
+    load(tile);
+    for (;;) {
+        __syncthreads();   // every wave is done with the previous tile's halo
+#pragma unroll
+        for (int k = 0; k < HV; ++k) {
+            const int e = tid + 256 * k;
+            if (e < LH_H * LH_W * 4) *reinterpret_cast<uint4*>(&hal[(e >> 2) * LH_PS + 8 * (e & 3)]) = hv[k];
+        }
+        __syncthreads();
+        const int ntile = tile + int(gridDim.x);
+        if (ntile < ntiles) load(ntile);   // in flight during this tile's MFMAs
+        f32x4 acc[CT][4];
+        for (int i = 0; i < CT; ++i)
+            for (int t = 0; t < 4; ++t) acc[i][t] = f32x4{0, 0, 0, 0};
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap % 3;
+            V8 bv[4];
+            for (int t = 0; t < 4; ++t) {
+                const int hp = (2 * wv + (t >> 1) + dy) * LH_W + 16 * (t & 1) + r + dx;
+                bv[t].u = *reinterpret_cast<const uint4*>(&hal[hp * LH_PS + 8 * g]);
+            }
+            for (int i = 0; i < CT; ++i) {
+                V8 av;
+                av.u = *reinterpret_cast<const uint4*>(&wls[(i * 16 + r) * LW_RS + tap * 32 + 8 * g]);
+                for (int t = 0; t < 4; ++t) acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av.v, bv[t].v, acc[i][t], 0, 0, 0);
+            }
+        }
+        const int bx = tile % tiles_x, q0 = tile / tiles_x, by = q0 % tiles_y, img = q0 / tiles_y;
+        const int oy = by * LT_H, ox = bx * LT_W;
+        for (int t = 0; t < 4; ++t) {
+            const int y = oy + 2 * wv + (t >> 1), x = ox + 16 * (t & 1) + r;
+            if (y >= a.h || x >= a.wd) continue;
+            const size_t p = size_t(img * a.h + y) * a.wd + x;
+            for (int i = 0; i < CT; ++i) {
+                const int co = a.co0 + i * 16 + 4 * g;
+                float v[4];
+                for (int j = 0; j < 4; ++j) {
+                    v[j] = acc[i][t][j] + (a.bias ? a.bias[co + j] : 0.0f);
+                    if (a.relu && v[j] < 0.0f) v[j] = 0.0f;
+                }
+                const uint32_t lo = uint32_t(f2bf(v[0])) | (uint32_t(f2bf(v[1])) << 16);
+                const uint32_t hi = uint32_t(f2bf(v[2])) | (uint32_t(f2bf(v[3])) << 16);
+                *reinterpret_cast<uint2*>(a.out + p * a.cout + co) = make_uint2(lo, hi);
+            }
+        }
+        if (ntile >= ntiles) break;
+        tile = ntile;
+    }
+}
+
 // The first layer's convolution (8 input channels, K = 72 padded to 96): one 32-wide k step
 // is 4 taps x 8 channels, so lane group g of a fragment reads tap 4*step + g, shifted, from
 // the staged 10 x 34 pixel tile (16 B a pixel). The 3 steps sum k in k_conv's order, so the
@@ -720,13 +806,19 @@ bool conv3x3(Ctx& c, const bf16_t* in0, const bf16_t* in1, int c0, int c1, const
         }
         return hip_ok(c, hipGetLastError(), "conv3x3");
     }
-    if (chunked && c.conv_kernel == 0 && (c1 == 0 || (h % 2 == 0 && wd % 2 == 0))) {
+    if (chunked && (c.conv_kernel == 0 || c.conv_kernel == 3) && (c1 == 0 || (h % 2 == 0 && wd % 2 == 0))) {
         if (c.cus == 0) {
             int cus = 0;
             c.cus = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) == hipSuccess && cus > 0 ? cus
                                                                                                                      : 256;
         }
         const int ntiles = int(size_t(n) * ((h + LT_H - 1) / LT_H) * ((wd + LT_W - 1) / LT_W));
+        if (c.conv_kernel == 0 && c0 == 32 && c1 == 0 && cout == 32) {   // one chunk in, one group out: persistent
+            a.co0 = 0;
+            const unsigned blocks = unsigned(std::min(ntiles, c.cus * PF_BLOCKS));
+            k_conv_pf<2><<<blocks, 256, 0, s>>>(a, ntiles);
+            return hip_ok(c, hipGetLastError(), "conv3x3");
+        }
         // the widest channel group (<= 64) that still gives every CU two work items;
         // channel groups of one width run in one launch, a narrower remainder in another
         int ct = 4;
