@@ -359,6 +359,29 @@ def test_wave_cut_does_not_change_ao(ctx, k, spp):
     assert rays == spp * int(mask.sum()) and visits > 0
 
 
+def test_wave_cut_full_frame(ctx):
+    """At the bench's full size (the 1M-triangle scene, 1920 x 1080, 64 spp, radius 1.0) the
+    cut kernel's AO equals the traversal from the root on every pixel, and so does the
+    visibility pass's (ao, sky, sun)."""
+    print("wave cut 1080p", flush=True)
+    tris, pos, nrm, mask = big_frame(1920, 1080, seed=1)
+    ctx.build_ploc(tris, 16)
+    to_light = np.array([0.4, 1.0, 0.3], np.float32) / np.float32(np.sqrt(1.25))
+    try:
+        ctx.set_option(raw_ao.OPT_WAVE_CUT, 0)
+        want = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
+        want_vis = ctx.compute_rt_visibility(pos, nrm, mask, to_light, 64, 1.0)
+        ctx.set_option(raw_ao.OPT_WAVE_CUT, WAVE_CUT_DEFAULT)
+        got = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
+        got_vis = ctx.compute_rt_visibility(pos, nrm, mask, to_light, 64, 1.0)
+    finally:
+        ctx.set_option(raw_ao.OPT_WAVE_CUT, WAVE_CUT_DEFAULT)
+    assert int(mask.sum()) > 1_000_000
+    assert np.array_equal(bits(got), bits(want))
+    for g, w in zip(got_vis, want_vis):
+        assert np.array_equal(bits(g), bits(w))
+
+
 def test_wave_cut_small_scenes(ctx):
     """The cut on the 12-triangle test scene (radius 2, the tree's leaves reached in the wave's
     walk) and on an empty tree."""
