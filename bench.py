@@ -344,11 +344,12 @@ ARGS = None
 
 
 def trace_kernel_name(a):
-    """The kernel that traces the 64-spp frame under these options."""
+    """The kernel that traces the 64-spp frame under these options (the library's defaults:
+    refill at 48 idle lanes, wave cut 2)."""
     if a.kernel == "binary":
         return "k_trace_ao"
-    if a.refill > 0:
-        return "k_trace_ao4r"
+    if a.refill != 0:
+        return "k_trace_ao4r" if a.wave_cut == 0 else "k_trace_ao4rc"
     return "k_trace_ao4" if a.wave_cut == 0 else "k_trace_ao4c"
 
 

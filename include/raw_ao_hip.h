@@ -71,9 +71,10 @@ enum {
     /* PLOC tuning, 1..4096 (default 4096): below this many clusters one workgroup runs every
      * remaining round. The tree is the same for every value; only the speed differs. */
     RAW_AO_OPT_PLOC_TAIL = 5,
-    /* 64-spp wide kernel, 0..64: with a value n > 0 a wave owns 4 pixels' 256 rays and a lane
-     * that finishes takes the next ray once n lanes are idle; 0 = one pixel's 64 rays a wave.
-     * Results are the same for every value; only the speed differs. */
+    /* 64-spp wide kernel, 0..64 (default 48): with a value n > 0 a wave owns 4 pixels' 256 rays
+     * (with the wave cut: from the cut of their union box) and a lane that finishes takes the
+     * next ray once n lanes are idle; 0 = one pixel's 64 rays a wave. Results are the same for
+     * every value; only the speed differs. */
     RAW_AO_OPT_REFILL = 6,
     /* 64-spp wide kernel, 0 or 1 (default 1): descend into the hit child box with the smallest
      * entry distance first (0: the first in slot order). Results are the same; only the speed

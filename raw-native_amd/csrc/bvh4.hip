@@ -1091,6 +1091,205 @@ __global__ __launch_bounds__(BT) void k_trace_ao4r(TraceArgs4 a, uint32_t refill
     finish();
 }
 
+// k_trace_ao4r from the wave's cut: the wave walks the tree once for the union of its PW pixels'
+// reach boxes (no walk when that union is wider than a.cut_span), and every ray starts from the
+// cut: its entered cut leaves wait with its node's leaf slots for the wave's triangle phases,
+// and when its stack runs empty it takes the next cut node it enters. Open counts and refills
+// as k_trace_ao4r. Held to 6 waves a SIMD (80 registers). COUNT: per-lane visit and test counts.
+template <int PW, bool COUNT>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_trace_ao4rc(TraceArgs4 a,
+                                                                                                uint32_t refill_min) {
+    __shared__ uint32_t stk[STACK_LDS * BT];
+    __shared__ uint32_t cutmem[CUT_WAVE * (BT / 64)];
+    uint32_t* const lds_col = stk + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t pix0 = (blockIdx.x * (BT / 64) + wave) * PW;
+    if (pix0 >= a.npix) return;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t cnt[PW];
+    for (int k = 0; k < PW; ++k) cnt[k] = 0;
+    const auto finish = [&]() {
+        for (int k = 0; k < PW; ++k) {
+            uint32_t v = cnt[k];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            const uint32_t pix = pix0 + uint32_t(k);
+            if (lane == 0 && pix < a.npix) a.ao[pix] = a.mask[pix] ? float(v) / 64.0f : 1.0f;
+        }
+    };
+    if (a.ntri == 0) {   // nothing to hit: every covered pixel's rays are open
+        for (int k = 0; k < PW; ++k) cnt[k] = 1;
+        finish();
+        return;
+    }
+    // the union of the covered pixels' reach boxes (as trace_ao4's), lane k for pixel pix0 + k
+    V3 lo = v3(kFar, kFar, kFar), hi = v3(-kFar, -kFar, -kFar);
+    if (lane < uint32_t(PW) && pix0 + lane < a.npix && a.mask[pix0 + lane]) {
+        const uint32_t pix = pix0 + lane;
+        const V3 p = v3(a.pos[3 * size_t(pix)], a.pos[3 * size_t(pix) + 1], a.pos[3 * size_t(pix) + 2]);
+        const V3 n = v3(a.nrm[3 * size_t(pix)], a.nrm[3 * size_t(pix) + 1], a.nrm[3 * size_t(pix) + 2]);
+        const V3 o = add(p, scl(n, 1e-3f));
+        const float big = fmaxf(fabsf(o.x), fmaxf(fabsf(o.y), fabsf(o.z)));
+        const float half = a.radius * 1.001f + (big + a.radius) * 1e-5f;
+        lo = v3(o.x - half, o.y - half, o.z - half);
+        hi = v3(o.x + half, o.y + half, o.z + half);
+    }
+    lo = v3(wave_min(lo.x), wave_min(lo.y), wave_min(lo.z));
+    hi = v3(wave_max(hi.x), wave_max(hi.y), wave_max(hi.z));
+    if (!(lo.x <= hi.x)) {   // no covered pixel
+        finish();
+        return;
+    }
+    uint32_t* const mine = cutmem + wave * CUT_WAVE;
+    CutView cv;
+    cv.id = mine;
+    cv.box = reinterpret_cast<float*>(mine + CUT_N);
+    cv.leaf = mine + 7 * CUT_N;
+    cv.lbox = reinterpret_cast<float*>(mine + 7 * CUT_N + CUT_LEAF);
+    cv.n = -1;
+    cv.nleaf = 0;
+    uint32_t visits = 0, tests = 0, rays = 0;
+    if (hi.x - lo.x <= a.cut_span && hi.y - lo.y <= a.cut_span && hi.z - lo.z <= a.cut_span) {
+        uint32_t steps = 0;
+        wave_cut(a.nodes, a.top, *a.ntop, lo, hi, a.cut_small, stk + (threadIdx.x & ~63u), cv, steps);
+        if (COUNT) visits += steps;
+    }
+    const char* const base = reinterpret_cast<const char*>(a.nodes);
+    const float reach = a.radius * kReach;
+    const int ncut = cv.n;
+    Ray r;
+    uint32_t ox = 0, oy = 0, oz = 0;
+    uint32_t spill[STACK_SPILL];
+    int sp = 0, pk = 0, ci = 0;
+    int32_t node = -1;
+    uint32_t pend = 0, lp = 0, work = 0;
+    int4 leafref = make_int4(0, 0, 0, 0);
+    const auto next_cut = [&]() -> int32_t {
+        while (ci < ncut) {
+            const int k = ci++;
+            if (CutView::enters(r, cv.box, CUT_N, k, reach, kExit)) return int32_t(cv.id[k]);
+        }
+        return -1;
+    };
+    // ray ri of the wave: pixel pix0 + ri / 64, sample ri % 64; false for an uncovered pixel
+    const auto setup = [&](uint32_t ri) -> bool {
+        pk = int(ri >> 6);
+        const uint32_t pix = pix0 + (ri >> 6), s = ri & 63u;
+        if (pix >= a.npix || !a.mask[pix]) return false;
+        const V3 p = v3(a.pos[3 * size_t(pix)], a.pos[3 * size_t(pix) + 1], a.pos[3 * size_t(pix) + 2]);
+        const V3 n = v3(a.nrm[3 * size_t(pix)], a.nrm[3 * size_t(pix) + 1], a.nrm[3 * size_t(pix) + 2]);
+        const uint32_t y = pix / a.w, x = pix - y * a.w;
+        V3 t, b;
+        basis(n, t, b);
+        const V3 origin = add(p, scl(n, 1e-3f));
+        const uint32_t k1 = hash24(x, y, 2 * s), k2 = hash24(x, y, 2 * s + 1);
+        const float u1 = float(k1) / float(0x1000000), u2 = float(k2) / float(0x1000000);
+        float sphi, cphi;
+        sincos_base(6.2831853f * u2, sphi, cphi);
+        if (a.trig) trig_fix(a.trig, k2, sphi, cphi);
+        r = make_ray(origin, ao_dir(u1, sphi, cphi, n, t, b));
+        ox = r.inv.x >= 0.0f ? 0u : 48u;
+        oy = r.inv.y >= 0.0f ? 0u : 48u;
+        oz = r.inv.z >= 0.0f ? 0u : 48u;
+        sp = 0;
+        pend = 0;
+        work = 0;
+        ci = 0;
+        lp = 0;
+        for (int i = 0; i < cv.nleaf; ++i)   // nleaf is 0 when the cut overflowed
+            if (cv.leaf[i] < a.ntri && CutView::enters(r, cv.lbox, CUT_LEAF, i, reach, kExit)) lp |= 1u << i;
+        node = ncut < 0 ? 0 : next_cut();   // the root, or the first cut node the ray enters
+        if (COUNT) ++rays;
+        return true;
+    };
+    const auto count_open = [&]() {
+        for (int k = 0; k < PW; ++k) cnt[k] += pk == k ? 1u : 0u;
+    };
+    const uint32_t total = PW * 64u;
+    bool live = setup(lane);
+    uint32_t next = 64;
+    while (true) {
+        if (live && pend == 0 && node >= 0) {
+            const uint32_t off = uint32_t(node) * uint32_t(sizeof(Node4));
+            if (COUNT) ++visits;
+            const uint32_t hit = slab4s(r, base, off, ox, oy, oz, reach, kExit);
+            const int4 ref = *reinterpret_cast<const int4*>(base + off + 96);
+            const uint32_t masks = *reinterpret_cast<const uint32_t*>(base + off + 112);
+            pend = hit & masks & 15u;
+            leafref = ref;
+            bool overflow = false;
+            uint32_t ih = hit & (masks >> 4);
+            if (ih) {
+                node = ref_of(ref, __builtin_ctz(ih));
+                ih &= ih - 1u;
+                while (ih) {
+                    const uint32_t v = uint32_t(ref_of(ref, __builtin_ctz(ih)));
+                    ih &= ih - 1u;
+                    if (sp < STACK_LDS) lds_col[sp * BT] = v;
+                    else if (sp < STACK_LDS + STACK_SPILL) spill[sp - STACK_LDS] = v;
+                    else overflow = true;
+                    ++sp;
+                }
+            } else if (sp > 0) {
+                --sp;
+                node = int32_t(sp < STACK_LDS ? lds_col[sp * BT] : spill[sp - STACK_LDS]);
+            } else {
+                node = next_cut();
+            }
+            if (++work > a.ntri + 4) overflow = true;
+            if (overflow) {   // answered by the scan of every triangle
+                if (!scan_occluded(r, a.radius, a.tris, a.ntri)) count_open();
+                live = false;
+                pend = 0;
+                lp = 0;
+            }
+        }
+        const bool waiting = live && (pend != 0 || lp != 0);
+        const bool can_step = live && pend == 0 && node >= 0;
+        const unsigned long long wb = __ballot(waiting), sb = __ballot(can_step), lb = __ballot(live);
+        if (lb != 0 && (sb == 0 || 64u * uint32_t(__popcll(wb)) >= a.tri_q * uint32_t(__popcll(lb)))) {
+            if (waiting) {
+                uint32_t slot;
+                if (pend) {
+                    const int k = __builtin_ctz(pend);
+                    pend &= pend - 1u;
+                    slot = uint32_t(~ref_of(leafref, k));
+                } else {
+                    slot = cv.leaf[__builtin_ctz(lp)];
+                    lp &= lp - 1u;
+                }
+                float t;
+                if (COUNT) ++tests;
+                if (slot < a.ntri && tri_hit(r, a.tris, slot, t) && t < a.radius) {
+                    live = false;   // occluded
+                    pend = 0;
+                    lp = 0;
+                }
+            }
+        }
+        if (live && pend == 0 && lp == 0 && node < 0) {   // finished: open
+            count_open();
+            live = false;
+        }
+        const unsigned long long idle = __ballot(!live);
+        const uint32_t nidle = uint32_t(__popcll(idle));
+        if (next < total && (nidle >= refill_min || nidle == 64u)) {
+            if (!live) {
+                const uint32_t ri = next + uint32_t(__popcll(idle & below));
+                if (ri < total) live = setup(ri);
+            }
+            next += nidle;
+        }
+        if (__ballot(live) == 0 && next >= total) break;
+    }
+    if (COUNT) {
+        atomicAdd(&a.counters->visits, (unsigned long long)visits);
+        atomicAdd(&a.counters->tests, (unsigned long long)tests);
+        atomicAdd(&a.counters->rays, (unsigned long long)rays);
+    }
+    finish();
+}
+
 // Lighting model 2's visibility (computeRTVisibility, src/renderer/ray_ao.cpp:20-63): the AO
 // rays of k_trace_ao4, and each open one traced again with tMax 1e30 for the sky.
 template <bool SPP64>
@@ -1414,8 +1613,11 @@ int trace_ao4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_m
     const uint32_t blocks = cdiv(uint64_t(npix) * uint64_t(spp), BT);
     const bool count = c.count_next;
     (void)hipEventRecord(c.ev[2], s);
-    if (spp64 && !count && c.refill > 0) {
-        constexpr int PW = 4;
+    constexpr int PW = 4;
+    if (spp64 && c.refill > 0 && c.wave_cut) {
+        if (count) k_trace_ao4rc<PW, true><<<cdiv(npix, uint32_t(PW * (BT / 64))), BT, 0, s>>>(a, uint32_t(c.refill));
+        else k_trace_ao4rc<PW, false><<<cdiv(npix, uint32_t(PW * (BT / 64))), BT, 0, s>>>(a, uint32_t(c.refill));
+    } else if (spp64 && !count && c.refill > 0) {
         k_trace_ao4r<PW><<<cdiv(npix, uint32_t(PW * (BT / 64))), BT, 0, s>>>(a, uint32_t(c.refill));
     } else if (spp64) {
         if (count) {

@@ -290,15 +290,18 @@ def test_ploc_tail_threshold_does_not_change_the_tree(ctx):
         assert (root, rounds) == trees[0][2:]
 
 
-@pytest.mark.parametrize("refill", [1, 24, 64])
-def test_refill_kernel_matches(ctx, refill):
-    """The 64-spp refill kernel (a wave owns 4 pixels' rays and refills idle lanes) gives the
-    one-pixel-a-wave kernel's AO bit for bit: on the 1M-triangle scene (odd width, so a wave's
-    pixels cross rows) and on the golden default frame."""
+@pytest.mark.parametrize("refill,cut", [(1, 2), (24, 2), (64, 2), (24, 0), (64, 0)])
+def test_refill_kernel_matches(ctx, refill, cut):
+    """The 64-spp refill kernel (a wave owns 4 pixels' rays and refills idle lanes; from the
+    wave's cut of their union box, or from the root) gives the one-pixel-a-wave kernel's AO bit
+    for bit: on the 1M-triangle scene (odd width, so a wave's pixels cross rows) and on the
+    golden default frame."""
     tris, pos, nrm, mask = big_frame(243, 131, seed=3)
     ctx.build_ploc(tris, 16)
+    ctx.set_option(raw_ao.OPT_REFILL, 0)
     base = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
     ctx.set_option(raw_ao.OPT_REFILL, refill)
+    ctx.set_option(raw_ao.OPT_WAVE_CUT, cut)
     try:
         got = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
         assert np.array_equal(bits(got), bits(base))
@@ -311,7 +314,8 @@ def test_refill_kernel_matches(ctx, refill):
         ctx.build_ploc(np.zeros((0, 9), np.float32), 16)
         assert np.all(ctx.compute_rtao(pos2, nrm2, mask2, 64, 2.0) == 1.0)
     finally:
-        ctx.set_option(raw_ao.OPT_REFILL, 0)
+        ctx.set_option(raw_ao.OPT_REFILL, REFILL_DEFAULT)
+        ctx.set_option(raw_ao.OPT_WAVE_CUT, WAVE_CUT_DEFAULT)
 
 
 def test_near_first_order_does_not_change_ao(ctx):
@@ -321,17 +325,20 @@ def test_near_first_order_does_not_change_ao(ctx):
     ctx.build_ploc(tris, 16)
     outs = []
     try:
-        ctx.set_option(raw_ao.OPT_WAVE_CUT, 0)   # the child order applies to the traversal from the root
+        ctx.set_option(raw_ao.OPT_WAVE_CUT, 0)   # the child order applies to the one-pixel kernel from the root
+        ctx.set_option(raw_ao.OPT_REFILL, 0)
         for nf in (1, 0):
             ctx.set_option(raw_ao.OPT_NEAR_FIRST, nf)
             outs.append(ctx.compute_rtao(pos, nrm, mask, 64, 1.0))
     finally:
         ctx.set_option(raw_ao.OPT_NEAR_FIRST, 1)
         ctx.set_option(raw_ao.OPT_WAVE_CUT, WAVE_CUT_DEFAULT)
+        ctx.set_option(raw_ao.OPT_REFILL, REFILL_DEFAULT)
     assert np.array_equal(bits(outs[0]), bits(outs[1]))
 
 
 WAVE_CUT_DEFAULT = 2
+REFILL_DEFAULT = 48
 
 
 @pytest.mark.parametrize("k,spp", [(1, 64), (2, 64), (8, 64), (64, 64), (2, 4), (2, 33), (8, 1)])
