@@ -251,12 +251,12 @@ __device__ __forceinline__ float4 ld4(const char* __restrict__ base, uint32_t of
 // base: the node array; off: the node's byte offset; ox/oy/oz: 0 or 48 (byte offset of
 // the far planes' float4 from the near ones' when 1/d < 0).
 __device__ __forceinline__ uint32_t slab4s(const Ray& r, const char* __restrict__ base, uint32_t off, uint32_t ox, uint32_t oy,
-                                           uint32_t oz, float reach, float exit_scale) {
+                                           uint32_t oz, float reach, float exit_scale, float tlo = 0.0f) {
     const float4 nx = fma4(ld4(base, off + ox), r.inv.x, -r.oinv.x), fx = fma4(ld4(base, off + 48 - ox), r.inv.x, -r.oinv.x);
     const float4 ny = fma4(ld4(base, off + 16 + oy), r.inv.y, -r.oinv.y), fy = fma4(ld4(base, off + 64 - oy), r.inv.y, -r.oinv.y);
     const float4 nz = fma4(ld4(base, off + 32 + oz), r.inv.z, -r.oinv.z), fz = fma4(ld4(base, off + 80 - oz), r.inv.z, -r.oinv.z);
     const auto one = [&](float a, float b, float c, float d, float e, float f) {
-        const float t0 = fmaxf(fmaxf(a, b), fmaxf(c, 0.0f));
+        const float t0 = fmaxf(fmaxf(a, b), fmaxf(c, tlo));
         const float t1 = fminf(fminf(d, e), fminf(f, reach));
         return t0 <= t1 * exit_scale;
     };
@@ -265,12 +265,12 @@ __device__ __forceinline__ uint32_t slab4s(const Ray& r, const char* __restrict_
 }
 // slab4s that also returns each slot's entry distance.
 __device__ __forceinline__ uint32_t slab4s_t(const Ray& r, const char* __restrict__ base, uint32_t off, uint32_t ox, uint32_t oy,
-                                             uint32_t oz, float reach, float exit_scale, float (&t0s)[4]) {
+                                             uint32_t oz, float reach, float exit_scale, float (&t0s)[4], float tlo = 0.0f) {
     const float4 nx = fma4(ld4(base, off + ox), r.inv.x, -r.oinv.x), fx = fma4(ld4(base, off + 48 - ox), r.inv.x, -r.oinv.x);
     const float4 ny = fma4(ld4(base, off + 16 + oy), r.inv.y, -r.oinv.y), fy = fma4(ld4(base, off + 64 - oy), r.inv.y, -r.oinv.y);
     const float4 nz = fma4(ld4(base, off + 32 + oz), r.inv.z, -r.oinv.z), fz = fma4(ld4(base, off + 80 - oz), r.inv.z, -r.oinv.z);
     const auto one = [&](float a, float b, float c, float d, float e, float f, float& t0) {
-        t0 = fmaxf(fmaxf(a, b), fmaxf(c, 0.0f));
+        t0 = fmaxf(fmaxf(a, b), fmaxf(c, tlo));
         const float t1 = fminf(fminf(d, e), fminf(f, reach));
         return t0 <= t1 * exit_scale;
     };
@@ -320,10 +320,11 @@ struct CutView {
 // cut (optional, the wave's shared cut of the tree, see wave_cut): instead of the root, the
 // traversal starts from the first cut node the ray enters and takes the next one whenever its
 // stack runs empty.
+// tlo: where the slab tests start along the ray (0, or sky_from for a sky ray).
 template <bool COUNT, bool NEAR = true>
 __device__ int occluded4(const Ray& r, float tmax, const Node4* __restrict__ nodes, const float4* __restrict__ ts,
                          uint32_t ntri, uint32_t* lds_col, uint32_t tri_q, uint32_t& visits, uint32_t& tests,
-                         bool active = true, const CutView* cut = nullptr) {
+                         bool active = true, const CutView* cut = nullptr, float tlo = 0.0f) {
     const float reach = tmax * kReach;
     const int ncut = cut ? cut->n : -1;
     int ci = 0;
@@ -353,7 +354,8 @@ __device__ int occluded4(const Ray& r, float tmax, const Node4* __restrict__ nod
             const uint32_t off = uint32_t(node) * uint32_t(sizeof(Node4));
             if (COUNT) ++visits;
             float t0s[4];
-            const uint32_t hit = NEAR ? slab4s_t(r, base, off, ox, oy, oz, reach, kExit, t0s) : slab4s(r, base, off, ox, oy, oz, reach, kExit);
+            const uint32_t hit = NEAR ? slab4s_t(r, base, off, ox, oy, oz, reach, kExit, t0s, tlo)
+                                      : slab4s(r, base, off, ox, oy, oz, reach, kExit, tlo);
             const int4 ref = *reinterpret_cast<const int4*>(base + off + 96);
             const uint32_t masks = *reinterpret_cast<const uint32_t*>(base + off + 112);   // leaf slots bits 0-3, inner 4-7
             pend = hit & masks & 15u;
@@ -421,6 +423,17 @@ __device__ int occluded4(const Ray& r, float tmax, const Node4* __restrict__ nod
         if (live && pend == 0 && node < 0) live = false;   // finished: open
     }
     return result;
+}
+
+// Where a sky ray's box tests start. An open AO ray has no triangle with t < radius, so every
+// triangle its sky ray (the same origin and direction) hits lies at t >= radius, inside each of
+// its ancestors' boxes, whose computed exit is then at least the radius less the slab test's
+// float rounding (a few ulp of the coordinates' magnitude). This bound stays 1% of the radius
+// plus 1e-4 of that magnitude below it, so boxes the ray leaves before it are skipped and the
+// answer is unchanged.
+__device__ __forceinline__ float sky_from(V3 o, float radius) {
+    const float big = fmaxf(fabsf(o.x), fmaxf(fabsf(o.y), fabsf(o.z)));
+    return fmaxf(0.0f, 0.99f * radius - 1e-4f * (big + radius));
 }
 
 __device__ bool scan_occluded(const Ray& r, float tmax, const float4* __restrict__ ts, uint32_t ntri) {
@@ -766,7 +779,7 @@ __device__ __forceinline__ void trace_ao4(const TraceArgs4& a, uint32_t* stk, ui
         constexpr float kFarT = 1e30f;
         const Ray r = make_ray(origin, dir);
         const int res2 = occluded4<false>(r, kFarT, a.nodes, a.tris, a.ntri, stk + threadIdx.x, a.tri_q, visits, tests,
-                                          live && !occ);
+                                          live && !occ, nullptr, sky_from(origin, a.radius));
         far_occ = occ || (res2 < 0 ? (live && scan_occluded(r, kFarT, a.tris, a.ntri)) : res2 == 1);
     }
     if (STAY && !live) return;   // past the wave's collective steps
@@ -1326,7 +1339,8 @@ __global__ __launch_bounds__(BT) void k_trace_vis4(TraceArgs4 a, float* __restri
         const int res = occluded4<false>(r, a.radius, a.nodes, a.tris, a.ntri, stk + threadIdx.x, a.tri_q, visits, tests);
         occ = res < 0 ? scan_occluded(r, a.radius, a.tris, a.ntri) : res == 1;
         // blocked within radius: blocked for the sky too
-        const int res2 = occluded4<false>(r, kFarT, a.nodes, a.tris, a.ntri, stk + threadIdx.x, a.tri_q, visits, tests, !occ);
+        const int res2 = occluded4<false>(r, kFarT, a.nodes, a.tris, a.ntri, stk + threadIdx.x, a.tri_q, visits, tests, !occ,
+                                          nullptr, sky_from(origin, a.radius));
         far_occ = occ || (res2 < 0 ? scan_occluded(r, kFarT, a.tris, a.ntri) : res2 == 1);
     }
     if (SPP64) {

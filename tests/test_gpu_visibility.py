@@ -85,3 +85,43 @@ def test_visibility_empty_scene(ctx):
     ctx.build_ploc(np.zeros((0, 9), np.float32))
     ao, sky, sun = ctx.compute_rt_visibility(pos, nrm, mask, DEFAULT_TO_LIGHT, 64, 2.0)
     assert (ao == 1).all() and (sky == 1).all() and (sun == 1).all()
+
+
+@pytest.mark.parametrize("offset,radius", [(0.0, 1.0), (30.0, 1.0), (1000.0, 0.5), (5.0, 0.01)])
+def test_visibility_blockers_just_past_radius(ctx, offset, radius):
+    """Sky rays skip the boxes they leave before the AO radius (an open AO ray has no triangle
+    nearer): a ceiling and a shell of small triangles just beyond the radius above a ground grid,
+    at coordinates near the origin and far from it, must block the sky exactly as the oracle says."""
+    print("visibility blockers past radius", offset, radius, flush=True)
+    rng = np.random.default_rng(11)
+    o = np.float32(offset)
+    w = h = 24
+    gx, gz = np.meshgrid(np.linspace(-1.5, 1.5, w, dtype=np.float32), np.linspace(-1.5, 1.5, h, dtype=np.float32))
+    pos = np.stack([gx + o, np.zeros_like(gx), gz + o], -1).astype(np.float32)
+    nrm = np.zeros_like(pos)
+    nrm[..., 1] = 1.0
+    mask = np.ones((h, w), np.uint8)
+    mask[0, :5] = 0
+    hc = np.float32(1e-3) + np.float32(radius) * np.float32(1.0005)   # ceiling: every hit at t > radius
+    s = np.float32(4.0 * max(radius, 0.1))
+    ceil = np.array([[[-s + o, hc, -s + o], [s + o, hc, -s + o], [s + o, hc, s + o]],
+                     [[-s + o, hc, -s + o], [s + o, hc, s + o], [-s + o, hc, s + o]]], np.float32)
+    ceil[:, :, 0] += np.float32(0.5 * s)   # an open side, so some sky rays pass
+    # small triangles in a shell just past the radius around the grid's centre
+    n = 400
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d[:, 1] = np.abs(d[:, 1])
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    c = d * np.float32(radius * 1.003) + np.array([o, 1e-3, o], np.float32)
+    e1 = rng.normal(size=(n, 3)).astype(np.float32) * np.float32(0.05 * radius)
+    e2 = rng.normal(size=(n, 3)).astype(np.float32) * np.float32(0.05 * radius)
+    shell = np.stack([c, c + e1, c + e2], 1).astype(np.float32)
+    ground = np.array([[[-50 + o, -1e-2, -50 + o], [50 + o, -1e-2, 50 + o], [50 + o, -1e-2, -50 + o]]], np.float32)
+    tris = np.ascontiguousarray(np.concatenate([ceil, shell, ground]).reshape(-1, 9))
+    ctx.build_ploc(tris)
+    got = ctx.compute_rt_visibility(pos, nrm, mask, (0.3, 1.0, 0.2), 64, radius)
+    want = O.rtvis(tris, pos, nrm, mask, (0.3, 1.0, 0.2), 64, radius)
+    for g, o_ in zip(got, want):
+        assert np.array_equal(bits(g), bits(o_))
+    sky, ao = want[1][mask != 0], want[0][mask != 0]
+    assert float(np.mean(sky)) < float(np.mean(ao))   # the blockers past the radius matter
