@@ -668,7 +668,7 @@ __device__ __forceinline__ float wave_max(float v) {
 // lanes' reach boxes (no walk when that union is wider than a.cut_span). stk: STACK_LDS x BT
 // words; cutmem: CUT_WAVE words a wave.
 // SKY (the visibility pass): each open AO ray is traced again to tMax 1e30 from the root (in
-// the same lane, right after), sky = the unblocked fraction (a.sky; counts in a.cnt_free below
+// the same lane, right after; its box tests start at sky_from), sky = the unblocked fraction (a.sky; counts in a.cnt_free below
 // 64 spp), 1 off the mask.
 template <bool SPP64, bool COUNT, bool NEAR, bool CUT, bool SKY = false>
 __device__ __forceinline__ void trace_ao4(const TraceArgs4& a, uint32_t* stk, uint32_t* cutmem) {
@@ -778,8 +778,9 @@ __device__ __forceinline__ void trace_ao4(const TraceArgs4& a, uint32_t* stk, ui
     if (SKY && a.ntri) {   // blocked within the radius: blocked for the sky too
         constexpr float kFarT = 1e30f;
         const Ray r = make_ray(origin, dir);
-        const int res2 = occluded4<false>(r, kFarT, a.nodes, a.tris, a.ntri, stk + threadIdx.x, a.tri_q, visits, tests,
-                                          live && !occ, nullptr, sky_from(origin, a.radius));
+        // children in slot order: faster here than nearest first (fewer registers, no sort)
+        const int res2 = occluded4<false, false>(r, kFarT, a.nodes, a.tris, a.ntri, stk + threadIdx.x, a.tri_q, visits,
+                                                 tests, live && !occ, nullptr, sky_from(origin, a.radius));
         far_occ = occ || (res2 < 0 ? (live && scan_occluded(r, kFarT, a.tris, a.ntri)) : res2 == 1);
     }
     if (STAY && !live) return;   // past the wave's collective steps
