@@ -52,30 +52,44 @@ __device__ bool finite(float x) { return (__float_as_uint(x) & 0x7f800000u) != 0
 // counts[0] += valid motion vectors, counts[1] += covered pixels (one atomic a wave).
 __global__ __launch_bounds__(FBT) void k_motion(const float* __restrict__ pos, const uint8_t* __restrict__ mask, uint32_t npix,
                                                 const Mat cur, const Mat prev, float* __restrict__ motion, uint32_t* __restrict__ counts) {
-    const uint32_t i = blockIdx.x * FBT + threadIdx.x;
-    bool covered = false, valid = false;
-    if (i < npix) {
+    __shared__ uint32_t red[2][FBT / 64];
+    uint32_t nv = 0, nc = 0;
+    for (size_t i = size_t(blockIdx.x) * FBT + threadIdx.x; i < npix; i += size_t(gridDim.x) * FBT) {
         float mx = 0.0f, my = 0.0f;
-        covered = mask[i] != 0;
+        const bool covered = mask[i] != 0;
         if (covered) {
-            const V3 p = v3(pos[3 * size_t(i)], pos[3 * size_t(i) + 1], pos[3 * size_t(i) + 2]);
+            const V3 p = v3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
             float cu, cv, pu, pv;
             if (project_uv(cur, p, cu, cv) && project_uv(prev, p, pu, pv)) {
                 const float dx = cu - pu, dy = cv - pv;
                 if (finite(dx) && finite(dy)) {
                     mx = dx;
                     my = dy;
-                    valid = true;
+                    ++nv;
                 }
             }
+            ++nc;
         }
-        motion[2 * size_t(i)] = mx;
-        motion[2 * size_t(i) + 1] = my;
+        motion[2 * i] = mx;
+        motion[2 * i + 1] = my;
     }
-    const unsigned long long bv = __ballot(valid), bc = __ballot(covered);
+    for (int o = 32; o > 0; o >>= 1) {
+        nv += __shfl_xor(nv, o);
+        nc += __shfl_xor(nc, o);
+    }
     if ((threadIdx.x & 63u) == 0u) {
-        if (bv) atomicAdd(&counts[0], uint32_t(__popcll(bv)));
-        if (bc) atomicAdd(&counts[1], uint32_t(__popcll(bc)));
+        red[0][threadIdx.x >> 6] = nv;
+        red[1][threadIdx.x >> 6] = nc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t v = 0, c = 0;
+        for (int k = 0; k < FBT / 64; ++k) {
+            v += red[0][k];
+            c += red[1][k];
+        }
+        if (v) atomicAdd(&counts[0], v);
+        if (c) atomicAdd(&counts[1], c);
     }
 }
 
@@ -89,9 +103,8 @@ __global__ __launch_bounds__(FBT) void k_shade(const float* __restrict__ nrm, co
                                                float lx, float ly, float lz, float li, float* __restrict__ frame,
                                                float* __restrict__ hdr, uint32_t* __restrict__ maxbits) {
     __shared__ uint32_t red[FBT / 64];
-    const uint32_t i = blockIdx.x * FBT + threadIdx.x;
     uint32_t mb = 0;
-    if (i < npix) {
+    for (size_t i = size_t(blockIdx.x) * FBT + threadIdx.x; i < npix; i += size_t(gridDim.x) * FBT) {
         float r[3] = {0.0f, 0.0f, 0.0f};
         if (mask[i]) {
             const float nx = nrm[3 * size_t(i)], ny = nrm[3 * size_t(i) + 1], nz = nrm[3 * size_t(i) + 2];
@@ -104,7 +117,7 @@ __global__ __launch_bounds__(FBT) void k_shade(const float* __restrict__ nrm, co
             // std::max(c.x, std::max(c.y, c.z)), then std::max(m, that) from m = 0
             const float yz = r[1] < r[2] ? r[2] : r[1];
             const float xyz = r[0] < yz ? yz : r[0];
-            if (0.0f < xyz) mb = __float_as_uint(xyz);
+            if (0.0f < xyz && __float_as_uint(xyz) > mb) mb = __float_as_uint(xyz);
         }
         for (int k = 0; k < 3; ++k) {
             hdr[3 * size_t(i) + k] = r[k];
@@ -130,6 +143,10 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
 }
 
 uint32_t blocks(size_t n) { return uint32_t(std::min<size_t>((n + FBT - 1) / FBT, 1u << 20)); }
+// The per-pixel passes that add counts or a maximum (k_shade, k_motion) run at most this many
+// blocks, each striding over the pixels and adding its result once: one atomic a wave on one
+// address over a 1080p frame (32k waves) serializes to ~0.6 ms.
+uint32_t reduce_blocks(size_t n) { return std::min<uint32_t>(blocks(n), 1024u); }
 
 }  // namespace
 
@@ -193,14 +210,14 @@ int render_frame(Ctx& c, const raw_ao_scene& sc, const raw_ao_render_opts& o, in
     }
     // shade + shade_hdr, maxRadiance in sc_dev[3]
     const float* plane_a = model == 2 ? d.sky : (o.rtao ? d.ao_rt : d.ao_ss);
-    k_shade<<<blocks(npix), FBT, 0, s>>>(d.normal, d.albedo, d.mask, npix, model, plane_a, d.sun, ldir[0], ldir[1], ldir[2], li,
+    k_shade<<<reduce_blocks(npix), FBT, 0, s>>>(d.normal, d.albedo, d.mask, npix, model, plane_a, d.sun, ldir[0], ldir[1], ldir[2], li,
                                          d.frame, d.hdr, sc_dev + 3);
     // motion against the previous view-projection (this frame's when none is given)
     Mat cur, prev;
     view_proj(sc.eye, sc.center, sc.up, sc.fovy, sc.aspect, sc.near_z, sc.far_z, cur.m);
     prev = cur;
     if (o.prev_viewproj) std::memcpy(prev.m, o.prev_viewproj, 64);
-    k_motion<<<blocks(npix), FBT, 0, s>>>(d.position, d.mask, npix, cur, prev, d.motion, sc_dev);
+    k_motion<<<reduce_blocks(npix), FBT, 0, s>>>(d.position, d.mask, npix, cur, prev, d.motion, sc_dev);
     (void)hipEventRecord(c.ev[9], s);
     if (!hip_ok(c, hipGetLastError(), "frame launch")) return -2;
 
