@@ -411,15 +411,18 @@ __global__ __launch_bounds__(TAIL_T) void k_ploc_tail(const PlocState* __restric
         // right candidate of i (kept in a register) and a left candidate of i + d (an LDS
         // atomic minimum). Keys are (area order, candidate position), so the minimum is the
         // smallest area at the lowest position: the reference's ascending scan with a strict
-        // `<` from FLT_MAX (areas that are not below FLT_MAX never count).
+        // `<` from FLT_MAX (areas that are not below FLT_MAX never count). In this step thread
+        // t takes positions t, t + TAIL_T, ..., so a wave's LDS reads and atomics are at
+        // consecutive positions (no bank conflicts); the steps after it, which scan in position
+        // order, read the result from snn.
         for (int k = 0; k < TAIL_PER; ++k) {
-            const int i = i0 + k;
+            const int i = int(threadIdx.x) + k * TAIL_T;
             if (i < int(m)) skey[i] = ~0ull;
         }
         __syncthreads();
         unsigned long long rbest[TAIL_PER];
         for (int k = 0; k < TAIL_PER; ++k) {
-            const int i = i0 + k;
+            const int i = int(threadIdx.x) + k * TAIL_T;
             rbest[k] = ~0ull;
             if (i >= int(m)) continue;
             const float al0 = sb[0][i], al1 = sb[1][i], al2 = sb[2][i], ah0 = sb[3][i], ah1 = sb[4][i], ah2 = sb[5][i];
@@ -436,7 +439,7 @@ __global__ __launch_bounds__(TAIL_T) void k_ploc_tail(const PlocState* __restric
         }
         __syncthreads();
         for (int k = 0; k < TAIL_PER; ++k) {
-            const int i = i0 + k;
+            const int i = int(threadIdx.x) + k * TAIL_T;
             if (i >= int(m)) break;
             const unsigned long long best = min(skey[i], rbest[k]);
             snn[i] = best == ~0ull ? int8_t(0) : int8_t(int(uint32_t(best)) - i);
