@@ -1532,6 +1532,13 @@ static int top_table(Ctx& c, hipStream_t s) {
     return hip_ok(c, hipGetLastError(), "wide build") ? 0 : -2;
 }
 
+namespace {
+__global__ void k_wide_init(WideState* st, uint32_t* frontier, uint32_t root) {
+    st[0] = WideState{1, 0, 1, 0};
+    frontier[0] = root;
+}
+}  // namespace
+
 int wide_build(Ctx& c, hipStream_t s) {
     const uint32_t n = c.ntri;
     if (n == 0) return 0;
@@ -1544,12 +1551,8 @@ int wide_build(Ctx& c, hipStream_t s) {
         c.wide_nodes = 1;
         return hip_ok(c, hipGetLastError(), "wide launch") ? top_table(c, s) : -2;
     }
-    WideState init{1, 0, 1, 0};
-    const uint32_t root = uint32_t(c.root);
-    if (!hip_ok(c, hipMemcpyAsync(c.wfront_a.p, &root, 4, hipMemcpyHostToDevice, s), "frontier") ||
-        !hip_ok(c, hipMemcpyAsync(c.wstate.p, &init, sizeof init, hipMemcpyHostToDevice, s), "wide state") ||
-        !hip_ok(c, hipStreamSynchronize(s), "wide init"))
-        return -2;
+    k_wide_init<<<1, 1, 0, s>>>(c.wstate.as<WideState>(), c.wfront_a.as<uint32_t>(), uint32_t(c.root));
+    if (!hip_ok(c, hipGetLastError(), "wide init")) return -2;
     WideState* d_st = c.wstate.as<WideState>();
     uint32_t *fin = c.wfront_a.as<uint32_t>(), *fout = c.wfront_b.as<uint32_t>();
     uint64_t ub = 1;   // upper bound of this level's size (grids are sized by it)
@@ -1577,13 +1580,6 @@ int wide_build(Ctx& c, hipStream_t s) {
     }
     return top_table(c, s);
 }
-
-namespace {
-__global__ void k_wide_init(WideState* st, uint32_t* frontier, uint32_t root) {
-    st[0] = WideState{1, 0, 1, 0};
-    frontier[0] = root;
-}
-}  // namespace
 
 // The collapse with the level grids of the last eager wide_build of this tree: no host
 // copies or waits, so a stream capture can record it.

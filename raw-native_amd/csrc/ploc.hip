@@ -519,6 +519,15 @@ uint32_t cdiv(uint64_t a, uint32_t b) { return uint32_t((a + b - 1) / b); }
 
 }  // namespace
 
+namespace {
+__global__ void k_ploc_init(PlocState* st, uint32_t n) {
+    PlocState init{};
+    init.m = n;
+    init.next_id = n;
+    st[0] = init;
+}
+}  // namespace
+
 int ploc_build(Ctx& c, const float* d_tris, uint32_t n, int radius, hipStream_t s) {
     c.ntri = n;
     c.radius = radius;
@@ -557,12 +566,8 @@ int ploc_build(Ctx& c, const float* d_tris, uint32_t n, int radius, hipStream_t 
     // surplus blocks find no positions and exit). Below TAIL_M clusters one workgroup
     // runs every remaining round.
     constexpr int kBatch = 6;
-    PlocState init{};
-    init.m = n;
-    init.next_id = n;
-    c.h_state[1] = init;
-    if (!hip_ok(c, hipMemcpyAsync(c.state.p, &c.h_state[1], sizeof init, hipMemcpyHostToDevice, s), "state upload")) return -2;
     PlocState* d_st = c.state.as<PlocState>();
+    k_ploc_init<<<1, 1, 0, s>>>(d_st, n);
     uint32_t m = n;
     int parity = 0;
     uint32_t *cin = c.clus_a.as<uint32_t>(), *cout = c.clus_b.as<uint32_t>();
@@ -599,14 +604,6 @@ int ploc_build(Ctx& c, const float* d_tris, uint32_t n, int radius, hipStream_t 
     return 0;
 }
 
-namespace {
-__global__ void k_ploc_init(PlocState* st, uint32_t n) {
-    PlocState init{};
-    init.m = n;
-    init.next_id = n;
-    st[0] = init;
-}
-}  // namespace
 
 // The build with the round grids of the last eager ploc_build of the same triangles: no
 // host copies or waits, so a stream capture can record it. PLOC is deterministic, so the
