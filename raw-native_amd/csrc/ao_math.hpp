@@ -125,6 +125,42 @@ AO_HD void trig_fix(const uint8_t* t, uint32_t k, float& s, float& c) {
     if ((word >> (k & 31u)) & 1u) trig_adjust((t[kTrigBitmapBytes + (k >> 1)] >> ((k & 1u) * 4)) & 15u, s, c);
 }
 
+// A float32 sin/cos of the same angles for callers that only round a product of them to an
+// integer (the SSAO taps): Cody-Waite reduction to |r| <= pi/4 with pi/2 in two float parts,
+// then Taylor polynomials (sin to r^9, cos to r^10), fused multiply-adds written out. Checked
+// against sincos_base and the host libm over all 2^24 angles (tests/cpp/fast_trig_check.cpp):
+// at most 6e-8 from either, inside kFastTrigErr.
+constexpr float kFastTrigErr = 4e-7f;
+AO_HD void sincos_fast(float phi, float& s, float& c) {
+    const int q = int(phi * 0.636619772f + 0.5f);   // nearest quadrant, phi >= 0
+    const float qf = float(q);
+    float r = fmaf(-qf, 1.57079637f, phi);
+    r = fmaf(-qf, -4.37113883e-8f, r);
+    const float r2 = r * r;
+    float ps = fmaf(r2, 2.75573192e-6f, -1.98412698e-4f);
+    ps = fmaf(r2, ps, 8.33333333e-3f);
+    ps = fmaf(r2, ps, -1.66666667e-1f);
+    const float sr = fmaf(r * r2, ps, r);
+    float pc = fmaf(r2, -2.75573192e-7f, 2.48015873e-5f);
+    pc = fmaf(r2, pc, -1.38888889e-3f);
+    pc = fmaf(r2, pc, 4.16666667e-2f);
+    pc = fmaf(r2, pc, -0.5f);
+    const float cr = fmaf(r2, pc, 1.0f);
+    switch (q & 3) {
+        case 0: s = sr; c = cr; break;
+        case 1: s = cr; c = -sr; break;
+        case 2: s = -sr; c = -cr; break;
+        default: s = -cr; c = sr; break;
+    }
+}
+// An SSAO tap offset (cos or sin) x rad, rad <= 6, computed from the fast pair differs from
+// the exact one by at most 6 x kFastTrigErr plus half an ulp of each product (2^-22 each
+// below 8): 2.9e-6. Where an offset is within this margin (over three times that) of a
+// half-integer, the tap takes the exact pair.
+constexpr float kSsaoTapMargin = 1e-5f;
+// True when v lies within m of a half-integer, where std::lround's result changes.
+AO_HD bool near_half(float v, float m) { return fabsf(v - (floorf(v) + 0.5f)) <= m; }
+
 // ---- watertight ray/triangle test (raw/math/primitives.hpp:17-64) ------------------
 // The part that depends only on the ray direction is computed once per ray.
 struct RayFrame {

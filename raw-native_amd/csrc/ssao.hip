@@ -35,11 +35,19 @@ __global__ __launch_bounds__(TX* TY) void k_ssao(const float* __restrict__ pos, 
             const uint32_t ka = hash24(uint32_t(x), uint32_t(y), uint32_t(s));
             const float a = (float(ka) / float(0x1000000)) * 6.2831853f;
             const float rad = (0.3f + 0.7f * hash01(uint32_t(x), uint32_t(y), uint32_t(s + 97))) * float(R);
+            // Only the rounded offsets are used, so a float32 sin/cos decides them unless an
+            // offset lies within kSsaoTapMargin of a half-integer; there the exact pair is computed.
             float sa, ca;
-            sincos_base(a, sa, ca);
-            if (trig) trig_fix(trig, ka, sa, ca);
-            const int sx = x + lround_f(ca * rad);
-            const int sy = y + lround_f(sa * rad);
+            sincos_fast(a, sa, ca);
+            float ox = ca * rad, oy = sa * rad;
+            if (near_half(ox, kSsaoTapMargin) || near_half(oy, kSsaoTapMargin)) {
+                sincos_base(a, sa, ca);
+                if (trig) trig_fix(trig, ka, sa, ca);
+                ox = ca * rad;
+                oy = sa * rad;
+            }
+            const int sx = x + lround_f(ox);
+            const int sy = y + lround_f(oy);
             inb[u] = s < samples && sx >= 0 && sy >= 0 && sx < w && sy < h;
             jj[u] = inb[u] ? size_t(sy) * size_t(w) + size_t(sx) : i;
         }

@@ -418,3 +418,18 @@ def test_timings_before_every_kernel_ran_leave_no_error():
         assert m["ssao"] == 0.0 and m["error"] == 0.0
         ao = c.compute_rtao(pos, nrm, mask, 64, 1.0)
         assert ao.shape == mask.shape
+
+
+def test_ssao_full_frame_matches_oracle(ctx):
+    """SSAO of the 1M-triangle scene's 1920x1080 G-buffer (about 38M taps, a few thousand of
+    them within the tap margin of a half-integer, where the exact sin/cos pair is taken)
+    against the oracle, bit for bit."""
+    import raw_ao.host as H
+    print("ssao full frame", flush=True)
+    vpos, vnrm, idx, _ = H.scene_seeded()
+    _, pos, nrm, mask = H.rasterize(vpos, vnrm, idx, H.orbit_camera(1), (0, 2, 0), (0, 1, 0), 0.9,
+                                    float(np.float32(1920) / np.float32(1080)), 0.1, 200.0, 1920, 1080)
+    got = ctx.compute_ssao(pos, nrm, mask, 24, 1.0)
+    want = O.ssao(pos, nrm, mask, 24, 1.0)
+    assert np.array_equal(bits(got), bits(want))
+    assert 0.0 < float(got[mask != 0].mean()) < 1.0
