@@ -278,32 +278,20 @@ def batched(raw_ao, torch, steps, wave_cut=-1):
     the autotuner picked the launch parameters."""
     import concurrent.futures as cf
     import numpy as np
-    rng = np.random.default_rng(0xBA7C)
     dev = torch.device("cuda:0")
-    base = raw_ao.host.SCENE_1M
-    tris_per_object = 0.385 * 1280 + 0.615 * 12
     b = raw_ao.Batch(0)
     if wave_cut >= 0:
         b.set_option(raw_ao.OPT_WAVE_CUT, wave_cut)
+    specs, views = raw_ao.host.batched_workload()
     meshes, ntris = [], []
-    for k in range(8):
-        target = float(np.exp(rng.uniform(np.log(5e4), np.log(2e6))))
-        spec = raw_ao.host.SceneSpec(0xBA7C00 + k, max(20, int(target / tris_per_object)), base.p_sphere, base.subdiv,
-                                     base.smin, base.smax, base.ground, base.height)
+    for spec in specs:
         vpos, vnrm, idx, _ = raw_ao.host.scene_seeded(spec)
         meshes.append((vpos, vnrm, idx))
         ntris.append(int(idx.shape[0]))
         b.add_scene(torch.from_numpy(raw_ao.host.mesh_tris(vpos, idx)).to(dev))
-    views = []
-    for v in range(64):
-        side = int(round(np.exp(rng.uniform(np.log(256), np.log(2048)))))
-        spp = int(round(np.exp(rng.uniform(np.log(16), np.log(128)))))
-        views.append((int(rng.integers(0, 8)), side, spp, raw_ao.host.orbit_camera(1000 + v)))
 
     def raster(vw):
-        sc, side, spp, eye = vw
-        vpos, vnrm, idx = meshes[sc]
-        return raw_ao.host.rasterize(vpos, vnrm, idx, eye, (0, 2, 0), (0, 1, 0), 0.9, 1.0, 0.1, 200.0, side, side)
+        return raw_ao.host.batched_gbuffer(meshes[vw[0]], vw)
 
     with cf.ThreadPoolExecutor(16) as pool:
         gbufs = list(pool.map(raster, views))

@@ -85,7 +85,11 @@ enum {
      * can reach and are no wider than n x radius, and every ray starts from those (n = 0: every
      * ray starts from the root; RAW_AO_OPT_NEAR_FIRST applies to that traversal only). Results
      * are the same for every value; only the speed differs. */
-    RAW_AO_OPT_WAVE_CUT = 8
+    RAW_AO_OPT_WAVE_CUT = 8,
+    /* 0 (default) or 1: the reference's skew control (RAW_NATIVE_AO_ORACLE=skew,
+     * src/renderer/accel.cpp:6-11,26-28): every AO query distance of raw_ao_trace_ao[_dev] is
+     * radius x 0.99f. A negative control for tests: the AO must then differ from the golden. */
+    RAW_AO_OPT_SKEW = 9
 };
 enum {
     RAW_AO_TRACE_WIDE = 0,     /* 4-wide nodes, LDS stack, triangle tests batched per wave */

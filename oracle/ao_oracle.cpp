@@ -828,10 +828,17 @@ int orc_bvh_occluded(void* bvh, const float* o, const float* d, float tmax) {
 void orc_rtao(void* bvh, const float* tris9, size_t ntri, int mode,
               const float* position, const float* normal, const uint8_t* mask,
               int w, int h, int y0, int y1, int samples, float radius, int threads, float* ao) {
+    orc_rtao_rows(bvh, tris9, ntri, mode, position, normal, mask, w, h, y0, y1, 1, samples, radius, threads, ao);
+}
+
+void orc_rtao_rows(void* bvh, const float* tris9, size_t ntri, int mode,
+                   const float* position, const float* normal, const uint8_t* mask,
+                   int w, int h, int y0, int y1, int ystep, int samples, float radius, int threads, float* ao) {
     const Bvh* B = static_cast<const Bvh*>(bvh);
-    const int nrows = y1 - y0;
+    if (ystep < 1) ystep = 1;
+    const int nrows = y1 > y0 ? (y1 - y0 + ystep - 1) / ystep : 0;
     rows(nrows, threads, [&](int r) {
-        const int y = y0 + r;
+        const int y = y0 + r * ystep;
         for (int x = 0; x < w; ++x) {
             const size_t i = size_t(y) * size_t(w) + size_t(x);
             if (!mask[i]) { ao[i] = 1.0f; continue; }

@@ -75,6 +75,11 @@ void orc_rtao(void* bvh, const float* tris9, size_t ntri, int mode,
               const float* position, const float* normal, const uint8_t* mask,
               int w, int h, int y0, int y1, int samples, float radius, int threads,
               float* ao);
+/* orc_rtao on rows y0, y0 + ystep, ... below y1 only (a bounded sample of a large frame). */
+void orc_rtao_rows(void* bvh, const float* tris9, size_t ntri, int mode,
+                   const float* position, const float* normal, const uint8_t* mask,
+                   int w, int h, int y0, int y1, int ystep, int samples, float radius, int threads,
+                   float* ao);
 
 /* computeRTVisibility (src/renderer/ray_ao.cpp:18-63), lighting model 2's visibility: ao
  * exactly as computeRTAO; sky = the fraction of the samples whose ray is open within radius

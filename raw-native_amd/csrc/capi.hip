@@ -130,6 +130,7 @@ int raw_ao_set_option(raw_ao_ctx* x, int option, int value) {
         return RAW_AO_OK;
     }
     if (option == RAW_AO_OPT_NEAR_FIRST) { c.near_first = value != 0; return RAW_AO_OK; }
+    if (option == RAW_AO_OPT_SKEW) { c.skew = value != 0; return RAW_AO_OK; }
     if (option == RAW_AO_OPT_WAVE_CUT) {
         if (value < 0 || value > 64) return fail(c, RAW_AO_ERR_ARG, "wave cut must be in 0..64");
         c.wave_cut = value;
@@ -253,6 +254,7 @@ int raw_ao_trace_ao_dev(raw_ao_ctx* x, const float* d_pos, const float* d_nrm, c
     }
     if (c.ntri && c.trace_kernel != c.built_kernel)
         return fail(c, RAW_AO_ERR_ARG, "the trace kernel option changed after the build: build again");
+    if (c.skew) radius *= 0.99f;   // accel.cpp:28, Mode::Skew
     if (c.trace_kernel == RAW_AO_TRACE_BINARY)
         return code_of(raw_ao::trace_ao(c, d_pos, d_nrm, d_mask, w, h, spp, radius, d_ao, pick(c, stream)));
     return code_of(raw_ao::trace_ao4(c, d_pos, d_nrm, d_mask, w, h, spp, radius, d_ao, pick(c, stream)));

@@ -46,6 +46,7 @@ struct Ctx {
     int refill{48};              // 64-spp wide kernel: refill lanes once this many are idle (0: off)
     int wave_cut{2};             // 64-spp wide kernel: > 0 starts rays from the wave's cut at wave_cut x radius
     bool near_first{true};       // 64-spp wide kernel: descend into the nearest hit child first
+    bool skew{false};            // the skew control: AO query distances x 0.99f
     bool count_next{false};
 
     // host-libm sin/cos correction table: a 2 MB bitmap of the corrected angle indices, then one nibble an index

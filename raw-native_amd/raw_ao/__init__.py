@@ -26,6 +26,7 @@ OK, ERR_ARG, ERR_DEVICE, ERR_BUILD, ERR_NO_DEVICE = 0, -1, -2, -3, -4
 OPT_HOST_LIBM_TRIG, OPT_TRACE_KERNEL, OPT_TRI_PHASE, OPT_CONV_KERNEL, OPT_PLOC_TAIL, OPT_REFILL = 1, 2, 3, 4, 5, 6
 OPT_NEAR_FIRST = 7
 OPT_WAVE_CUT = 8
+OPT_SKEW = 9
 TRACE_WIDE, TRACE_BINARY = 0, 1
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")

@@ -227,3 +227,33 @@ def cli_gbuffer(w=256, h=256, eye=(4, 4, 6), target=(0, 1, 0), fovy=0.9):
     vpos, vnrm, idx = test_scene_mesh()
     aspect = float(np.float32(w) / np.float32(h))
     return rasterize(vpos, vnrm, idx, eye, target, (0, 1, 0), fovy, aspect, 0.1, 100.0, w, h)
+
+
+def batched_workload(seed=0xBA7C):
+    """The batched uneven workload: 8 seeded scenes of the 1M-triangle scene's family with
+    50k-2M triangles (log-uniform), and 64 square views, sides log-uniform in [256, 2048] and
+    16-128 rays a pixel (log-uniform), each from a seeded orbit camera of one of the scenes.
+    Returns (specs, views) with views as (scene, side, spp, eye); build the meshes with
+    scene_seeded(spec) and the G-buffers with batched_gbuffer."""
+    rng = np.random.default_rng(seed)
+    base = SCENE_1M
+    tris_per_object = 0.385 * 1280 + 0.615 * 12
+    specs = []
+    for k in range(8):
+        target = float(np.exp(rng.uniform(np.log(5e4), np.log(2e6))))
+        specs.append(SceneSpec(0xBA7C00 + k, max(20, int(target / tris_per_object)), base.p_sphere, base.subdiv, base.smin,
+                               base.smax, base.ground, base.height))
+    views = []
+    for v in range(64):
+        side = int(round(np.exp(rng.uniform(np.log(256), np.log(2048)))))
+        spp = int(round(np.exp(rng.uniform(np.log(16), np.log(128)))))
+        views.append((int(rng.integers(0, 8)), side, spp, orbit_camera(1000 + v)))
+    return specs, views
+
+
+def batched_gbuffer(mesh, view):
+    """One view's G-buffer (the host rasterizer; fovy 0.9, square, near 0.1, far 200, looking at
+    (0, 2, 0)): (depth, position, normal, mask)."""
+    vpos, vnrm, idx = mesh[:3]
+    _, side, _, eye = view
+    return rasterize(vpos, vnrm, idx, eye, (0, 2, 0), (0, 1, 0), 0.9, 1.0, 0.1, 200.0, side, side)

@@ -6,6 +6,9 @@ import ctypes as C
 import os
 import numpy as np
 
+# the host threads the oracle uses: the machine's cores, at most 16 (a GPU box's CPU share)
+NTHREADS = max(1, min(16, os.cpu_count() or 1))
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "build", "liboracle_ao.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref_ao.so")
@@ -44,6 +47,8 @@ def lib():
         L.orc_bvh_occluded.restype = C.c_int
         L.orc_rtao.argtypes = [C.c_void_p, f32p, SZ, C.c_int, f32p, f32p, u8p, C.c_int, C.c_int, C.c_int, C.c_int,
                                C.c_int, C.c_float, C.c_int, f32p]
+        L.orc_rtao_rows.argtypes = [C.c_void_p, f32p, SZ, C.c_int, f32p, f32p, u8p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_int, C.c_int, C.c_float, C.c_int, f32p]
         L.orc_ssao.argtypes = [f32p, f32p, u8p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, f32p]
         L.orc_rtvis.argtypes = [C.c_void_p, f32p, SZ, C.c_int, f32p, f32p, u8p, C.c_int, C.c_int, C.c_int, C.c_float,
                                 f32p, C.c_int, C.c_int, f32p, f32p, f32p]
@@ -160,13 +165,15 @@ class OracleBvh:
             lib().orc_bvh_free(self.h)
             self.h = None
 
-    def rtao(self, pos, nrm, mask, samples, radius, mode=0, threads=None, rows=None):
+    def rtao(self, pos, nrm, mask, samples, radius, mode=0, threads=None, rows=None, row_step=1):
+        """computeRTAO on rows y0, y0 + row_step, ... below y1 (rows=(y0, y1); all rows by
+        default); the other rows are left at 1.0."""
         h, w = mask.shape
         y0, y1 = rows if rows is not None else (0, h)
         ao = np.ones((h, w), np.float32)
-        lib().orc_rtao(self.h, self.tris, self.tris.shape[0], mode, np.ascontiguousarray(pos, np.float32),
-                       np.ascontiguousarray(nrm, np.float32), np.ascontiguousarray(mask, np.uint8), w, h, y0, y1,
-                       samples, radius, threads or os.cpu_count() or 1, ao)
+        lib().orc_rtao_rows(self.h, self.tris, self.tris.shape[0], mode, np.ascontiguousarray(pos, np.float32),
+                            np.ascontiguousarray(nrm, np.float32), np.ascontiguousarray(mask, np.uint8), w, h, y0, y1,
+                            row_step, samples, radius, threads or NTHREADS, ao)
         return ao
 
 
@@ -176,7 +183,7 @@ def rtvis(tris9, pos, nrm, mask, to_light, samples=64, radius=2.0, sun_forced=Fa
     h, w = mask.shape
     out = [np.zeros((h, w), np.float32) for _ in range(3)]
     args = (np.ascontiguousarray(pos, np.float32), np.ascontiguousarray(nrm, np.float32), np.ascontiguousarray(mask, np.uint8),
-            w, h, samples, radius, np.asarray(to_light, np.float32), int(bool(sun_forced)), threads or os.cpu_count() or 1)
+            w, h, samples, radius, np.asarray(to_light, np.float32), int(bool(sun_forced)), threads or NTHREADS)
     if impl is not None:
         impl.ref_rtvis(tris9, tris9.shape[0], *args, *out)
     else:
@@ -193,7 +200,7 @@ def ssao(pos, nrm, mask, samples=24, radius=2.0, threads=None):
     h, w = mask.shape
     ao = np.zeros((h, w), np.float32)
     lib().orc_ssao(np.ascontiguousarray(pos, np.float32), np.ascontiguousarray(nrm, np.float32),
-                   np.ascontiguousarray(mask, np.uint8), w, h, samples, radius, threads or os.cpu_count() or 1, ao)
+                   np.ascontiguousarray(mask, np.uint8), w, h, samples, radius, threads or NTHREADS, ao)
     return ao
 
 
@@ -254,7 +261,7 @@ def scan_rays(tris9, rays6, tmax, any_hit, threads=None):
     t = np.zeros(n, np.float32)
     tri = np.zeros(n, np.int32)
     lib().orc_scan_rays(tris9, tris9.shape[0], rays6, np.ascontiguousarray(tmax, np.float32), n, int(any_hit),
-                        threads or os.cpu_count() or 1, t, tri)
+                        threads or NTHREADS, t, tri)
     return t, tri
 
 
@@ -386,7 +393,7 @@ def ref_frame(sc, w, h, prev=None, lighting=2, no_sun=False, rtao_on=True, tol=0
                            np.ascontiguousarray(sc["albedo"], np.float32), len(sc["mesh_end"]), sc["eye"], sc["center"],
                            sc["up"], sc["fovy"], sc["aspect"], sc["near"], sc["far"], int(bool(sc["has_light"])),
                            np.asarray(sc["light_dir"], np.float32), float(sc["light_intensity"]), w, h, prev_p, lighting,
-                           int(bool(no_sun)), int(bool(rtao_on)), tol, threads or os.cpu_count() or 1, depth, pos, nrm,
+                           int(bool(no_sun)), int(bool(rtao_on)), tol, threads or NTHREADS, depth, pos, nrm,
                            alb, mask, motion, ao_rt, ao_ss, em, fr, hdr, st, vp)
     out = dict(depth=depth, position=pos, normal=nrm, albedo=alb, mask=mask, motion=motion, ao_ss=ao_ss, error=em,
                frame=fr, hdr=hdr, rmse=float(st[0]), max_error=float(st[1]), pixels=int(st[2]), within=bool(st[3]),
@@ -431,7 +438,7 @@ def scan_hits(tris9, rays8, threads=None):
     n = rays8.shape[0]
     hits = np.zeros(n, HIT)
     t2 = np.zeros(n, np.float32)
-    lib().orc_scan_hits(tris9, tris9.shape[0], rays8, n, threads or os.cpu_count() or 1, hits.view(np.float32), t2.ctypes.data)
+    lib().orc_scan_hits(tris9, tris9.shape[0], rays8, n, threads or NTHREADS, hits.view(np.float32), t2.ctypes.data)
     return hits, t2
 
 
@@ -444,5 +451,5 @@ def tree_closest(boxes, links, root, tris9, rays8, threads=None, impl=None):
         impl.ref_tree_closest(tris9, tris9.shape[0], rays8, rays8.shape[0], hits.view(np.float32))
     else:
         lib().orc_tree_closest(np.ascontiguousarray(boxes, np.float32), np.ascontiguousarray(links, np.int32), root, tris9,
-                               rays8, rays8.shape[0], threads or os.cpu_count() or 1, hits.view(np.float32))
+                               rays8, rays8.shape[0], threads or NTHREADS, hits.view(np.float32))
     return hits
