@@ -125,6 +125,17 @@ int raw_ao_trace_ao(raw_ao_ctx* ctx, const float* position, const float* normal,
 int raw_ao_trace_ao_dev(raw_ao_ctx* ctx, const float* d_position, const float* d_normal, const uint8_t* d_mask,
                         int w, int h, int spp, float radius, float* d_ao, void* stream);
 
+/* No reference counterpart: raw_ao_trace_ao_dev for callers that cannot allocate device memory
+ * (code under HIP stream capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments and results, bit for bit; it calls no hipMalloc, hipFree or allocating library
+ * routine. Its scratch (trace counters, per-pixel open counts below 64 spp) is one library-wide
+ * array per device, so w * h <= RAW_AO_NOALLOC_MAX_PIXELS (larger is RAW_AO_ERR_ARG). Needs the
+ * scene built with the wide traversal form (the build sets up everything else it reads). Calls
+ * that overlap on one device (several streams or host threads) race on that array. */
+#define RAW_AO_NOALLOC_MAX_PIXELS (1u << 22)
+int raw_ao_trace_ao_dev_noalloc(raw_ao_ctx* ctx, const float* d_position, const float* d_normal, const uint8_t* d_mask,
+                                int w, int h, int spp, float radius, float* d_ao, void* stream);
+
 /* ---- lighting model 2's visibility ---------------------------------------------------
  * Replaces computeRTVisibility (raw/renderer/ray_ao.hpp:12-14, src/renderer/ray_ao.cpp:18-63),
  * the pass the default render runs (src/renderer/render.cpp:31-36): ao exactly as

@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 BUILD = os.path.join(HERE, "build")
 HIP_SOURCES = ["csrc/capi.hip", "csrc/ploc.hip", "csrc/trace.hip", "csrc/bvh4.hip", "csrc/ssao.hip", "csrc/unet.hip", "csrc/batch.hip", "csrc/raster.hip", "csrc/frame.hip",
-               "csrc/trig_table.cpp"]
+               "csrc/noalloc.hip", "csrc/trig_table.cpp"]
 ARCH = os.environ.get("RAW_AO_ARCH", "gfx950")
 
 HIPCC_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",

@@ -1606,24 +1606,39 @@ int wide_build_fixed(Ctx& c, hipStream_t s) {
     return top_table(c, s);
 }
 
+namespace {
+// Zeroes n words with a kernel (a stream capture records it as a kernel node, not a memset node).
+__global__ __launch_bounds__(BT) void k_zero_u32(uint32_t* __restrict__ p, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * BT + threadIdx.x; i < n; i += size_t(gridDim.x) * BT) p[i] = 0u;
+}
+}  // namespace
+
 int trace_ao4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int spp, float radius,
               float* d_ao, hipStream_t s) {
     const uint32_t npix = uint32_t(w) * uint32_t(h);
     if (!ensure(c, c.counters, sizeof(TraceCounters))) return -2;
-    if (!hip_ok(c, hipMemsetAsync(c.counters.p, 0, sizeof(TraceCounters), s), "counter clear")) return -2;
+    if (spp != 64 && !ensure(c, c.st_cnt, 4 * size_t(npix))) return -2;
+    return trace_ao4_into(c, d_pos, d_nrm, d_mask, w, h, spp, radius, d_ao, c.counters.as<TraceCounters>(),
+                          spp != 64 ? c.st_cnt.as<uint32_t>() : nullptr, true, s);
+}
+
+int trace_ao4_into(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int spp, float radius,
+                   float* d_ao, TraceCounters* d_counters, uint32_t* d_cnt, bool timed, hipStream_t s) {
+    const uint32_t npix = uint32_t(w) * uint32_t(h);
+    const bool spp64 = spp == 64;
+    if (!spp64 && !d_cnt) { c.err = "no per-pixel count buffer"; return -1; }
+    k_zero_u32<<<1, BT, 0, s>>>(reinterpret_cast<uint32_t*>(d_counters), sizeof(TraceCounters) / 4);
     TraceArgs4 a{d_pos, d_nrm, d_mask, d_ao, nullptr, c.nodes4.as<Node4>(), c.tris_sorted.as<float4>(),
-                 c.host_trig ? c.d_trig : nullptr, c.counters.as<TraceCounters>(), uint32_t(w), npix, uint32_t(spp),
+                 c.host_trig ? c.d_trig : nullptr, d_counters, uint32_t(w), npix, uint32_t(spp),
                  c.ntri, uint32_t(c.tri_q), radius, float(c.wave_cut) * radius, 4.0f * radius, c.top.as<TopEntry>(),
                  c.top.p ? c.top.as<uint32_t>() + TOP_MAX * (sizeof(TopEntry) / 4) : nullptr};
-    const bool spp64 = spp == 64;
     if (!spp64) {
-        if (!ensure(c, c.st_cnt, 4 * size_t(npix))) return -2;
-        a.cnt = c.st_cnt.as<uint32_t>();
-        if (!hip_ok(c, hipMemsetAsync(a.cnt, 0, 4 * size_t(npix), s), "count clear")) return -2;
+        a.cnt = d_cnt;
+        k_zero_u32<<<std::min<uint32_t>(2048, cdiv(npix, BT)), BT, 0, s>>>(a.cnt, npix);
     }
     const uint32_t blocks = cdiv(uint64_t(npix) * uint64_t(spp), BT);
-    const bool count = c.count_next;
-    (void)hipEventRecord(c.ev[2], s);
+    const bool count = c.count_next && timed;
+    if (timed) (void)hipEventRecord(c.ev[2], s);
     constexpr int PW = 4;
     if (spp64 && c.refill > 0 && c.wave_cut) {
         if (count) k_trace_ao4rc<PW, true><<<cdiv(npix, uint32_t(PW * (BT / 64))), BT, 0, s>>>(a, uint32_t(c.refill));
@@ -1652,10 +1667,10 @@ int trace_ao4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_m
             k_trace_ao4<false, false><<<blocks, BT, 0, s>>>(a);
         }
     }
-    (void)hipEventRecord(c.ev[3], s);
+    if (timed) (void)hipEventRecord(c.ev[3], s);
     if (!spp64) ao_finish(c, d_mask, a.cnt, npix, uint32_t(spp), d_ao, s);
     if (!hip_ok(c, hipGetLastError(), "trace launch")) return -2;
-    c.have_counts = count;
+    if (timed) c.have_counts = count;
     return 0;
 }
 

@@ -90,7 +90,7 @@ int raw_ao_open(int device_ordinal, raw_ao_ctx** out) {
         return RAW_AO_ERR_DEVICE;
     for (auto& e : c.ev)
         if (!hip_ok(c, hipEventCreate(&e), "hipEventCreate")) return RAW_AO_ERR_DEVICE;
-    return RAW_AO_OK;
+    return raw_ao::static_scratch_setup(c);
 }
 
 void raw_ao_close(raw_ao_ctx* x) {
@@ -166,6 +166,10 @@ int raw_ao_build_ploc_dev(raw_ao_ctx* x, const float* d_tris9, uint32_t ntri, in
     if (ntri && !d_tris9) return fail(c, RAW_AO_ERR_ARG, "null triangles");
     if (ntri > RAW_AO_MAX_TRIANGLES) return fail(c, RAW_AO_ERR_ARG, "more than RAW_AO_MAX_TRIANGLES triangles");
     hipStream_t s = pick(c, stream);
+    if (c.host_trig) {   // the traces' angle table, set up here so raw_ao_trace_ao_dev_noalloc allocates nothing
+        const int r = raw_ao::trig_table_upload(c);
+        if (r) return r;
+    }
     if (ntri) {
         if (!ensure(c, c.tris_in, 36 * size_t(ntri))) return RAW_AO_ERR_DEVICE;
         if (d_tris9 != c.tris_in.p &&

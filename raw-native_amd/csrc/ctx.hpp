@@ -125,6 +125,14 @@ int ploc_fixed_check(Ctx& c);
 int wide_build_fixed(Ctx& c, hipStream_t s);
 int trace_ao4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int spp,
               float radius, float* d_ao, hipStream_t s);
+// trace_ao4 with its scratch given: d_counters (cleared here), d_cnt (npix words, cleared here; only
+// read below 64 spp, may be null at 64). timed: record the context's events and honour
+// raw_ao_count_next_trace; otherwise neither, and nothing but kernels is enqueued.
+int trace_ao4_into(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int spp,
+                   float radius, float* d_ao, TraceCounters* d_counters, uint32_t* d_cnt, bool timed, hipStream_t s);
+// Looks up this device's copy of the no-allocation entry point's scratch array (once per device,
+// with the device current); RAW_AO_ERR_DEVICE when the lookup fails.
+int static_scratch_setup(Ctx& c);
 int trace_vis4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int spp, float radius,
                const float to_light[3], int sun_forced, float* d_ao, float* d_sky, float* d_sun, hipStream_t s);
 int trace_rays4(Ctx& c, const float* d_rays, uint32_t n, int any, float far_scale, float* d_out, hipStream_t s);

@@ -27,6 +27,7 @@ OPT_HOST_LIBM_TRIG, OPT_TRACE_KERNEL, OPT_TRI_PHASE, OPT_CONV_KERNEL, OPT_PLOC_T
 OPT_NEAR_FIRST = 7
 OPT_WAVE_CUT = 8
 OPT_SKEW = 9
+NOALLOC_MAX_PIXELS = 1 << 22
 TRACE_WIDE, TRACE_BINARY = 0, 1
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
@@ -51,6 +52,7 @@ SIGNATURES = [
     ("raw_ao_read_keys", C.c_int, [_vp, _u32p, _u32p]),
     ("raw_ao_trace_ao", C.c_int, [_vp, _f32p, _f32p, _u8p, C.c_int, C.c_int, C.c_int, C.c_float, _f32p]),
     ("raw_ao_trace_ao_dev", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp]),
+    ("raw_ao_trace_ao_dev_noalloc", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _vp, _vp]),
     ("raw_ao_trace_visibility", C.c_int, [_vp, _f32p, _f32p, _u8p, C.c_int, C.c_int, C.c_int, C.c_float, _f32p, C.c_int,
                                           _f32p, _f32p, _f32p]),
     ("raw_ao_trace_visibility_dev", C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, _f32p, C.c_int,
@@ -273,6 +275,12 @@ class Context:
     def compute_rtao_dev(self, position, normal, mask, w, h, samples, radius, out, stream=None):
         self._check(lib().raw_ao_trace_ao_dev(self._h, _ptr(position), _ptr(normal), _ptr(mask), w, h, samples, radius,
                                               _ptr(out), _vp(stream)))
+
+    def compute_rtao_dev_noalloc(self, position, normal, mask, w, h, samples, radius, out, stream=None):
+        """compute_rtao_dev through the entry point that allocates nothing (one scratch array per
+        device, at most NOALLOC_MAX_PIXELS pixels)."""
+        self._check(lib().raw_ao_trace_ao_dev_noalloc(self._h, _ptr(position), _ptr(normal), _ptr(mask), w, h, samples,
+                                                      radius, _ptr(out), _vp(stream)))
 
     def compute_rt_visibility(self, position, normal, mask, to_light, samples=64, radius=2.0, sun_forced=False):
         """computeRTVisibility: (ao, sky, sun)."""
