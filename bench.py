@@ -11,9 +11,12 @@ frame's GPU work; every frame is complete inside the timed region).
 
 value = AO rays per second of whole frames (covered pixels x spp x steps / timed wall
 time). Also reported: the trace kernel's own ray rate, per-kernel milliseconds, the trace
-kernel's roofline from a counting run (node and triangle fetches), and the CPU baseline:
-the reference's own computeRTAO (oracle/_ref, built from its sources) on a row sample of
-the same frame, timed on this host's cores.
+kernel's roofline (its algorithmic FP32 work from a counting launch's node visits and
+triangle tests, against the FP32 vector peak, with the HBM and L2 rates and the VALU issue
+and lane figures of the committed counter runs beside it), and the CPU baseline: the
+reference's own code (oracle/_ref, built from its sources) timed on this host's cores by the
+reference's benchJson protocol (median of 5 after a warm-up): BVH build, computeRTAO on a
+row sample of the same frame, computeSSAO and reconcile, and the CPU ms/frame they add to.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--calib-copy MiB]
 """
@@ -28,6 +31,7 @@ sys.path.insert(0, os.path.join(ROOT, "raw-native_amd"))
 
 METRIC = "AO rays/s + ms/frame at 1080p×64spp; learned-AO ms/frame at RMSE≤0.02 vs RT"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 
 
 def log(msg):
@@ -35,39 +39,74 @@ def log(msg):
     sys.stderr.flush()
 
 
-def cpu_baseline(tris, pos, nrm, mask, spp, radius, row_step):
-    """The reference's computeRTAO (its own sources, oracle/_ref) on every row_step-th row."""
+def cpu_baseline(tris, pos, nrm, mask, spp, radius, row_step, runs=5):
+    """The CPU baseline, timed on this host as the reference's own benchJson times a frame
+    (src/tools/run.cpp:65-87): each part once untimed, then the median of `runs` timed runs
+    (time.perf_counter, a steady clock). With oracle/_ref built, the reference's own code from
+    its sources ("reference"): LinearAccel::build (its binned-SAH BVH, one thread as the reference
+    builds it), computeRTAO on every row_step-th row of the frame (the bounded sample; its rate
+    scales to the whole frame), computeSSAO and reconcile on the whole frame, on `threads` host
+    threads. Without it, the oracle's restatement ("port")."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
     threads = max(1, min(threads, os.cpu_count() or 1))
-    sub = slice(0, pos.shape[0], row_step)
+    h, w = mask.shape
+    sub = slice(0, h, row_step)
     p2, n2, m2 = [np.ascontiguousarray(x[sub]) for x in (pos, nrm, mask)]
-    ao = np.zeros(m2.shape, np.float32)
+    ao_s = np.zeros(m2.shape, np.float32)
+    ao = np.ones((h, w), np.float32)
+    ss = np.zeros((h, w), np.float32)
+    em = np.zeros((h, w), np.float32)
+    rm, mx = O.C.c_float(), O.C.c_float()
+    px, wi = O.C.c_int32(), O.C.c_int32()
     if O.ref_available():
         R = O.ref()
         kind = "reference"
-        t0 = time.perf_counter()
-        acc = R.ref_accel_build(tris, tris.shape[0])
-        build_s = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        R.ref_rtao_accel(acc, p2, n2, m2, m2.shape[1], m2.shape[0], spp, radius, threads, ao)
-        trace_s = time.perf_counter() - t0
-        R.ref_accel_free(acc)
+        build = lambda: R.ref_accel_build(tris, tris.shape[0])
+        free = R.ref_accel_free
+        trace = lambda acc: R.ref_rtao_accel(acc, p2, n2, m2, w, m2.shape[0], spp, radius, threads, ao_s)
+        ssao = lambda: R.ref_ssao(pos, nrm, mask, w, h, 24, radius, threads, ss)
+        recon = lambda: R.ref_reconcile(ss, ao, mask, w, h, 0.12, em, O.C.byref(rm), O.C.byref(mx), O.C.byref(px),
+                                        O.C.byref(wi))
     else:
         kind = "port"
-        t0 = time.perf_counter()
-        b = O.OracleBvh(tris)
-        build_s = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        ao = b.rtao(p2, n2, m2, spp, radius, threads=threads)
-        trace_s = time.perf_counter() - t0
-    rays = int(m2.sum()) * spp
+        build = lambda: O.OracleBvh(tris)
+        free = lambda acc: None
+        trace = lambda acc: ao_s.__setitem__(slice(None), acc.rtao(p2, n2, m2, spp, radius, threads=threads))
+        ssao = lambda: ss.__setitem__(slice(None), O.ssao(pos, nrm, mask, 24, radius, threads=threads))
+        recon = lambda: O.reconcile(ss, ao, mask, 0.12)
+
+    def median_s(fn, pre=None):
+        ts = []
+        for k in range(runs + 1):
+            t0 = time.perf_counter()
+            r = fn()
+            dt = time.perf_counter() - t0
+            if pre is not None:
+                pre(r)
+            if k:
+                ts.append(dt)
+        return float(np.median(ts))
+
+    build_s = median_s(build, free)
+    acc = build()
+    trace_s = median_s(lambda: trace(acc))
+    free(acc)
+    ssao_s = median_s(ssao)
+    recon_s = median_s(recon)
+    rays = int(m2.astype(bool).sum()) * spp
+    full_rays = int(mask.astype(bool).sum()) * spp
+    trace_full_s = trace_s * full_rays / max(1, rays)
+    frame_ms = (build_s + trace_full_s + ssao_s + recon_s) * 1e3
     return {"value": rays / trace_s, "unit": "rays/s", "cores": threads, "kind": kind,
-            "sample": "computeRTAO on every %dth row of the same frame (%d rays, %.2f s); BVH build (1 thread) %.2f s"
-                      % (row_step, rays, trace_s, build_s),
-            "build_s": build_s}
+            "sample": "computeRTAO on every %dth row of the same frame (%d rays, median %.3f s of %d runs after one "
+                      "untimed); the BVH build (1 thread), computeSSAO and reconcile on the whole frame, timed the same way"
+                      % (row_step, rays, trace_s, runs),
+            "ms_per_frame": frame_ms,
+            "ms_frame_parts": {"bvh_build_1_thread": build_s * 1e3, "ao_trace_whole_frame": trace_full_s * 1e3,
+                               "ssao": ssao_s * 1e3, "reconcile": recon_s * 1e3}}
 
 
 WEIGHTS = os.path.join(ROOT, "raw-native_amd", "weights")
@@ -283,12 +322,13 @@ def batched(raw_ao, torch, steps, wave_cut=-1):
     if wave_cut >= 0:
         b.set_option(raw_ao.OPT_WAVE_CUT, wave_cut)
     specs, views = raw_ao.host.batched_workload()
-    meshes, ntris = [], []
+    meshes, ntris, scene_tris = [], [], []
     for spec in specs:
         vpos, vnrm, idx, _ = raw_ao.host.scene_seeded(spec)
         meshes.append((vpos, vnrm, idx))
         ntris.append(int(idx.shape[0]))
-        b.add_scene(torch.from_numpy(raw_ao.host.mesh_tris(vpos, idx)).to(dev))
+        scene_tris.append(torch.from_numpy(raw_ao.host.mesh_tris(vpos, idx)).to(dev))
+        b.add_scene(scene_tris[-1])
 
     def raster(vw):
         return raw_ao.host.batched_gbuffer(meshes[vw[0]], vw)
@@ -318,6 +358,21 @@ def batched(raw_ao, torch, steps, wave_cut=-1):
     rmse, mx, px, rays = b.results()
     chunk, tq, bpc = b.tuning()
     b.close()
+    # every view's AO against the single-view path (which the oracle tests pin), on the device
+    same = 0
+    with raw_ao.Context(0) as c:
+        for sc in range(len(specs)):
+            mine = [k for k, v in enumerate(views) if v[0] == sc]
+            if not mine:
+                continue
+            c.build_ploc_dev(scene_tris[sc], 16)
+            for k in mine:
+                P, N, M, out = bufs[k]
+                side, spp = views[k][1], views[k][2]
+                ref = torch.empty_like(out[0])
+                c.compute_rtao_dev(P, N, M, side, side, spp, 1.0, ref)
+                torch.cuda.synchronize()
+                same += int(torch.equal(ref.view(torch.int32), out[0].view(torch.int32)))
     log("batched: %d views, %.3g rays in %.1f ms/step (%.3g rays/s)" % (len(views), rays, ms, rays / (ms * 1e-3)))
     return {"workload": "8 seeded scenes (%s triangles), 64 views %d-%d px square, 16-128 spp, radius 1.0, SSAO 24 + "
                         "reconcile each; graph of builds + one persistent trace launch + reconciles"
@@ -325,7 +380,8 @@ def batched(raw_ao, torch, steps, wave_cut=-1):
             "rays_per_step": int(rays), "ms_per_step": ms, "rays_per_s": rays / (ms * 1e-3),
             "covered_pixels": int(px.sum()), "tuning": {"chunk": chunk, "tri_q": tq, "blocks_per_cu": bpc,
                                                          "tuned_eager_ms": tuned_ms, "tune_seconds": tune_s},
-            "rmse_ssao_vs_rt_mean": float(np.mean(rmse)), "max_error_max": float(np.max(mx))}
+            "rmse_ssao_vs_rt_mean": float(np.mean(rmse)), "max_error_max": float(np.max(mx)),
+            "views_equal_single_view": "%d/%d" % (same, len(views))}
 
 
 ARGS = None
@@ -514,8 +570,13 @@ def main():
     value = rays * a.steps / wall
     trace_ms = float(np.mean(ms["trace"]))
     node_bytes, tri_bytes, rec = ctx.bvh_bytes()
-    alg_bytes = visits * rec + tests * 48 + covered * 24 + W * H * (1 + 4)
-    achieved = alg_bytes / (trace_ms * 1e-3) / 1e9
+    # the trace kernel's algorithmic work (node visits and triangle tests from the counting launch):
+    # 20 flops a box test (4 a node visit), 45 a triangle test, and the bytes those fetches name
+    alg_flops = visits * 4 * 20 + tests * 45
+    tflops = alg_flops / (trace_ms * 1e-3) / 1e12
+    work_bytes = visits * rec + tests * 48 + covered * 24 + W * H * (1 + 4)
+    traffic = hbm_traffic_per_launch()
+    counters = trace_counters()
     out = {
         "metric": METRIC, "value": value, "unit": "rays/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": wall * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -531,13 +592,20 @@ def main():
                            "ssao": float(np.mean(ms["ssao"])), "error_map": float(np.mean(ms["error"])),
                            "host_reconcile_sum": float(np.mean(host_ms))},
         "rmse_ssao_vs_rt": float(rmse), "reconcile_pixels": px,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": hbm_traffic_per_launch(),
+        "roofline": {"bound": "valu", "achieved": tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tflops / FP32_PEAK_TFLOPS, "traffic": traffic,
                      "traffic_source": os.path.relpath(HBM_COUNTERS, ROOT),
                      "kernel": trace_kernel_name(a),
-                     "algorithmic_bytes_per_launch": alg_bytes,
-                     "node_visits": visits, "triangle_tests": tests,
-                     "l2": l2_figures(trace_ms)},
+                     "algorithmic_flops_per_launch": alg_flops,
+                     "node_visits": visits, "triangle_tests": tests, "rays": rays_counted,
+                     "valu_issue_per_simd_cycle": counters.get("valu_issue_per_simd_cycle"),
+                     "valu_issue_frac": (counters["valu_issue_per_simd_cycle"] / 0.5
+                                         if "valu_issue_per_simd_cycle" in counters else None),
+                     "valu_lane_utilization": counters.get("valu_lane_utilization"),
+                     "hbm": ({"achieved": traffic / (trace_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": traffic / (trace_ms * 1e-3) / 1e9 / HBM_PEAK_GBS} if traffic else None),
+                     "l2": l2_figures(trace_ms),
+                     "work_bytes_per_launch": work_bytes},
     }
     out["gbuffer"] = gbuffer_gpu(ctx, torch, vpos, vnrm, idx, eye, W, H, gbuf_host, host_raster_ms, a.steps, stream)
     if not a.no_whole_frame:
@@ -560,8 +628,9 @@ def main():
     if not a.no_cpu:
         log("CPU baseline")
         cb = cpu_baseline(tris, pos, nrm, mask, spp, radius, a.cpu_row_step)
-        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        out["cpu_baseline"] = cb
         out["vs_cpu_trace"] = out["trace_rays_per_s"] / cb["value"]
+        out["vs_cpu_frame"] = cb["ms_per_frame"] / out["ms_per_step"]
     for c in ctxs:
         c.close()
     print(json.dumps(out))

@@ -209,12 +209,21 @@ struct Ray {
     V3 o, d, inv, oinv;
     RayFrame f;
 };
+// The slab tests' 1/d: the hardware reciprocal (within 1 ulp of the correctly rounded quotient),
+// +-1e30 for d = 0. The slab test only decides which boxes a ray visits, never a hit: its plane
+// distances come out within a relative 2^-22 of those from the exact quotient, far inside the
+// 1e-5 the exit is widened by and the boxes' padding, so every box the exact test admits is
+// still admitted and the any-hit answer (the triangle test's) is unchanged. An overflow to
+// infinity (|d| below 2^-128) makes that axis' distances infinite or NaN, which the min/max
+// chain drops, leaving the other axes to decide: again no box is lost.
+__device__ __forceinline__ float slab_inv(float d) {
+    return d != 0.0f ? __builtin_amdgcn_rcpf(d) : copysignf(1e30f, d);
+}
 __device__ __forceinline__ Ray make_ray(V3 o, V3 d) {
     Ray r;
     r.o = o;
     r.d = d;
-    r.inv = v3(d.x != 0.0f ? 1.0f / d.x : copysignf(1e30f, d.x), d.y != 0.0f ? 1.0f / d.y : copysignf(1e30f, d.y),
-               d.z != 0.0f ? 1.0f / d.z : copysignf(1e30f, d.z));
+    r.inv = v3(slab_inv(d.x), slab_inv(d.y), slab_inv(d.z));
     r.oinv = v3(o.x * r.inv.x, o.y * r.inv.y, o.z * r.inv.z);
     r.f = ray_frame(d);
     return r;
@@ -415,8 +424,8 @@ __device__ int occluded4(const Ray& r, float tmax, const Node4* __restrict__ nod
                 const int k = __builtin_ctz(pend);
                 pend &= pend - 1u;
                 const uint32_t slot = uint32_t(~ref_of(leafref, k));
-                float t;
                 if (COUNT) ++tests;
+                float t;
                 if (slot < ntri && tri_hit(r, ts, slot, t) && t < tmax) { result = 1; live = false; }
             }
         }
@@ -1110,11 +1119,18 @@ __global__ __launch_bounds__(BT) void k_trace_ao4r(TraceArgs4 a, uint32_t refill
 // cut: its entered cut leaves wait with its node's leaf slots for the wave's triangle phases,
 // and when its stack runs empty it takes the next cut node it enters. Open counts and refills
 // as k_trace_ao4r. Held to 6 waves a SIMD (80 registers). COUNT: per-lane visit and test counts.
+// The per-pixel part of a ray's set-up (origin, basis, x and y) is computed once for each of the
+// wave's pixels into LDS, and the host-libm angle steps of all PW x 64 rays are read at the start,
+// their loads in flight together, and kept as one nibble a ray (lane L holds sample L of each
+// pixel); a refilled ray takes its nibble from that lane with a shuffle.
+constexpr int PIX_WORDS = 16;   // per pixel: origin, n, t, b (12 floats), x, y
 template <int PW, bool COUNT>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_trace_ao4rc(TraceArgs4 a,
                                                                                                 uint32_t refill_min) {
+    static_assert(PW * 4 <= 32, "one 4-bit angle step a pixel in a 32-bit register");
     __shared__ uint32_t stk[STACK_LDS * BT];
     __shared__ uint32_t cutmem[CUT_WAVE * (BT / 64)];
+    __shared__ float pixmem[PIX_WORDS * PW * (BT / 64)];
     uint32_t* const lds_col = stk + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1136,23 +1152,55 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6, 6))) void
         finish();
         return;
     }
-    // the union of the covered pixels' reach boxes (as trace_ao4's), lane k for pixel pix0 + k
+    // lane k < PW: pixel pix0 + k's set-up (as computeRTAO's per pixel: ray_ao.cpp:24-31) and its
+    // reach box; the union of the covered pixels' boxes
+    float* const pm = pixmem + wave * (PIX_WORDS * PW);
     V3 lo = v3(kFar, kFar, kFar), hi = v3(-kFar, -kFar, -kFar);
-    if (lane < uint32_t(PW) && pix0 + lane < a.npix && a.mask[pix0 + lane]) {
+    const bool cov = lane < uint32_t(PW) && pix0 + lane < a.npix && a.mask[pix0 + lane];
+    if (cov) {
         const uint32_t pix = pix0 + lane;
         const V3 p = v3(a.pos[3 * size_t(pix)], a.pos[3 * size_t(pix) + 1], a.pos[3 * size_t(pix) + 2]);
         const V3 n = v3(a.nrm[3 * size_t(pix)], a.nrm[3 * size_t(pix) + 1], a.nrm[3 * size_t(pix) + 2]);
         const V3 o = add(p, scl(n, 1e-3f));
+        V3 t, b;
+        basis(n, t, b);
+        const uint32_t y = pix / a.w, x = pix - y * a.w;
+        float* q = pm + lane * PIX_WORDS;
+        q[0] = o.x; q[1] = o.y; q[2] = o.z;
+        q[3] = n.x; q[4] = n.y; q[5] = n.z;
+        q[6] = t.x; q[7] = t.y; q[8] = t.z;
+        q[9] = b.x; q[10] = b.y; q[11] = b.z;
+        q[12] = __uint_as_float(x);
+        q[13] = __uint_as_float(y);
         const float big = fmaxf(fabsf(o.x), fmaxf(fabsf(o.y), fabsf(o.z)));
         const float half = a.radius * 1.001f + (big + a.radius) * 1e-5f;
         lo = v3(o.x - half, o.y - half, o.z - half);
         hi = v3(o.x + half, o.y + half, o.z + half);
     }
+    const uint32_t covmask = uint32_t(__ballot(cov));   // bit k: pixel pix0 + k is covered
     lo = v3(wave_min(lo.x), wave_min(lo.y), wave_min(lo.z));
     hi = v3(wave_max(hi.x), wave_max(hi.y), wave_max(hi.z));
-    if (!(lo.x <= hi.x)) {   // no covered pixel
+    if (covmask == 0u) {   // no covered pixel
         finish();
         return;
+    }
+    wave_sync();
+    // the angle steps: nibble k of lane L = sample L of pixel pix0 + k (0 where no step applies)
+    uint32_t fixes = 0;
+    if (a.trig) {
+        uint32_t words[PW], idx[PW];
+        for (int k = 0; k < PW; ++k) {
+            idx[k] = 0;
+            words[k] = 0;
+            if ((covmask >> k) & 1u) {
+                const uint32_t x = __float_as_uint(pm[k * PIX_WORDS + 12]), y = __float_as_uint(pm[k * PIX_WORDS + 13]);
+                idx[k] = hash24(x, y, 2 * lane + 1);
+                words[k] = reinterpret_cast<const uint32_t*>(a.trig)[idx[k] >> 5];
+            }
+        }
+        for (int k = 0; k < PW; ++k)
+            if ((words[k] >> (idx[k] & 31u)) & 1u)
+                fixes |= ((uint32_t(a.trig[kTrigBitmapBytes + (idx[k] >> 1)]) >> ((idx[k] & 1u) * 4)) & 15u) << (4 * k);
     }
     uint32_t* const mine = cutmem + wave * CUT_WAVE;
     CutView cv;
@@ -1185,22 +1233,20 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6, 6))) void
         }
         return -1;
     };
-    // ray ri of the wave: pixel pix0 + ri / 64, sample ri % 64; false for an uncovered pixel
-    const auto setup = [&](uint32_t ri) -> bool {
+    // ray ri of the wave: pixel pix0 + ri / 64, sample ri % 64, angle step nib; false for an
+    // uncovered pixel (or one past the image)
+    const auto setup = [&](uint32_t ri, uint32_t nib) -> bool {
         pk = int(ri >> 6);
-        const uint32_t pix = pix0 + (ri >> 6), s = ri & 63u;
-        if (pix >= a.npix || !a.mask[pix]) return false;
-        const V3 p = v3(a.pos[3 * size_t(pix)], a.pos[3 * size_t(pix) + 1], a.pos[3 * size_t(pix) + 2]);
-        const V3 n = v3(a.nrm[3 * size_t(pix)], a.nrm[3 * size_t(pix) + 1], a.nrm[3 * size_t(pix) + 2]);
-        const uint32_t y = pix / a.w, x = pix - y * a.w;
-        V3 t, b;
-        basis(n, t, b);
-        const V3 origin = add(p, scl(n, 1e-3f));
+        const uint32_t s = ri & 63u;
+        if (!((covmask >> pk) & 1u)) return false;
+        const float* q = pm + pk * PIX_WORDS;
+        const V3 origin = v3(q[0], q[1], q[2]), n = v3(q[3], q[4], q[5]), t = v3(q[6], q[7], q[8]), b = v3(q[9], q[10], q[11]);
+        const uint32_t x = __float_as_uint(q[12]), y = __float_as_uint(q[13]);
         const uint32_t k1 = hash24(x, y, 2 * s), k2 = hash24(x, y, 2 * s + 1);
         const float u1 = float(k1) / float(0x1000000), u2 = float(k2) / float(0x1000000);
         float sphi, cphi;
         sincos_base(6.2831853f * u2, sphi, cphi);
-        if (a.trig) trig_fix(a.trig, k2, sphi, cphi);
+        trig_adjust(nib, sphi, cphi);
         r = make_ray(origin, ao_dir(u1, sphi, cphi, n, t, b));
         ox = r.inv.x >= 0.0f ? 0u : 48u;
         oy = r.inv.y >= 0.0f ? 0u : 48u;
@@ -1220,7 +1266,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6, 6))) void
         for (int k = 0; k < PW; ++k) cnt[k] += pk == k ? 1u : 0u;
     };
     const uint32_t total = PW * 64u;
-    bool live = setup(lane);
+    bool live = setup(lane, fixes & 15u);
     uint32_t next = 64;
     while (true) {
         if (live && pend == 0 && node >= 0) {
@@ -1272,8 +1318,8 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6, 6))) void
                     slot = cv.leaf[__builtin_ctz(lp)];
                     lp &= lp - 1u;
                 }
-                float t;
                 if (COUNT) ++tests;
+                float t;
                 if (slot < a.ntri && tri_hit(r, a.tris, slot, t) && t < a.radius) {
                     live = false;   // occluded
                     pend = 0;
@@ -1288,10 +1334,10 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6, 6))) void
         const unsigned long long idle = __ballot(!live);
         const uint32_t nidle = uint32_t(__popcll(idle));
         if (next < total && (nidle >= refill_min || nidle == 64u)) {
-            if (!live) {
-                const uint32_t ri = next + uint32_t(__popcll(idle & below));
-                if (ri < total) live = setup(ri);
-            }
+            // every lane takes part in the shuffle (its source lane may be busy tracing)
+            const uint32_t ri = next + uint32_t(__popcll(idle & below));
+            const uint32_t nib = (uint32_t(__shfl(int(fixes), int(ri & 63u))) >> (4u * ((ri >> 6) & uint32_t(PW - 1)))) & 15u;
+            if (!live && ri < total) live = setup(ri, nib);
             next += nidle;
         }
         if (__ballot(live) == 0 && next >= total) break;
@@ -1491,8 +1537,8 @@ __global__ __launch_bounds__(BT) void k_batch_trace(BatchDev d) {
                 if (CUT) {
                     const float reach = bv.radius * kReach;
                     for (int i = 0; i < cv.nleaf && !hit && active; ++i) {
-                        float th;
                         if (!CutView::enters(r, cv.lbox, CUT_LEAF, i, reach, kExit)) continue;
+                        float th;
                         if (cv.leaf[i] < sc.ntri && tri_hit(r, sc.tris, cv.leaf[i], th) && th < bv.radius) hit = true;
                     }
                 }
