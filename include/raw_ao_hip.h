@@ -428,6 +428,44 @@ int raw_ao_conv3x3_dev(raw_ao_ctx* ctx, const uint16_t* d_in0, const uint16_t* d
 int raw_ao_conv3x3_wgrad_dev(raw_ao_ctx* ctx, const uint16_t* d_dy, const uint16_t* d_in0, const uint16_t* d_in1,
                              int c0, int c1, int cout, int n, int w, int h, float* d_gw, float* d_gb, void* stream);
 
+/* ---- the versioned function table ---------------------------------------------------
+ * The ABI as one symbol a loader resolves (the versioned table the reference's GPU AO path
+ * would bind; no reference function): raw_ao_hip_api(RAW_AO_ABI_VERSION) returns this
+ * library's table, raw_ao_hip_api(v) NULL for a version it does not implement. The entries are
+ * the functions declared above, with the same behaviour. */
+typedef struct raw_ao_hip_api_v1 {
+    uint32_t abi_version;   /* 1 */
+    uint32_t size;          /* sizeof(raw_ao_hip_api_v1) */
+    int (*open)(int device_ordinal, raw_ao_ctx** out);
+    void (*close)(raw_ao_ctx* ctx);
+    const char* (*last_error)(const raw_ao_ctx* ctx);
+    int (*set_option)(raw_ao_ctx* ctx, int option, int value);
+    int (*build_ploc)(raw_ao_ctx* ctx, const float* tris9, uint32_t ntri, int radius);
+    int (*build_ploc_dev)(raw_ao_ctx* ctx, const float* d_tris9, uint32_t ntri, int radius, void* stream);
+    int (*tree_info)(raw_ao_ctx* ctx, uint32_t* nnodes, int32_t* root, uint32_t* rounds);
+    int (*read_tree)(raw_ao_ctx* ctx, float* boxes6, int32_t* links3);
+    int (*read_keys)(raw_ao_ctx* ctx, uint32_t* keys, uint32_t* order);
+    int (*trace_ao)(raw_ao_ctx* ctx, const float* position, const float* normal, const uint8_t* mask, int w, int h, int spp,
+                    float radius, float* ao);
+    int (*trace_ao_dev)(raw_ao_ctx* ctx, const float* d_position, const float* d_normal, const uint8_t* d_mask, int w, int h,
+                        int spp, float radius, float* d_ao, void* stream);
+    int (*trace_visibility)(raw_ao_ctx* ctx, const float* position, const float* normal, const uint8_t* mask, int w, int h,
+                            int spp, float radius, const float to_light[3], int sun_forced, float* ao, float* sky, float* sun);
+    int (*ssao)(raw_ao_ctx* ctx, const float* position, const float* normal, const uint8_t* mask, int w, int h, int samples,
+                float radius, float* ao);
+    int (*ssao_dev)(raw_ao_ctx* ctx, const float* d_position, const float* d_normal, const uint8_t* d_mask, int w, int h,
+                    int samples, float radius, float* d_ao, void* stream);
+    int (*error_map)(raw_ao_ctx* ctx, const float* approx, const float* truth, const uint8_t* mask, int w, int h, float* err,
+                     float* max_error);
+    int (*error_map_dev)(raw_ao_ctx* ctx, const float* d_approx, const float* d_truth, const uint8_t* d_mask, int w, int h,
+                         float* d_err, float* d_max_error, void* stream);
+    int (*trace_rays)(raw_ao_ctx* ctx, const float* rays8, uint32_t nrays, int any, float far_scale, float* out4);
+    int (*closest_hits)(raw_ao_ctx* ctx, const float* rays8, uint32_t nrays, void* hits4);
+    int (*render_frame)(raw_ao_ctx* ctx, const raw_ao_scene* scene, const raw_ao_render_opts* opts, int w, int h,
+                        raw_ao_frame* out, raw_ao_frame_stats* stats);
+} raw_ao_hip_api_v1;
+const raw_ao_hip_api_v1* raw_ao_hip_api(uint32_t abi_version);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,7 +16,7 @@ HIP_SOURCES = ["csrc/capi.hip", "csrc/ploc.hip", "csrc/trace.hip", "csrc/bvh4.hi
                "csrc/noalloc.hip", "csrc/trig_table.cpp"]
 ARCH = os.environ.get("RAW_AO_ARCH", "gfx950")
 
-HIPCC_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+HIPCC_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++23", "-fPIC", "-shared", "-ffp-contract=off",
                "-fno-gpu-flush-denormals-to-zero", "-fhip-fp32-correctly-rounded-divide-sqrt",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall"]
 
@@ -38,12 +38,12 @@ def build(verbose=False):
     os.replace(tmp, hip_out)
     host_out = os.path.join(BUILD, "libraw_ao_host.so")
     tmp = host_out + ".tmp"
-    out += run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-Wall", "host/raw_ao_host.cpp", "host/raw_ao_outputs.cpp",
+    out += run(["g++", "-O2", "-std=c++23", "-fPIC", "-shared", "-ffp-contract=off", "-Wall", "host/raw_ao_host.cpp", "host/raw_ao_outputs.cpp",
                 "-L" + BUILD, "-lraw_ao_hip", "-Wl,-rpath,$ORIGIN", "-o", tmp], HERE)
     os.replace(tmp, host_out)
     # the C++ host interface's frame test (tests/cpp), linked to the host library
     exe = os.path.join(BUILD, "host_interface_frame")
-    out += run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "../tests/cpp/host_interface_frame.cpp",
+    out += run(["g++", "-O2", "-std=c++23", "-ffp-contract=off", "-Wall", "../tests/cpp/host_interface_frame.cpp",
                 "-L" + BUILD, "-lraw_ao_host", "-lraw_ao_hip", "-Wl,-rpath,$ORIGIN", "-o", exe + ".tmp"], HERE)
     os.replace(exe + ".tmp", exe)
     if verbose and out.strip():

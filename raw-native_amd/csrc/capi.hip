@@ -735,4 +735,13 @@ int raw_ao_bvh_bytes(raw_ao_ctx* x, uint64_t* node_bytes, uint64_t* tri_bytes, u
     return RAW_AO_OK;
 }
 
+const raw_ao_hip_api_v1* raw_ao_hip_api(uint32_t abi_version) {
+    static const raw_ao_hip_api_v1 v1 = {
+        1u, uint32_t(sizeof(raw_ao_hip_api_v1)), raw_ao_open, raw_ao_close, raw_ao_last_error, raw_ao_set_option,
+        raw_ao_build_ploc, raw_ao_build_ploc_dev, raw_ao_tree_info, raw_ao_read_tree, raw_ao_read_keys, raw_ao_trace_ao,
+        raw_ao_trace_ao_dev, raw_ao_trace_visibility, raw_ao_ssao, raw_ao_ssao_dev, raw_ao_error_map, raw_ao_error_map_dev,
+        raw_ao_trace_rays, raw_ao_closest_hits, raw_ao_render_frame};
+    return abi_version == 1u ? &v1 : nullptr;
+}
+
 }  // extern "C"

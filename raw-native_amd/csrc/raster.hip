@@ -20,7 +20,6 @@
 #include "../../include/raw_ao_hip.h"
 #include "ao_math.hpp"
 #include "ctx.hpp"
-#include <hipcub/hipcub.hpp>
 #include <cmath>
 #include <limits>
 #include <vector>
@@ -223,7 +222,73 @@ __global__ __launch_bounds__(RBT) void k_rast_resolve(const RTri* __restrict__ t
 __global__ void k_fill_keys(unsigned long long* keys, uint32_t n) {
     for (uint32_t i = blockIdx.x * RBT + threadIdx.x; i < n; i += gridDim.x * RBT) keys[i] = ~0ull;
 }
-__global__ void k_total(uint64_t* offs, const uint64_t* counts, uint32_t n) { offs[n] = offs[n - 1] + counts[n - 1]; }
+// ---- exclusive prefix sum of the boxes' 64-bit pixel counts ---------------------------
+// Three passes: each workgroup scans a tile of SCAN_TILE counts (loaded and stored coalesced
+// through LDS, a thread summing SCAN_PER consecutive ones, then wave and workgroup scans) and
+// leaves its total; one workgroup scans the totals and writes the grand total at out[n] (the
+// cover pass's sentinel); then every tile adds its offset. Integer sums, so the order of
+// additions does not change the result.
+constexpr int SCAN_T = 256, SCAN_PER = 16, SCAN_TILE = SCAN_T * SCAN_PER;
+__global__ __launch_bounds__(SCAN_T) void k_scan_tiles(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint32_t n,
+                                                       uint64_t* __restrict__ part) {
+    __shared__ uint64_t tile[SCAN_TILE];
+    __shared__ uint64_t wsum[SCAN_T / 64];
+    const size_t t0 = size_t(blockIdx.x) * SCAN_TILE;
+    for (int k = threadIdx.x; k < SCAN_TILE; k += SCAN_T) tile[k] = t0 + k < n ? in[t0 + k] : 0ull;
+    __syncthreads();
+    uint64_t v[SCAN_PER], sum = 0;
+    for (int k = 0; k < SCAN_PER; ++k) {
+        v[k] = tile[threadIdx.x * SCAN_PER + k];
+        sum += v[k];
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t x = sum;   // inclusive scan across the wave
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint64_t run = x - sum;
+    for (int k = 0; k < w; ++k) run += wsum[k];
+    for (int k = 0; k < SCAN_PER; ++k) {
+        tile[threadIdx.x * SCAN_PER + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == SCAN_T - 1) part[blockIdx.x] = run;   // the tile's total
+    __syncthreads();
+    for (int k = threadIdx.x; k < SCAN_TILE; k += SCAN_T)
+        if (t0 + k < n) out[t0 + k] = tile[k];
+}
+__global__ __launch_bounds__(1024) void k_scan_parts(uint64_t* __restrict__ part, uint32_t nparts, uint64_t* __restrict__ total) {
+    __shared__ uint64_t s[1024];
+    const uint32_t per = (nparts + 1023) / 1024;
+    const uint32_t a = threadIdx.x * per, b = min(nparts, a + per);
+    uint64_t sum = 0;
+    for (uint32_t i = a; i < b; ++i) sum += part[i];
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint64_t v = threadIdx.x >= uint32_t(off) ? s[threadIdx.x - off] : 0ull;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint64_t run = s[threadIdx.x] - sum;
+    for (uint32_t i = a; i < b; ++i) {
+        const uint64_t v = part[i];
+        part[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 1023) *total = s[1023];
+}
+__global__ __launch_bounds__(SCAN_T) void k_scan_add(uint64_t* __restrict__ out, uint32_t n, const uint64_t* __restrict__ part) {
+    const uint64_t add = part[blockIdx.x];
+    if (add == 0) return;
+    const size_t t0 = size_t(blockIdx.x) * SCAN_TILE;
+    for (int k = threadIdx.x; k < SCAN_TILE; k += SCAN_T)
+        if (t0 + k < n) out[t0 + k] += add;
+}
 
 // The camera matrices of raw/math/mat.hpp:7-38 on the host, as the reference computes them.
 struct M4 {
@@ -294,16 +359,13 @@ int rasterize(Ctx& c, const float* d_vpos, const float* d_vnrm, const int32_t* d
     if (ntri) {
         k_rast_setup<<<(ntri + RBT - 1) / RBT, RBT, 0, s>>>(d_vpos, d_vnrm, d_idx, ntri, cam, nearz, w, h,
                                                            c.r_tris.as<RTri>(), c.r_counts.as<uint64_t>());
-        size_t tmp = 0;
-        if (!hip_ok(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, c.r_counts.as<uint64_t>(), c.r_offs.as<uint64_t>(), nsub, s),
-                    "scan size") ||
-            !ensure(c, c.r_scan_tmp, tmp) ||
-            !hip_ok(c, hipcub::DeviceScan::ExclusiveSum(c.r_scan_tmp.p, tmp, c.r_counts.as<uint64_t>(), c.r_offs.as<uint64_t>(), nsub,
-                                                        s),
-                    "scan"))
-            return -2;
-        // offs[nsub] = the candidate total, also the search's sentinel
-        k_total<<<1, 1, 0, s>>>(c.r_offs.as<uint64_t>(), c.r_counts.as<uint64_t>(), nsub);
+        // offs = the exclusive prefix sum of the counts; offs[nsub] = the candidate total, also the
+        // search's sentinel
+        const uint32_t ntiles = uint32_t((size_t(nsub) + SCAN_TILE - 1) / SCAN_TILE);
+        if (!ensure(c, c.r_scan_tmp, 8 * size_t(ntiles))) return -2;
+        k_scan_tiles<<<ntiles, SCAN_T, 0, s>>>(c.r_counts.as<uint64_t>(), c.r_offs.as<uint64_t>(), nsub, c.r_scan_tmp.as<uint64_t>());
+        k_scan_parts<<<1, 1024, 0, s>>>(c.r_scan_tmp.as<uint64_t>(), ntiles, c.r_offs.as<uint64_t>() + nsub);
+        k_scan_add<<<ntiles, SCAN_T, 0, s>>>(c.r_offs.as<uint64_t>(), nsub, c.r_scan_tmp.as<uint64_t>());
         k_rast_cover<<<8192, RBT, 0, s>>>(c.r_tris.as<RTri>(), c.r_offs.as<uint64_t>(), nsub, c.r_offs.as<uint64_t>() + nsub, w,
                                          c.r_keys.as<unsigned long long>());
     }

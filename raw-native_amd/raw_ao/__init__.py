@@ -41,6 +41,7 @@ _lib = None
 # (name, restype, argtypes) of every symbol include/raw_ao_hip.h declares
 SIGNATURES = [
     ("raw_ao_abi_version", C.c_uint32, []),
+    ("raw_ao_hip_api", C.c_void_p, [C.c_uint32]),
     ("raw_ao_open", C.c_int, [C.c_int, C.POINTER(_vp)]),
     ("raw_ao_close", None, [_vp]),
     ("raw_ao_last_error", C.c_char_p, [_vp]),

@@ -30,6 +30,26 @@ def test_every_declared_symbol_is_exported_and_bound():
     assert L.raw_ao_abi_version() == 1
 
 
+def test_versioned_api_table():
+    """raw_ao_hip_api(1): the version, its size and every entry the function of the same name;
+    an unknown version gets NULL."""
+    L = raw_ao.lib()
+    names = ["open", "close", "last_error", "set_option", "build_ploc", "build_ploc_dev", "tree_info", "read_tree",
+             "read_keys", "trace_ao", "trace_ao_dev", "trace_visibility", "ssao", "ssao_dev", "error_map", "error_map_dev",
+             "trace_rays", "closest_hits", "render_frame"]
+
+    class Api(C.Structure):
+        _fields_ = [("abi_version", C.c_uint32), ("size", C.c_uint32)] + [(n, C.c_void_p) for n in names]
+
+    p = L.raw_ao_hip_api(1)
+    assert p
+    api = Api.from_address(p)
+    assert api.abi_version == 1 and api.size == C.sizeof(Api)
+    for n in names:
+        assert getattr(api, n) == C.cast(getattr(L, "raw_ao_" + n), C.c_void_p).value, n
+    assert not L.raw_ao_hip_api(2) and not L.raw_ao_hip_api(0)
+
+
 def gpu_present():
     try:
         import torch
