@@ -89,7 +89,13 @@ enum {
     /* 0 (default) or 1: the reference's skew control (RAW_NATIVE_AO_ORACLE=skew,
      * src/renderer/accel.cpp:6-11,26-28): every AO query distance of raw_ao_trace_ao[_dev] is
      * radius x 0.99f. A negative control for tests: the AO must then differ from the golden. */
-    RAW_AO_OPT_SKEW = 9
+    RAW_AO_OPT_SKEW = 9,
+    /* Build tuning, 0..4096 (default 0): the build enqueues its PLOC rounds and wide-collapse
+     * levels without reading the cluster count back, as many as the last build of as many
+     * triangles needed (0: that, or an estimate for a first build), or this many; rounds or
+     * levels past the end do nothing, and a build left short continues with read-backs. The tree
+     * is the same for every value; only the speed differs. */
+    RAW_AO_OPT_BUILD_ROUNDS = 10
 };
 enum {
     RAW_AO_TRACE_WIDE = 0,     /* 4-wide nodes, LDS stack, triangle tests batched per wave */

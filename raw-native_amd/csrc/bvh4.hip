@@ -36,7 +36,7 @@ constexpr int STACK_SPILL = 48;
 
 struct alignas(16) Node4 { float4 lx, ly, lz, hx, hy, hz; int4 ref; float4 spare; };
 
-struct WideState { uint32_t count, base, next_base, pad; };
+struct WideState { uint32_t count, base, next_base, pad; };   // pad: levels run with a frontier
 
 __device__ __forceinline__ float pad_of(float l, float h) {
     return (h - l) * 6.103515625e-05f + (fabsf(l) + fabsf(h)) * 9.5367431640625e-07f + 1e-30f;   // 2^-14, 2^-20
@@ -71,116 +71,122 @@ __device__ __forceinline__ int expand(const float* __restrict__ boxes, const int
 }
 
 // Pass 1 of a level: each wide node's inner-slot count, summed per block.
+// Two launches a level, both with a fixed grid that strides over the tiles of BT frontier
+// entries and reads the frontier's size from the device state, so the host enqueues levels
+// without reading the size back; a level with an empty frontier does nothing (its state is
+// copied).
+// Pass 1: each tile's inner-slot count (blk[tile]).
 __global__ __launch_bounds__(BT) void k_wide_count(const WideState* __restrict__ st, const uint32_t* __restrict__ frontier,
                                                    const float* __restrict__ boxes, const int32_t* __restrict__ links,
                                                    uint32_t n, uint32_t* __restrict__ blk) {
     __shared__ uint32_t ws[BT / 64];
-    const uint32_t p = blockIdx.x * BT + threadIdx.x;
-    uint32_t c = 0;
-    if (p < st->count) {
-        int32_t slot[4];
-        const int cnt = expand(boxes, links, n, int32_t(frontier[p]), slot);
-        for (int k = 0; k < cnt; ++k) c += uint32_t(slot[k]) >= n;
-    }
-    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
-    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t s = 0;
-        for (int k = 0; k < BT / 64; ++k) s += ws[k];
-        blk[blockIdx.x] = s;
+    const uint32_t count = st->count;
+    const uint32_t ntile = (count + BT - 1) / BT;
+    for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        __syncthreads();   // the previous tile's ws reads are done
+        const uint32_t p = tile * BT + threadIdx.x;
+        uint32_t c = 0;
+        if (p < count) {
+            int32_t slot[4];
+            const int cnt = expand(boxes, links, n, int32_t(frontier[p]), slot);
+            for (int k = 0; k < cnt; ++k) c += uint32_t(slot[k]) >= n;
+        }
+        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+        if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = c;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t s = 0;
+            for (int k = 0; k < BT / 64; ++k) s += ws[k];
+            blk[tile] = s;
+        }
     }
 }
 
-// Pass 2: block offsets (exclusive scan, one block of 1024) and the next level's size.
-__global__ __launch_bounds__(1024) void k_wide_scan(const WideState* __restrict__ in, WideState* __restrict__ out,
-                                                    uint32_t* __restrict__ blk, uint32_t nblk) {
-    __shared__ uint32_t s[1024];
-    const uint32_t per = (nblk + 1023) / 1024;
-    const uint32_t a = threadIdx.x * per, b = min(nblk, a + per);
-    uint32_t sum = 0;
-    for (uint32_t i = a; i < b; ++i) sum += blk[i];
-    s[threadIdx.x] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = threadIdx.x >= uint32_t(off) ? s[threadIdx.x - off] : 0u;
-        __syncthreads();
-        s[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = s[threadIdx.x] - sum;
-    for (uint32_t i = a; i < b; ++i) { const uint32_t v = blk[i]; blk[i] = run; run += v; }
-    if (threadIdx.x == 1023) {
-        const WideState i = *in;
-        WideState o;
-        o.count = s[1023];
-        o.base = i.base + i.count;
-        o.next_base = o.base + o.count;
-        o.pad = 0;
-        *out = o;
-    }
-}
-
-// Pass 3: write the records of this level and the next level's frontier.
-__global__ __launch_bounds__(BT) void k_wide_write(const WideState* __restrict__ st, const uint32_t* __restrict__ frontier,
-                                                   uint32_t* __restrict__ next, const uint32_t* __restrict__ blkoff,
-                                                   const float* __restrict__ boxes, const int32_t* __restrict__ links,
-                                                   uint32_t n, Node4* __restrict__ nodes) {
-    __shared__ uint32_t ws[BT / 64];
-    const uint32_t p = blockIdx.x * BT + threadIdx.x;
+// Pass 2: each tile's offset among all tiles (the sum of the earlier tiles' counts, read by the
+// tile's own workgroup), then the records of this level and the next level's frontier in entry
+// order; the last tile writes the next level's state.
+__global__ __launch_bounds__(BT) void k_wide_write(const WideState* __restrict__ st, WideState* __restrict__ st_next,
+                                                   const uint32_t* __restrict__ frontier, uint32_t* __restrict__ next,
+                                                   const uint32_t* __restrict__ blk, const float* __restrict__ boxes,
+                                                   const int32_t* __restrict__ links, uint32_t n, Node4* __restrict__ nodes) {
+    __shared__ uint32_t ws[BT / 64], red[BT / 64];
     const WideState s = *st;
-    int32_t slot[4] = {EMPTY, EMPTY, EMPTY, EMPTY};
-    int cnt = 0;
-    uint32_t c = 0;
-    if (p < s.count) {
-        cnt = expand(boxes, links, n, int32_t(frontier[p]), slot);
-        for (int k = 0; k < cnt; ++k) c += uint32_t(slot[k]) >= n;
+    if (s.count == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *st_next = s;
+        return;
     }
-    uint32_t x = c;   // inclusive wave scan of the inner-slot counts
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(x, off);
-        if (lane >= off) x += y;
+    const uint32_t ntile = (s.count + BT - 1) / BT;
+    for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        // the tile's offset: the sum of blk[0 .. tile-1] over the workgroup
+        uint32_t pre = 0;
+        for (uint32_t i = threadIdx.x; i < tile; i += BT) pre += blk[i];
+        for (int off = 32; off > 0; off >>= 1) pre += __shfl_xor(pre, off);
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        __syncthreads();   // the previous tile's ws and red reads are done
+        if (lane == 0) red[w] = pre;
+        const uint32_t p = tile * BT + threadIdx.x;
+        int32_t slot[4] = {EMPTY, EMPTY, EMPTY, EMPTY};
+        int cnt = 0;
+        uint32_t c = 0;
+        if (p < s.count) {
+            cnt = expand(boxes, links, n, int32_t(frontier[p]), slot);
+            for (int k = 0; k < cnt; ++k) c += uint32_t(slot[k]) >= n;
+        }
+        uint32_t x = c;   // inclusive wave scan of the inner-slot counts
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) ws[w] = x;
+        __syncthreads();
+        pre = 0;
+        for (int k = 0; k < BT / 64; ++k) pre += red[k];
+        if (tile == ntile - 1 && threadIdx.x == 0) {   // the last tile: the next level's state
+            WideState o;
+            o.count = pre + blk[tile];
+            o.base = s.base + s.count;
+            o.next_base = o.base + o.count;
+            o.pad = s.pad + 1u;   // levels that had a frontier
+            *st_next = o;
+        }
+        if (p >= s.count) continue;
+        uint32_t off = pre + x - c;
+        for (int k = 0; k < w; ++k) off += ws[k];
+        float bl[3][4], bh[3][4];
+        int32_t ref[4];
+        uint32_t masks = 0;   // leaf slots in bits 0-3, inner slots in bits 4-7
+        for (int k = 0; k < 4; ++k) {
+            if (k >= cnt) {
+                for (int a = 0; a < 3; ++a) { bl[a][k] = kFar; bh[a][k] = kFar; }
+                ref[k] = EMPTY;
+                continue;
+            }
+            masks |= uint32_t(slot[k]) < n ? (1u << k) : (16u << k);
+            const float* b = boxes + 6 * size_t(slot[k]);
+            for (int a = 0; a < 3; ++a) {
+                const float pd = pad_of(b[a], b[3 + a]);
+                bl[a][k] = b[a] - pd;
+                bh[a][k] = b[3 + a] + pd;
+            }
+            if (uint32_t(slot[k]) < n) {
+                ref[k] = ~slot[k];
+            } else {
+                ref[k] = int32_t(s.next_base + off);
+                next[off] = uint32_t(slot[k]);
+                ++off;
+            }
+        }
+        Node4 r;
+        r.lx = make_float4(bl[0][0], bl[0][1], bl[0][2], bl[0][3]);
+        r.ly = make_float4(bl[1][0], bl[1][1], bl[1][2], bl[1][3]);
+        r.lz = make_float4(bl[2][0], bl[2][1], bl[2][2], bl[2][3]);
+        r.hx = make_float4(bh[0][0], bh[0][1], bh[0][2], bh[0][3]);
+        r.hy = make_float4(bh[1][0], bh[1][1], bh[1][2], bh[1][3]);
+        r.hz = make_float4(bh[2][0], bh[2][1], bh[2][2], bh[2][3]);
+        r.ref = make_int4(ref[0], ref[1], ref[2], ref[3]);
+        r.spare = make_float4(__uint_as_float(masks), 0, 0, 0);
+        nodes[s.base + p] = r;
     }
-    if (lane == 63) ws[w] = x;
-    __syncthreads();
-    if (p >= s.count) return;
-    uint32_t off = blkoff[blockIdx.x] + x - c;
-    for (int k = 0; k < w; ++k) off += ws[k];
-    float bl[3][4], bh[3][4];
-    int32_t ref[4];
-    uint32_t masks = 0;   // leaf slots in bits 0-3, inner slots in bits 4-7
-    for (int k = 0; k < 4; ++k) {
-        if (k >= cnt) {
-            for (int a = 0; a < 3; ++a) { bl[a][k] = kFar; bh[a][k] = kFar; }
-            ref[k] = EMPTY;
-            continue;
-        }
-        masks |= uint32_t(slot[k]) < n ? (1u << k) : (16u << k);
-        const float* b = boxes + 6 * size_t(slot[k]);
-        for (int a = 0; a < 3; ++a) {
-            const float pd = pad_of(b[a], b[3 + a]);
-            bl[a][k] = b[a] - pd;
-            bh[a][k] = b[3 + a] + pd;
-        }
-        if (uint32_t(slot[k]) < n) {
-            ref[k] = ~slot[k];
-        } else {
-            ref[k] = int32_t(s.next_base + off);
-            next[off] = uint32_t(slot[k]);
-            ++off;
-        }
-    }
-    Node4 r;
-    r.lx = make_float4(bl[0][0], bl[0][1], bl[0][2], bl[0][3]);
-    r.ly = make_float4(bl[1][0], bl[1][1], bl[1][2], bl[1][3]);
-    r.lz = make_float4(bl[2][0], bl[2][1], bl[2][2], bl[2][3]);
-    r.hx = make_float4(bh[0][0], bh[0][1], bh[0][2], bh[0][3]);
-    r.hy = make_float4(bh[1][0], bh[1][1], bh[1][2], bh[1][3]);
-    r.hz = make_float4(bh[2][0], bh[2][1], bh[2][2], bh[2][3]);
-    r.ref = make_int4(ref[0], ref[1], ref[2], ref[3]);
-    r.spare = make_float4(__uint_as_float(masks), 0, 0, 0);
-    nodes[s.base + p] = r;
 }
 
 // A single-triangle scene: one record, the leaf in slot 0.
@@ -1579,9 +1585,45 @@ static int top_table(Ctx& c, hipStream_t s) {
 }
 
 namespace {
+// The first level's state: the frontier is the binary root.
 __global__ void k_wide_init(WideState* st, uint32_t* frontier, uint32_t root) {
     st[0] = WideState{1, 0, 1, 0};
     frontier[0] = root;
+}
+void wide_init(Ctx& c, uint32_t root, hipStream_t s) {
+    k_wide_init<<<1, 1, 0, s>>>(c.wstate.as<WideState>(), c.wfront_a.as<uint32_t>(), root);
+    c.wide_parity = 0;
+    c.wide_in_a = true;
+}
+
+uint32_t wide_grid(uint32_t n) { return std::min<uint32_t>(cdiv(n, BT), 2048u); }
+
+// Collapse levels to enqueue for n triangles: as many as the last finished build of n triangles
+// needed, else an estimate (20 levels for 1.03 M triangles); a collapse whose levels fall short
+// continues with read-backs (wide_continue), so the estimate sets only the speed.
+uint32_t predicted_levels(const Ctx& c, uint32_t n) {
+    if (c.build_rounds > 0) return uint32_t(c.build_rounds);
+    if (c.last_rounds_n == n && c.last_wide_levels) return c.last_wide_levels;
+    uint32_t l4 = 0;
+    while ((uint64_t(1) << (2 * l4)) < n) ++l4;
+    return 2 * l4 + 6;
+}
+
+// Enqueues `levels` collapse levels from the state at c.wide_parity (frontier in the buffer
+// c.wide_in_a names). The state's pad counts the levels that had a frontier.
+void enqueue_levels(Ctx& c, uint32_t levels, hipStream_t s) {
+    const uint32_t n = c.ntri, G = wide_grid(n);
+    WideState* d_st = c.wstate.as<WideState>();
+    for (uint32_t l = 0; l < levels; ++l) {
+        uint32_t* fin = c.wide_in_a ? c.wfront_a.as<uint32_t>() : c.wfront_b.as<uint32_t>();
+        uint32_t* fout = c.wide_in_a ? c.wfront_b.as<uint32_t>() : c.wfront_a.as<uint32_t>();
+        const int parity = c.wide_parity;
+        k_wide_count<<<G, BT, 0, s>>>(d_st + parity, fin, c.boxes.as<float>(), c.links.as<int32_t>(), n, c.wblk.as<uint32_t>());
+        k_wide_write<<<G, BT, 0, s>>>(d_st + parity, d_st + (parity ^ 1), fin, fout, c.wblk.as<uint32_t>(), c.boxes.as<float>(),
+                                      c.links.as<int32_t>(), n, c.nodes4.as<Node4>());
+        c.wide_parity ^= 1;
+        c.wide_in_a = !c.wide_in_a;
+    }
 }
 }  // namespace
 
@@ -1595,40 +1637,39 @@ int wide_build(Ctx& c, hipStream_t s) {
     if (n == 1) {
         k_wide_single<<<1, 64, 0, s>>>(c.boxes.as<float>(), c.nodes4.as<Node4>());
         c.wide_nodes = 1;
+        c.plan_wide_nb.clear();
         return hip_ok(c, hipGetLastError(), "wide launch") ? top_table(c, s) : -2;
     }
-    k_wide_init<<<1, 1, 0, s>>>(c.wstate.as<WideState>(), c.wfront_a.as<uint32_t>(), uint32_t(c.root));
+    // The binary root is the last merge's node (2n - 2); build_finish confirms it and that the
+    // enqueued levels emptied the frontier.
+    wide_init(c, 2 * n - 2, s);
     if (!hip_ok(c, hipGetLastError(), "wide init")) return -2;
-    WideState* d_st = c.wstate.as<WideState>();
-    uint32_t *fin = c.wfront_a.as<uint32_t>(), *fout = c.wfront_b.as<uint32_t>();
-    uint64_t ub = 1;   // upper bound of this level's size (grids are sized by it)
-    int parity = 0;
-    c.plan_wide_nb.clear();
-    for (int level = 0;; ++level) {
-        const uint32_t nb = cdiv(ub, BT);
-        c.plan_wide_nb.push_back(nb);
-        k_wide_count<<<nb, BT, 0, s>>>(d_st + parity, fin, c.boxes.as<float>(), c.links.as<int32_t>(), n, c.wblk.as<uint32_t>());
-        k_wide_scan<<<1, 1024, 0, s>>>(d_st + parity, d_st + (parity ^ 1), c.wblk.as<uint32_t>(), nb);
-        k_wide_write<<<nb, BT, 0, s>>>(d_st + parity, fin, fout, c.wblk.as<uint32_t>(), c.boxes.as<float>(),
-                                       c.links.as<int32_t>(), n, c.nodes4.as<Node4>());
-        parity ^= 1;
-        std::swap(fin, fout);
-        ub = std::min<uint64_t>(4 * ub, n);
-        if ((level & 3) == 3 || ub >= n) {   // read the real size back now and then
-            WideState h;
-            if (!hip_ok(c, hipMemcpyAsync(&h, d_st + parity, sizeof h, hipMemcpyDeviceToHost, s), "wide state") ||
-                !hip_ok(c, hipStreamSynchronize(s), "wide level"))
-                return -2;
-            if (h.count == 0) { c.wide_nodes = h.base; break; }
-            ub = h.count;
-        }
-        if (level > 4 * 64) { c.err = "wide collapse did not finish"; return -2; }
-    }
+    const uint32_t levels = predicted_levels(c, n);
+    enqueue_levels(c, levels, s);
+    c.plan_wide_nb.assign(levels, wide_grid(n));
     return top_table(c, s);
 }
 
-// The collapse with the level grids of the last eager wide_build of this tree: no host
-// copies or waits, so a stream capture can record it.
+int wide_continue(Ctx& c, hipStream_t s) {
+    const uint32_t n = c.ntri;
+    WideState h{};
+    for (int batch = 0;; ++batch) {
+        if (!hip_ok(c, hipMemcpyAsync(&h, c.wstate.as<WideState>() + c.wide_parity, sizeof h, hipMemcpyDeviceToHost, s),
+                    "wide state") ||
+            !hip_ok(c, hipStreamSynchronize(s), "wide level"))
+            return -2;
+        if (h.count == 0) break;
+        if (batch > 64) { c.err = "wide collapse did not finish"; return -2; }
+        enqueue_levels(c, 4, s);
+        c.plan_wide_nb.insert(c.plan_wide_nb.end(), 4, wide_grid(n));
+    }
+    c.wide_nodes = h.base;
+    c.last_wide_levels = h.pad;
+    return top_table(c, s);
+}
+
+// The collapse with the level count of the last eager wide_build of this tree: no host copies
+// or waits, so a stream capture can record it.
 int wide_build_fixed(Ctx& c, hipStream_t s) {
     const uint32_t n = c.ntri;
     if (n == 0) return 0;
@@ -1637,19 +1678,52 @@ int wide_build_fixed(Ctx& c, hipStream_t s) {
         return hip_ok(c, hipGetLastError(), "wide launch") ? top_table(c, s) : -2;
     }
     if (c.plan_wide_nb.empty()) { c.err = "no recorded wide collapse plan"; return -1; }
-    WideState* d_st = c.wstate.as<WideState>();
-    uint32_t *fin = c.wfront_a.as<uint32_t>(), *fout = c.wfront_b.as<uint32_t>();
-    k_wide_init<<<1, 1, 0, s>>>(d_st, fin, uint32_t(c.root));
-    int parity = 0;
-    for (const uint32_t nb : c.plan_wide_nb) {
-        k_wide_count<<<nb, BT, 0, s>>>(d_st + parity, fin, c.boxes.as<float>(), c.links.as<int32_t>(), n, c.wblk.as<uint32_t>());
-        k_wide_scan<<<1, 1024, 0, s>>>(d_st + parity, d_st + (parity ^ 1), c.wblk.as<uint32_t>(), nb);
-        k_wide_write<<<nb, BT, 0, s>>>(d_st + parity, fin, fout, c.wblk.as<uint32_t>(), c.boxes.as<float>(),
-                                       c.links.as<int32_t>(), n, c.nodes4.as<Node4>());
-        parity ^= 1;
-        std::swap(fin, fout);
-    }
+    wide_init(c, uint32_t(c.root), s);
+    enqueue_levels(c, uint32_t(c.plan_wide_nb.size()), s);
     return top_table(c, s);
+}
+
+int build_finish(Ctx& c, hipStream_t s) {
+    const uint32_t n = c.ntri;
+    if (n == 0) return 0;
+    const bool wide = c.trace_kernel != RAW_AO_TRACE_BINARY && n > 1;
+    // the build's one read-back: the PLOC tail's final state and the collapse's last state
+    auto* h_ploc = c.h_state;
+    auto* h_wide = reinterpret_cast<WideState*>(reinterpret_cast<char*>(c.h_state) + 128);
+    const auto read_wide = [&]() {
+        return hip_ok(c, hipMemcpyAsync(h_wide, c.wstate.as<WideState>() + c.wide_parity, sizeof(WideState), hipMemcpyDeviceToHost, s),
+                      "wide state");
+    };
+    if (!hip_ok(c, hipMemcpyAsync(h_ploc, c.state.as<PlocState>() + 2, sizeof(PlocState), hipMemcpyDeviceToHost, s), "state read") ||
+        (wide && !read_wide()) || !hip_ok(c, hipStreamSynchronize(s), "build"))
+        return -2;
+    bool redo = false;
+    if (h_ploc->stalled) { c.err = "PLOC round made no merge (degenerate input)"; return -3; }
+    if (h_ploc->m != 1) {   // the enqueued rounds fell short: the rest with read-backs, then the later steps again
+        const int r = ploc_continue(c, s);
+        if (r) return r;
+        redo = true;
+    }
+    const PlocState f = *h_ploc;
+    if (f.m != 1 || int32_t(f.root) != (n > 1 ? int32_t(2 * n - 2) : 0)) {
+        c.err = "PLOC build ended with an unexpected root";
+        return -3;
+    }
+    c.root = int32_t(f.root);
+    c.rounds = f.rounds;
+    c.last_rounds_n = n;
+    c.last_grid_rounds = f.grid_rounds;
+    if (redo) {
+        int r = bvh_layout(c, s);
+        if (!r && wide) r = wide_build(c, s);
+        if (r) return r;
+        if (wide && (!read_wide() || !hip_ok(c, hipStreamSynchronize(s), "build"))) return -2;
+    }
+    if (!wide) return 0;
+    if (h_wide->count != 0) return wide_continue(c, s);   // the enqueued levels fell short
+    c.wide_nodes = h_wide->base;
+    c.last_wide_levels = h_wide->pad;
+    return 0;
 }
 
 namespace {

@@ -131,6 +131,11 @@ int raw_ao_set_option(raw_ao_ctx* x, int option, int value) {
     }
     if (option == RAW_AO_OPT_NEAR_FIRST) { c.near_first = value != 0; return RAW_AO_OK; }
     if (option == RAW_AO_OPT_SKEW) { c.skew = value != 0; return RAW_AO_OK; }
+    if (option == RAW_AO_OPT_BUILD_ROUNDS) {
+        if (value < 0 || value > 4096) return fail(c, RAW_AO_ERR_ARG, "build rounds must be in 0..4096");
+        c.build_rounds = value;
+        return RAW_AO_OK;
+    }
     if (option == RAW_AO_OPT_WAVE_CUT) {
         if (value < 0 || value > 64) return fail(c, RAW_AO_ERR_ARG, "wave cut must be in 0..64");
         c.wave_cut = value;
@@ -182,8 +187,11 @@ int raw_ao_build_ploc_dev(raw_ao_ctx* x, const float* d_tris9, uint32_t ntri, in
     if (r == 0 && c.trace_kernel != RAW_AO_TRACE_BINARY) r = raw_ao::wide_build(c, s);
     c.built_kernel = c.trace_kernel;
     (void)hipEventRecord(c.ev[1], s);
-    if (r != 0) return code_of(r);
-    if (!hip_ok(c, hipStreamSynchronize(s), "build")) return RAW_AO_ERR_DEVICE;
+    if (r == 0) r = raw_ao::build_finish(c, s);   // the build's one wait
+    if (r != 0) {
+        c.root = -1;
+        return code_of(r);
+    }
     c.ms_build = elapsed(c, 0, 1);
     return RAW_AO_OK;
 }

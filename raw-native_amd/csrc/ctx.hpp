@@ -25,7 +25,8 @@ struct PlocState {
     uint32_t stalled;  // sticky: some round with m > 1 made no merge
     uint32_t rounds;   // rounds with m > 1 so far
     uint32_t root;     // the last cluster (final state only)
-    uint32_t pad[2];
+    uint32_t grid_rounds;   // rounds run by the grid kernels (the rest by the one-workgroup tail)
+    uint32_t pad;
 };
 
 struct UNet;
@@ -47,6 +48,7 @@ struct Ctx {
     int wave_cut{2};             // 64-spp wide kernel: > 0 starts rays from the wave's cut at wave_cut x radius
     bool near_first{true};       // 64-spp wide kernel: descend into the nearest hit child first
     bool skew{false};            // the skew control: AO query distances x 0.99f
+    int build_rounds{0};         // PLOC rounds and collapse levels a build enqueues (0: automatic)
     bool count_next{false};
 
     // host-libm sin/cos correction table: a 2 MB bitmap of the corrected angle indices, then one nibble an index
@@ -63,7 +65,7 @@ struct Ctx {
     DevBuf boxes, links;         // (2n-1) x 6 floats, (2n-1) x 3 ints: the reference's node order
     DevBuf clus_a, clus_b, nn, flags, blk;
     DevBuf state;                // PlocState[2]
-    PlocState* h_state{nullptr}; // pinned mirror of the state read back each round
+    PlocState* h_state{nullptr}; // pinned read-back of the build's final states (PLOC, then the collapse's)
     DevBuf g4;                   // centroid box lo xyz and scale
     DevBuf bounds_part;
     // traversal forms
@@ -94,6 +96,13 @@ struct Ctx {
     std::vector<uint32_t> plan_ploc_nb, plan_wide_nb;
     uint32_t plan_n{0};
     int plan_radius{0};
+    // the grid rounds and wide-collapse levels of the last finished build, and of how many
+    // triangles: the next build of that many enqueues as many (a first build estimates them)
+    uint32_t last_rounds_n{0}, last_grid_rounds{0}, last_wide_levels{0};
+    // where the enqueued PLOC rounds and collapse levels leave their state (buffer parity) and
+    // clusters or frontier (true: the first buffer), for a build that has to continue them
+    int ploc_parity{0}, wide_parity{0};
+    bool ploc_in_a{true}, wide_in_a{true};
 
     // the G-buffer rasterizer: per-pixel (depth, draw id) keys, drawn triangles, their box
     // pixel counts and the counts' exclusive scan
@@ -110,7 +119,14 @@ bool ensure(Ctx& c, DevBuf& b, size_t bytes);
 bool hip_ok(Ctx& c, hipError_t e, const char* what);
 
 // Kernel-side entry points (each in its own translation unit).
+// Enqueues the whole PLOC build (bounds to root) with no host wait; build_finish() checks it.
 int ploc_build(Ctx& c, const float* d_tris, uint32_t n, int radius, hipStream_t s);
+// After ploc_build, bvh_layout and wide_build have been enqueued: one read-back of the final
+// PLOC and collapse states (the build's only wait); where the enqueued rounds or levels fell
+// short, the rest run with the cluster count read back, and the later steps again.
+int build_finish(Ctx& c, hipStream_t s);
+// The rest of an unfinished PLOC build (rounds with read-backs, then the tail).
+int ploc_continue(Ctx& c, hipStream_t s);
 int bvh_layout(Ctx& c, hipStream_t s);
 int trig_table_upload(Ctx& c);
 int trace_ao(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_mask, int w, int h, int spp,
@@ -118,6 +134,8 @@ int trace_ao(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_ma
 int angle_table(Ctx& c, float* d_sin, float* d_cos, hipStream_t s);
 int ao_finish(Ctx& c, const uint8_t* d_mask, const uint32_t* d_cnt, uint32_t npix, uint32_t spp, float* d_ao, hipStream_t s);
 int wide_build(Ctx& c, hipStream_t s);
+// The rest of an unfinished collapse (levels with read-backs), then the top table.
+int wide_continue(Ctx& c, hipStream_t s);
 // The word offset of the entry count in Ctx::top.
 size_t top_count_offset();
 int ploc_build_fixed(Ctx& c, const float* d_tris, hipStream_t s);

@@ -185,7 +185,11 @@ int render_frame(Ctx& c, const raw_ao_scene& sc, const raw_ao_render_opts& o, in
         if (r == 0) r = bvh_layout(c, s);
         if (r == 0 && c.trace_kernel != RAW_AO_TRACE_BINARY) r = wide_build(c, s);
         c.built_kernel = c.trace_kernel;
-        if (r) return r;
+        if (r == 0) r = build_finish(c, s);
+        if (r) {
+            c.root = -1;
+            return r;
+        }
         if (model == 2) {
             const float to_light[3] = {ldir[0] * -1.0f, ldir[1] * -1.0f, ldir[2] * -1.0f};
             r = trace_vis4(c, d.position, d.normal, d.mask, w, h, o.rt_samples, o.radius, to_light, o.no_sun, d.ao_rt, d.sky, d.sun,

@@ -15,7 +15,9 @@
 //            not depend on scheduling.
 #include "ao_math.hpp"
 #include "ctx.hpp"
+#include <algorithm>
 #include <cfloat>
+#include <cmath>
 
 namespace raw_ao {
 namespace {
@@ -235,152 +237,178 @@ __global__ __launch_bounds__(BT) void k_leaves(const float* __restrict__ tris, c
 }
 
 // ---- PLOC rounds (rt_bvh.cpp:84-111) ------------------------------------------------
-// Nearest neighbour of every position of this block and of a halo of `radius` on each
-// side, then the merge/survive flags of this block's positions and its two counts.
+// Two launches a round, both with a fixed grid that strides over the tiles of BT positions and
+// reads the cluster count from the device state, so the host enqueues rounds without reading the
+// count back. A round that starts at or below tail_at clusters is a no-op (the state and clusters
+// carry over to the next buffer); the single-workgroup tail runs the rest.
+// Pass 1: the nearest neighbour of every position of a tile and of a halo of `radius` on each
+// side (so mutual pairs are decided inside the tile), then the tile's merge and survivor flags
+// and counts (counts[k * maxt + tile]).
 __global__ __launch_bounds__(BT) void k_ploc_nn(const PlocState* __restrict__ st, int radius, const float* __restrict__ boxes,
                                                 const uint32_t* __restrict__ clus, uint32_t* __restrict__ nn,
-                                                uint8_t* __restrict__ flags, uint32_t* __restrict__ counts, uint32_t nblk) {
+                                                uint8_t* __restrict__ flags, uint32_t* __restrict__ counts, uint32_t maxt,
+                                                uint32_t tail_at) {
     __shared__ float sb[6][BT + 4 * MAX_R];
     __shared__ int32_t snn[BT + 2 * MAX_R];
     __shared__ unsigned long long skey[BT + 2 * MAX_R];
     __shared__ uint32_t wsum[BT / 64][2];
     const int32_t m = int32_t(st->m);
-    const int32_t b0 = int32_t(blockIdx.x) * BT;
-    const int32_t base = b0 - 2 * radius;          // position of sb[*][0]
-    const int span = BT + 4 * radius;
-    for (int k = threadIdx.x; k < span; k += BT) {
-        const int32_t pos = base + k;
-        if (pos >= 0 && pos < m) {
-            const float* b = boxes + 6 * size_t(clus[pos]);
-            for (int a = 0; a < 6; ++a) sb[a][k] = b[a];
-        }
-    }
-    __syncthreads();
-    const int nspan = BT + 2 * radius;             // neighbours of positions b0-radius ..
-    const int32_t q0 = b0 - radius;                // position of snn[0] and skey[0]
-    for (int q = threadIdx.x; q < nspan; q += BT) skey[q] = ~0ull;
-    __syncthreads();
-    // Each pair (p, p + d), d = 1..radius, that some position of the span needs is evaluated
-    // once, by the owner of p (positions b0-2radius .. b0+BT+radius-1): a right candidate of p
-    // and a left candidate of p + d, both by an LDS atomic minimum of (area order, candidate
-    // position). The minimum is the smallest area at the lowest position: the reference's
-    // ascending scan with a strict `<` from FLT_MAX (areas not below FLT_MAX never count).
-    for (int k = threadIdx.x; k < BT + 3 * radius; k += BT) {
-        const int32_t p = base + k;
-        if (p < 0 || p >= m) continue;
-        const bool needs = p >= q0;                // p's own neighbour is wanted
-        const float al0 = sb[0][k], al1 = sb[1][k], al2 = sb[2][k], ah0 = sb[3][k], ah1 = sb[4][k], ah2 = sb[5][k];
-        unsigned long long rb = ~0ull;
-        const int32_t j1 = min(m - 1, p + radius);
-        for (int32_t j = p + 1; j <= j1; ++j) {
-            const int jp = j - base;
-            const float d = area6(stdmin(al0, sb[0][jp]), stdmin(al1, sb[1][jp]), stdmin(al2, sb[2][jp]),
-                                  stdmax(ah0, sb[3][jp]), stdmax(ah1, sb[4][jp]), stdmax(ah2, sb[5][jp]));
-            if (d < FLT_MAX) {
-                const unsigned long long kd = (unsigned long long)area_order(d) << 32;
-                rb = min(rb, kd | uint32_t(j));
-                const int qj = j - q0;
-                if (qj >= 0 && qj < nspan) atomicMin(&skey[qj], kd | uint32_t(p));
+    if (uint32_t(m) <= tail_at) return;
+    const uint32_t ntile = (uint32_t(m) + BT - 1) / BT;
+    for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        __syncthreads();   // the previous tile's LDS reads are done
+        const int32_t b0 = int32_t(tile) * BT;
+        const int32_t base = b0 - 2 * radius;          // position of sb[*][0]
+        const int span = BT + 4 * radius;
+        for (int k = threadIdx.x; k < span; k += BT) {
+            const int32_t pos = base + k;
+            if (pos >= 0 && pos < m) {
+                const float* b = boxes + 6 * size_t(clus[pos]);
+                for (int a = 0; a < 6; ++a) sb[a][k] = b[a];
             }
         }
-        if (needs && rb != ~0ull) atomicMin(&skey[p - q0], rb);
+        const int nspan = BT + 2 * radius;             // neighbours of positions b0-radius ..
+        const int32_t q0 = b0 - radius;                // position of snn[0] and skey[0]
+        for (int q = threadIdx.x; q < nspan; q += BT) skey[q] = ~0ull;
+        __syncthreads();
+        // Each pair (p, p + d), d = 1..radius, that some position of the span needs is evaluated
+        // once, by the owner of p (positions b0-2radius .. b0+BT+radius-1): a right candidate of p
+        // and a left candidate of p + d, both by an LDS atomic minimum of (area order, candidate
+        // position). The minimum is the smallest area at the lowest position: the reference's
+        // ascending scan with a strict `<` from FLT_MAX (areas not below FLT_MAX never count).
+        for (int k = threadIdx.x; k < BT + 3 * radius; k += BT) {
+            const int32_t p = base + k;
+            if (p < 0 || p >= m) continue;
+            const bool needs = p >= q0;                // p's own neighbour is wanted
+            const float al0 = sb[0][k], al1 = sb[1][k], al2 = sb[2][k], ah0 = sb[3][k], ah1 = sb[4][k], ah2 = sb[5][k];
+            unsigned long long rb = ~0ull;
+            const int32_t j1 = min(m - 1, p + radius);
+            for (int32_t j = p + 1; j <= j1; ++j) {
+                const int jp = j - base;
+                const float d = area6(stdmin(al0, sb[0][jp]), stdmin(al1, sb[1][jp]), stdmin(al2, sb[2][jp]),
+                                      stdmax(ah0, sb[3][jp]), stdmax(ah1, sb[4][jp]), stdmax(ah2, sb[5][jp]));
+                if (d < FLT_MAX) {
+                    const unsigned long long kd = (unsigned long long)area_order(d) << 32;
+                    rb = min(rb, kd | uint32_t(j));
+                    const int qj = j - q0;
+                    if (qj >= 0 && qj < nspan) atomicMin(&skey[qj], kd | uint32_t(p));
+                }
+            }
+            if (needs && rb != ~0ull) atomicMin(&skey[p - q0], rb);
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < nspan; q += BT) {
+            const int32_t pos = q0 + q;
+            if (pos < 0 || pos >= m) continue;
+            const unsigned long long best = skey[q];
+            snn[q] = best == ~0ull ? pos : int32_t(uint32_t(best));
+        }
+        __syncthreads();
+        const int32_t pos = b0 + int32_t(threadIdx.x);
+        bool merge = false, survive = false;
+        if (pos < m) {
+            const int32_t j = snn[threadIdx.x + radius];
+            const bool mutual = j != pos && snn[j - (b0 - radius)] == pos;
+            merge = mutual && pos < j;
+            survive = !(mutual && pos > j);
+            nn[pos] = uint32_t(j);
+            flags[pos] = uint8_t((merge ? 1 : 0) | (survive ? 2 : 0));
+        }
+        const unsigned long long bm = __ballot(merge), bs = __ballot(survive);
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) { wsum[w][0] = __popcll(bm); wsum[w][1] = __popcll(bs); }
+        __syncthreads();
+        if (threadIdx.x < 2) {
+            uint32_t s = 0;
+            for (int k = 0; k < BT / 64; ++k) s += wsum[k][threadIdx.x];
+            counts[threadIdx.x * maxt + tile] = s;   // merges then survivors, tile-major
+        }
     }
-    __syncthreads();
-    for (int q = threadIdx.x; q < nspan; q += BT) {
-        const int32_t pos = q0 + q;
-        if (pos < 0 || pos >= m) continue;
-        const unsigned long long best = skey[q];
-        snn[q] = best == ~0ull ? pos : int32_t(uint32_t(best));
+}
+
+// The sums of counts[0 .. t-1] and counts[maxt .. maxt + t-1] over the workgroup (each thread
+// a strided share, then wave and workgroup reductions): a tile's offsets among all tiles. Called
+// by every thread of the workgroup; red: BT / 64 x 2 words of LDS.
+__device__ __forceinline__ void tile_prefix(const uint32_t* __restrict__ counts, uint32_t maxt, uint32_t t, uint32_t (*red)[2],
+                                            uint32_t& pm, uint32_t& ps) {
+    uint32_t a = 0, b = 0;
+    for (uint32_t i = threadIdx.x; i < t; i += BT) {
+        a += counts[i];
+        b += counts[maxt + i];
     }
-    __syncthreads();
-    const int32_t pos = b0 + int32_t(threadIdx.x);
-    bool merge = false, survive = false;
-    if (pos < m) {
-        const int32_t j = snn[threadIdx.x + radius];
-        const bool mutual = j != pos && snn[j - (b0 - radius)] == pos;
-        merge = mutual && pos < j;
-        survive = !(mutual && pos > j);
-        nn[pos] = uint32_t(j);
-        flags[pos] = uint8_t((merge ? 1 : 0) | (survive ? 2 : 0));
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
     }
-    const unsigned long long bm = __ballot(merge), bs = __ballot(survive);
     const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { wsum[w][0] = __popcll(bm); wsum[w][1] = __popcll(bs); }
+    __syncthreads();   // red's earlier readers are done
+    if ((threadIdx.x & 63) == 0) { red[w][0] = a; red[w][1] = b; }
     __syncthreads();
-    if (threadIdx.x < 2) {
-        uint32_t s = 0;
-        for (int k = 0; k < BT / 64; ++k) s += wsum[k][threadIdx.x];
-        counts[threadIdx.x * nblk + blockIdx.x] = s;   // merges then survivors, block-major
-    }
+    pm = ps = 0;
+    for (int k = 0; k < BT / 64; ++k) { pm += red[k][0]; ps += red[k][1]; }
 }
 
-// Exclusive scans of the blocks' merge and survivor counts and the next round's state.
-__global__ __launch_bounds__(1024) void k_ploc_scan(const PlocState* __restrict__ in, PlocState* __restrict__ out,
-                                                    uint32_t* __restrict__ counts, uint32_t nblk) {
-    __shared__ uint32_t s[2][1024];
-    const uint32_t per = (nblk + 1023) / 1024;
-    const uint32_t a = threadIdx.x * per, b = min(nblk, a + per);
-    uint32_t sum[2] = {0, 0};
-    for (int k = 0; k < 2; ++k)
-        for (uint32_t i = a; i < b; ++i) sum[k] += counts[k * nblk + i];
-    s[0][threadIdx.x] = sum[0];
-    s[1][threadIdx.x] = sum[1];
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const uint32_t v0 = threadIdx.x >= uint32_t(off) ? s[0][threadIdx.x - off] : 0u;
-        const uint32_t v1 = threadIdx.x >= uint32_t(off) ? s[1][threadIdx.x - off] : 0u;
-        __syncthreads();
-        s[0][threadIdx.x] += v0;
-        s[1][threadIdx.x] += v1;
-        __syncthreads();
-    }
-    for (int k = 0; k < 2; ++k) {
-        uint32_t run = s[k][threadIdx.x] - sum[k];
-        for (uint32_t i = a; i < b; ++i) { const uint32_t v = counts[k * nblk + i]; counts[k * nblk + i] = run; run += v; }
-    }
-    if (threadIdx.x == 1023) {
-        const PlocState i = *in;
-        PlocState o{};
-        o.m = s[1][1023];
-        o.merges = s[0][1023];
-        o.next_id = i.next_id + o.merges;
-        o.stalled = i.stalled | ((i.m > 1 && o.merges == 0) ? 1u : 0u);
-        o.rounds = i.rounds + (i.m > 1 ? 1u : 0u);
-        *out = o;
-    }
-}
-
-// New nodes from mutual pairs (id = next_id + merge rank, in position order) and the
-// survivors compacted in position order.
-__global__ __launch_bounds__(BT) void k_ploc_merge(const PlocState* __restrict__ st, const uint32_t* __restrict__ nn,
-                                                   const uint8_t* __restrict__ flags, const uint32_t* __restrict__ offs,
-                                                   uint32_t nblk, const uint32_t* __restrict__ cin, uint32_t* __restrict__ cout,
-                                                   float* __restrict__ boxes, int32_t* __restrict__ links) {
+// Pass 2: each tile's offsets among all tiles (the sums of the earlier tiles' counts, read by the
+// tile's own workgroup), then the new nodes from mutual pairs (id = next_id + merge rank, in
+// position order) and the survivors compacted in position order; the last tile writes the next
+// round's state (a copy of this one in a no-op round).
+__global__ __launch_bounds__(BT) void k_ploc_merge(const PlocState* __restrict__ st, PlocState* __restrict__ st_next,
+                                                   const uint32_t* __restrict__ nn, const uint8_t* __restrict__ flags,
+                                                   const uint32_t* __restrict__ counts, uint32_t maxt,
+                                                   const uint32_t* __restrict__ cin, uint32_t* __restrict__ cout,
+                                                   float* __restrict__ boxes, int32_t* __restrict__ links, uint32_t tail_at) {
     __shared__ uint32_t wsum[BT / 64][2];
-    const uint32_t m = st->m;
-    const uint32_t pos = blockIdx.x * BT + threadIdx.x;
-    const uint8_t f = pos < m ? flags[pos] : uint8_t(0);
-    const bool merge = f & 1u, survive = (f >> 1) & 1u;
-    const unsigned long long lt = lanemask_lt();
-    const unsigned long long bm = __ballot(merge), bs = __ballot(survive);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { wsum[w][0] = __popcll(bm); wsum[w][1] = __popcll(bs); }
-    __syncthreads();
-    if (!survive) return;
-    uint32_t S = offs[blockIdx.x] + __popcll(bm & lt), SV = offs[nblk + blockIdx.x] + __popcll(bs & lt);
-    for (int k = 0; k < w; ++k) { S += wsum[k][0]; SV += wsum[k][1]; }
-    const uint32_t a = cin[pos];
-    if (!merge) { cout[SV] = a; return; }
-    const uint32_t id = st->next_id + S;
-    const uint32_t b = cin[nn[pos]];
-    const float* A = boxes + 6 * size_t(a);
-    const float* B = boxes + 6 * size_t(b);
-    float* N = boxes + 6 * size_t(id);
-    for (int k = 0; k < 3; ++k) { N[k] = stdmin(A[k], B[k]); N[3 + k] = stdmax(A[3 + k], B[3 + k]); }
-    links[3 * size_t(id)] = int32_t(a);
-    links[3 * size_t(id) + 1] = int32_t(b);
-    links[3 * size_t(id) + 2] = -1;
-    cout[SV] = id;
+    __shared__ uint32_t red[BT / 64][2];
+    const PlocState in = *st;
+    const uint32_t m = in.m;
+    if (m <= tail_at) {   // a no-op round: the clusters move to the next buffer unchanged
+        for (uint32_t i = blockIdx.x * BT + threadIdx.x; i < m; i += gridDim.x * BT) cout[i] = cin[i];
+        if (blockIdx.x == 0 && threadIdx.x == 0) *st_next = in;
+        return;
+    }
+    const uint32_t ntile = (m + BT - 1) / BT;
+    for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        uint32_t pm, ps;
+        tile_prefix(counts, maxt, tile, red, pm, ps);
+        const uint32_t pos = tile * BT + threadIdx.x;
+        const uint8_t f = pos < m ? flags[pos] : uint8_t(0);
+        const bool merge = f & 1u, survive = (f >> 1) & 1u;
+        const unsigned long long lt = lanemask_lt();
+        const unsigned long long bm = __ballot(merge), bs = __ballot(survive);
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) { wsum[w][0] = __popcll(bm); wsum[w][1] = __popcll(bs); }
+        __syncthreads();
+        if (tile == ntile - 1 && threadIdx.x == 0) {   // the last tile: the next round's state
+            PlocState o{};
+            o.m = ps + counts[maxt + tile];
+            o.merges = pm + counts[tile];
+            o.next_id = in.next_id + o.merges;
+            o.stalled = in.stalled | ((in.m > 1 && o.merges == 0) ? 1u : 0u);
+            o.rounds = in.rounds + (in.m > 1 ? 1u : 0u);
+            o.grid_rounds = in.grid_rounds + 1;
+            *st_next = o;
+        }
+        if (survive) {
+            uint32_t S = pm + __popcll(bm & lt), SV = ps + __popcll(bs & lt);
+            for (int k = 0; k < w; ++k) { S += wsum[k][0]; SV += wsum[k][1]; }
+            const uint32_t a = cin[pos];
+            if (!merge) {
+                cout[SV] = a;
+            } else {
+                const uint32_t id = in.next_id + S;
+                const uint32_t b = cin[nn[pos]];
+                const float* A = boxes + 6 * size_t(a);
+                const float* B = boxes + 6 * size_t(b);
+                float* N = boxes + 6 * size_t(id);
+                for (int k = 0; k < 3; ++k) { N[k] = stdmin(A[k], B[k]); N[3 + k] = stdmax(A[3 + k], B[3 + k]); }
+                links[3 * size_t(id)] = int32_t(a);
+                links[3 * size_t(id) + 1] = int32_t(b);
+                links[3 * size_t(id) + 2] = -1;
+                cout[SV] = id;
+            }
+        }
+        __syncthreads();   // wsum's readers are done before the next tile
+    }
 }
 
 // All remaining rounds once at most TAIL_M clusters are left, in one workgroup with the
@@ -397,7 +425,13 @@ __global__ __launch_bounds__(TAIL_T) void k_ploc_tail(const PlocState* __restric
     __shared__ uint32_t wsum[TAIL_T / 64][2];
     PlocState s = *st_in;
     uint32_t m = s.m;
-    if (m > uint32_t(TAIL_M)) return;   // the host launches this only below TAIL_M
+    if (m > uint32_t(TAIL_M)) {   // too many clusters left for one workgroup: report them, unfinished
+        if (threadIdx.x == 0) {
+            s.root = 0xffffffffu;
+            *st_out = s;
+        }
+        return;
+    }
     for (uint32_t i = threadIdx.x; i < m; i += TAIL_T) {
         const uint32_t c = cin[i];
         sid[i] = c;
@@ -528,6 +562,69 @@ __global__ void k_ploc_init(PlocState* st, uint32_t n) {
 }
 }  // namespace
 
+namespace {
+
+// The grid of the round kernels (they stride over the tiles of the current clusters).
+uint32_t round_grid(uint32_t n) { return std::min<uint32_t>(cdiv(n, BT), 2048u); }
+uint32_t tail_threshold(const Ctx& c) { return uint32_t(std::max(1, std::min(c.ploc_tail, TAIL_M))); }
+
+// Grid rounds to enqueue for n triangles: as many as the last finished build of n triangles ran,
+// else an estimate. The clusters of these scenes shrink by a factor of about 0.83 a round with
+// radius 16 (1.03 M to 4,096 in 30 rounds); 0.87 leaves room, and a build whose rounds fall short
+// continues with read-backs (ploc_continue), so the estimate sets only the speed.
+uint32_t predicted_rounds(const Ctx& c, uint32_t n) {
+    const uint32_t tail_at = tail_threshold(c);
+    if (n <= tail_at) return 0;
+    if (c.build_rounds > 0) return uint32_t(c.build_rounds);
+    if (c.last_rounds_n == n && c.last_grid_rounds) return c.last_grid_rounds;
+    return uint32_t(std::ceil(std::log(double(n) / double(tail_at)) / std::log(1.0 / 0.87))) + 2;
+}
+
+// Enqueues `rounds` grid rounds from the state at c.ploc_parity (clusters in the buffer
+// c.ploc_in_a names), then the tail into state 2.
+void enqueue_rounds(Ctx& c, uint32_t rounds, int radius, hipStream_t s) {
+    const uint32_t n = c.ntri, maxt = cdiv(n, BT), G = round_grid(n), tail_at = tail_threshold(c);
+    PlocState* d_st = c.state.as<PlocState>();
+    for (uint32_t r = 0; r < rounds; ++r) {
+        uint32_t* cin = c.ploc_in_a ? c.clus_a.as<uint32_t>() : c.clus_b.as<uint32_t>();
+        uint32_t* cout = c.ploc_in_a ? c.clus_b.as<uint32_t>() : c.clus_a.as<uint32_t>();
+        const int parity = c.ploc_parity;
+        k_ploc_nn<<<G, BT, 0, s>>>(d_st + parity, radius, c.boxes.as<float>(), cin, c.nn.as<uint32_t>(), c.flags.as<uint8_t>(),
+                                   c.blk.as<uint32_t>(), maxt, tail_at);
+        k_ploc_merge<<<G, BT, 0, s>>>(d_st + parity, d_st + (parity ^ 1), c.nn.as<uint32_t>(), c.flags.as<uint8_t>(),
+                                      c.blk.as<uint32_t>(), maxt, cin, cout, c.boxes.as<float>(), c.links.as<int32_t>(), tail_at);
+        c.ploc_parity ^= 1;
+        c.ploc_in_a = !c.ploc_in_a;
+    }
+    const uint32_t* cin = c.ploc_in_a ? c.clus_a.as<uint32_t>() : c.clus_b.as<uint32_t>();
+    k_ploc_tail<<<1, TAIL_T, 0, s>>>(d_st + c.ploc_parity, d_st + 2, radius, cin, c.boxes.as<float>(), c.links.as<int32_t>());
+}
+
+// Bounds, Morton keys, sort and leaves, then the rounds' initial state.
+void enqueue_front(Ctx& c, const float* d_tris, uint32_t n, hipStream_t s) {
+    const uint32_t nblk_rs = cdiv(n, RS_TILE);
+    const uint32_t nblk = cdiv(n, BT);
+    const uint32_t nparts = std::min<uint32_t>(1024, nblk);
+    k_bounds_part<<<nparts, BT, 0, s>>>(d_tris, n, c.bounds_part.as<float>());
+    k_bounds_final<<<1, BT, 0, s>>>(c.bounds_part.as<float>(), int(nparts), c.g4.as<float>());
+    k_morton<<<nblk, BT, 0, s>>>(d_tris, n, c.g4.as<float>(), c.keys.as<uint32_t>(), c.vals.as<uint32_t>());
+    uint32_t *ki = c.keys.as<uint32_t>(), *vi = c.vals.as<uint32_t>(), *ko = c.keys2.as<uint32_t>(), *vo = c.vals2.as<uint32_t>();
+    for (int shift = 0; shift < 30; shift += 8) {
+        k_radix_hist<<<nblk_rs, BT, 0, s>>>(ki, n, shift, c.hist.as<uint32_t>(), nblk_rs);
+        k_radix_colscan<<<256, BT, 0, s>>>(c.hist.as<uint32_t>(), nblk_rs, c.rs_tot.as<uint32_t>());
+        k_radix_scatter<<<nblk_rs, BT, 0, s>>>(ki, vi, ko, vo, n, shift, c.hist.as<uint32_t>(), nblk_rs, c.rs_tot.as<uint32_t>());
+        std::swap(ki, ko);
+        std::swap(vi, vo);
+    }
+    // four passes: the sorted pairs are back in keys/vals
+    k_leaves<<<nblk, BT, 0, s>>>(d_tris, vi, n, c.boxes.as<float>(), c.links.as<int32_t>(), c.clus_a.as<uint32_t>());
+    k_ploc_init<<<1, 1, 0, s>>>(c.state.as<PlocState>(), n);
+    c.ploc_parity = 0;
+    c.ploc_in_a = true;
+}
+
+}  // namespace
+
 int ploc_build(Ctx& c, const float* d_tris, uint32_t n, int radius, hipStream_t s) {
     c.ntri = n;
     c.radius = radius;
@@ -545,103 +642,54 @@ int ploc_build(Ctx& c, const float* d_tris, uint32_t n, int radius, hipStream_t 
         !ensure(c, c.blk, 8 * size_t(nblk)) || !ensure(c, c.state, 3 * sizeof(PlocState)) || !ensure(c, c.g4, 16) ||
         !ensure(c, c.bounds_part, 6 * 4 * 1024) || !ensure(c, c.rs_tot, 4 * 256))
         return -2;
-    if (!c.h_state && !hip_ok(c, hipHostMalloc(reinterpret_cast<void**>(&c.h_state), 2 * sizeof(PlocState)), "hipHostMalloc"))
-        return -2;
-    const uint32_t nparts = std::min<uint32_t>(1024, nblk);
-    k_bounds_part<<<nparts, BT, 0, s>>>(d_tris, n, c.bounds_part.as<float>());
-    k_bounds_final<<<1, BT, 0, s>>>(c.bounds_part.as<float>(), int(nparts), c.g4.as<float>());
-    k_morton<<<nblk, BT, 0, s>>>(d_tris, n, c.g4.as<float>(), c.keys.as<uint32_t>(), c.vals.as<uint32_t>());
-    uint32_t *ki = c.keys.as<uint32_t>(), *vi = c.vals.as<uint32_t>(), *ko = c.keys2.as<uint32_t>(), *vo = c.vals2.as<uint32_t>();
-    for (int shift = 0; shift < 30; shift += 8) {
-        k_radix_hist<<<nblk_rs, BT, 0, s>>>(ki, n, shift, c.hist.as<uint32_t>(), nblk_rs);
-        k_radix_colscan<<<256, BT, 0, s>>>(c.hist.as<uint32_t>(), nblk_rs, c.rs_tot.as<uint32_t>());
-        k_radix_scatter<<<nblk_rs, BT, 0, s>>>(ki, vi, ko, vo, n, shift, c.hist.as<uint32_t>(), nblk_rs, c.rs_tot.as<uint32_t>());
-        std::swap(ki, ko);
-        std::swap(vi, vo);
-    }
-    // four passes: the sorted pairs are back in keys/vals
-    k_leaves<<<nblk, BT, 0, s>>>(d_tris, vi, n, c.boxes.as<float>(), c.links.as<int32_t>(), c.clus_a.as<uint32_t>());
-    // Rounds: the device keeps the cluster count; the host reads it back only every
-    // kBatch rounds and sizes the grids by the last count it read (an upper bound: the
-    // surplus blocks find no positions and exit). Below TAIL_M clusters one workgroup
-    // runs every remaining round.
-    constexpr int kBatch = 6;
-    PlocState* d_st = c.state.as<PlocState>();
-    k_ploc_init<<<1, 1, 0, s>>>(d_st, n);
-    uint32_t m = n;
-    int parity = 0;
-    uint32_t *cin = c.clus_a.as<uint32_t>(), *cout = c.clus_b.as<uint32_t>();
-    c.plan_ploc_nb.clear();
-    const uint32_t tail_at = uint32_t(std::max(1, std::min(c.ploc_tail, TAIL_M)));
-    while (m > tail_at) {
-        const uint32_t nb = cdiv(m, BT);
-        for (int r = 0; r < kBatch; ++r) {
-            c.plan_ploc_nb.push_back(nb);
-            k_ploc_nn<<<nb, BT, 0, s>>>(d_st + parity, radius, c.boxes.as<float>(), cin, c.nn.as<uint32_t>(),
-                                        c.flags.as<uint8_t>(), c.blk.as<uint32_t>(), nb);
-            k_ploc_scan<<<1, 1024, 0, s>>>(d_st + parity, d_st + (parity ^ 1), c.blk.as<uint32_t>(), nb);
-            k_ploc_merge<<<nb, BT, 0, s>>>(d_st + parity, c.nn.as<uint32_t>(), c.flags.as<uint8_t>(), c.blk.as<uint32_t>(),
-                                           nb, cin, cout, c.boxes.as<float>(), c.links.as<int32_t>());
-            parity ^= 1;
-            std::swap(cin, cout);
-        }
-        if (!hip_ok(c, hipMemcpyAsync(c.h_state, d_st + parity, sizeof(PlocState), hipMemcpyDeviceToHost, s), "state read") ||
-            !hip_ok(c, hipStreamSynchronize(s), "PLOC rounds"))
-            return -2;
-        if (c.h_state->stalled) { c.err = "PLOC round made no merge (degenerate input)"; return -3; }
-        m = c.h_state->m;
-    }
-    k_ploc_tail<<<1, TAIL_T, 0, s>>>(d_st + parity, d_st + 2, radius, cin, c.boxes.as<float>(), c.links.as<int32_t>());
-    if (!hip_ok(c, hipGetLastError(), "PLOC tail launch") ||
-        !hip_ok(c, hipMemcpyAsync(c.h_state, d_st + 2, sizeof(PlocState), hipMemcpyDeviceToHost, s), "state read") ||
-        !hip_ok(c, hipStreamSynchronize(s), "PLOC build"))
-        return -2;
-    if (c.h_state->stalled || c.h_state->m != 1) { c.err = "PLOC round made no merge (degenerate input)"; return -3; }
-    c.rounds = c.h_state->rounds;
-    c.root = int32_t(c.h_state->root);
+    if (!c.h_state && !hip_ok(c, hipHostMalloc(reinterpret_cast<void**>(&c.h_state), 256), "hipHostMalloc")) return -2;
+    // Everything from the bounds to the root is enqueued at once: the round kernels read the
+    // cluster count on the device, the tail finishes below its threshold, and build_finish reads
+    // the final state back once, at the end of the whole build.
+    enqueue_front(c, d_tris, n, s);
+    const uint32_t rounds = predicted_rounds(c, n);
+    enqueue_rounds(c, rounds, radius, s);
+    c.plan_ploc_nb.assign(rounds, round_grid(n));
+    c.root = n > 1 ? int32_t(2 * n - 2) : 0;   // the last merge's node; build_finish confirms it
     c.plan_n = n;
     c.plan_radius = radius;
     return 0;
 }
 
+int ploc_continue(Ctx& c, hipStream_t s) {
+    const uint32_t n = c.ntri;
+    constexpr int kBatch = 6;
+    PlocState* d_st = c.state.as<PlocState>();
+    PlocState* h = c.h_state;
+    for (int batches = 0;; ++batches) {
+        if (!hip_ok(c, hipMemcpyAsync(h, d_st + c.ploc_parity, sizeof(PlocState), hipMemcpyDeviceToHost, s), "state read") ||
+            !hip_ok(c, hipStreamSynchronize(s), "PLOC rounds"))
+            return -2;
+        if (h->stalled) { c.err = "PLOC round made no merge (degenerate input)"; return -3; }
+        if (h->m <= tail_threshold(c)) break;
+        if (batches > int(n)) { c.err = "PLOC rounds did not finish"; return -3; }
+        enqueue_rounds(c, kBatch, c.radius, s);   // its tail reports unfinished while m is above its threshold
+        c.plan_ploc_nb.insert(c.plan_ploc_nb.end(), kBatch, round_grid(n));
+    }
+    enqueue_rounds(c, 0, c.radius, s);
+    if (!hip_ok(c, hipGetLastError(), "PLOC tail launch") ||
+        !hip_ok(c, hipMemcpyAsync(h, d_st + 2, sizeof(PlocState), hipMemcpyDeviceToHost, s), "state read") ||
+        !hip_ok(c, hipStreamSynchronize(s), "PLOC build"))
+        return -2;
+    if (h->stalled || h->m != 1) { c.err = "PLOC round made no merge (degenerate input)"; return -3; }
+    return 0;
+}
 
-// The build with the round grids of the last eager ploc_build of the same triangles: no
-// host copies or waits, so a stream capture can record it. PLOC is deterministic, so the
-// same input takes the same rounds; ploc_fixed_check() confirms it after the run.
+// The build with the round count of the last eager ploc_build of the same triangles: no host
+// copies or waits, so a stream capture can record it. PLOC is deterministic, so the same input
+// takes the same rounds; ploc_fixed_check() confirms it after the run.
 int ploc_build_fixed(Ctx& c, const float* d_tris, hipStream_t s) {
     const uint32_t n = c.ntri;
     const int radius = c.radius;
     if (n == 0) return 0;
     if (c.plan_n != n || c.plan_radius != radius || c.root < 0) { c.err = "no recorded PLOC plan for this scene"; return -1; }
-    const uint32_t nblk_rs = cdiv(n, RS_TILE);
-    const uint32_t nblk = cdiv(n, BT);
-    const uint32_t nparts = std::min<uint32_t>(1024, nblk);
-    k_bounds_part<<<nparts, BT, 0, s>>>(d_tris, n, c.bounds_part.as<float>());
-    k_bounds_final<<<1, BT, 0, s>>>(c.bounds_part.as<float>(), int(nparts), c.g4.as<float>());
-    k_morton<<<nblk, BT, 0, s>>>(d_tris, n, c.g4.as<float>(), c.keys.as<uint32_t>(), c.vals.as<uint32_t>());
-    uint32_t *ki = c.keys.as<uint32_t>(), *vi = c.vals.as<uint32_t>(), *ko = c.keys2.as<uint32_t>(), *vo = c.vals2.as<uint32_t>();
-    for (int shift = 0; shift < 30; shift += 8) {
-        k_radix_hist<<<nblk_rs, BT, 0, s>>>(ki, n, shift, c.hist.as<uint32_t>(), nblk_rs);
-        k_radix_colscan<<<256, BT, 0, s>>>(c.hist.as<uint32_t>(), nblk_rs, c.rs_tot.as<uint32_t>());
-        k_radix_scatter<<<nblk_rs, BT, 0, s>>>(ki, vi, ko, vo, n, shift, c.hist.as<uint32_t>(), nblk_rs, c.rs_tot.as<uint32_t>());
-        std::swap(ki, ko);
-        std::swap(vi, vo);
-    }
-    k_leaves<<<nblk, BT, 0, s>>>(d_tris, vi, n, c.boxes.as<float>(), c.links.as<int32_t>(), c.clus_a.as<uint32_t>());
-    PlocState* d_st = c.state.as<PlocState>();
-    k_ploc_init<<<1, 1, 0, s>>>(d_st, n);
-    int parity = 0;
-    uint32_t *cin = c.clus_a.as<uint32_t>(), *cout = c.clus_b.as<uint32_t>();
-    for (const uint32_t nb : c.plan_ploc_nb) {
-        k_ploc_nn<<<nb, BT, 0, s>>>(d_st + parity, radius, c.boxes.as<float>(), cin, c.nn.as<uint32_t>(), c.flags.as<uint8_t>(),
-                                    c.blk.as<uint32_t>(), nb);
-        k_ploc_scan<<<1, 1024, 0, s>>>(d_st + parity, d_st + (parity ^ 1), c.blk.as<uint32_t>(), nb);
-        k_ploc_merge<<<nb, BT, 0, s>>>(d_st + parity, c.nn.as<uint32_t>(), c.flags.as<uint8_t>(), c.blk.as<uint32_t>(), nb, cin,
-                                       cout, c.boxes.as<float>(), c.links.as<int32_t>());
-        parity ^= 1;
-        std::swap(cin, cout);
-    }
-    k_ploc_tail<<<1, TAIL_T, 0, s>>>(d_st + parity, d_st + 2, radius, cin, c.boxes.as<float>(), c.links.as<int32_t>());
+    enqueue_front(c, d_tris, n, s);
+    enqueue_rounds(c, uint32_t(c.plan_ploc_nb.size()), radius, s);
     return hip_ok(c, hipGetLastError(), "PLOC launch") ? 0 : -2;
 }
 

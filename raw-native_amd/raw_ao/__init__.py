@@ -27,6 +27,7 @@ OPT_HOST_LIBM_TRIG, OPT_TRACE_KERNEL, OPT_TRI_PHASE, OPT_CONV_KERNEL, OPT_PLOC_T
 OPT_NEAR_FIRST = 7
 OPT_WAVE_CUT = 8
 OPT_SKEW = 9
+OPT_BUILD_ROUNDS = 10
 NOALLOC_MAX_PIXELS = 1 << 22
 TRACE_WIDE, TRACE_BINARY = 0, 1
 

@@ -311,6 +311,30 @@ def test_repeatable(ctx):
     assert np.array_equal(bits(a1), bits(a2))
 
 
+@pytest.mark.parametrize("rounds", [1, 3, 7, 200])
+def test_enqueued_build_rounds_do_not_change_the_tree(ctx, rounds):
+    """The build enqueues its PLOC rounds and collapse levels without reading the cluster count
+    back; too few (1, 3, 7: the rest continue with read-backs) or far too many (200: the extra
+    ones do nothing) give the same tree node for node, the same root and rounds, and the same AO
+    as the automatic count."""
+    tris, pos, nrm, mask = big_frame(160, 90, seed=13)
+    ctx.build_ploc(tris, 16)
+    want_tree = ctx.read_tree()
+    want_ao = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
+    ctx.set_option(raw_ao.OPT_BUILD_ROUNDS, rounds)
+    try:
+        ctx.build_ploc(tris, 16)
+        got_tree = ctx.read_tree()
+        got_ao = ctx.compute_rtao(pos, nrm, mask, 64, 1.0)
+    finally:
+        ctx.set_option(raw_ao.OPT_BUILD_ROUNDS, 0)
+    assert got_tree[2:] == want_tree[2:]
+    assert np.array_equal(got_tree[1], want_tree[1]) and np.array_equal(bits(got_tree[0]), bits(want_tree[0]))
+    assert np.array_equal(bits(got_ao), bits(want_ao))
+    ob, ol, oroot, orounds = O.ploc(tris, 16)
+    assert (got_tree[2], got_tree[3]) == (oroot, orounds) and np.array_equal(got_tree[1], ol)
+
+
 def test_ploc_tail_threshold_does_not_change_the_tree(ctx):
     """The single-workgroup tail and the grid-wide rounds run the same rounds: the tree is the
     same whichever takes over where."""
