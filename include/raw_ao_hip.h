@@ -68,7 +68,7 @@ enum {
      * general kernel's sum order: identical outputs to 1), 3 as 0 without the persistent
      * form (identical outputs to 0). */
     RAW_AO_OPT_CONV_KERNEL = 4,
-    /* PLOC tuning, 1..4096 (default 4096): below this many clusters one workgroup runs every
+    /* PLOC tuning, 1..4096 (default 1024): at or below this many clusters one workgroup runs every
      * remaining round. The tree is the same for every value; only the speed differs. */
     RAW_AO_OPT_PLOC_TAIL = 5,
     /* 64-spp wide kernel, 0..64 (default 48): with a value n > 0 a wave owns 4 pixels' 256 rays

@@ -43,7 +43,7 @@ struct Ctx {
     int trace_kernel{0};         // RAW_AO_TRACE_WIDE or RAW_AO_TRACE_BINARY
     int built_kernel{0};         // the form the last build made
     int tri_q{16};               // wide kernel: triangle phase when tri_q/64 of live lanes wait
-    int ploc_tail{4096};         // the single-workgroup PLOC tail takes over at this many clusters
+    int ploc_tail{1024};         // the single-workgroup PLOC tail takes over at this many clusters
     int refill{48};              // 64-spp wide kernel: refill lanes once this many are idle (0: off)
     int wave_cut{2};             // 64-spp wide kernel: > 0 starts rays from the wave's cut at wave_cut x radius
     bool near_first{true};       // 64-spp wide kernel: descend into the nearest hit child first

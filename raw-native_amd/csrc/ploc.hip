@@ -244,7 +244,9 @@ __global__ __launch_bounds__(BT) void k_leaves(const float* __restrict__ tris, c
 // Pass 1: the nearest neighbour of every position of a tile and of a halo of `radius` on each
 // side (so mutual pairs are decided inside the tile), then the tile's merge and survivor flags
 // and counts (counts[k * maxt + tile]).
-__global__ __launch_bounds__(BT) void k_ploc_nn(const PlocState* __restrict__ st, int radius, const float* __restrict__ boxes,
+// R > 0: the radius is R (the reference's kPlocRadius, 16, unrolled); R = 0: `radius` at run time.
+template <int R>
+__global__ __launch_bounds__(BT) void k_ploc_nn(const PlocState* __restrict__ st, int radius_rt, const float* __restrict__ boxes,
                                                 const uint32_t* __restrict__ clus, uint32_t* __restrict__ nn,
                                                 uint8_t* __restrict__ flags, uint32_t* __restrict__ counts, uint32_t maxt,
                                                 uint32_t tail_at) {
@@ -252,6 +254,7 @@ __global__ __launch_bounds__(BT) void k_ploc_nn(const PlocState* __restrict__ st
     __shared__ int32_t snn[BT + 2 * MAX_R];
     __shared__ unsigned long long skey[BT + 2 * MAX_R];
     __shared__ uint32_t wsum[BT / 64][2];
+    const int radius = R > 0 ? R : radius_rt;
     const int32_t m = int32_t(st->m);
     if (uint32_t(m) <= tail_at) return;
     const uint32_t ntile = (uint32_t(m) + BT - 1) / BT;
@@ -283,12 +286,18 @@ __global__ __launch_bounds__(BT) void k_ploc_nn(const PlocState* __restrict__ st
             const float al0 = sb[0][k], al1 = sb[1][k], al2 = sb[2][k], ah0 = sb[3][k], ah1 = sb[4][k], ah2 = sb[5][k];
             unsigned long long rb = ~0ull;
             const int32_t j1 = min(m - 1, p + radius);
-            for (int32_t j = p + 1; j <= j1; ++j) {
+            // candidates p+1 .. p+radius; with R fixed the loop unrolls and its LDS loads issue
+            // together (j - base < BT + 4 radius always: a position past j1 reads an unused entry,
+            // whose result is dropped)
+#pragma unroll
+            for (int d = 1; d <= (R > 0 ? R : MAX_R); ++d) {
+                if (R == 0 && d > radius) break;
+                const int32_t j = p + d;
                 const int jp = j - base;
-                const float d = area6(stdmin(al0, sb[0][jp]), stdmin(al1, sb[1][jp]), stdmin(al2, sb[2][jp]),
-                                      stdmax(ah0, sb[3][jp]), stdmax(ah1, sb[4][jp]), stdmax(ah2, sb[5][jp]));
-                if (d < FLT_MAX) {
-                    const unsigned long long kd = (unsigned long long)area_order(d) << 32;
+                const float ar = area6(stdmin(al0, sb[0][jp]), stdmin(al1, sb[1][jp]), stdmin(al2, sb[2][jp]),
+                                       stdmax(ah0, sb[3][jp]), stdmax(ah1, sb[4][jp]), stdmax(ah2, sb[5][jp]));
+                if (j <= j1 && ar < FLT_MAX) {
+                    const unsigned long long kd = (unsigned long long)area_order(ar) << 32;
                     rb = min(rb, kd | uint32_t(j));
                     const int qj = j - q0;
                     if (qj >= 0 && qj < nspan) atomicMin(&skey[qj], kd | uint32_t(p));
@@ -589,8 +598,12 @@ void enqueue_rounds(Ctx& c, uint32_t rounds, int radius, hipStream_t s) {
         uint32_t* cin = c.ploc_in_a ? c.clus_a.as<uint32_t>() : c.clus_b.as<uint32_t>();
         uint32_t* cout = c.ploc_in_a ? c.clus_b.as<uint32_t>() : c.clus_a.as<uint32_t>();
         const int parity = c.ploc_parity;
-        k_ploc_nn<<<G, BT, 0, s>>>(d_st + parity, radius, c.boxes.as<float>(), cin, c.nn.as<uint32_t>(), c.flags.as<uint8_t>(),
-                                   c.blk.as<uint32_t>(), maxt, tail_at);
+        if (radius == 16)
+            k_ploc_nn<16><<<G, BT, 0, s>>>(d_st + parity, radius, c.boxes.as<float>(), cin, c.nn.as<uint32_t>(),
+                                           c.flags.as<uint8_t>(), c.blk.as<uint32_t>(), maxt, tail_at);
+        else
+            k_ploc_nn<0><<<G, BT, 0, s>>>(d_st + parity, radius, c.boxes.as<float>(), cin, c.nn.as<uint32_t>(),
+                                          c.flags.as<uint8_t>(), c.blk.as<uint32_t>(), maxt, tail_at);
         k_ploc_merge<<<G, BT, 0, s>>>(d_st + parity, d_st + (parity ^ 1), c.nn.as<uint32_t>(), c.flags.as<uint8_t>(),
                                       c.blk.as<uint32_t>(), maxt, cin, cout, c.boxes.as<float>(), c.links.as<int32_t>(), tail_at);
         c.ploc_parity ^= 1;

@@ -346,7 +346,7 @@ def test_ploc_tail_threshold_does_not_change_the_tree(ctx):
             ctx.build_ploc(tris)
             trees.append(ctx.read_tree())
     finally:
-        ctx.set_option(raw_ao.OPT_PLOC_TAIL, 4096)
+        ctx.set_option(raw_ao.OPT_PLOC_TAIL, 1024)
     for b, l, root, rounds in trees[1:]:
         assert np.array_equal(b.view(np.uint32), trees[0][0].view(np.uint32)) and np.array_equal(l, trees[0][1])
         assert (root, rounds) == trees[0][2:]
