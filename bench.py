@@ -267,6 +267,42 @@ def learned_surrogate(ctx, raw_ao, torch, d_pos, d_nrm, d_mask, d_rt, d_ss, mask
     return res
 
 
+def denoise_1080p(ctx, raw_ao, torch, d_pos, d_nrm, d_mask, d_rt, mask, eye, W, H, radius, steps, trace_ms, frame_ms, stream):
+    """The denoiser form on the same 1080p frame: k rays a pixel of ray-traced AO (the first k of
+    the 64-spp frame's own samples), SSAO and the G-buffer into the network trained to give the
+    64-spp ray-traced AO, timed end to end (trace, SSAO, features, network), with RMSE / max error
+    against this frame's 64-spp AO (and a 256-spp reference)."""
+    import raw_ao.learned as L
+    path = L.DENOISER_1080P
+    if not os.path.exists(path):
+        return {"skipped": "no trained weights at raw-native_amd/weights/ao_denoiser_1080p.raow"}
+    _, meta = L.load_weights(path)
+    k = int(meta["noisy_spp"])
+    ctx.net_load(path)
+    d_k, d_dn, d_ref = (torch.empty_like(d_rt) for _ in range(3))
+
+    def frame():
+        ctx.compute_rtao_dev(d_pos, d_nrm, d_mask, W, H, k, radius, d_k, stream)
+        return ctx.learned_ao_dev(d_pos, d_nrm, d_mask, d_k, eye, W, H, d_dn, rt_radius=radius, stream=stream)
+
+    ms, _ = _timed(torch, frame, steps)
+    ms_k, _ = _timed(torch, lambda: ctx.compute_rtao_dev(d_pos, d_nrm, d_mask, W, H, k, radius, d_k, stream), steps)
+    ctx.compute_rtao_dev(d_pos, d_nrm, d_mask, W, H, 256, radius, d_ref, stream)
+    rmse, mx = _rmse(ctx, raw_ao, torch, d_dn, d_rt, d_mask, mask, W, H, stream)
+    rk, mk = _rmse(ctx, raw_ao, torch, d_k, d_rt, d_mask, mask, W, H, stream)
+    r256, _ = _rmse(ctx, raw_ao, torch, d_dn, d_ref, d_mask, mask, W, H, stream)
+    rt64_256, _ = _rmse(ctx, raw_ao, torch, d_rt, d_ref, d_mask, mask, W, H, stream)
+    log("learned denoise 1080p (%d spp in): %.3f ms/frame, RMSE %.4f vs 64-spp RT (%d-spp RT %.4f)" % (k, ms, rmse, k, rk))
+    return {"weights": os.path.basename(path), "noisy_spp": k, "target": meta.get("target"),
+            "trained_on": "%d views of %d seeded scenes (seeds %s) at %dx%d, held-out scene"
+                          % (meta["scenes"] * meta["views_per_scene"], meta["scenes"], meta["scene_seeds"], meta["width"],
+                             meta["height"]),
+            "ms_per_frame": ms, "ms_rt_k_trace": ms_k, "rmse_vs_rt": rmse, "max_error_vs_rt": mx,
+            "rt_k_rmse_vs_rt": rk, "rt_k_max_error_vs_rt": mk, "rmse_vs_rt256": r256, "rt64_rmse_vs_rt256": rt64_256,
+            "rt64_trace_ms": trace_ms, "rt64_frame_ms": frame_ms, "speedup_vs_rt64_trace": trace_ms / ms,
+            "speedup_vs_rt64_frame": frame_ms / ms, "meets_rmse_0.02": rmse <= 0.02}
+
+
 def denoise_4k(ctx, raw_ao, torch, steps, stream):
     """4 rays/pixel ray-traced AO at 3840 x 2160 on the seeded ~2M-triangle scene, denoised by
     the network, against a 256-spp ray-traced reference; beside plain 64-spp ray tracing."""
@@ -565,6 +601,8 @@ def main():
     if not a.no_learned:
         learned = learned_surrogate(ctx, raw_ao, torch, d_pos, d_nrm, d_mask, d_ao, d_ss, mask, eye, W, H, spp, radius,
                                     a.steps, float(np.mean(ms["trace"])), float(np.mean(ms["build"])), frame_ms, stream)
+        learned_dn = denoise_1080p(ctx, raw_ao, torch, d_pos, d_nrm, d_mask, d_ao, mask, eye, W, H, radius, a.steps,
+                                   float(np.mean(ms["trace"])), frame_ms, stream)
 
     rays = covered * spp
     value = rays * a.steps / wall
@@ -612,6 +650,7 @@ def main():
         out["whole_frame"] = whole_frame(ctx, torch, vpos, vnrm, idx, eye, W, H, radius, a.steps, stream)
     if learned is not None:
         out["learned_ao"] = learned
+        out["learned_ao_denoise_1080p"] = learned_dn
     if not a.no_denoise:
         out["denoise_4k"] = denoise_4k(ctx, raw_ao, torch, a.steps, stream)
     if not a.no_batch:

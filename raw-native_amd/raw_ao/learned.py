@@ -5,9 +5,11 @@ Data: seeded scenes from the same generator as the 1M-triangle workload (ground 
 boxes and icospheres, sizes log-uniform in [0.05, 2.0], 40 x 40 x 10 volume) with varying
 object counts, viewed from random orbit cameras; the G-buffer comes from the host
 rasterizer (the reference's raw::rasterize), SSAO and the 64-spp ray-traced AO target from
-this library's GPU kernels. For the denoiser the noisy input is the first 4 samples of the
-same rays and the target the mean of the other 60 ((64 a64 - 4 a4) / 60, exact: AO is a
-count over samples), so input noise and target noise are independent.
+this library's GPU kernels. For the denoiser the noisy input is the first k samples of the
+same rays (--noisy-spp, default 4) and the target either the mean of the other 64 - k
+((64 a64 - k ak) / (64 - k), exact: AO is a count over samples; input noise and target noise
+independent) or the 64-spp AO itself (--target rt64: the network learns the ray-traced frame
+it stands in for, the k input samples being part of it).
 
 The network: include/raw_ao_hip.h raw_ao_net_* (a 4-level U-Net, bf16 MFMA kernels, fp32
 master weights and Adam on the device). Nothing here computes the network on the CPU.
@@ -26,6 +28,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS_DIR = os.path.join(os.path.dirname(HERE), "weights")
 SURROGATE = os.path.join(WEIGHTS_DIR, "ao_surrogate.raow")
 DENOISER = os.path.join(WEIGHTS_DIR, "ao_denoiser.raow")
+# trained at 1920 x 1080 on 8-spp inputs to give the 64-spp frame they are the first samples of
+DENOISER_1080P = os.path.join(WEIGHTS_DIR, "ao_denoiser_1080p.raow")
 FEATURES = 8
 
 # (c0, c1, cout): layer shapes in parameter order (as raw_ao_net_param_count lays them out)
@@ -111,9 +115,13 @@ class ViewSet:
 
 
 def render_views(ctx, n_scenes, views_per_scene, w, h, seed=1000, spp=64, radius=1.0, ssao_samples=24, ssao_radius=1.0,
-                 noisy_spp=0, threads=16, log=print, stream=None, objects=(600, 3000)):
+                 noisy_spp=0, threads=16, log=print, stream=None, objects=(600, 3000), target="independent"):
     """Renders n_scenes x views_per_scene views into a ViewSet. The scene seeds are
-    seed + i; the 1M-triangle workload's scene (seed 0x5EED) is never among them."""
+    seed + i; the 1M-triangle workload's scene (seed 0x5EED) is never among them. With a noisy
+    input, target "independent" is the mean of the spp - noisy_spp samples the input does not
+    hold (an estimate of the expectation), "rt64" the spp-spp ray-traced AO itself, whose first
+    noisy_spp samples are the input's (the network then learns the ray-traced frame it stands
+    in for)."""
     import torch
     assert not (seed <= host.SCENE_1M.seed < seed + n_scenes)
     rng = np.random.default_rng(seed)
@@ -139,8 +147,8 @@ def render_views(ctx, n_scenes, views_per_scene, w, h, seed=1000, spp=64, radius
                 ctx.compute_rtao_dev(pos_d, nrm_d, vs.mask[v], w, h, spp, radius, vs.target[v], st)
                 if noisy_spp:
                     ctx.compute_rtao_dev(pos_d, nrm_d, vs.mask[v], w, h, noisy_spp, radius, vs.noisy[v], st)
-                    # the target from the other samples only: independent of the input's noise
-                    vs.target[v] = (vs.target[v] * spp - vs.noisy[v] * noisy_spp) / (spp - noisy_spp)
+                    if target == "independent":   # the target from the other samples only
+                        vs.target[v] = (vs.target[v] * spp - vs.noisy[v] * noisy_spp) / (spp - noisy_spp)
                 ctx.net_features_dev(pos_d, nrm_d, vs.mask[v], vs.ssao[v], vs.noisy[v] if noisy_spp else None, cam.eye, w,
                                      h, vs.x[v], st)
                 vs.eyes[v] = cam.eye
@@ -297,6 +305,9 @@ def main(argv=None):
     ap.add_argument("--eval-every", type=int, default=500)
     ap.add_argument("--init", default=None, help="start from this weights file")
     ap.add_argument("--objects", type=int, nargs=2, default=(600, 3000), help="object count range of the scenes")
+    ap.add_argument("--noisy-spp", type=int, default=4, help="denoiser: rays a pixel of the noisy input")
+    ap.add_argument("--target", choices=["independent", "rt64"], default="independent",
+                    help="denoiser: the mean of the samples the input does not hold, or the 64-spp AO itself")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
 
@@ -307,16 +318,17 @@ def main(argv=None):
     import torch
     torch.cuda.init()
     ctx = raw_ao.Context(0)
-    noisy = 4 if a.mode == "denoiser" else 0
+    noisy = a.noisy_spp if a.mode == "denoiser" else 0
     t0 = time.time()
     vs = render_views(ctx, a.scenes, a.views, a.width, a.height, seed=a.seed, noisy_spp=noisy, log=log,
-                      objects=tuple(a.objects))
+                      objects=tuple(a.objects), target=a.target)
     gen_s = time.time() - t0
     init = load_weights(a.init)[0] if a.init else None
     params, hist = train(ctx, vs, a.minutes, batch=a.batch, crop=a.crop, lr_max=a.lr, seed=a.seed, log=log,
                          eval_every=a.eval_every, params=init)
     meta = dict(mode=a.mode, scenes=a.scenes, views_per_scene=a.views, width=a.width, height=a.height, rt_spp=64,
-                rt_radius=1.0, ssao_samples=24, ssao_radius=1.0, noisy_spp=noisy, scene_seeds=[a.seed, a.seed + a.scenes - 1],
+                rt_radius=1.0, ssao_samples=24, ssao_radius=1.0, noisy_spp=noisy, target=a.target if noisy else "rt64",
+                scene_seeds=[a.seed, a.seed + a.scenes - 1],
                 objects=list(a.objects),
                 batch=a.batch, crop=a.crop, lr=a.lr, minutes=a.minutes, render_seconds=round(gen_s, 1),
                 history=hist, init=os.path.basename(a.init) if a.init else None)

@@ -6,6 +6,8 @@
              1920 x 1080 on the ~1M-triangle scene's seeded orbit view.
   denoiser   RMSE against 256-spp ray tracing: the network's AO from 4-spp ray tracing vs
              the 4-spp ray tracing itself, at 3840 x 2160 on the ~2M-triangle scene.
+  1080p      the denoiser trained on 8-spp inputs to give the 64-spp frame: against that frame
+             and against 256-spp ray tracing, at 1920 x 1080 on the ~1M-triangle scene.
 """
 import os
 
@@ -69,3 +71,25 @@ def test_denoiser_beats_low_spp_ray_tracing(ctx):
     r_dn, r_4 = rmse(ctx, dn, ref, M, mask, w, h), rmse(ctx, rt4, ref, M, mask, w, h)
     print("denoised RMSE %.4f vs 4-spp %.4f" % (r_dn, r_4))
     assert r_dn < 0.5 * r_4
+
+
+@pytest.mark.skipif(not os.path.exists(L.DENOISER_1080P), reason="no trained 1080p denoiser weights")
+def test_1080p_denoiser_from_8_spp(ctx):
+    w, h = 1920, 1080
+    tris, eye, mask, P, N, M = frame(raw_ao.host.SCENE_1M, w, h)
+    ctx.build_ploc(tris)
+    img = lambda: torch.empty((h, w), dtype=torch.float32, device="cuda")
+    ref, rt64, rt8, dn = img(), img(), img(), img()
+    ctx.compute_rtao_dev(P, N, M, w, h, 256, 1.0, ref)
+    ctx.compute_rtao_dev(P, N, M, w, h, 64, 1.0, rt64)
+    ctx.compute_rtao_dev(P, N, M, w, h, 8, 1.0, rt8)
+    ctx.net_load(L.DENOISER_1080P)
+    ctx.learned_ao_dev(P, N, M, rt8, eye, w, h, dn, rt_radius=1.0)
+    torch.cuda.synchronize()
+    r_dn, r_8 = rmse(ctx, dn, rt64, M, mask, w, h), rmse(ctx, rt8, rt64, M, mask, w, h)
+    r_dn256, r_64_256 = rmse(ctx, dn, ref, M, mask, w, h), rmse(ctx, rt64, ref, M, mask, w, h)
+    print("1080p denoised RMSE %.4f vs 64 spp (8-spp input %.4f); vs 256 spp %.4f (64 spp %.4f)"
+          % (r_dn, r_8, r_dn256, r_64_256))
+    assert r_dn < 0.5 * r_8
+    # closer to the 256-spp frame than the 64-spp frame it was trained to give
+    assert r_dn256 < r_64_256
