@@ -666,6 +666,7 @@ struct TraceArgs4 {
     const uint32_t* ntop;
     float* sky;            // SKY: sky out (64 spp) or the unblocked counts (below)
     uint32_t* cnt_free;
+    uint64_t* open_mask;   // k_trace_ao4rc<.., true> out, k_sky4r in: bit s of a pixel = its AO sample s is open
 };
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -1129,14 +1130,17 @@ __global__ __launch_bounds__(BT) void k_trace_ao4r(TraceArgs4 a, uint32_t refill
 // wave's pixels into LDS, and the host-libm angle steps of all PW x 64 rays are read at the start,
 // their loads in flight together, and kept as one nibble a ray (lane L holds sample L of each
 // pixel); a refilled ray takes its nibble from that lane with a shuffle.
+// MASK: also a.open_mask, each pixel's open samples as bits (gathered by LDS atomics as the rays
+// end; 0 for a pixel off the mask), for the visibility pass's sky rays (k_sky4r).
 constexpr int PIX_WORDS = 16;   // per pixel: origin, n, t, b (12 floats), x, y
-template <int PW, bool COUNT>
+template <int PW, bool COUNT, bool MASK = false>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_trace_ao4rc(TraceArgs4 a,
                                                                                                 uint32_t refill_min) {
     static_assert(PW * 4 <= 32, "one 4-bit angle step a pixel in a 32-bit register");
     __shared__ uint32_t stk[STACK_LDS * BT];
     __shared__ uint32_t cutmem[CUT_WAVE * (BT / 64)];
     __shared__ float pixmem[PIX_WORDS * PW * (BT / 64)];
+    __shared__ uint32_t openmem[MASK ? 2 * PW * (BT / 64) : 1];
     uint32_t* const lds_col = stk + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1145,16 +1149,22 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6, 6))) void
     const unsigned long long below = (1ull << lane) - 1ull;
     uint32_t cnt[PW];
     for (int k = 0; k < PW; ++k) cnt[k] = 0;
+    uint32_t* const om = openmem + (MASK ? wave * 2 * PW : 0);
+    if (MASK && lane < 2u * PW) om[lane] = 0u;
     const auto finish = [&]() {
+        if (MASK) wave_sync();
         for (int k = 0; k < PW; ++k) {
             uint32_t v = cnt[k];
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
             const uint32_t pix = pix0 + uint32_t(k);
             if (lane == 0 && pix < a.npix) a.ao[pix] = a.mask[pix] ? float(v) / 64.0f : 1.0f;
+            if (MASK && lane == 0 && pix < a.npix)
+                a.open_mask[pix] = a.mask[pix] ? (uint64_t(om[2 * k + 1]) << 32) | om[2 * k] : 0ull;
         }
     };
     if (a.ntri == 0) {   // nothing to hit: every covered pixel's rays are open
         for (int k = 0; k < PW; ++k) cnt[k] = 1;
+        if (MASK && lane < 2u * PW) om[lane] = ~0u;
         finish();
         return;
     }
@@ -1230,7 +1240,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6, 6))) void
     uint32_t spill[STACK_SPILL];
     int sp = 0, pk = 0, ci = 0;
     int32_t node = -1;
-    uint32_t pend = 0, lp = 0, work = 0;
+    uint32_t pend = 0, lp = 0, work = 0, cur_s = 0;
     int4 leafref = make_int4(0, 0, 0, 0);
     const auto next_cut = [&]() -> int32_t {
         while (ci < ncut) {
@@ -1244,6 +1254,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6, 6))) void
     const auto setup = [&](uint32_t ri, uint32_t nib) -> bool {
         pk = int(ri >> 6);
         const uint32_t s = ri & 63u;
+        if (MASK) cur_s = s;
         if (!((covmask >> pk) & 1u)) return false;
         const float* q = pm + pk * PIX_WORDS;
         const V3 origin = v3(q[0], q[1], q[2]), n = v3(q[3], q[4], q[5]), t = v3(q[6], q[7], q[8]), b = v3(q[9], q[10], q[11]);
@@ -1270,6 +1281,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6, 6))) void
     };
     const auto count_open = [&]() {
         for (int k = 0; k < PW; ++k) cnt[k] += pk == k ? 1u : 0u;
+        if (MASK) atomicOr(&om[2 * pk + (cur_s >> 5)], 1u << (cur_s & 31u));
     };
     const uint32_t total = PW * 64u;
     bool live = setup(lane, fixes & 15u);
@@ -1354,6 +1366,195 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6, 6))) void
         atomicAdd(&a.counters->rays, (unsigned long long)rays);
     }
     finish();
+}
+
+// The sky rays of the visibility pass at 64 spp, traced apart from its AO rays: a wave takes G
+// consecutive pixels, lists the open AO samples of each (a.open_mask, from
+// k_trace_ao4rc<.., true>) in LDS, and traces them one a lane to tMax 1e30 from the root, the
+// box tests starting at sky_from, children in slot order. A lane whose ray ends takes the next
+// listed ray once refill_min lanes are idle (or all are), so the lanes stay busy until the list
+// runs out, where a wave of one pixel's rays idles while its longest ray runs (measured with 2, 4,
+// 8, 16 and 32 pixels a wave at 1080p on the 1M-triangle scene: 8 is the fastest). A pixel's sky is
+// its unblocked rays / 64 (counted in LDS), 1 off the mask: the same rays and the same any-hit
+// answers as k_vis4c, so the same values.
+template <int G>
+__global__ __launch_bounds__(BT) void k_sky4r(TraceArgs4 a, uint32_t refill_min) {
+    static_assert(G <= 16 && (G & (G - 1)) == 0, "one pixel a lane for the set-up, 3 bits a pixel in klass");
+    __shared__ uint32_t stk[STACK_LDS * BT];
+    __shared__ float pixmem[PIX_WORDS * G * (BT / 64)];
+    __shared__ uint16_t raylist[64 * G * (BT / 64)];
+    __shared__ uint32_t nfree[G * (BT / 64)];
+    uint32_t* const lds_col = stk + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t pix0 = (blockIdx.x * (BT / 64) + wave) * G;
+    if (pix0 >= a.npix) return;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    float* const pm = pixmem + wave * (PIX_WORDS * G);
+    uint16_t* const lst = raylist + wave * (64 * G);
+    uint32_t* const nf = nfree + wave * G;
+    // lane k < G: pixel pix0 + k's set-up (as k_trace_ao4rc's) and its open samples
+    const uint32_t pixk = pix0 + lane;
+    const bool inimg = lane < uint32_t(G) && pixk < a.npix;
+    const bool cov = inimg && a.mask[pixk];
+    uint64_t m = 0;
+    if (cov) {
+        const V3 p = v3(a.pos[3 * size_t(pixk)], a.pos[3 * size_t(pixk) + 1], a.pos[3 * size_t(pixk) + 2]);
+        const V3 n = v3(a.nrm[3 * size_t(pixk)], a.nrm[3 * size_t(pixk) + 1], a.nrm[3 * size_t(pixk) + 2]);
+        const V3 o = add(p, scl(n, 1e-3f));
+        V3 t, b;
+        basis(n, t, b);
+        const uint32_t y = pixk / a.w, x = pixk - y * a.w;
+        float* q = pm + lane * PIX_WORDS;
+        q[0] = o.x; q[1] = o.y; q[2] = o.z;
+        q[3] = n.x; q[4] = n.y; q[5] = n.z;
+        q[6] = t.x; q[7] = t.y; q[8] = t.z;
+        q[9] = b.x; q[10] = b.y; q[11] = b.z;
+        q[12] = __uint_as_float(x);
+        q[13] = __uint_as_float(y);
+        m = a.open_mask[pixk];
+    }
+    if (lane < uint32_t(G)) nf[lane] = a.ntri ? 0u : uint32_t(__popcll(m));   // nothing to hit: every open ray is unblocked
+    wave_sync();
+    // The wave's list of rays, entries (pixel << 6) | sample, the most grazing first: by the top
+    // two bits of the sample's u1 (the squared sine of its angle from the normal), descending. A
+    // grazing sky ray crosses more of the scene than a steep one, so the long rays start first and
+    // the list ends with short ones, which keeps the lanes busy to the end. One pass counts the
+    // four classes, a second writes them; lane s handles sample s of each pixel.
+    uint32_t total = 0;
+    uint64_t klass = 0;
+    uint32_t base_c[4] = {0, 0, 0, 0};
+    if (a.ntri) {
+        for (int k = 0; k < G; ++k) {
+            const uint32_t mlo = uint32_t(__shfl(int(uint32_t(m)), k)), mhi = uint32_t(__shfl(int(uint32_t(m >> 32)), k));
+            const bool open = ((lane < 32u ? mlo >> lane : mhi >> (lane - 32u)) & 1u) != 0u;
+            const uint32_t x = __float_as_uint(pm[k * PIX_WORDS + 12]), y = __float_as_uint(pm[k * PIX_WORDS + 13]);
+            const uint32_t cl = open ? 1u + (hash24(x, y, 2 * lane) >> 22) : 0u;   // 0: closed, else 1 + class
+            klass |= uint64_t(cl) << (3 * k);
+            for (int c4 = 0; c4 < 4; ++c4) base_c[c4] += uint32_t(__popcll(__ballot(cl == uint32_t(c4) + 1u)));
+        }
+        // classes 3, 2, 1, 0 in that order
+        const uint32_t n3 = base_c[3], n2 = base_c[2], n1 = base_c[1];
+        total = n3 + n2 + n1 + base_c[0];
+        base_c[3] = 0;
+        base_c[2] = n3;
+        base_c[1] = n3 + n2;
+        base_c[0] = n3 + n2 + n1;
+        for (int k = 0; k < G; ++k) {
+            const uint32_t cl = uint32_t(klass >> (3 * k)) & 7u;
+            for (int c4 = 0; c4 < 4; ++c4) {
+                const unsigned long long bal = __ballot(cl == uint32_t(c4) + 1u);
+                if (cl == uint32_t(c4) + 1u) lst[base_c[c4] + uint32_t(__popcll(bal & below))] = uint16_t((uint32_t(k) << 6) | lane);
+                base_c[c4] += uint32_t(__popcll(bal));
+            }
+        }
+    }
+    wave_sync();
+    constexpr float kFarT = 1e30f;
+    const float reach = kFarT * kReach;
+    const char* const base = reinterpret_cast<const char*>(a.nodes);
+    Ray r;
+    uint32_t ox = 0, oy = 0, oz = 0;
+    float tlo = 0.0f;
+    uint32_t spill[STACK_SPILL];
+    int sp = 0, pk = 0;
+    int32_t node = -1;
+    uint32_t pend = 0, work = 0;
+    int4 leafref = make_int4(0, 0, 0, 0);
+    const auto setup = [&](uint32_t e) {
+        pk = int(e >> 6);
+        const uint32_t s = e & 63u;
+        const float* q = pm + pk * PIX_WORDS;
+        const V3 origin = v3(q[0], q[1], q[2]), n = v3(q[3], q[4], q[5]), t = v3(q[6], q[7], q[8]), b = v3(q[9], q[10], q[11]);
+        const uint32_t x = __float_as_uint(q[12]), y = __float_as_uint(q[13]);
+        const uint32_t k1 = hash24(x, y, 2 * s), k2 = hash24(x, y, 2 * s + 1);
+        const float u1 = float(k1) / float(0x1000000), u2 = float(k2) / float(0x1000000);
+        float sphi, cphi;
+        sincos_base(6.2831853f * u2, sphi, cphi);
+        if (a.trig) trig_fix(a.trig, k2, sphi, cphi);
+        r = make_ray(origin, ao_dir(u1, sphi, cphi, n, t, b));
+        ox = r.inv.x >= 0.0f ? 0u : 48u;
+        oy = r.inv.y >= 0.0f ? 0u : 48u;
+        oz = r.inv.z >= 0.0f ? 0u : 48u;
+        tlo = sky_from(origin, a.radius);
+        sp = 0;
+        pend = 0;
+        work = 0;
+        node = 0;   // the root
+    };
+    bool live = false;
+    uint32_t next = 0;
+    while (true) {
+        const unsigned long long idle = __ballot(!live);
+        const uint32_t nidle = uint32_t(__popcll(idle));
+        if (next < total && (nidle >= refill_min || nidle == 64u)) {
+            if (!live) {
+                const uint32_t ri = next + uint32_t(__popcll(idle & below));
+                if (ri < total) {
+                    setup(lst[ri]);
+                    live = true;
+                }
+            }
+            next += nidle;
+        }
+        // with every lane idle the refill above takes the rest of the list, so none live = done
+        if (__ballot(live) == 0ull) break;
+        if (live && pend == 0 && node >= 0) {
+            const uint32_t off = uint32_t(node) * uint32_t(sizeof(Node4));
+            const uint32_t hit = slab4s(r, base, off, ox, oy, oz, reach, kExit, tlo);
+            const int4 ref = *reinterpret_cast<const int4*>(base + off + 96);
+            const uint32_t masks = *reinterpret_cast<const uint32_t*>(base + off + 112);
+            pend = hit & masks & 15u;
+            leafref = ref;
+            bool overflow = false;
+            uint32_t ih = hit & (masks >> 4);
+            if (ih) {
+                node = ref_of(ref, __builtin_ctz(ih));
+                ih &= ih - 1u;
+                while (ih) {
+                    const uint32_t v = uint32_t(ref_of(ref, __builtin_ctz(ih)));
+                    ih &= ih - 1u;
+                    if (sp < STACK_LDS) lds_col[sp * BT] = v;
+                    else if (sp < STACK_LDS + STACK_SPILL) spill[sp - STACK_LDS] = v;
+                    else overflow = true;
+                    ++sp;
+                }
+            } else if (sp > 0) {
+                --sp;
+                node = int32_t(sp < STACK_LDS ? lds_col[sp * BT] : spill[sp - STACK_LDS]);
+            } else {
+                node = -1;
+            }
+            if (++work > a.ntri + 4) overflow = true;
+            if (overflow) {   // answered by the scan of every triangle
+                if (!scan_occluded(r, kFarT, a.tris, a.ntri)) atomicAdd(&nf[pk], 1u);
+                atomicAdd(&a.counters->overflows, 1ull);
+                live = false;
+                pend = 0;
+            }
+        }
+        const bool waiting = live && pend != 0;
+        const bool can_step = live && pend == 0 && node >= 0;
+        const unsigned long long wb = __ballot(waiting), sb = __ballot(can_step), lb = __ballot(live);
+        if (lb != 0 && (sb == 0 || 64u * uint32_t(__popcll(wb)) >= a.tri_q * uint32_t(__popcll(lb)))) {
+            if (waiting) {
+                const int k = __builtin_ctz(pend);
+                pend &= pend - 1u;
+                const uint32_t slot = uint32_t(~ref_of(leafref, k));
+                float t;
+                if (slot < a.ntri && tri_hit(r, a.tris, slot, t) && t < kFarT) {
+                    live = false;   // blocked
+                    pend = 0;
+                }
+            }
+        }
+        if (live && pend == 0 && node < 0) {   // finished: unblocked
+            atomicAdd(&nf[pk], 1u);
+            live = false;
+        }
+    }
+    wave_sync();
+    if (inimg) a.sky[pixk] = cov ? float(nf[lane]) / 64.0f : 1.0f;
 }
 
 // Lighting model 2's visibility (computeRTVisibility, src/renderer/ray_ao.cpp:20-63): the AO
@@ -1822,7 +2023,7 @@ int trace_vis4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_
                const float to_light[3], int sun_forced, float* d_ao, float* d_sky, float* d_sun, hipStream_t s) {
     const uint32_t npix = uint32_t(w) * uint32_t(h);
     if (!ensure(c, c.counters, sizeof(TraceCounters))) return -2;
-    if (!hip_ok(c, hipMemsetAsync(c.counters.p, 0, sizeof(TraceCounters), s), "counter clear")) return -2;
+    k_zero_u32<<<1, BT, 0, s>>>(c.counters.as<uint32_t>(), sizeof(TraceCounters) / 4);
     TraceArgs4 a{d_pos, d_nrm, d_mask, d_ao, nullptr, c.nodes4.as<Node4>(), c.tris_sorted.as<float4>(),
                  c.host_trig ? c.d_trig : nullptr, c.counters.as<TraceCounters>(), uint32_t(w), npix, uint32_t(spp),
                  c.ntri, uint32_t(c.tri_q), radius};
@@ -1832,7 +2033,7 @@ int trace_vis4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_
         if (!ensure(c, c.st_cnt, 8 * size_t(npix))) return -2;
         a.cnt = c.st_cnt.as<uint32_t>();
         cnt_free = a.cnt + npix;
-        if (!hip_ok(c, hipMemsetAsync(a.cnt, 0, 8 * size_t(npix), s), "count clear")) return -2;
+        k_zero_u32<<<std::min<uint32_t>(2048, cdiv(2 * npix, BT)), BT, 0, s>>>(a.cnt, 2 * size_t(npix));
     }
     const uint32_t blocks = cdiv(uint64_t(npix) * uint64_t(spp), BT);
     const bool cut = c.wave_cut > 0 && c.ntri > 0;
@@ -1844,8 +2045,17 @@ int trace_vis4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_
         a.top = c.top.as<TopEntry>();
         a.ntop = c.top.as<uint32_t>() + top_count_offset();
     }
+    // 64 spp: the AO rays by the refill kernel (which also records each pixel's open samples),
+    // then the open ones' sky rays by k_sky4r
+    const bool split = cut && spp64 && c.refill > 0 && c.sky_refill > 0;
+    if (split && !ensure(c, c.vis_open, 8 * size_t(npix))) return -2;
     (void)hipEventRecord(c.ev[2], s);
-    if (cut && spp64) k_vis4c<true><<<blocks, BT, 0, s>>>(a);
+    constexpr int PW = 4, SKY_G = 8;
+    if (split) {
+        a.open_mask = c.vis_open.as<uint64_t>();
+        k_trace_ao4rc<PW, false, true><<<cdiv(npix, uint32_t(PW * (BT / 64))), BT, 0, s>>>(a, uint32_t(c.refill));
+        k_sky4r<SKY_G><<<cdiv(npix, uint32_t(SKY_G * (BT / 64))), BT, 0, s>>>(a, uint32_t(c.sky_refill));
+    } else if (cut && spp64) k_vis4c<true><<<blocks, BT, 0, s>>>(a);
     else if (cut) k_vis4c<false><<<blocks, BT, 0, s>>>(a);
     else if (spp64) k_trace_vis4<true><<<blocks, BT, 0, s>>>(a, d_sky, nullptr);
     else k_trace_vis4<false><<<blocks, BT, 0, s>>>(a, d_sky, cnt_free);

@@ -102,7 +102,7 @@ void raw_ao_close(raw_ao_ctx* x) {
         raw_ao::net_release(c);
         raw_ao::DevBuf* bufs[] = {&c.tris_in, &c.keys, &c.vals, &c.keys2, &c.vals2, &c.hist, &c.rs_tot, &c.boxes, &c.links,
                                   &c.clus_a, &c.clus_b, &c.nn, &c.flags, &c.blk, &c.state, &c.g4, &c.bounds_part,
-                                  &c.nodes, &c.tris_sorted, &c.nodes4, &c.wfront_a, &c.wfront_b, &c.wblk, &c.wstate, &c.st_pos, &c.st_nrm, &c.st_mask, &c.st_out, &c.st_out2,
+                                  &c.nodes, &c.tris_sorted, &c.nodes4, &c.wfront_a, &c.wfront_b, &c.wblk, &c.wstate, &c.vis_open, &c.st_pos, &c.st_nrm, &c.st_mask, &c.st_out, &c.st_out2,
                                   &c.st_cnt, &c.st_rays, &c.st_max, &c.counters, &c.r_keys, &c.r_tris, &c.r_counts,
                                   &c.r_offs, &c.r_scan_tmp, &c.fr_mesh, &c.fr_scalars, &c.fr_planes};
         for (auto* b : bufs)
@@ -139,6 +139,11 @@ int raw_ao_set_option(raw_ao_ctx* x, int option, int value) {
     if (option == RAW_AO_OPT_WAVE_CUT) {
         if (value < 0 || value > 64) return fail(c, RAW_AO_ERR_ARG, "wave cut must be in 0..64");
         c.wave_cut = value;
+        return RAW_AO_OK;
+    }
+    if (option == RAW_AO_OPT_SKY_REFILL) {
+        if (value < 0 || value > 64) return fail(c, RAW_AO_ERR_ARG, "sky refill threshold must be in 0..64");
+        c.sky_refill = value;
         return RAW_AO_OK;
     }
     if (option == RAW_AO_OPT_REFILL) {

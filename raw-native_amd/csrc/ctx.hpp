@@ -47,6 +47,7 @@ struct Ctx {
     int refill{48};              // 64-spp wide kernel: refill lanes once this many are idle (0: off)
     int wave_cut{2};             // 64-spp wide kernel: > 0 starts rays from the wave's cut at wave_cut x radius
     bool near_first{true};       // 64-spp wide kernel: descend into the nearest hit child first
+    int sky_refill{32};          // 64-spp visibility: sky rays apart from the AO rays, lanes refilled at this many idle (0: off)
     bool skew{false};            // the skew control: AO query distances x 0.99f
     int build_rounds{0};         // PLOC rounds and collapse levels a build enqueues (0: automatic)
     bool count_next{false};
@@ -75,6 +76,7 @@ struct Ctx {
     DevBuf nodes4;               // 4-wide records, 128 B each, root at 0
     DevBuf top;                  // the wide tree's top table (slots to depth 3) and its entry count
     DevBuf wfront_a, wfront_b, wblk, wstate;
+    DevBuf vis_open;             // the visibility pass's per-pixel open AO samples (64 bits a pixel)
     uint32_t wide_nodes{0};
 
     // staging for host-pointer entry points

@@ -28,6 +28,7 @@ OPT_NEAR_FIRST = 7
 OPT_WAVE_CUT = 8
 OPT_SKEW = 9
 OPT_BUILD_ROUNDS = 10
+OPT_SKY_REFILL = 11
 NOALLOC_MAX_PIXELS = 1 << 22
 TRACE_WIDE, TRACE_BINARY = 0, 1
 

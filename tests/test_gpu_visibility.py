@@ -125,3 +125,26 @@ def test_visibility_blockers_just_past_radius(ctx, offset, radius):
         assert np.array_equal(bits(g), bits(o_))
     sky, ao = want[1][mask != 0], want[0][mask != 0]
     assert float(np.mean(sky)) < float(np.mean(ao))   # the blockers past the radius matter
+
+
+@pytest.mark.parametrize("refill", [0, 1, 40, 64])
+def test_visibility_sky_refill(ctx, refill):
+    """64 spp on the 1M-triangle scene: the sky rays traced apart from the AO rays with lanes
+    refilled at `refill` idle lanes (0: each sky ray right after its AO ray, in the same lane)
+    against the default threshold: ao, sky and sun bit for bit."""
+    import raw_ao
+    import raw_ao.host as H
+    print("visibility sky refill", refill, flush=True)
+    vpos, vnrm, idx, _ = H.scene_seeded()
+    _, pos, nrm, mask = H.rasterize(vpos, vnrm, idx, H.orbit_camera(2), (0, 2, 0), (0, 1, 0), 0.9, 320 / 180, 0.1, 200.0,
+                                    320, 180)
+    ctx.build_ploc(H.mesh_tris(vpos, idx))
+    want = ctx.compute_rt_visibility(pos, nrm, mask, DEFAULT_TO_LIGHT, 64, 1.0)
+    try:
+        ctx.set_option(raw_ao.OPT_SKY_REFILL, refill)
+        got = ctx.compute_rt_visibility(pos, nrm, mask, DEFAULT_TO_LIGHT, 64, 1.0)
+    finally:
+        ctx.set_option(raw_ao.OPT_SKY_REFILL, 32)
+    for g, o in zip(got, want):
+        assert np.array_equal(bits(g), bits(o))
+    assert 0.0 < float(np.mean(got[1][mask != 0])) < 1.0
