@@ -1377,10 +1377,10 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6, 6))) void
 // 8, 16 and 32 pixels a wave at 1080p on the 1M-triangle scene: 8 is the fastest). A pixel's sky is
 // its unblocked rays / 64 (counted in LDS), 1 off the mask: the same rays and the same any-hit
 // answers as k_vis4c, so the same values.
-template <int G>
+template <int G, int SL>
 __global__ __launch_bounds__(BT) void k_sky4r(TraceArgs4 a, uint32_t refill_min) {
     static_assert(G <= 16 && (G & (G - 1)) == 0, "one pixel a lane for the set-up, 3 bits a pixel in klass");
-    __shared__ uint32_t stk[STACK_LDS * BT];
+    __shared__ uint32_t stk[SL * BT];
     __shared__ float pixmem[PIX_WORDS * G * (BT / 64)];
     __shared__ uint16_t raylist[64 * G * (BT / 64)];
     __shared__ uint32_t nfree[G * (BT / 64)];
@@ -1503,25 +1503,27 @@ __global__ __launch_bounds__(BT) void k_sky4r(TraceArgs4 a, uint32_t refill_min)
             const uint32_t off = uint32_t(node) * uint32_t(sizeof(Node4));
             const uint32_t hit = slab4s(r, base, off, ox, oy, oz, reach, kExit, tlo);
             const int4 ref = *reinterpret_cast<const int4*>(base + off + 96);
-            const uint32_t masks = *reinterpret_cast<const uint32_t*>(base + off + 112);
-            pend = hit & masks & 15u;
+            // leaf slots have negative refs, inner ones non-negative, and an empty slot's box
+            // is never entered, so the refs give the slot kinds without the masks word's load
+            const uint32_t neg = (ref.x < 0 ? 1u : 0u) | (ref.y < 0 ? 2u : 0u) | (ref.z < 0 ? 4u : 0u) | (ref.w < 0 ? 8u : 0u);
+            pend = hit & neg;
             leafref = ref;
             bool overflow = false;
-            uint32_t ih = hit & (masks >> 4);
+            uint32_t ih = hit & ~neg & 15u;
             if (ih) {
                 node = ref_of(ref, __builtin_ctz(ih));
                 ih &= ih - 1u;
                 while (ih) {
                     const uint32_t v = uint32_t(ref_of(ref, __builtin_ctz(ih)));
                     ih &= ih - 1u;
-                    if (sp < STACK_LDS) lds_col[sp * BT] = v;
-                    else if (sp < STACK_LDS + STACK_SPILL) spill[sp - STACK_LDS] = v;
+                    if (sp < SL) lds_col[sp * BT] = v;
+                    else if (sp < SL + STACK_SPILL) spill[sp - SL] = v;
                     else overflow = true;
                     ++sp;
                 }
             } else if (sp > 0) {
                 --sp;
-                node = int32_t(sp < STACK_LDS ? lds_col[sp * BT] : spill[sp - STACK_LDS]);
+                node = int32_t(sp < SL ? lds_col[sp * BT] : spill[sp - SL]);
             } else {
                 node = -1;
             }
@@ -2050,11 +2052,11 @@ int trace_vis4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_
     const bool split = cut && spp64 && c.refill > 0 && c.sky_refill > 0;
     if (split && !ensure(c, c.vis_open, 8 * size_t(npix))) return -2;
     (void)hipEventRecord(c.ev[2], s);
-    constexpr int PW = 4, SKY_G = 8;
+    constexpr int PW = 4, SKY_G = 8, SKY_STACK = STACK_LDS;
     if (split) {
         a.open_mask = c.vis_open.as<uint64_t>();
         k_trace_ao4rc<PW, false, true><<<cdiv(npix, uint32_t(PW * (BT / 64))), BT, 0, s>>>(a, uint32_t(c.refill));
-        k_sky4r<SKY_G><<<cdiv(npix, uint32_t(SKY_G * (BT / 64))), BT, 0, s>>>(a, uint32_t(c.sky_refill));
+        k_sky4r<SKY_G, SKY_STACK><<<cdiv(npix, uint32_t(SKY_G * (BT / 64))), BT, 0, s>>>(a, uint32_t(c.sky_refill));
     } else if (cut && spp64) k_vis4c<true><<<blocks, BT, 0, s>>>(a);
     else if (cut) k_vis4c<false><<<blocks, BT, 0, s>>>(a);
     else if (spp64) k_trace_vis4<true><<<blocks, BT, 0, s>>>(a, d_sky, nullptr);
