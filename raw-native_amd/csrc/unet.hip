@@ -56,6 +56,7 @@ struct ConvArgs {
     bf16_t* out;         // N,H,W,Cout
     int n, h, wd, c0, c1, cout, kpad, relu;
     int co0;             // first output channel of this launch's group
+    bf16_t* pool;        // or null: also the 2x2 average pool of out, N,H/2,W/2,Cout (k_conv_lds, k_conv_pf)
 };
 
 // One wave: 32 pixels (2 B tiles) x CT*16 output channels starting at co0.
@@ -214,6 +215,54 @@ __global__ __launch_bounds__(256) void k_conv_t(ConvArgs a) {
 // run chunk-major (taps inside a chunk), so the results may differ from k_conv's by the
 // rounding of the float32 sums.
 constexpr int LT_H = 8, LT_W = 32, LH_H = LT_H + 2, LH_W = LT_W + 2;
+
+// The output of an 8 x 32 tile's wave (rows oy + 2 wv and oy + 2 wv + 1, tile t: row t >> 1,
+// columns ox + 16 (t & 1) + r) with bias and ReLU, rounded to bf16; with a.pool also its 2x2
+// average pool: a 2x2 block is rows 2 wv, 2 wv + 1 of one tile column and lanes r, r ^ 1, summed
+// (x0 + x1) + (x0' + x1') in float from the rounded outputs, times 0.25, rounded (k_pool_fwd's
+// operations, so the same bits).
+template <int CT>
+__device__ __forceinline__ void conv_tile_out(const ConvArgs& a, const f32x4 (&acc)[CT][4], int img, int oy, int ox, int wv,
+                                              int g, int r, int co0) {
+    const auto pack = [&](int i, int t) {
+        const int co = co0 + i * 16 + 4 * g;
+        float v[4];
+        for (int j = 0; j < 4; ++j) {
+            v[j] = acc[i][t][j] + (a.bias ? a.bias[co + j] : 0.0f);
+            if (a.relu && v[j] < 0.0f) v[j] = 0.0f;
+        }
+        return make_uint2(uint32_t(f2bf(v[0])) | (uint32_t(f2bf(v[1])) << 16), uint32_t(f2bf(v[2])) | (uint32_t(f2bf(v[3])) << 16));
+    };
+    for (int t = 0; t < 4; ++t) {
+        const int y = oy + 2 * wv + (t >> 1), x = ox + 16 * (t & 1) + r;
+        if (y >= a.h || x >= a.wd) continue;
+        const size_t p = size_t(img * a.h + y) * a.wd + x;
+        for (int i = 0; i < CT; ++i) *reinterpret_cast<uint2*>(a.out + p * a.cout + co0 + i * 16 + 4 * g) = pack(i, t);
+    }
+    if (!a.pool) return;
+    const int ph = a.h >> 1, pw = a.wd >> 1;
+    const int y = oy + 2 * wv;
+    for (int half = 0; half < 2; ++half) {
+        const int x = ox + 16 * half + r;
+        for (int i = 0; i < CT; ++i) {
+            const uint2 u0 = pack(i, half), u1 = pack(i, half + 2);
+            const uint32_t n0x = uint32_t(__shfl_xor(int(u0.x), 1)), n0y = uint32_t(__shfl_xor(int(u0.y), 1));
+            const uint32_t n1x = uint32_t(__shfl_xor(int(u1.x), 1)), n1y = uint32_t(__shfl_xor(int(u1.y), 1));
+            if ((r & 1) || y + 1 >= a.h || x + 1 >= a.wd) continue;
+            const uint32_t w0[2] = {u0.x, u0.y}, w1[2] = {n0x, n0y}, w2[2] = {u1.x, u1.y}, w3[2] = {n1x, n1y};
+            bf16_t o[4];
+            for (int j = 0; j < 4; ++j) {
+                const int sh = 16 * (j & 1);
+                const float s = (bf2f(bf16_t(w0[j >> 1] >> sh)) + bf2f(bf16_t(w1[j >> 1] >> sh))) +
+                                (bf2f(bf16_t(w2[j >> 1] >> sh)) + bf2f(bf16_t(w3[j >> 1] >> sh)));
+                o[j] = f2bf(s * 0.25f);
+            }
+            const size_t q = size_t(img * ph + (y >> 1)) * pw + (x >> 1);
+            *reinterpret_cast<uint2*>(a.pool + q * a.cout + co0 + i * 16 + 4 * g) =
+                make_uint2(uint32_t(o[0]) | (uint32_t(o[1]) << 16), uint32_t(o[2]) | (uint32_t(o[3]) << 16));
+        }
+    }
+}
 constexpr int LH_PS = 40;            // bf16 a staged pixel: 32 channels + 8 pad
 constexpr int LW_RS = 9 * 32 + 8;    // bf16 a staged weight row: 9 taps x 32 channels + 8 pad
 template <int CT>
@@ -273,22 +322,7 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, int ntiles) {
             }
         }
     }
-    for (int t = 0; t < 4; ++t) {
-        const int y = oy + 2 * wv + (t >> 1), x = ox + 16 * (t & 1) + r;
-        if (y >= a.h || x >= a.wd) continue;
-        const size_t p = size_t(img * a.h + y) * a.wd + x;
-        for (int i = 0; i < CT; ++i) {
-            const int co = co0 + i * 16 + 4 * g;
-            float v[4];
-            for (int j = 0; j < 4; ++j) {
-                v[j] = acc[i][t][j] + (a.bias ? a.bias[co + j] : 0.0f);
-                if (a.relu && v[j] < 0.0f) v[j] = 0.0f;
-            }
-            const uint32_t lo = uint32_t(f2bf(v[0])) | (uint32_t(f2bf(v[1])) << 16);
-            const uint32_t hi = uint32_t(f2bf(v[2])) | (uint32_t(f2bf(v[3])) << 16);
-            *reinterpret_cast<uint2*>(a.out + p * a.cout + co) = make_uint2(lo, hi);
-        }
-    }
+    conv_tile_out<CT>(a, acc, img, oy, ox, wv, g, r, co0);
 }
 
 // The persistent form of k_conv_lds for layers of one 32-channel chunk in and one channel
@@ -355,23 +389,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
             }
         }
         const int bx = tile % tiles_x, q0 = tile / tiles_x, by = q0 % tiles_y, img = q0 / tiles_y;
-        const int oy = by * LT_H, ox = bx * LT_W;
-        for (int t = 0; t < 4; ++t) {
-            const int y = oy + 2 * wv + (t >> 1), x = ox + 16 * (t & 1) + r;
-            if (y >= a.h || x >= a.wd) continue;
-            const size_t p = size_t(img * a.h + y) * a.wd + x;
-            for (int i = 0; i < CT; ++i) {
-                const int co = a.co0 + i * 16 + 4 * g;
-                float v[4];
-                for (int j = 0; j < 4; ++j) {
-                    v[j] = acc[i][t][j] + (a.bias ? a.bias[co + j] : 0.0f);
-                    if (a.relu && v[j] < 0.0f) v[j] = 0.0f;
-                }
-                const uint32_t lo = uint32_t(f2bf(v[0])) | (uint32_t(f2bf(v[1])) << 16);
-                const uint32_t hi = uint32_t(f2bf(v[2])) | (uint32_t(f2bf(v[3])) << 16);
-                *reinterpret_cast<uint2*>(a.out + p * a.cout + co) = make_uint2(lo, hi);
-            }
-        }
+        conv_tile_out<CT>(a, acc, img, by * LT_H, bx * LT_W, wv, g, r, a.co0);
         if (ntile >= ntiles) break;
         tile = ntile;
     }
@@ -787,8 +805,13 @@ bool conv_shape_ok(int c0, int c1, int cout) {
 }
 
 bool conv3x3(Ctx& c, const bf16_t* in0, const bf16_t* in1, int c0, int c1, const bf16_t* w, int kpad, const float* bias,
-             int cout, int relu, int n, int h, int wd, bf16_t* out, hipStream_t s) {
-    ConvArgs a{in0, in1, w, bias, out, n, h, wd, c0, c1, cout, kpad, relu, 0};
+             int cout, int relu, int n, int h, int wd, bf16_t* out, hipStream_t s, bf16_t* pool) {
+    ConvArgs a{in0, in1, w, bias, out, n, h, wd, c0, c1, cout, kpad, relu, 0, nullptr};
+    // the kernels without the fused pool: a pool pass after them
+    const auto pool_after = [&]() {
+        if (pool)
+            k_pool_fwd<<<cdivu(size_t(n) * (h / 2) * (wd / 2) * cout, 256), 256, 0, s>>>(out, pool, n, h, wd, cout);
+    };
     const bool chunked = (c0 + c1) % 32 == 0 && c0 % 32 == 0;
     if (c0 == 8 && c1 == 0 && kpad == 96 && c.conv_kernel == 0 && (cout == 16 || cout == 32 || cout == 64)) {
         if (c.cus == 0) {
@@ -804,6 +827,7 @@ bool conv3x3(Ctx& c, const bf16_t* in0, const bf16_t* in1, int c0, int c1, const
             case 2: k_conv_c8<2><<<blocks, 256, 0, s>>>(a, ntiles); break;
             default: k_conv_c8<4><<<blocks, 256, 0, s>>>(a, ntiles); break;
         }
+        pool_after();
         return hip_ok(c, hipGetLastError(), "conv3x3");
     }
     if (chunked && (c.conv_kernel == 0 || c.conv_kernel == 3) && (c1 == 0 || (h % 2 == 0 && wd % 2 == 0))) {
@@ -813,6 +837,7 @@ bool conv3x3(Ctx& c, const bf16_t* in0, const bf16_t* in1, int c0, int c1, const
                                                                                                                      : 256;
         }
         const int ntiles = int(size_t(n) * ((h + LT_H - 1) / LT_H) * ((wd + LT_W - 1) / LT_W));
+        a.pool = pool;
         if (c.conv_kernel == 0 && c0 == 32 && c1 == 0 && cout == 32) {   // one chunk in, one group out: persistent
             a.co0 = 0;
             const unsigned blocks = unsigned(std::min(ntiles, c.cus * PF_BLOCKS));
@@ -865,6 +890,7 @@ bool conv3x3(Ctx& c, const bf16_t* in0, const bf16_t* in1, int c0, int c1, const
             default: launch_conv<8>(a, s); break;
         }
     }
+    pool_after();
     return hip_ok(c, hipGetLastError(), "conv3x3");
 }
 
@@ -996,15 +1022,12 @@ bool UNet::forward(const bf16_t* x, const uint8_t* mask, int n, int h, int w, fl
     for (int i = 0; i < kNetLayers; ++i) {
         Layer& l = layers[i];
         const int lh = h >> l.level, lw = w >> l.level;
-        if (i == 2 || i == 4 || i == 6) {   // pool the previous level's last output
-            const int cp = layers[i - 1].cout;
-            k_pool_fwd<<<cdivu(size_t(n) * lh * lw * cp, 256), 256, 0, s>>>(act[i - 1].as<bf16_t>(), pooled[i / 2 - 1].as<bf16_t>(),
-                                                                             n, 2 * lh, 2 * lw, cp);
-        }
         const bf16_t *in0, *in1;
         layer_inputs(*this, i, x, in0, in1);
+        // an encoder level's last output is pooled for the next level by the same launch
+        bf16_t* pool = (i == 1 || i == 3 || i == 5) ? pooled[i / 2].as<bf16_t>() : nullptr;
         if (!conv3x3(c, in0, in1, l.c0, l.c1, w16.as<bf16_t>() + l.wofs16, l.kpad, params.as<float>() + l.boff, l.cout, 1, n, lh,
-                     lw, act[i].as<bf16_t>(), s))
+                     lw, act[i].as<bf16_t>(), s, pool))
             return false;
     }
     k_head<<<cdivu(np, 256), 256, 0, s>>>(act[kNetLayers - 1].as<bf16_t>(), params.as<float>() + head_off, mask, np,

@@ -73,8 +73,10 @@ int trace_built(Ctx& c, const float* pos, const float* nrm, const uint8_t* mask,
                 hipStream_t s);
 
 // Single-layer kernels for the tests and the host driver.
+// pool (or null): also out's 2x2 average pool, N,H/2,W/2,cout (h and w even)
 bool conv3x3(Ctx& c, const unsigned short* in0, const unsigned short* in1, int c0, int c1, const unsigned short* w, int kpad,
-             const float* bias, int cout, int relu, int n, int h, int wd, unsigned short* out, hipStream_t s);
+             const float* bias, int cout, int relu, int n, int h, int wd, unsigned short* out, hipStream_t s,
+             unsigned short* pool = nullptr);
 bool conv3x3_wgrad(Ctx& c, DevBuf& part, const unsigned short* dy, const unsigned short* in0, const unsigned short* in1, int c0,
                    int c1, int cout, int n, int h, int wd, float* gw, float* gb, hipStream_t s);
 
