@@ -44,7 +44,7 @@ __global__ __launch_bounds__(BT) void k_layout(const float* __restrict__ boxes, 
         o[2] = make_float4(v[8], __int_as_float(int32_t(t)), 0.0f, 0.0f);
     }
     const uint32_t inner = n > 1 ? n - 1 : 1;
-    if (i >= inner) return;
+    if (!nodes || i >= inner) return;
     int32_t ch[2];
     if (n == 1) { ch[0] = 0; ch[1] = -1; }   // a single leaf under a one-record root
     else { ch[0] = links[3 * size_t(n + i)]; ch[1] = links[3 * size_t(n + i) + 1]; }
@@ -313,10 +313,13 @@ int bvh_layout(Ctx& c, hipStream_t s) {
     const uint32_t n = c.ntri;
     if (n == 0) return 0;
     const size_t inner = n > 1 ? n - 1 : 1;
-    if (!ensure(c, c.nodes, sizeof(Node) * inner) || !ensure(c, c.tris_sorted, 48 * size_t(n))) return -2;
-    k_layout<<<cdiv(std::max<size_t>(n, inner), BT), BT, 0, s>>>(c.boxes.as<float>(), c.links.as<int32_t>(), c.tris_in.as<float>(),
-                                                                 c.vals.as<uint32_t>(), n, c.root, c.nodes.as<Node>(),
-                                                                 c.tris_sorted.as<float4>());
+    // the binary records only for the binary traversal; the wide one reads the leaf-order triangles
+    const bool binary = c.trace_kernel == RAW_AO_TRACE_BINARY;
+    if ((binary && !ensure(c, c.nodes, sizeof(Node) * inner)) || !ensure(c, c.tris_sorted, 48 * size_t(n))) return -2;
+    k_layout<<<cdiv(binary ? std::max<size_t>(n, inner) : n, BT), BT, 0, s>>>(c.boxes.as<float>(), c.links.as<int32_t>(),
+                                                                              c.tris_in.as<float>(), c.vals.as<uint32_t>(), n, c.root,
+                                                                              binary ? c.nodes.as<Node>() : nullptr,
+                                                                              c.tris_sorted.as<float4>());
     c.trav_root = n > 1 ? uint32_t(c.root) - n : 0u;
     return hip_ok(c, hipGetLastError(), "layout launch") ? 0 : -2;
 }
