@@ -56,7 +56,7 @@ struct ConvArgs {
     bf16_t* out;         // N,H,W,Cout
     int n, h, wd, c0, c1, cout, kpad, relu;
     int co0;             // first output channel of this launch's group
-    bf16_t* pool;        // or null: also the 2x2 average pool of out, N,H/2,W/2,Cout (k_conv_lds, k_conv_pf)
+    bf16_t* pool;        // or null: also the 2x2 average pool of out, N,H/2,W/2,Cout (k_conv_lds, k_conv_pf with POOL)
 };
 
 // One wave: 32 pixels (2 B tiles) x CT*16 output channels starting at co0.
@@ -221,7 +221,7 @@ constexpr int LT_H = 8, LT_W = 32, LH_H = LT_H + 2, LH_W = LT_W + 2;
 // average pool: a 2x2 block is rows 2 wv, 2 wv + 1 of one tile column and lanes r, r ^ 1, summed
 // (x0 + x1) + (x0' + x1') in float from the rounded outputs, times 0.25, rounded (k_pool_fwd's
 // operations, so the same bits).
-template <int CT>
+template <int CT, bool POOL>
 __device__ __forceinline__ void conv_tile_out(const ConvArgs& a, const f32x4 (&acc)[CT][4], int img, int oy, int ox, int wv,
                                               int g, int r, int co0) {
     const auto pack = [&](int i, int t) {
@@ -239,7 +239,7 @@ __device__ __forceinline__ void conv_tile_out(const ConvArgs& a, const f32x4 (&a
         const size_t p = size_t(img * a.h + y) * a.wd + x;
         for (int i = 0; i < CT; ++i) *reinterpret_cast<uint2*>(a.out + p * a.cout + co0 + i * 16 + 4 * g) = pack(i, t);
     }
-    if (!a.pool) return;
+    if (!POOL) return;
     const int ph = a.h >> 1, pw = a.wd >> 1;
     const int y = oy + 2 * wv;
     for (int half = 0; half < 2; ++half) {
@@ -265,7 +265,7 @@ __device__ __forceinline__ void conv_tile_out(const ConvArgs& a, const f32x4 (&a
 }
 constexpr int LH_PS = 40;            // bf16 a staged pixel: 32 channels + 8 pad
 constexpr int LW_RS = 9 * 32 + 8;    // bf16 a staged weight row: 9 taps x 32 channels + 8 pad
-template <int CT>
+template <int CT, bool POOL = false>
 __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, int ntiles) {
     __shared__ __attribute__((aligned(16))) bf16_t hal[LH_H * LH_W * LH_PS];
     __shared__ __attribute__((aligned(16))) bf16_t wls[CT * 16 * LW_RS];
@@ -322,7 +322,7 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, int ntiles) {
             }
         }
     }
-    conv_tile_out<CT>(a, acc, img, oy, ox, wv, g, r, co0);
+    conv_tile_out<CT, POOL>(a, acc, img, oy, ox, wv, g, r, co0);
 }
 
 // The persistent form of k_conv_lds for layers of one 32-channel chunk in and one channel
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, int ntiles) {
 // outputs are the same bit for bit. (Measured: the same form for layers of several chunks,
 // with each chunk's weights prefetched too, and for 64 outputs ran slower than k_conv_lds.)
 constexpr int PF_BLOCKS = 3;   // k_conv_pf<2> blocks a CU: 46 KB of LDS each
-template <int CT>
+template <int CT, bool POOL = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_conv_pf(ConvArgs a, int ntiles) {
     __shared__ __attribute__((aligned(16))) bf16_t hal[LH_H * LH_W * LH_PS];
     __shared__ __attribute__((aligned(16))) bf16_t wls[CT * 16 * LW_RS];
@@ -389,7 +389,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
             }
         }
         const int bx = tile % tiles_x, q0 = tile / tiles_x, by = q0 % tiles_y, img = q0 / tiles_y;
-        conv_tile_out<CT>(a, acc, img, by * LT_H, bx * LT_W, wv, g, r, a.co0);
+        conv_tile_out<CT, POOL>(a, acc, img, by * LT_H, bx * LT_W, wv, g, r, a.co0);
         if (ntile >= ntiles) break;
         tile = ntile;
     }
@@ -841,7 +841,8 @@ bool conv3x3(Ctx& c, const bf16_t* in0, const bf16_t* in1, int c0, int c1, const
         if (c.conv_kernel == 0 && c0 == 32 && c1 == 0 && cout == 32) {   // one chunk in, one group out: persistent
             a.co0 = 0;
             const unsigned blocks = unsigned(std::min(ntiles, c.cus * PF_BLOCKS));
-            k_conv_pf<2><<<blocks, 256, 0, s>>>(a, ntiles);
+            if (pool) k_conv_pf<2, true><<<blocks, 256, 0, s>>>(a, ntiles);
+            else k_conv_pf<2><<<blocks, 256, 0, s>>>(a, ntiles);
             return hip_ok(c, hipGetLastError(), "conv3x3");
         }
         // the widest channel group (<= 64) that still gives every CU two work items;
@@ -853,11 +854,20 @@ bool conv3x3(Ctx& c, const bf16_t* in0, const bf16_t* in1, int c0, int c1, const
             const int groups = (cout - co0) / (16 * w16);
             const unsigned blocks = unsigned(ntiles * groups);
             a.co0 = co0;
-            switch (w16) {
-                case 1: k_conv_lds<1><<<blocks, 256, 0, s>>>(a, ntiles); break;
-                case 2: k_conv_lds<2><<<blocks, 256, 0, s>>>(a, ntiles); break;
-                case 3: k_conv_lds<3><<<blocks, 256, 0, s>>>(a, ntiles); break;
-                default: k_conv_lds<4><<<blocks, 256, 0, s>>>(a, ntiles); break;
+            if (pool) {
+                switch (w16) {
+                    case 1: k_conv_lds<1, true><<<blocks, 256, 0, s>>>(a, ntiles); break;
+                    case 2: k_conv_lds<2, true><<<blocks, 256, 0, s>>>(a, ntiles); break;
+                    case 3: k_conv_lds<3, true><<<blocks, 256, 0, s>>>(a, ntiles); break;
+                    default: k_conv_lds<4, true><<<blocks, 256, 0, s>>>(a, ntiles); break;
+                }
+            } else {
+                switch (w16) {
+                    case 1: k_conv_lds<1><<<blocks, 256, 0, s>>>(a, ntiles); break;
+                    case 2: k_conv_lds<2><<<blocks, 256, 0, s>>>(a, ntiles); break;
+                    case 3: k_conv_lds<3><<<blocks, 256, 0, s>>>(a, ntiles); break;
+                    default: k_conv_lds<4><<<blocks, 256, 0, s>>>(a, ntiles); break;
+                }
             }
             co0 += groups * 16 * w16;
         }
