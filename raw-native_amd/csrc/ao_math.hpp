@@ -89,15 +89,13 @@ AO_HD void sincos_base(float phi, float& s, float& c) {
     pc = pc * r2 - 1.0 / 24.0;                               // -1/4!
     pc = pc * r2 + 1.0 / 2.0;                                //  1/2!
     const double cr = 1.0 - r2 * pc;
-    double sd, cd;
-    switch (q & 3) {
-        case 0: sd = sr; cd = cr; break;
-        case 1: sd = cr; cd = -sr; break;
-        case 2: sd = -sr; cd = -cr; break;
-        default: sd = -cr; cd = sr; break;
-    }
-    s = float(sd);
-    c = float(cd);
+    // quadrant q: (sin, cos) = (sr, cr), (cr, -sr), (-sr, -cr), (-cr, sr), by selects and sign
+    // flips (rounding to float commutes with negation)
+    const float fs = float(sr), fc = float(cr);
+    const bool odd = (q & 1) != 0;
+    const float as = odd ? fc : fs, ac = odd ? fs : fc;
+    s = (q & 2) ? -as : as;
+    c = ((q + 1) & 2) ? -ac : ac;
 }
 
 AO_HD float bits_step(float v, int d) {
@@ -167,6 +165,7 @@ struct RayFrame {
     int kz;
     bool sw;
     float sx, sy, sz;
+    int ix, iy;   // the components of v that wframe(v, kz, sw) puts first and second (its third is kz)
 };
 AO_HD V3 wframe(V3 v, int kz, bool swap) {
     const V3 p = kz == 0 ? v3(v.y, v.z, v.x) : (kz == 1 ? v3(v.z, v.x, v.y) : v);
@@ -182,11 +181,18 @@ AO_HD RayFrame ray_frame(V3 rd) {
     f.sx = d.x / d.z;
     f.sy = d.y / d.z;
     f.sz = 1.0f / d.z;
+    const int i0 = f.kz == 2 ? 0 : f.kz + 1, i1 = f.kz == 0 ? 2 : f.kz - 1;   // (kz + 1) % 3, (kz + 2) % 3
+    f.ix = f.sw ? i1 : i0;
+    f.iy = f.sw ? i0 : i1;
     return f;
 }
+// Component i of v (0, 1, 2: x, y, z) by selects: wframe by per-ray indices, without the
+// per-lane branches a rotation of the whole vector compiles to.
+AO_HD float comp(V3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+AO_HD V3 wframe_f(V3 v, const RayFrame& f) { return v3(comp(v, f.ix), comp(v, f.iy), comp(v, f.kz)); }
 // intersectTri's hit decision and t: true when the ray hits with t > 1e-4 (t written).
 AO_HD bool intersect_t(V3 ro, V3 rd, const RayFrame& f, V3 ta, V3 tb, V3 tc, float& t) {
-    const V3 A = wframe(sub(ta, ro), f.kz, f.sw), B = wframe(sub(tb, ro), f.kz, f.sw), C = wframe(sub(tc, ro), f.kz, f.sw);
+    const V3 A = wframe_f(sub(ta, ro), f), B = wframe_f(sub(tb, ro), f), C = wframe_f(sub(tc, ro), f);
     const float Ax = A.x - f.sx * A.z, Ay = A.y - f.sy * A.z;
     const float Bx = B.x - f.sx * B.z, By = B.y - f.sy * B.z;
     const float Cx = C.x - f.sx * C.z, Cy = C.y - f.sy * C.z;
@@ -209,7 +215,7 @@ AO_HD bool intersect_t(V3 ro, V3 rd, const RayFrame& f, V3 ta, V3 tb, V3 tc, flo
 
 // The whole of intersectTri: the hit decision, t and the barycentric weights u, v of b and c.
 AO_HD bool intersect_tuv(V3 ro, V3 rd, const RayFrame& f, V3 ta, V3 tb, V3 tc, float& t, float& u, float& v) {
-    const V3 A = wframe(sub(ta, ro), f.kz, f.sw), B = wframe(sub(tb, ro), f.kz, f.sw), C = wframe(sub(tc, ro), f.kz, f.sw);
+    const V3 A = wframe_f(sub(ta, ro), f), B = wframe_f(sub(tb, ro), f), C = wframe_f(sub(tc, ro), f);
     const float Ax = A.x - f.sx * A.z, Ay = A.y - f.sy * A.z;
     const float Bx = B.x - f.sx * B.z, By = B.y - f.sy * B.z;
     const float Cx = C.x - f.sx * C.z, Cy = C.y - f.sy * C.z;

@@ -267,9 +267,11 @@ __device__ __forceinline__ float4 ld4(const char* __restrict__ base, uint32_t of
 // the far planes' float4 from the near ones' when 1/d < 0).
 __device__ __forceinline__ uint32_t slab4s(const Ray& r, const char* __restrict__ base, uint32_t off, uint32_t ox, uint32_t oy,
                                            uint32_t oz, float reach, float exit_scale, float tlo = 0.0f) {
-    const float4 nx = fma4(ld4(base, off + ox), r.inv.x, -r.oinv.x), fx = fma4(ld4(base, off + 48 - ox), r.inv.x, -r.oinv.x);
-    const float4 ny = fma4(ld4(base, off + 16 + oy), r.inv.y, -r.oinv.y), fy = fma4(ld4(base, off + 64 - oy), r.inv.y, -r.oinv.y);
-    const float4 nz = fma4(ld4(base, off + 32 + oz), r.inv.z, -r.oinv.z), fz = fma4(ld4(base, off + 80 - oz), r.inv.z, -r.oinv.z);
+    // 32-bit offsets from the (uniform) base and base + 16, base + 32: one add an address
+    const uint32_t o48 = off + 48u;
+    const float4 nx = fma4(ld4(base, off + ox), r.inv.x, -r.oinv.x), fx = fma4(ld4(base, o48 - ox), r.inv.x, -r.oinv.x);
+    const float4 ny = fma4(ld4(base + 16, off + oy), r.inv.y, -r.oinv.y), fy = fma4(ld4(base + 16, o48 - oy), r.inv.y, -r.oinv.y);
+    const float4 nz = fma4(ld4(base + 32, off + oz), r.inv.z, -r.oinv.z), fz = fma4(ld4(base + 32, o48 - oz), r.inv.z, -r.oinv.z);
     const auto one = [&](float a, float b, float c, float d, float e, float f) {
         const float t0 = fmaxf(fmaxf(a, b), fmaxf(c, tlo));
         const float t1 = fminf(fminf(d, e), fminf(f, reach));
@@ -281,9 +283,11 @@ __device__ __forceinline__ uint32_t slab4s(const Ray& r, const char* __restrict_
 // slab4s that also returns each slot's entry distance.
 __device__ __forceinline__ uint32_t slab4s_t(const Ray& r, const char* __restrict__ base, uint32_t off, uint32_t ox, uint32_t oy,
                                              uint32_t oz, float reach, float exit_scale, float (&t0s)[4], float tlo = 0.0f) {
-    const float4 nx = fma4(ld4(base, off + ox), r.inv.x, -r.oinv.x), fx = fma4(ld4(base, off + 48 - ox), r.inv.x, -r.oinv.x);
-    const float4 ny = fma4(ld4(base, off + 16 + oy), r.inv.y, -r.oinv.y), fy = fma4(ld4(base, off + 64 - oy), r.inv.y, -r.oinv.y);
-    const float4 nz = fma4(ld4(base, off + 32 + oz), r.inv.z, -r.oinv.z), fz = fma4(ld4(base, off + 80 - oz), r.inv.z, -r.oinv.z);
+    // 32-bit offsets from the (uniform) base and base + 16, base + 32: one add an address
+    const uint32_t o48 = off + 48u;
+    const float4 nx = fma4(ld4(base, off + ox), r.inv.x, -r.oinv.x), fx = fma4(ld4(base, o48 - ox), r.inv.x, -r.oinv.x);
+    const float4 ny = fma4(ld4(base + 16, off + oy), r.inv.y, -r.oinv.y), fy = fma4(ld4(base + 16, o48 - oy), r.inv.y, -r.oinv.y);
+    const float4 nz = fma4(ld4(base + 32, off + oz), r.inv.z, -r.oinv.z), fz = fma4(ld4(base + 32, o48 - oz), r.inv.z, -r.oinv.z);
     const auto one = [&](float a, float b, float c, float d, float e, float f, float& t0) {
         t0 = fmaxf(fmaxf(a, b), fmaxf(c, tlo));
         const float t1 = fminf(fminf(d, e), fminf(f, reach));
@@ -535,13 +539,14 @@ __global__ __launch_bounds__(64) void k_top_table(const Node4* __restrict__ node
 }
 
 // The walk starts from the top table when it has entries (else from the root).
+template <int SB = BT>
 __device__ void wave_cut(const Node4* __restrict__ nodes, const TopEntry* __restrict__ top, uint32_t ntop, V3 lo, V3 hi,
                          float small, uint32_t* rows, CutView& cv, uint32_t& steps) {
     const uint32_t lane = threadIdx.x & 63u;
     const unsigned long long below = (1ull << lane) - 1ull;
     const char* const base = reinterpret_cast<const char*>(nodes);
     // frontier f's row k, entry i: rows[(7 f + k) BT + i]
-    const auto fr = [&](int f, int k) { return rows + (7 * f + k) * BT; };
+    const auto fr = [&](int f, int k) { return rows + (7 * f + k) * SB; };
     int cur = 0, nn = 0;
     cv.n = 0;
     cv.nleaf = 0;
@@ -1133,29 +1138,27 @@ __global__ __launch_bounds__(BT) void k_trace_ao4r(TraceArgs4 a, uint32_t refill
 // MASK: also a.open_mask, each pixel's open samples as bits (gathered by LDS atomics as the rays
 // end; 0 for a pixel off the mask), for the visibility pass's sky rays (k_sky4r).
 constexpr int PIX_WORDS = 16;   // per pixel: origin, n, t, b (12 floats), x, y
-template <int PW, bool COUNT, bool MASK = false>
-__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_trace_ao4rc(TraceArgs4 a,
-                                                                                                uint32_t refill_min) {
-    static_assert(PW * 4 <= 32, "one 4-bit angle step a pixel in a 32-bit register");
-    __shared__ uint32_t stk[STACK_LDS * BT];
-    __shared__ uint32_t cutmem[CUT_WAVE * (BT / 64)];
-    __shared__ float pixmem[PIX_WORDS * PW * (BT / 64)];
-    __shared__ uint32_t openmem[MASK ? 2 * PW * (BT / 64) : 1];
+constexpr int RC_WAVES = 2;     // k_trace_ao4rc: waves a block
+// One group of PW pixels (from pix0) for the calling wave. stk: the block's traversal stack
+// (STACK_LDS rows of SB words, SB the block's threads); cutmem: the wave's CUT_WAVE words; pm: the wave's PIX_WORDS x PW
+// floats; om: the wave's 2 PW words (MASK). COUNT: the wave's lanes add their node visits,
+// triangle tests and rays to visits, tests, rays.
+template <int PW, bool COUNT, bool MASK, int SB>
+__device__ __forceinline__ void trace_group_rc(const TraceArgs4& a, uint32_t refill_min, uint32_t pix0, uint32_t* stk,
+                                               uint32_t* cutmem, float* pm, uint32_t* om, uint32_t& visits,
+                                               uint32_t& tests, uint32_t& rays) {
     uint32_t* const lds_col = stk + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t pix0 = (blockIdx.x * (BT / 64) + wave) * PW;
-    if (pix0 >= a.npix) return;
     const unsigned long long below = (1ull << lane) - 1ull;
-    uint32_t cnt[PW];
-    for (int k = 0; k < PW; ++k) cnt[k] = 0;
-    uint32_t* const om = openmem + (MASK ? wave * 2 * PW : 0);
+    static_assert(PW <= 4, "a lane's open counts: one byte a pixel in a 32-bit register");
+    uint32_t cnt = 0;   // byte k: this lane's open rays of pixel pix0 + k (at most 64 each)
     if (MASK && lane < 2u * PW) om[lane] = 0u;
     const auto finish = [&]() {
         if (MASK) wave_sync();
+        uint32_t all = cnt;   // the wave's sums stay below 256 a byte: no carries between pixels
+        for (int off = 32; off > 0; off >>= 1) all += __shfl_xor(all, off);
         for (int k = 0; k < PW; ++k) {
-            uint32_t v = cnt[k];
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            const uint32_t v = (all >> (8 * k)) & 255u;
             const uint32_t pix = pix0 + uint32_t(k);
             if (lane == 0 && pix < a.npix) a.ao[pix] = a.mask[pix] ? float(v) / 64.0f : 1.0f;
             if (MASK && lane == 0 && pix < a.npix)
@@ -1163,14 +1166,13 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6, 6))) void
         }
     };
     if (a.ntri == 0) {   // nothing to hit: every covered pixel's rays are open
-        for (int k = 0; k < PW; ++k) cnt[k] = 1;
+        cnt = 0x01010101u;
         if (MASK && lane < 2u * PW) om[lane] = ~0u;
         finish();
         return;
     }
     // lane k < PW: pixel pix0 + k's set-up (as computeRTAO's per pixel: ray_ao.cpp:24-31) and its
     // reach box; the union of the covered pixels' boxes
-    float* const pm = pixmem + wave * (PIX_WORDS * PW);
     V3 lo = v3(kFar, kFar, kFar), hi = v3(-kFar, -kFar, -kFar);
     const bool cov = lane < uint32_t(PW) && pix0 + lane < a.npix && a.mask[pix0 + lane];
     if (cov) {
@@ -1218,7 +1220,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6, 6))) void
             if ((words[k] >> (idx[k] & 31u)) & 1u)
                 fixes |= ((uint32_t(a.trig[kTrigBitmapBytes + (idx[k] >> 1)]) >> ((idx[k] & 1u) * 4)) & 15u) << (4 * k);
     }
-    uint32_t* const mine = cutmem + wave * CUT_WAVE;
+    uint32_t* const mine = cutmem;
     CutView cv;
     cv.id = mine;
     cv.box = reinterpret_cast<float*>(mine + CUT_N);
@@ -1226,10 +1228,9 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6, 6))) void
     cv.lbox = reinterpret_cast<float*>(mine + 7 * CUT_N + CUT_LEAF);
     cv.n = -1;
     cv.nleaf = 0;
-    uint32_t visits = 0, tests = 0, rays = 0;
     if (hi.x - lo.x <= a.cut_span && hi.y - lo.y <= a.cut_span && hi.z - lo.z <= a.cut_span) {
         uint32_t steps = 0;
-        wave_cut(a.nodes, a.top, *a.ntop, lo, hi, a.cut_small, stk + (threadIdx.x & ~63u), cv, steps);
+        wave_cut<SB>(a.nodes, a.top, *a.ntop, lo, hi, a.cut_small, stk + (threadIdx.x & ~63u), cv, steps);
         if (COUNT) visits += steps;
     }
     const char* const base = reinterpret_cast<const char*>(a.nodes);
@@ -1280,7 +1281,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6, 6))) void
         return true;
     };
     const auto count_open = [&]() {
-        for (int k = 0; k < PW; ++k) cnt[k] += pk == k ? 1u : 0u;
+        cnt += 1u << (8 * pk);
         if (MASK) atomicOr(&om[2 * pk + (cur_s >> 5)], 1u << (cur_s & 31u));
     };
     const uint32_t total = PW * 64u;
@@ -1303,14 +1304,14 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6, 6))) void
                 while (ih) {
                     const uint32_t v = uint32_t(ref_of(ref, __builtin_ctz(ih)));
                     ih &= ih - 1u;
-                    if (sp < STACK_LDS) lds_col[sp * BT] = v;
+                    if (sp < STACK_LDS) lds_col[sp * SB] = v;
                     else if (sp < STACK_LDS + STACK_SPILL) spill[sp - STACK_LDS] = v;
                     else overflow = true;
                     ++sp;
                 }
             } else if (sp > 0) {
                 --sp;
-                node = int32_t(sp < STACK_LDS ? lds_col[sp * BT] : spill[sp - STACK_LDS]);
+                node = int32_t(sp < STACK_LDS ? lds_col[sp * SB] : spill[sp - STACK_LDS]);
             } else {
                 node = next_cut();
             }
@@ -1360,12 +1361,33 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6, 6))) void
         }
         if (__ballot(live) == 0 && next >= total) break;
     }
+    finish();
+}
+
+// The kernel: WB waves a block, each tracing one group of PW consecutive pixels. One wave a block
+// frees a finished wave's slot at once for the next group (with more, it waits for the block's
+// slowest wave).
+template <int PW, bool COUNT, bool MASK = false, int WB = RC_WAVES>
+__global__ __launch_bounds__(64 * WB) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_trace_ao4rc(TraceArgs4 a,
+                                                                                                     uint32_t refill_min) {
+    static_assert(PW * 4 <= 32, "one 4-bit angle step a pixel in a 32-bit register");
+    constexpr int SB = 64 * WB;
+    static_assert(STACK_LDS >= 14, "the wave cut's two frontiers use 14 rows of the traversal stack");
+    __shared__ uint32_t stk[STACK_LDS * SB];
+    __shared__ uint32_t cutmem[CUT_WAVE * WB];
+    __shared__ float pixmem[PIX_WORDS * PW * WB];
+    __shared__ uint32_t openmem[MASK ? 2 * PW * WB : 1];
+    const uint32_t wave = WB == 1 ? 0u : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t pix0 = (blockIdx.x * WB + wave) * PW;
+    if (pix0 >= a.npix) return;
+    uint32_t visits = 0, tests = 0, rays = 0;
+    trace_group_rc<PW, COUNT, MASK, SB>(a, refill_min, pix0, stk, cutmem + wave * CUT_WAVE, pixmem + wave * (PIX_WORDS * PW),
+                                        openmem + (MASK ? wave * 2 * PW : 0), visits, tests, rays);
     if (COUNT) {
         atomicAdd(&a.counters->visits, (unsigned long long)visits);
         atomicAdd(&a.counters->tests, (unsigned long long)tests);
         atomicAdd(&a.counters->rays, (unsigned long long)rays);
     }
-    finish();
 }
 
 // The sky rays of the visibility pass at 64 spp, traced apart from its AO rays: a wave takes G
@@ -1964,8 +1986,9 @@ int trace_ao4_into(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t
     if (timed) (void)hipEventRecord(c.ev[2], s);
     constexpr int PW = 4;
     if (spp64 && c.refill > 0 && c.wave_cut) {
-        if (count) k_trace_ao4rc<PW, true><<<cdiv(npix, uint32_t(PW * (BT / 64))), BT, 0, s>>>(a, uint32_t(c.refill));
-        else k_trace_ao4rc<PW, false><<<cdiv(npix, uint32_t(PW * (BT / 64))), BT, 0, s>>>(a, uint32_t(c.refill));
+        const uint32_t grid = cdiv(npix, uint32_t(PW * RC_WAVES));
+        if (count) k_trace_ao4rc<PW, true><<<grid, 64 * RC_WAVES, 0, s>>>(a, uint32_t(c.refill));
+        else k_trace_ao4rc<PW, false><<<grid, 64 * RC_WAVES, 0, s>>>(a, uint32_t(c.refill));
     } else if (spp64 && !count && c.refill > 0) {
         k_trace_ao4r<PW><<<cdiv(npix, uint32_t(PW * (BT / 64))), BT, 0, s>>>(a, uint32_t(c.refill));
     } else if (spp64) {
@@ -2055,7 +2078,7 @@ int trace_vis4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_
     constexpr int PW = 4, SKY_G = 8, SKY_STACK = STACK_LDS;
     if (split) {
         a.open_mask = c.vis_open.as<uint64_t>();
-        k_trace_ao4rc<PW, false, true><<<cdiv(npix, uint32_t(PW * (BT / 64))), BT, 0, s>>>(a, uint32_t(c.refill));
+        k_trace_ao4rc<PW, false, true><<<cdiv(npix, uint32_t(PW * RC_WAVES)), 64 * RC_WAVES, 0, s>>>(a, uint32_t(c.refill));
         k_sky4r<SKY_G, SKY_STACK><<<cdiv(npix, uint32_t(SKY_G * (BT / 64))), BT, 0, s>>>(a, uint32_t(c.sky_refill));
     } else if (cut && spp64) k_vis4c<true><<<blocks, BT, 0, s>>>(a);
     else if (cut) k_vis4c<false><<<blocks, BT, 0, s>>>(a);
