@@ -191,8 +191,14 @@ AO_HD RayFrame ray_frame(V3 rd) {
 AO_HD float comp(V3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
 AO_HD V3 wframe_f(V3 v, const RayFrame& f) { return v3(comp(v, f.ix), comp(v, f.iy), comp(v, f.kz)); }
 // intersectTri's hit decision and t: true when the ray hits with t > 1e-4 (t written).
+// SEL: the vertices' frame change by per-component selects (wframe_f), else by rotating and
+// swapping whole vectors (wframe); the same values either way. Which compiles faster depends on
+// the calling kernel (how coherent its rays' dominant axes are, and its register pressure).
+template <bool SEL = true>
 AO_HD bool intersect_t(V3 ro, V3 rd, const RayFrame& f, V3 ta, V3 tb, V3 tc, float& t) {
-    const V3 A = wframe_f(sub(ta, ro), f), B = wframe_f(sub(tb, ro), f), C = wframe_f(sub(tc, ro), f);
+    const V3 A = SEL ? wframe_f(sub(ta, ro), f) : wframe(sub(ta, ro), f.kz, f.sw);
+    const V3 B = SEL ? wframe_f(sub(tb, ro), f) : wframe(sub(tb, ro), f.kz, f.sw);
+    const V3 C = SEL ? wframe_f(sub(tc, ro), f) : wframe(sub(tc, ro), f.kz, f.sw);
     const float Ax = A.x - f.sx * A.z, Ay = A.y - f.sy * A.z;
     const float Bx = B.x - f.sx * B.z, By = B.y - f.sy * B.z;
     const float Cx = C.x - f.sx * C.z, Cy = C.y - f.sy * C.z;

@@ -300,9 +300,10 @@ __device__ __forceinline__ int32_t ref_of(const int4& r, int k) {
     const int32_t lo = (k & 1) ? r.y : r.x, hi = (k & 1) ? r.w : r.z;
     return (k & 2) ? hi : lo;
 }
+template <bool SEL = true>
 __device__ __forceinline__ bool tri_hit(const Ray& r, const float4* __restrict__ ts, uint32_t slot, float& t) {
     const float4 p0 = ts[3 * size_t(slot)], p1 = ts[3 * size_t(slot) + 1], p2 = ts[3 * size_t(slot) + 2];
-    return intersect_t(r.o, r.d, r.f, v3(p0.x, p0.y, p0.z), v3(p0.w, p1.x, p1.y), v3(p1.z, p1.w, p2.x), t);
+    return intersect_t<SEL>(r.o, r.d, r.f, v3(p0.x, p0.y, p0.z), v3(p0.w, p1.x, p1.y), v3(p1.z, p1.w, p2.x), t);
 }
 
 constexpr int CUT_N = 32, CUT_LEAF = 16;
@@ -1139,6 +1140,7 @@ __global__ __launch_bounds__(BT) void k_trace_ao4r(TraceArgs4 a, uint32_t refill
 // end; 0 for a pixel off the mask), for the visibility pass's sky rays (k_sky4r).
 constexpr int PIX_WORDS = 16;   // per pixel: origin, n, t, b (12 floats), x, y
 constexpr int RC_WAVES = 2;     // k_trace_ao4rc: waves a block
+constexpr int SKY_WAVES = 2;    // k_sky4r: waves a block
 // One group of PW pixels (from pix0) for the calling wave. stk: the block's traversal stack
 // (STACK_LDS rows of SB words, SB the block's threads); cutmem: the wave's CUT_WAVE words; pm: the wave's PIX_WORDS x PW
 // floats; om: the wave's 2 PW words (MASK). COUNT: the wave's lanes add their node visits,
@@ -1399,17 +1401,18 @@ __global__ __launch_bounds__(64 * WB) __attribute__((amdgpu_waves_per_eu(6, 6)))
 // 8, 16 and 32 pixels a wave at 1080p on the 1M-triangle scene: 8 is the fastest). A pixel's sky is
 // its unblocked rays / 64 (counted in LDS), 1 off the mask: the same rays and the same any-hit
 // answers as k_vis4c, so the same values.
-template <int G, int SL>
-__global__ __launch_bounds__(BT) void k_sky4r(TraceArgs4 a, uint32_t refill_min) {
+template <int G, int SL, int WB = SKY_WAVES>
+__global__ __launch_bounds__(64 * WB) void k_sky4r(TraceArgs4 a, uint32_t refill_min) {
     static_assert(G <= 16 && (G & (G - 1)) == 0, "one pixel a lane for the set-up, 3 bits a pixel in klass");
-    __shared__ uint32_t stk[SL * BT];
-    __shared__ float pixmem[PIX_WORDS * G * (BT / 64)];
-    __shared__ uint16_t raylist[64 * G * (BT / 64)];
-    __shared__ uint32_t nfree[G * (BT / 64)];
+    constexpr int SB = 64 * WB;   // the block's threads: the stack's row stride
+    __shared__ uint32_t stk[SL * SB];
+    __shared__ float pixmem[PIX_WORDS * G * WB];
+    __shared__ uint16_t raylist[64 * G * WB];
+    __shared__ uint32_t nfree[G * WB];
     uint32_t* const lds_col = stk + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t pix0 = (blockIdx.x * (BT / 64) + wave) * G;
+    const uint32_t wave = WB == 1 ? 0u : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t pix0 = (blockIdx.x * WB + wave) * G;
     if (pix0 >= a.npix) return;
     const unsigned long long below = (1ull << lane) - 1ull;
     float* const pm = pixmem + wave * (PIX_WORDS * G);
@@ -1538,14 +1541,14 @@ __global__ __launch_bounds__(BT) void k_sky4r(TraceArgs4 a, uint32_t refill_min)
                 while (ih) {
                     const uint32_t v = uint32_t(ref_of(ref, __builtin_ctz(ih)));
                     ih &= ih - 1u;
-                    if (sp < SL) lds_col[sp * BT] = v;
+                    if (sp < SL) lds_col[sp * SB] = v;
                     else if (sp < SL + STACK_SPILL) spill[sp - SL] = v;
                     else overflow = true;
                     ++sp;
                 }
             } else if (sp > 0) {
                 --sp;
-                node = int32_t(sp < SL ? lds_col[sp * BT] : spill[sp - SL]);
+                node = int32_t(sp < SL ? lds_col[sp * SB] : spill[sp - SL]);
             } else {
                 node = -1;
             }
@@ -1566,7 +1569,7 @@ __global__ __launch_bounds__(BT) void k_sky4r(TraceArgs4 a, uint32_t refill_min)
                 pend &= pend - 1u;
                 const uint32_t slot = uint32_t(~ref_of(leafref, k));
                 float t;
-                if (slot < a.ntri && tri_hit(r, a.tris, slot, t) && t < kFarT) {
+                if (slot < a.ntri && tri_hit<false>(r, a.tris, slot, t) && t < kFarT) {
                     live = false;   // blocked
                     pend = 0;
                 }
@@ -2079,7 +2082,7 @@ int trace_vis4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_
     if (split) {
         a.open_mask = c.vis_open.as<uint64_t>();
         k_trace_ao4rc<PW, false, true><<<cdiv(npix, uint32_t(PW * RC_WAVES)), 64 * RC_WAVES, 0, s>>>(a, uint32_t(c.refill));
-        k_sky4r<SKY_G, SKY_STACK><<<cdiv(npix, uint32_t(SKY_G * (BT / 64))), BT, 0, s>>>(a, uint32_t(c.sky_refill));
+        k_sky4r<SKY_G, SKY_STACK><<<cdiv(npix, uint32_t(SKY_G * SKY_WAVES)), 64 * SKY_WAVES, 0, s>>>(a, uint32_t(c.sky_refill));
     } else if (cut && spp64) k_vis4c<true><<<blocks, BT, 0, s>>>(a);
     else if (cut) k_vis4c<false><<<blocks, BT, 0, s>>>(a);
     else if (spp64) k_trace_vis4<true><<<blocks, BT, 0, s>>>(a, d_sky, nullptr);
