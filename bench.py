@@ -110,6 +110,7 @@ def cpu_baseline(tris, pos, nrm, mask, spp, radius, row_step, runs=5):
 
 
 WEIGHTS = os.path.join(ROOT, "raw-native_amd", "weights")
+BACKOFF_FRAMES = 64
 HBM_COUNTERS = os.path.join(ROOT, "profiles", "ao_trace_1m_1080p_hbm_counters.md")
 L2_PEAK_GBS = 8 * 16 * 128 * 2.4   # 8 XCDs x 16 channels x 128 B a clock x 2.4 GHz
 
@@ -259,23 +260,40 @@ def learned_surrogate(ctx, raw_ao, torch, d_pos, d_nrm, d_mask, d_rt, d_ss, mask
         ms_fb, rep_fb = _timed(torch, lambda: ctx.learned_ao_dev(*args, rt_radius=radius, probe_stride=stride,
                                                                   fallback_rmse=thr, stream=stream), max(2, steps // 2))
         rmse_fb, mx_fb = _rmse(ctx, raw_ao, torch, d_l, d_rt, d_mask, mask, W, H, stream)
+        # a serving loop of BACKOFF_FRAMES frames with the fall-back back-off on: after a frame that
+        # falls back, the next 1, 2, 4, ... 64 ray trace at once, then the network is tried again
+        ctx.set_option(raw_ao.OPT_LEARNED_BACKOFF, 64)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tried = 0
+        for _ in range(BACKOFF_FRAMES):
+            rep_bo = ctx.learned_ao_dev(*args, rt_radius=radius, probe_stride=stride, fallback_rmse=thr, stream=stream)
+            tried += not rep_bo["network_skipped"]
+        torch.cuda.synchronize()
+        ms_bo = (time.perf_counter() - t0) * 1e3 / BACKOFF_FRAMES
+        ctx.set_option(raw_ao.OPT_LEARNED_BACKOFF, 0)
+        ms_rt, _ = _timed(torch, lambda: ctx.compute_rtao_dev(d_pos, d_nrm, d_mask, W, H, spp, radius, d_l, stream), steps)
         res["probe"] = {"stride": stride, "ms_per_frame_with_probe": ms_probe, "probe_pixels": rep["probe_pixels"],
                         "probe_rmse": rep["probe_rmse"], "fallback_rmse": thr, "fell_back": rep_fb["fell_back"],
                         "ms_per_frame_delivered": ms_fb, "delivered_rmse_vs_rt": rmse_fb,
-                        "delivered_max_error_vs_rt": mx_fb}
+                        "delivered_max_error_vs_rt": mx_fb,
+                        "backoff": {"frames": BACKOFF_FRAMES, "network_tried": tried, "ms_per_frame_delivered": ms_bo,
+                                    "ms_per_frame_rt64_trace_call": ms_rt, "vs_rt64_trace_call": ms_bo / ms_rt}}
+        log("learned fall-back back-off: %d frames, network tried %d, %.3f ms/frame vs %.3f ms a plain trace call"
+            % (BACKOFF_FRAMES, tried, ms_bo, ms_rt))
     log("learned surrogate: %.3f ms/frame, RMSE %.4f vs RT (SSAO %.4f)" % (ms_net, rmse, ss_rmse))
     return res
 
 
-def denoise_1080p(ctx, raw_ao, torch, d_pos, d_nrm, d_mask, d_rt, mask, eye, W, H, radius, steps, trace_ms, frame_ms, stream):
+def denoise_1080p(ctx, raw_ao, torch, d_pos, d_nrm, d_mask, d_rt, mask, eye, W, H, radius, steps, trace_ms, frame_ms, stream,
+                  path):
     """The denoiser form on the same 1080p frame: k rays a pixel of ray-traced AO (the first k of
-    the 64-spp frame's own samples), SSAO and the G-buffer into the network trained to give the
-    64-spp ray-traced AO, timed end to end (trace, SSAO, features, network), with RMSE / max error
-    against this frame's 64-spp AO (and a 256-spp reference)."""
+    the 64-spp frame's own samples), SSAO and the G-buffer into the network trained (weights
+    `path`) to give the 64-spp ray-traced AO, timed end to end (trace, SSAO, features, network),
+    with RMSE / max error against this frame's 64-spp AO (and a 256-spp reference)."""
     import raw_ao.learned as L
-    path = L.DENOISER_1080P
     if not os.path.exists(path):
-        return {"skipped": "no trained weights at raw-native_amd/weights/ao_denoiser_1080p.raow"}
+        return {"skipped": "no trained weights at %s" % os.path.relpath(path, ROOT)}
     _, meta = L.load_weights(path)
     k = int(meta["noisy_spp"])
     ctx.net_load(path)
@@ -601,8 +619,11 @@ def main():
     if not a.no_learned:
         learned = learned_surrogate(ctx, raw_ao, torch, d_pos, d_nrm, d_mask, d_ao, d_ss, mask, eye, W, H, spp, radius,
                                     a.steps, float(np.mean(ms["trace"])), float(np.mean(ms["build"])), frame_ms, stream)
-        learned_dn = denoise_1080p(ctx, raw_ao, torch, d_pos, d_nrm, d_mask, d_ao, mask, eye, W, H, radius, a.steps,
-                                   float(np.mean(ms["trace"])), frame_ms, stream)
+        import raw_ao.learned as L
+        learned_dn = {k: denoise_1080p(ctx, raw_ao, torch, d_pos, d_nrm, d_mask, d_ao, mask, eye, W, H, radius, a.steps,
+                                       float(np.mean(ms["trace"])), frame_ms, stream, path)
+                      for k, path in (("24spp", L.DENOISER_1080P_24), ("16spp", L.DENOISER_1080P_16),
+                                      ("8spp", L.DENOISER_1080P))}
 
     rays = covered * spp
     value = rays * a.steps / wall
@@ -650,7 +671,13 @@ def main():
         out["whole_frame"] = whole_frame(ctx, torch, vpos, vnrm, idx, eye, W, H, radius, a.steps, stream)
     if learned is not None:
         out["learned_ao"] = learned
-        out["learned_ao_denoise_1080p"] = learned_dn
+        # the operating point: the fewest input samples whose frame keeps the bound 0.02 against the
+        # 64-spp frame (24), then the others, fewer samples and a larger error
+        out["learned_ao_denoise_1080p"] = learned_dn["24spp"]
+        out["learned_ao_denoise_1080p_fewer_samples"] = {
+            k: {f: v.get(f) for f in ("noisy_spp", "ms_per_frame", "ms_rt_k_trace", "rmse_vs_rt", "rmse_vs_rt256",
+                                      "speedup_vs_rt64_trace", "meets_rmse_0.02")}
+            for k, v in learned_dn.items() if k != "24spp"}
     if not a.no_denoise:
         out["denoise_4k"] = denoise_4k(ctx, raw_ao, torch, a.steps, stream)
     if not a.no_batch:

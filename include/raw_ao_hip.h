@@ -102,7 +102,14 @@ enum {
      * rays and a lane whose ray ends the next one once n lanes are idle; 0 = each sky ray right
      * after its AO ray in the same lane. Results are the same for every value; only the speed
      * differs. */
-    RAW_AO_OPT_SKY_REFILL = 11
+    RAW_AO_OPT_SKY_REFILL = 11,
+    /* Learned frames with a probe (raw_ao_learned_ao[_dev], probe_stride > 0), 0..1024 (default
+     * 0, off): with n > 0, a frame that fell back makes the next calls of the same image size ray
+     * trace at once, without the network and the probe (fell_back = 2), for 1, 2, 4, ... up to n
+     * calls after consecutive fall-backs, then try the network again; a frame whose probe passes
+     * clears it. For serving loops where the network misses the bound for a while: the frames
+     * that skip the network cost what plain ray tracing costs. */
+    RAW_AO_OPT_LEARNED_BACKOFF = 12
 };
 enum {
     RAW_AO_TRACE_WIDE = 0,     /* 4-wide nodes, LDS stack, triangle tests batched per wave */
@@ -419,7 +426,8 @@ typedef struct raw_ao_learned_report {
     uint32_t probe_pixels;   /* covered probe pixels */
     float probe_rmse;        /* the network against rt_spp ray tracing on them */
     float probe_max_error;
-    int32_t fell_back;       /* 1: d_ao holds the ray-traced frame */
+    int32_t fell_back;       /* 1: d_ao holds the ray-traced frame; 2: the same, the network and the
+                              * probe skipped (RAW_AO_OPT_LEARNED_BACKOFF; probe fields 0) */
 } raw_ao_learned_report;
 int raw_ao_learned_ao_dev(raw_ao_ctx* ctx, const float* d_pos, const float* d_nrm, const uint8_t* d_mask,
                           const float* d_noisy, const float eye[3], int w, int h, int ssao_samples, float ssao_radius,

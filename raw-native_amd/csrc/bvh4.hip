@@ -1141,35 +1141,48 @@ __global__ __launch_bounds__(BT) void k_trace_ao4r(TraceArgs4 a, uint32_t refill
 constexpr int PIX_WORDS = 16;   // per pixel: origin, n, t, b (12 floats), x, y
 constexpr int RC_WAVES = 2;     // k_trace_ao4rc: waves a block
 constexpr int SKY_WAVES = 2;    // k_sky4r: waves a block
-// One group of PW pixels (from pix0) for the calling wave. stk: the block's traversal stack
-// (STACK_LDS rows of SB words, SB the block's threads); cutmem: the wave's CUT_WAVE words; pm: the wave's PIX_WORDS x PW
-// floats; om: the wave's 2 PW words (MASK). COUNT: the wave's lanes add their node visits,
-// triangle tests and rays to visits, tests, rays.
-template <int PW, bool COUNT, bool MASK, int SB>
+// One group of PW = 256 / SPP pixels (from pix0) for the calling wave: its PW x SPP (at most 256)
+// rays, ray ri = sample ri % SPP of pixel pix0 + ri / SPP. stk: the block's traversal stack (STACK_LDS rows of
+// SB words, SB the block's threads); cutmem: the wave's CUT_WAVE words; pm: the wave's
+// PIX_WORDS x PW floats; om: the wave's 2 PW words (MASK, 64 spp); cl: the wave's PW open counts
+// (below 64 spp). COUNT: the wave's lanes add their node visits, triangle tests and rays to
+// visits, tests, rays.
+template <int SPP, bool COUNT, bool MASK, int SB>
 __device__ __forceinline__ void trace_group_rc(const TraceArgs4& a, uint32_t refill_min, uint32_t pix0, uint32_t* stk,
-                                               uint32_t* cutmem, float* pm, uint32_t* om, uint32_t& visits,
+                                               uint32_t* cutmem, float* pm, uint32_t* om, uint32_t* cl, uint32_t& visits,
                                                uint32_t& tests, uint32_t& rays) {
+    static_assert(SPP == 8 || SPP == 16 || SPP == 24 || SPP == 32 || SPP == 64, "8, 16, 24, 32 or 64 rays a pixel");
+    static_assert(!MASK || SPP == 64, "the open masks are 64 bits a pixel");
+    constexpr int PW = 256 / SPP;
+    // 64 spp: a lane's open counts in one register, a byte a pixel; below, LDS counters
+    constexpr bool REGCNT = PW <= 4;
     uint32_t* const lds_col = stk + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
     const unsigned long long below = (1ull << lane) - 1ull;
-    static_assert(PW <= 4, "a lane's open counts: one byte a pixel in a 32-bit register");
-    uint32_t cnt = 0;   // byte k: this lane's open rays of pixel pix0 + k (at most 64 each)
+    uint32_t cnt = 0;   // REGCNT: byte k holds this lane's open rays of pixel pix0 + k (at most 64 each)
     if (MASK && lane < 2u * PW) om[lane] = 0u;
+    if (!REGCNT && lane < uint32_t(PW)) cl[lane] = 0u;
     const auto finish = [&]() {
-        if (MASK) wave_sync();
-        uint32_t all = cnt;   // the wave's sums stay below 256 a byte: no carries between pixels
-        for (int off = 32; off > 0; off >>= 1) all += __shfl_xor(all, off);
-        for (int k = 0; k < PW; ++k) {
-            const uint32_t v = (all >> (8 * k)) & 255u;
-            const uint32_t pix = pix0 + uint32_t(k);
-            if (lane == 0 && pix < a.npix) a.ao[pix] = a.mask[pix] ? float(v) / 64.0f : 1.0f;
-            if (MASK && lane == 0 && pix < a.npix)
-                a.open_mask[pix] = a.mask[pix] ? (uint64_t(om[2 * k + 1]) << 32) | om[2 * k] : 0ull;
+        if (MASK || !REGCNT) wave_sync();
+        if (REGCNT) {
+            uint32_t all = cnt;   // the wave's sums stay below 256 a byte: no carries between pixels
+            for (int off = 32; off > 0; off >>= 1) all += __shfl_xor(all, off);
+            for (int k = 0; k < PW; ++k) {
+                const uint32_t v = (all >> (8 * k)) & 255u;
+                const uint32_t pix = pix0 + uint32_t(k);
+                if (lane == 0 && pix < a.npix) a.ao[pix] = a.mask[pix] ? float(v) / float(SPP) : 1.0f;
+                if (MASK && lane == 0 && pix < a.npix)
+                    a.open_mask[pix] = a.mask[pix] ? (uint64_t(om[2 * k + 1]) << 32) | om[2 * k] : 0ull;
+            }
+        } else {
+            const uint32_t pix = pix0 + lane;
+            if (lane < uint32_t(PW) && pix < a.npix) a.ao[pix] = a.mask[pix] ? float(cl[lane]) / float(SPP) : 1.0f;
         }
     };
     if (a.ntri == 0) {   // nothing to hit: every covered pixel's rays are open
-        cnt = 0x01010101u;
+        cnt = 0x01010101u;   // 64 lanes x 1 = 64 a pixel
         if (MASK && lane < 2u * PW) om[lane] = ~0u;
+        if (!REGCNT && lane < uint32_t(PW)) cl[lane] = SPP;
         finish();
         return;
     }
@@ -1205,20 +1218,21 @@ __device__ __forceinline__ void trace_group_rc(const TraceArgs4& a, uint32_t ref
         return;
     }
     wave_sync();
-    // the angle steps: nibble k of lane L = sample L of pixel pix0 + k (0 where no step applies)
+    // the angle steps: nibble k of lane L = ray L + 64 k (0 where no step applies)
     uint32_t fixes = 0;
     if (a.trig) {
-        uint32_t words[PW], idx[PW];
-        for (int k = 0; k < PW; ++k) {
+        uint32_t words[4], idx[4];
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t ri = lane + 64u * uint32_t(k), pk = ri / SPP;
             idx[k] = 0;
             words[k] = 0;
-            if ((covmask >> k) & 1u) {
-                const uint32_t x = __float_as_uint(pm[k * PIX_WORDS + 12]), y = __float_as_uint(pm[k * PIX_WORDS + 13]);
-                idx[k] = hash24(x, y, 2 * lane + 1);
+            if (ri < uint32_t(PW * SPP) && ((covmask >> pk) & 1u)) {
+                const uint32_t x = __float_as_uint(pm[pk * PIX_WORDS + 12]), y = __float_as_uint(pm[pk * PIX_WORDS + 13]);
+                idx[k] = hash24(x, y, 2 * (ri % SPP) + 1);
                 words[k] = reinterpret_cast<const uint32_t*>(a.trig)[idx[k] >> 5];
             }
         }
-        for (int k = 0; k < PW; ++k)
+        for (int k = 0; k < 4; ++k)
             if ((words[k] >> (idx[k] & 31u)) & 1u)
                 fixes |= ((uint32_t(a.trig[kTrigBitmapBytes + (idx[k] >> 1)]) >> ((idx[k] & 1u) * 4)) & 15u) << (4 * k);
     }
@@ -1252,11 +1266,11 @@ __device__ __forceinline__ void trace_group_rc(const TraceArgs4& a, uint32_t ref
         }
         return -1;
     };
-    // ray ri of the wave: pixel pix0 + ri / 64, sample ri % 64, angle step nib; false for an
+    // ray ri of the wave: pixel pix0 + ri / SPP, sample ri % SPP, angle step nib; false for an
     // uncovered pixel (or one past the image)
     const auto setup = [&](uint32_t ri, uint32_t nib) -> bool {
-        pk = int(ri >> 6);
-        const uint32_t s = ri & 63u;
+        pk = int(ri / SPP);
+        const uint32_t s = ri % SPP;
         if (MASK) cur_s = s;
         if (!((covmask >> pk) & 1u)) return false;
         const float* q = pm + pk * PIX_WORDS;
@@ -1283,10 +1297,11 @@ __device__ __forceinline__ void trace_group_rc(const TraceArgs4& a, uint32_t ref
         return true;
     };
     const auto count_open = [&]() {
-        cnt += 1u << (8 * pk);
+        if (REGCNT) cnt += 1u << (8 * pk);
+        else atomicAdd(&cl[pk], 1u);
         if (MASK) atomicOr(&om[2 * pk + (cur_s >> 5)], 1u << (cur_s & 31u));
     };
-    const uint32_t total = PW * 64u;
+    constexpr uint32_t total = PW * SPP;   // 256 (240 at 24 spp)
     bool live = setup(lane, fixes & 15u);
     uint32_t next = 64;
     while (true) {
@@ -1357,7 +1372,7 @@ __device__ __forceinline__ void trace_group_rc(const TraceArgs4& a, uint32_t ref
         if (next < total && (nidle >= refill_min || nidle == 64u)) {
             // every lane takes part in the shuffle (its source lane may be busy tracing)
             const uint32_t ri = next + uint32_t(__popcll(idle & below));
-            const uint32_t nib = (uint32_t(__shfl(int(fixes), int(ri & 63u))) >> (4u * ((ri >> 6) & uint32_t(PW - 1)))) & 15u;
+            const uint32_t nib = (uint32_t(__shfl(int(fixes), int(ri & 63u))) >> (4u * ((ri >> 6) & 3u))) & 15u;
             if (!live && ri < total) live = setup(ri, nib);
             next += nidle;
         }
@@ -1369,22 +1384,23 @@ __device__ __forceinline__ void trace_group_rc(const TraceArgs4& a, uint32_t ref
 // The kernel: WB waves a block, each tracing one group of PW consecutive pixels. One wave a block
 // frees a finished wave's slot at once for the next group (with more, it waits for the block's
 // slowest wave).
-template <int PW, bool COUNT, bool MASK = false, int WB = RC_WAVES>
+template <int SPP, bool COUNT, bool MASK = false, int WB = RC_WAVES>
 __global__ __launch_bounds__(64 * WB) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_trace_ao4rc(TraceArgs4 a,
                                                                                                      uint32_t refill_min) {
-    static_assert(PW * 4 <= 32, "one 4-bit angle step a pixel in a 32-bit register");
-    constexpr int SB = 64 * WB;
+    constexpr int PW = 256 / SPP, SB = 64 * WB;
     static_assert(STACK_LDS >= 14, "the wave cut's two frontiers use 14 rows of the traversal stack");
     __shared__ uint32_t stk[STACK_LDS * SB];
     __shared__ uint32_t cutmem[CUT_WAVE * WB];
     __shared__ float pixmem[PIX_WORDS * PW * WB];
     __shared__ uint32_t openmem[MASK ? 2 * PW * WB : 1];
+    __shared__ uint32_t cntmem[PW > 4 ? PW * WB : 1];
     const uint32_t wave = WB == 1 ? 0u : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t pix0 = (blockIdx.x * WB + wave) * PW;
     if (pix0 >= a.npix) return;
     uint32_t visits = 0, tests = 0, rays = 0;
-    trace_group_rc<PW, COUNT, MASK, SB>(a, refill_min, pix0, stk, cutmem + wave * CUT_WAVE, pixmem + wave * (PIX_WORDS * PW),
-                                        openmem + (MASK ? wave * 2 * PW : 0), visits, tests, rays);
+    trace_group_rc<SPP, COUNT, MASK, SB>(a, refill_min, pix0, stk, cutmem + wave * CUT_WAVE, pixmem + wave * (PIX_WORDS * PW),
+                                         openmem + (MASK ? wave * 2 * PW : 0), cntmem + (PW > 4 ? wave * PW : 0), visits, tests,
+                                         rays);
     if (COUNT) {
         atomicAdd(&a.counters->visits, (unsigned long long)visits);
         atomicAdd(&a.counters->tests, (unsigned long long)tests);
@@ -1980,7 +1996,9 @@ int trace_ao4_into(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t
                  c.host_trig ? c.d_trig : nullptr, d_counters, uint32_t(w), npix, uint32_t(spp),
                  c.ntri, uint32_t(c.tri_q), radius, float(c.wave_cut) * radius, 4.0f * radius, c.top.as<TopEntry>(),
                  c.top.p ? c.top.as<uint32_t>() + TOP_MAX * (sizeof(TopEntry) / 4) : nullptr};
-    if (!spp64) {
+    // 8, 16, 24 and 32 spp: the refill kernel with 256 / spp pixels a wave, which writes the AO itself
+    const bool rc_low = (spp == 8 || spp == 16 || spp == 24 || spp == 32) && c.refill > 0 && c.wave_cut;
+    if (!spp64 && !rc_low) {
         a.cnt = d_cnt;
         k_zero_u32<<<std::min<uint32_t>(2048, cdiv(npix, BT)), BT, 0, s>>>(a.cnt, npix);
     }
@@ -1988,10 +2006,25 @@ int trace_ao4_into(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t
     const bool count = c.count_next && timed;
     if (timed) (void)hipEventRecord(c.ev[2], s);
     constexpr int PW = 4;
+    const uint32_t rcgrid = cdiv(npix, uint32_t(256 / spp) * RC_WAVES);   // a group of 256 / spp pixels a wave
     if (spp64 && c.refill > 0 && c.wave_cut) {
-        const uint32_t grid = cdiv(npix, uint32_t(PW * RC_WAVES));
-        if (count) k_trace_ao4rc<PW, true><<<grid, 64 * RC_WAVES, 0, s>>>(a, uint32_t(c.refill));
-        else k_trace_ao4rc<PW, false><<<grid, 64 * RC_WAVES, 0, s>>>(a, uint32_t(c.refill));
+        if (count) k_trace_ao4rc<64, true><<<rcgrid, 64 * RC_WAVES, 0, s>>>(a, uint32_t(c.refill));
+        else k_trace_ao4rc<64, false><<<rcgrid, 64 * RC_WAVES, 0, s>>>(a, uint32_t(c.refill));
+    } else if (rc_low) {
+        const uint32_t rf = uint32_t(c.refill);
+        if (spp == 8) {
+            if (count) k_trace_ao4rc<8, true><<<rcgrid, 64 * RC_WAVES, 0, s>>>(a, rf);
+            else k_trace_ao4rc<8, false><<<rcgrid, 64 * RC_WAVES, 0, s>>>(a, rf);
+        } else if (spp == 16) {
+            if (count) k_trace_ao4rc<16, true><<<rcgrid, 64 * RC_WAVES, 0, s>>>(a, rf);
+            else k_trace_ao4rc<16, false><<<rcgrid, 64 * RC_WAVES, 0, s>>>(a, rf);
+        } else if (spp == 24) {
+            if (count) k_trace_ao4rc<24, true><<<rcgrid, 64 * RC_WAVES, 0, s>>>(a, rf);
+            else k_trace_ao4rc<24, false><<<rcgrid, 64 * RC_WAVES, 0, s>>>(a, rf);
+        } else {
+            if (count) k_trace_ao4rc<32, true><<<rcgrid, 64 * RC_WAVES, 0, s>>>(a, rf);
+            else k_trace_ao4rc<32, false><<<rcgrid, 64 * RC_WAVES, 0, s>>>(a, rf);
+        }
     } else if (spp64 && !count && c.refill > 0) {
         k_trace_ao4r<PW><<<cdiv(npix, uint32_t(PW * (BT / 64))), BT, 0, s>>>(a, uint32_t(c.refill));
     } else if (spp64) {
@@ -2017,7 +2050,7 @@ int trace_ao4_into(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t
         }
     }
     if (timed) (void)hipEventRecord(c.ev[3], s);
-    if (!spp64) ao_finish(c, d_mask, a.cnt, npix, uint32_t(spp), d_ao, s);
+    if (!spp64 && !rc_low) ao_finish(c, d_mask, a.cnt, npix, uint32_t(spp), d_ao, s);
     if (!hip_ok(c, hipGetLastError(), "trace launch")) return -2;
     if (timed) c.have_counts = count;
     return 0;
@@ -2081,7 +2114,7 @@ int trace_vis4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_
     constexpr int PW = 4, SKY_G = 8, SKY_STACK = STACK_LDS;
     if (split) {
         a.open_mask = c.vis_open.as<uint64_t>();
-        k_trace_ao4rc<PW, false, true><<<cdiv(npix, uint32_t(PW * RC_WAVES)), 64 * RC_WAVES, 0, s>>>(a, uint32_t(c.refill));
+        k_trace_ao4rc<64, false, true><<<cdiv(npix, uint32_t(PW * RC_WAVES)), 64 * RC_WAVES, 0, s>>>(a, uint32_t(c.refill));
         k_sky4r<SKY_G, SKY_STACK><<<cdiv(npix, uint32_t(SKY_G * SKY_WAVES)), 64 * SKY_WAVES, 0, s>>>(a, uint32_t(c.sky_refill));
     } else if (cut && spp64) k_vis4c<true><<<blocks, BT, 0, s>>>(a);
     else if (cut) k_vis4c<false><<<blocks, BT, 0, s>>>(a);

@@ -141,6 +141,12 @@ int raw_ao_set_option(raw_ao_ctx* x, int option, int value) {
         c.wave_cut = value;
         return RAW_AO_OK;
     }
+    if (option == RAW_AO_OPT_LEARNED_BACKOFF) {
+        if (value < 0 || value > 1024) return fail(c, RAW_AO_ERR_ARG, "learned back-off must be in 0..1024");
+        c.learned_backoff = value;
+        c.lb_next = c.lb_left = 0;
+        return RAW_AO_OK;
+    }
     if (option == RAW_AO_OPT_SKY_REFILL) {
         if (value < 0 || value > 64) return fail(c, RAW_AO_ERR_ARG, "sky refill threshold must be in 0..64");
         c.sky_refill = value;

@@ -47,6 +47,10 @@ struct Ctx {
     int refill{48};              // 64-spp wide kernel: refill lanes once this many are idle (0: off)
     int wave_cut{2};             // 64-spp wide kernel: > 0 starts rays from the wave's cut at wave_cut x radius
     bool near_first{true};       // 64-spp wide kernel: descend into the nearest hit child first
+    int learned_backoff{0};      // learned frames: skip the network for up to this many calls after a fall-back (0: off)
+    int lb_next{0};              // the skip length after the next fall-back (1, 2, 4, ..., learned_backoff)
+    int lb_left{0};              // calls left to skip
+    int lb_w{0}, lb_h{0};        // the image size the skipping applies to
     int sky_refill{32};          // 64-spp visibility: sky rays apart from the AO rays, lanes refilled at this many idle (0: off)
     bool skew{false};            // the skew control: AO query distances x 0.99f
     int build_rounds{0};         // PLOC rounds and collapse levels a build enqueues (0: automatic)

@@ -1203,6 +1203,27 @@ int trace_built(Ctx& c, const float* pos, const float* nrm, const uint8_t* mask,
 
 int learned_frame(Ctx& c, UNet& u, const LearnedArgs& a, raw_ao_learned_report* rep, hipStream_t s) {
     const size_t np = size_t(a.w) * a.h;
+    // back-off after fall-backs (RAW_AO_OPT_LEARNED_BACKOFF): ray trace at once while calls to skip remain
+    const bool backoff = a.probe_stride > 0 && c.learned_backoff > 0;
+    if (backoff && (c.lb_w != a.w || c.lb_h != a.h)) {
+        c.lb_w = a.w;
+        c.lb_h = a.h;
+        c.lb_next = c.lb_left = 0;
+    }
+    if (backoff && c.lb_left > 0) {
+        --c.lb_left;
+        if (c.host_trig) {
+            const int r = trig_table_upload(c);
+            if (r) return r;
+        }
+        const int r = trace_built(c, a.pos, a.nrm, a.mask, a.w, a.h, a.rt_spp, a.rt_radius, a.ao, s);
+        if (r) return r;
+        if (rep) {
+            *rep = raw_ao_learned_report{};
+            rep->fell_back = 2;
+        }
+        return hip_ok(c, hipGetLastError(), "learned frame") ? 0 : -2;
+    }
     if (!ensure(c, u.f_ssao, 4 * np) || !ensure(c, u.f_x, 2 * kNetFeatures * np)) return -2;
     if (c.host_trig) {
         const int r = trig_table_upload(c);
@@ -1243,6 +1264,12 @@ int learned_frame(Ctx& c, UNet& u, const LearnedArgs& a, raw_ao_learned_report* 
             r = trace_built(c, a.pos, a.nrm, a.mask, a.w, a.h, a.rt_spp, a.rt_radius, a.ao, s);
             if (r) return r;
             out.fell_back = 1;
+            if (backoff) {   // skip 1, 2, 4, ... up to learned_backoff calls
+                c.lb_next = c.lb_next <= 0 ? 1 : std::min(2 * c.lb_next, c.learned_backoff);
+                c.lb_left = c.lb_next;
+            }
+        } else if (backoff) {
+            c.lb_next = c.lb_left = 0;
         }
     }
     if (!hip_ok(c, hipGetLastError(), "learned frame")) return -2;

@@ -29,6 +29,7 @@ OPT_WAVE_CUT = 8
 OPT_SKEW = 9
 OPT_BUILD_ROUNDS = 10
 OPT_SKY_REFILL = 11
+OPT_LEARNED_BACKOFF = 12
 NOALLOC_MAX_PIXELS = 1 << 22
 TRACE_WIDE, TRACE_BINARY = 0, 1
 
@@ -124,7 +125,7 @@ class LearnedReport(C.Structure):
 
     def as_dict(self):
         return dict(probe_pixels=self.probe_pixels, probe_rmse=self.probe_rmse, probe_max_error=self.probe_max_error,
-                    fell_back=bool(self.fell_back))
+                    fell_back=bool(self.fell_back), network_skipped=self.fell_back == 2)
 
 
 _F3 = C.c_float * 3

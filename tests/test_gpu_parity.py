@@ -140,6 +140,26 @@ def test_1m_scene_ao_matches_oracle(ctx):
     assert np.array_equal(bits(ao), bits(ref))
 
 
+@pytest.mark.parametrize("spp", [8, 16, 24, 32])
+def test_low_spp_refill_kernel_on_1m_scene(ctx, spp):
+    """8, 16, 24 and 32 spp run the refill kernel with 256 / spp pixels a wave: on the ~1M-triangle
+    scene at 1920 x 1080, radius 1.0, every pixel equals the one-ray-a-lane kernel's (refill off)
+    and, on every 16th row, the oracle's, bit for bit."""
+    print("low-spp refill kernel", spp, flush=True)
+    tris, pos, nrm, mask = big_frame(1920, 1080)
+    ctx.build_ploc(tris, 16)
+    ao = ctx.compute_rtao(pos, nrm, mask, spp, 1.0)
+    ctx.set_option(raw_ao.OPT_REFILL, 0)
+    try:
+        plain = ctx.compute_rtao(pos, nrm, mask, spp, 1.0)
+    finally:
+        ctx.set_option(raw_ao.OPT_REFILL, 48)
+    assert np.array_equal(bits(ao), bits(plain))
+    ref = O.OracleBvh(tris).rtao(pos, nrm, mask, spp, 1.0, row_step=16)
+    rows = slice(0, 1080, 16)
+    assert np.array_equal(bits(ao[rows]), bits(ref[rows]))
+
+
 @pytest.mark.parametrize("name", ["default", "close-512"])
 def test_skew_control_breaks_the_golden(ctx, name):
     """Negative control (src/renderer/accel.cpp:6-11,26-28; .github/workflows/ao-oracle.yml):
@@ -176,7 +196,7 @@ def test_skew_control_on_1m_scene(ctx):
     assert np.array_equal(bits(skew), bits(O.rtao(tris, pos, nrm, mask, 64, 1.0, mode=2)))
 
 
-@pytest.mark.parametrize("spp", [1, 16, 33, 128])
+@pytest.mark.parametrize("spp", [1, 8, 16, 24, 32, 33, 128])
 def test_spp_other_than_64(ctx, spp):
     p = params("default")
     _, pos, nrm, mask = raw_ao.host.cli_gbuffer(96, 96)
