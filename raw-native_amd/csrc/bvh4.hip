@@ -1139,7 +1139,12 @@ __global__ __launch_bounds__(BT) void k_trace_ao4r(TraceArgs4 a, uint32_t refill
 // MASK: also a.open_mask, each pixel's open samples as bits (gathered by LDS atomics as the rays
 // end; 0 for a pixel off the mask), for the visibility pass's sky rays (k_sky4r).
 constexpr int PIX_WORDS = 16;   // per pixel: origin, n, t, b (12 floats), x, y
-constexpr int RC_WAVES = 2;     // k_trace_ao4rc: waves a block
+// k_trace_ao4rc's waves a block, measured at 1080p on the 1M-triangle scene. Two waves free a
+// finished wave's slot sooner (the 64-spp trace alone 5.45 ms against 5.56 with four), but at 64
+// spp without the visibility masks, the frames-in-flight case, four leave the partly idle slots
+// that the next frame's build kernels on other streams fill (6.40 ms a frame in flight against
+// 6.80 with two); eight: 6.63 ms alone. Below 64 spp, 24 spp runs faster with four.
+constexpr int rc_waves(int spp, bool mask) { return (spp == 64 && !mask) || spp == 24 ? 4 : 2; }
 constexpr int SKY_WAVES = 2;    // k_sky4r: waves a block
 // One group of PW = 256 / SPP pixels (from pix0) for the calling wave: its PW x SPP (at most 256)
 // rays, ray ri = sample ri % SPP of pixel pix0 + ri / SPP. stk: the block's traversal stack (STACK_LDS rows of
@@ -1384,7 +1389,7 @@ __device__ __forceinline__ void trace_group_rc(const TraceArgs4& a, uint32_t ref
 // The kernel: WB waves a block, each tracing one group of PW consecutive pixels. One wave a block
 // frees a finished wave's slot at once for the next group (with more, it waits for the block's
 // slowest wave).
-template <int SPP, bool COUNT, bool MASK = false, int WB = RC_WAVES>
+template <int SPP, bool COUNT, bool MASK = false, int WB = rc_waves(SPP, MASK)>
 __global__ __launch_bounds__(64 * WB) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_trace_ao4rc(TraceArgs4 a,
                                                                                                      uint32_t refill_min) {
     constexpr int PW = 256 / SPP, SB = 64 * WB;
@@ -2006,24 +2011,27 @@ int trace_ao4_into(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t
     const bool count = c.count_next && timed;
     if (timed) (void)hipEventRecord(c.ev[2], s);
     constexpr int PW = 4;
-    const uint32_t rcgrid = cdiv(npix, uint32_t(256 / spp) * RC_WAVES);   // a group of 256 / spp pixels a wave
+    // a group of 256 / spp pixels a wave, rc_waves(spp) waves a block
+    const auto rc_launch = [&](auto kernel, int spp_k) {
+        const uint32_t wb = uint32_t(rc_waves(spp_k, false));
+        kernel<<<cdiv(npix, uint32_t(256 / spp_k) * wb), 64 * wb, 0, s>>>(a, uint32_t(c.refill));
+    };
     if (spp64 && c.refill > 0 && c.wave_cut) {
-        if (count) k_trace_ao4rc<64, true><<<rcgrid, 64 * RC_WAVES, 0, s>>>(a, uint32_t(c.refill));
-        else k_trace_ao4rc<64, false><<<rcgrid, 64 * RC_WAVES, 0, s>>>(a, uint32_t(c.refill));
+        if (count) rc_launch(k_trace_ao4rc<64, true>, 64);
+        else rc_launch(k_trace_ao4rc<64, false>, 64);
     } else if (rc_low) {
-        const uint32_t rf = uint32_t(c.refill);
         if (spp == 8) {
-            if (count) k_trace_ao4rc<8, true><<<rcgrid, 64 * RC_WAVES, 0, s>>>(a, rf);
-            else k_trace_ao4rc<8, false><<<rcgrid, 64 * RC_WAVES, 0, s>>>(a, rf);
+            if (count) rc_launch(k_trace_ao4rc<8, true>, 8);
+            else rc_launch(k_trace_ao4rc<8, false>, 8);
         } else if (spp == 16) {
-            if (count) k_trace_ao4rc<16, true><<<rcgrid, 64 * RC_WAVES, 0, s>>>(a, rf);
-            else k_trace_ao4rc<16, false><<<rcgrid, 64 * RC_WAVES, 0, s>>>(a, rf);
+            if (count) rc_launch(k_trace_ao4rc<16, true>, 16);
+            else rc_launch(k_trace_ao4rc<16, false>, 16);
         } else if (spp == 24) {
-            if (count) k_trace_ao4rc<24, true><<<rcgrid, 64 * RC_WAVES, 0, s>>>(a, rf);
-            else k_trace_ao4rc<24, false><<<rcgrid, 64 * RC_WAVES, 0, s>>>(a, rf);
+            if (count) rc_launch(k_trace_ao4rc<24, true>, 24);
+            else rc_launch(k_trace_ao4rc<24, false>, 24);
         } else {
-            if (count) k_trace_ao4rc<32, true><<<rcgrid, 64 * RC_WAVES, 0, s>>>(a, rf);
-            else k_trace_ao4rc<32, false><<<rcgrid, 64 * RC_WAVES, 0, s>>>(a, rf);
+            if (count) rc_launch(k_trace_ao4rc<32, true>, 32);
+            else rc_launch(k_trace_ao4rc<32, false>, 32);
         }
     } else if (spp64 && !count && c.refill > 0) {
         k_trace_ao4r<PW><<<cdiv(npix, uint32_t(PW * (BT / 64))), BT, 0, s>>>(a, uint32_t(c.refill));
@@ -2114,7 +2122,8 @@ int trace_vis4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_
     constexpr int PW = 4, SKY_G = 8, SKY_STACK = STACK_LDS;
     if (split) {
         a.open_mask = c.vis_open.as<uint64_t>();
-        k_trace_ao4rc<64, false, true><<<cdiv(npix, uint32_t(PW * RC_WAVES)), 64 * RC_WAVES, 0, s>>>(a, uint32_t(c.refill));
+        constexpr int WBM = rc_waves(64, true);
+        k_trace_ao4rc<64, false, true><<<cdiv(npix, uint32_t(PW * WBM)), 64 * WBM, 0, s>>>(a, uint32_t(c.refill));
         k_sky4r<SKY_G, SKY_STACK><<<cdiv(npix, uint32_t(SKY_G * SKY_WAVES)), 64 * SKY_WAVES, 0, s>>>(a, uint32_t(c.sky_refill));
     } else if (cut && spp64) k_vis4c<true><<<blocks, BT, 0, s>>>(a);
     else if (cut) k_vis4c<false><<<blocks, BT, 0, s>>>(a);
