@@ -70,6 +70,49 @@ __device__ __forceinline__ int expand(const float* __restrict__ boxes, const int
     return cnt;
 }
 
+// The wide record of a frontier entry at index rec (slots from expand, cnt of them): slot boxes
+// padded outward (pad_of), leaf slots as ~triangle position, inner slots as references to the
+// next level's records from first_next on, their binary ids appended at next[0..].
+__device__ __forceinline__ void write_record(const float* __restrict__ boxes, uint32_t n, const int32_t (&slot)[4], int cnt,
+                                             uint32_t rec, uint32_t first_next, uint32_t* __restrict__ next,
+                                             Node4* __restrict__ nodes) {
+    uint32_t off = 0;
+    float bl[3][4], bh[3][4];
+    int32_t ref[4];
+    uint32_t masks = 0;   // leaf slots in bits 0-3, inner slots in bits 4-7
+    for (int k = 0; k < 4; ++k) {
+        if (k >= cnt) {
+            for (int a = 0; a < 3; ++a) { bl[a][k] = kFar; bh[a][k] = kFar; }
+            ref[k] = EMPTY;
+            continue;
+        }
+        masks |= uint32_t(slot[k]) < n ? (1u << k) : (16u << k);
+        const float* b = boxes + 6 * size_t(slot[k]);
+        for (int a = 0; a < 3; ++a) {
+            const float pd = pad_of(b[a], b[3 + a]);
+            bl[a][k] = b[a] - pd;
+            bh[a][k] = b[3 + a] + pd;
+        }
+        if (uint32_t(slot[k]) < n) {
+            ref[k] = ~slot[k];
+        } else {
+            ref[k] = int32_t(first_next + off);
+            next[off] = uint32_t(slot[k]);
+            ++off;
+        }
+    }
+    Node4 r;
+    r.lx = make_float4(bl[0][0], bl[0][1], bl[0][2], bl[0][3]);
+    r.ly = make_float4(bl[1][0], bl[1][1], bl[1][2], bl[1][3]);
+    r.lz = make_float4(bl[2][0], bl[2][1], bl[2][2], bl[2][3]);
+    r.hx = make_float4(bh[0][0], bh[0][1], bh[0][2], bh[0][3]);
+    r.hy = make_float4(bh[1][0], bh[1][1], bh[1][2], bh[1][3]);
+    r.hz = make_float4(bh[2][0], bh[2][1], bh[2][2], bh[2][3]);
+    r.ref = make_int4(ref[0], ref[1], ref[2], ref[3]);
+    r.spare = make_float4(__uint_as_float(masks), 0, 0, 0);
+    nodes[rec] = r;
+}
+
 // Pass 1 of a level: each wide node's inner-slot count, summed per block.
 // Two launches a level, both with a fixed grid that strides over the tiles of BT frontier
 // entries and reads the frontier's size from the device state, so the host enqueues levels
@@ -78,9 +121,10 @@ __device__ __forceinline__ int expand(const float* __restrict__ boxes, const int
 // Pass 1: each tile's inner-slot count (blk[tile]).
 __global__ __launch_bounds__(BT) void k_wide_count(const WideState* __restrict__ st, const uint32_t* __restrict__ frontier,
                                                    const float* __restrict__ boxes, const int32_t* __restrict__ links,
-                                                   uint32_t n, uint32_t* __restrict__ blk) {
+                                                   uint32_t n, uint32_t* __restrict__ blk, uint32_t defer_max) {
     __shared__ uint32_t ws[BT / 64];
     const uint32_t count = st->count;
+    if (count <= defer_max) return;   // left to k_wide_small
     const uint32_t ntile = (count + BT - 1) / BT;
     for (uint32_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
         __syncthreads();   // the previous tile's ws reads are done
@@ -108,11 +152,13 @@ __global__ __launch_bounds__(BT) void k_wide_count(const WideState* __restrict__
 __global__ __launch_bounds__(BT) void k_wide_write(const WideState* __restrict__ st, WideState* __restrict__ st_next,
                                                    const uint32_t* __restrict__ frontier, uint32_t* __restrict__ next,
                                                    const uint32_t* __restrict__ blk, const float* __restrict__ boxes,
-                                                   const int32_t* __restrict__ links, uint32_t n, Node4* __restrict__ nodes) {
+                                                   const int32_t* __restrict__ links, uint32_t n, Node4* __restrict__ nodes,
+                                                   uint32_t defer_max) {
     __shared__ uint32_t ws[BT / 64], red[BT / 64];
     const WideState s = *st;
-    if (s.count == 0) {
+    if (s.count <= defer_max) {   // empty, or left to k_wide_small: the state and frontier move on unchanged
         if (blockIdx.x == 0 && threadIdx.x == 0) *st_next = s;
+        for (uint32_t i = blockIdx.x * BT + threadIdx.x; i < s.count; i += gridDim.x * BT) next[i] = frontier[i];
         return;
     }
     const uint32_t ntile = (s.count + BT - 1) / BT;
@@ -152,41 +198,64 @@ __global__ __launch_bounds__(BT) void k_wide_write(const WideState* __restrict__
         if (p >= s.count) continue;
         uint32_t off = pre + x - c;
         for (int k = 0; k < w; ++k) off += ws[k];
-        float bl[3][4], bh[3][4];
-        int32_t ref[4];
-        uint32_t masks = 0;   // leaf slots in bits 0-3, inner slots in bits 4-7
-        for (int k = 0; k < 4; ++k) {
-            if (k >= cnt) {
-                for (int a = 0; a < 3; ++a) { bl[a][k] = kFar; bh[a][k] = kFar; }
-                ref[k] = EMPTY;
-                continue;
-            }
-            masks |= uint32_t(slot[k]) < n ? (1u << k) : (16u << k);
-            const float* b = boxes + 6 * size_t(slot[k]);
-            for (int a = 0; a < 3; ++a) {
-                const float pd = pad_of(b[a], b[3 + a]);
-                bl[a][k] = b[a] - pd;
-                bh[a][k] = b[3 + a] + pd;
-            }
-            if (uint32_t(slot[k]) < n) {
-                ref[k] = ~slot[k];
-            } else {
-                ref[k] = int32_t(s.next_base + off);
-                next[off] = uint32_t(slot[k]);
-                ++off;
-            }
-        }
-        Node4 r;
-        r.lx = make_float4(bl[0][0], bl[0][1], bl[0][2], bl[0][3]);
-        r.ly = make_float4(bl[1][0], bl[1][1], bl[1][2], bl[1][3]);
-        r.lz = make_float4(bl[2][0], bl[2][1], bl[2][2], bl[2][3]);
-        r.hx = make_float4(bh[0][0], bh[0][1], bh[0][2], bh[0][3]);
-        r.hy = make_float4(bh[1][0], bh[1][1], bh[1][2], bh[1][3]);
-        r.hz = make_float4(bh[2][0], bh[2][1], bh[2][2], bh[2][3]);
-        r.ref = make_int4(ref[0], ref[1], ref[2], ref[3]);
-        r.spare = make_float4(__uint_as_float(masks), 0, 0, 0);
-        nodes[s.base + p] = r;
+        write_record(boxes, n, slot, cnt, s.base + p, s.next_base + off, next + off, nodes);
     }
+}
+
+// The levels of a small frontier (at most SMALL_W entries) in one workgroup: the top of the tree,
+// where the frontier grows from the root, and its bottom, where it shrinks again. One entry a
+// thread, the next frontier's offsets by a workgroup scan, the frontier kept in LDS between
+// levels; the records are the grid kernels' (write_record) and land where theirs would. It stops
+// at an empty frontier or one wider than SMALL_W (left to the grid levels) and writes the state
+// and frontier back in place. The state's pad (levels the grid kernels ran) is left as it is.
+constexpr int SMALL_W = 1024;
+__global__ __launch_bounds__(SMALL_W) void k_wide_small(WideState* __restrict__ st, uint32_t* __restrict__ frontier,
+                                                        const float* __restrict__ boxes, const int32_t* __restrict__ links,
+                                                        uint32_t n, Node4* __restrict__ nodes) {
+    __shared__ uint32_t cur[SMALL_W], nxt[4 * SMALL_W];
+    __shared__ uint32_t wsum[SMALL_W / 64];
+    WideState s = *st;
+    if (s.count == 0 || s.count > uint32_t(SMALL_W)) return;
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    if (t < s.count) cur[t] = frontier[t];
+    __syncthreads();
+    while (s.count > 0 && s.count <= uint32_t(SMALL_W)) {
+        int32_t slot[4] = {EMPTY, EMPTY, EMPTY, EMPTY};
+        int cnt = 0;
+        uint32_t c = 0;
+        if (t < s.count) {
+            cnt = expand(boxes, links, n, int32_t(cur[t]), slot);
+            for (int k = 0; k < cnt; ++k) c += uint32_t(slot[k]) >= n;
+        }
+        uint32_t x = c;   // inclusive wave scan
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
+        __syncthreads();
+        uint32_t off = x - c, total = 0;
+        for (uint32_t k = 0; k < SMALL_W / 64; ++k) {
+            if (k < w) off += wsum[k];
+            total += wsum[k];
+        }
+        if (t < s.count) write_record(boxes, n, slot, cnt, s.base + t, s.next_base + off, nxt + off, nodes);
+        __syncthreads();   // nxt written, wsum and cur read
+        WideState o;
+        o.count = total;
+        o.base = s.base + s.count;
+        o.next_base = o.base + total;
+        o.pad = s.pad;
+        s = o;
+        if (s.count <= uint32_t(SMALL_W)) {
+            if (t < s.count) cur[t] = nxt[t];
+        } else {
+            for (uint32_t i = t; i < s.count; i += SMALL_W) frontier[i] = nxt[i];
+        }
+        __syncthreads();
+    }
+    if (s.count <= uint32_t(SMALL_W) && t < s.count) frontier[t] = cur[t];
+    if (t == 0) *st = s;
 }
 
 // A single-triangle scene: one record, the leaf in slot 0.
@@ -1859,20 +1928,28 @@ uint32_t predicted_levels(const Ctx& c, uint32_t n) {
 }
 
 // Enqueues `levels` collapse levels from the state at c.wide_parity (frontier in the buffer
-// c.wide_in_a names). The state's pad counts the levels that had a frontier.
-void enqueue_levels(Ctx& c, uint32_t levels, hipStream_t s) {
-    const uint32_t n = c.ntri, G = wide_grid(n);
+// c.wide_in_a names). The state's pad counts the levels these grid kernels ran. With defer, a
+// frontier of at most SMALL_W entries is left to k_wide_small (enqueue_small).
+void enqueue_levels(Ctx& c, uint32_t levels, hipStream_t s, bool defer = true) {
+    const uint32_t n = c.ntri, G = wide_grid(n), defer_max = defer ? uint32_t(SMALL_W) : 0u;
     WideState* d_st = c.wstate.as<WideState>();
     for (uint32_t l = 0; l < levels; ++l) {
         uint32_t* fin = c.wide_in_a ? c.wfront_a.as<uint32_t>() : c.wfront_b.as<uint32_t>();
         uint32_t* fout = c.wide_in_a ? c.wfront_b.as<uint32_t>() : c.wfront_a.as<uint32_t>();
         const int parity = c.wide_parity;
-        k_wide_count<<<G, BT, 0, s>>>(d_st + parity, fin, c.boxes.as<float>(), c.links.as<int32_t>(), n, c.wblk.as<uint32_t>());
+        k_wide_count<<<G, BT, 0, s>>>(d_st + parity, fin, c.boxes.as<float>(), c.links.as<int32_t>(), n, c.wblk.as<uint32_t>(),
+                                      defer_max);
         k_wide_write<<<G, BT, 0, s>>>(d_st + parity, d_st + (parity ^ 1), fin, fout, c.wblk.as<uint32_t>(), c.boxes.as<float>(),
-                                      c.links.as<int32_t>(), n, c.nodes4.as<Node4>());
+                                      c.links.as<int32_t>(), n, c.nodes4.as<Node4>(), defer_max);
         c.wide_parity ^= 1;
         c.wide_in_a = !c.wide_in_a;
     }
+}
+// The small-frontier levels in one workgroup, on the current state and frontier.
+void enqueue_small(Ctx& c, hipStream_t s) {
+    k_wide_small<<<1, SMALL_W, 0, s>>>(c.wstate.as<WideState>() + c.wide_parity,
+                                       c.wide_in_a ? c.wfront_a.as<uint32_t>() : c.wfront_b.as<uint32_t>(), c.boxes.as<float>(),
+                                       c.links.as<int32_t>(), c.ntri, c.nodes4.as<Node4>());
 }
 }  // namespace
 
@@ -1893,8 +1970,11 @@ int wide_build(Ctx& c, hipStream_t s) {
     // enqueued levels emptied the frontier.
     wide_init(c, 2 * n - 2, s);
     if (!hip_ok(c, hipGetLastError(), "wide init")) return -2;
+    // the top levels in one workgroup, the wide middle on the grid, the bottom in one workgroup
     const uint32_t levels = predicted_levels(c, n);
+    enqueue_small(c, s);
     enqueue_levels(c, levels, s);
+    enqueue_small(c, s);
     c.plan_wide_nb.assign(levels, wide_grid(n));
     return top_table(c, s);
 }
@@ -1909,11 +1989,11 @@ int wide_continue(Ctx& c, hipStream_t s) {
             return -2;
         if (h.count == 0) break;
         if (batch > 64) { c.err = "wide collapse did not finish"; return -2; }
-        enqueue_levels(c, 4, s);
+        enqueue_levels(c, 4, s, false);
         c.plan_wide_nb.insert(c.plan_wide_nb.end(), 4, wide_grid(n));
     }
     c.wide_nodes = h.base;
-    c.last_wide_levels = h.pad;
+    c.last_wide_levels = std::max(1u, h.pad);   // grid levels (0 when the small-frontier kernels did all)
     return top_table(c, s);
 }
 
@@ -1928,7 +2008,9 @@ int wide_build_fixed(Ctx& c, hipStream_t s) {
     }
     if (c.plan_wide_nb.empty()) { c.err = "no recorded wide collapse plan"; return -1; }
     wide_init(c, uint32_t(c.root), s);
+    enqueue_small(c, s);
     enqueue_levels(c, uint32_t(c.plan_wide_nb.size()), s);
+    enqueue_small(c, s);
     return top_table(c, s);
 }
 
@@ -1971,7 +2053,7 @@ int build_finish(Ctx& c, hipStream_t s) {
     if (!wide) return 0;
     if (h_wide->count != 0) return wide_continue(c, s);   // the enqueued levels fell short
     c.wide_nodes = h_wide->base;
-    c.last_wide_levels = h_wide->pad;
+    c.last_wide_levels = std::max(1u, h_wide->pad);
     return 0;
 }
 
