@@ -64,9 +64,11 @@ enum {
      * layer's input channels are a multiple of 32 (its float32 sums run chunk-major, so
      * outputs can differ from the others' by one bf16 rounding), in its persistent form
      * (the next tile's data loaded while the current one runs) for layers of 32 inputs and
-     * 32 outputs, 1 the general kernel everywhere, 2 the tap-major kernel where it applies (the
-     * general kernel's sum order: identical outputs to 1), 3 as 0 without the persistent
-     * form (identical outputs to 0). */
+     * 32 outputs and in its direct-copy persistent form (each chunk's input and weights
+     * copied to LDS while the previous chunk runs) for a concatenated input with 32 outputs,
+     * 1 the general kernel everywhere, 2 the tap-major kernel where it applies (the general
+     * kernel's sum order: identical outputs to 1), 3 as 0 without the persistent forms, 4 the
+     * direct-copy form wherever 0 uses the LDS-tiled kernel (3 and 4: identical outputs to 0). */
     RAW_AO_OPT_CONV_KERNEL = 4,
     /* PLOC tuning, 1..4096 (default 1024): at or below this many clusters one workgroup runs every
      * remaining round. The tree is the same for every value; only the speed differs. */

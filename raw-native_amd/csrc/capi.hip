@@ -163,7 +163,7 @@ int raw_ao_set_option(raw_ao_ctx* x, int option, int value) {
         return RAW_AO_OK;
     }
     if (option == RAW_AO_OPT_CONV_KERNEL) {
-        if (value < 0 || value > 3) return fail(c, RAW_AO_ERR_ARG, "convolution kernel must be 0, 1, 2 or 3");
+        if (value < 0 || value > 4) return fail(c, RAW_AO_ERR_ARG, "convolution kernel must be in 0..4");
         c.conv_kernel = value;
         return RAW_AO_OK;
     }

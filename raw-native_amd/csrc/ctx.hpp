@@ -93,8 +93,9 @@ struct Ctx {
     float ms_build{0}, ms_trace{0}, ms_ssao{0}, ms_error{0};
 
     UNet* net{nullptr};          // the learned AO surrogate, made on first use
-    int conv_kernel{0};          // 0: LDS-tiled (persistent for 32 in, 32 out) where it applies, 1: general,
-                                 // 2: tap-major where it applies, 3: LDS-tiled, never persistent
+    int conv_kernel{0};          // 0: LDS-tiled (persistent for 32 in, 32 out; direct-copy persistent for a
+                                 // concatenated input with 32 out) where it applies, 1: general, 2: tap-major where
+                                 // it applies, 3: LDS-tiled, never persistent, 4: direct-copy persistent where it applies
     int cus{0};                  // compute units of the device (read on first use)
 
     // grids of the last eager build, replayed by the capturable build (ploc_build_fixed,

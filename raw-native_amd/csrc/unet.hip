@@ -41,6 +41,17 @@ __device__ __forceinline__ bf16_t f2bf(float f) {   // round to nearest even
     return bf16_t(u >> 16);
 }
 __device__ __forceinline__ float bf2f(bf16_t b) { return __uint_as_float(uint32_t(b) << 16); }
+// f2bf's bits from the hardware conversion (v_cvt_pk_bf16_f32 rounds to nearest even): the same for
+// every non-NaN float; a NaN gives f2bf's 0x7fc0
+__device__ __forceinline__ uint32_t f2bf2(float lo, float hi) {
+    typedef __attribute__((ext_vector_type(2))) __bf16 bf16v2;
+    typedef __attribute__((ext_vector_type(2))) float f32v2;
+    const bf16v2 b = __builtin_convertvector(f32v2{lo, hi}, bf16v2);
+    uint32_t u = __builtin_bit_cast(uint32_t, b);
+    if (lo != lo) u = (u & 0xffff0000u) | 0x7fc0u;
+    if (hi != hi) u = (u & 0x0000ffffu) | 0x7fc00000u;
+    return u;
+}
 
 union V8 {
     uint4 u;
@@ -220,53 +231,84 @@ constexpr int LT_H = 8, LT_W = 32, LH_H = LT_H + 2, LH_W = LT_W + 2;
 // columns ox + 16 (t & 1) + r) with bias and ReLU, rounded to bf16; with a.pool also its 2x2
 // average pool: a 2x2 block is rows 2 wv, 2 wv + 1 of one tile column and lanes r, r ^ 1, summed
 // (x0 + x1) + (x0' + x1') in float from the rounded outputs, times 0.25, rounded (k_pool_fwd's
-// operations, so the same bits).
+// operations, so the same bits). With lb (LDS) the group's biases are read from there.
+// In two steps, so a persistent kernel can hold a finished tile's outputs in registers and
+// store them later: conv_tile_pack rounds the outputs (and forms the pool, lanes r even) into
+// registers, conv_tile_store writes what lies inside the image.
 template <int CT, bool POOL>
-__device__ __forceinline__ void conv_tile_out(const ConvArgs& a, const f32x4 (&acc)[CT][4], int img, int oy, int ox, int wv,
-                                              int g, int r, int co0) {
-    const auto pack = [&](int i, int t) {
-        const int co = co0 + i * 16 + 4 * g;
-        float v[4];
-        for (int j = 0; j < 4; ++j) {
-            v[j] = acc[i][t][j] + (a.bias ? a.bias[co + j] : 0.0f);
-            if (a.relu && v[j] < 0.0f) v[j] = 0.0f;
+struct TileOut {
+    uint2 o[CT][4];
+    uint2 p[CT][POOL ? 2 : 1];
+};
+// the lane's biases (output channels co0 + 16 i + 4 g + j), zero without a bias
+template <int CT>
+__device__ __forceinline__ void conv_lane_bias(const ConvArgs& a, int co0, int g, float (&b)[CT][4]) {
+    for (int i = 0; i < CT; ++i)
+        for (int j = 0; j < 4; ++j) b[i][j] = a.bias ? a.bias[co0 + i * 16 + 4 * g + j] : 0.0f;
+}
+template <int CT, bool POOL>
+__device__ __forceinline__ void conv_tile_pack(const ConvArgs& a, const f32x4 (&acc)[CT][4], const float (&bias)[CT][4],
+                                               TileOut<CT, POOL>& out) {
+    for (int i = 0; i < CT; ++i)
+        for (int t = 0; t < 4; ++t) {
+            float v[4];
+            for (int j = 0; j < 4; ++j) {
+                v[j] = acc[i][t][j] + bias[i][j];
+                if (a.relu && v[j] < 0.0f) v[j] = 0.0f;
+            }
+            out.o[i][t] = make_uint2(f2bf2(v[0], v[1]), f2bf2(v[2], v[3]));
         }
-        return make_uint2(uint32_t(f2bf(v[0])) | (uint32_t(f2bf(v[1])) << 16), uint32_t(f2bf(v[2])) | (uint32_t(f2bf(v[3])) << 16));
-    };
+    if constexpr (POOL) {
+        for (int half = 0; half < 2; ++half)
+            for (int i = 0; i < CT; ++i) {
+                const uint2 u0 = out.o[i][half], u1 = out.o[i][half + 2];
+                const uint32_t n0x = uint32_t(__shfl_xor(int(u0.x), 1)), n0y = uint32_t(__shfl_xor(int(u0.y), 1));
+                const uint32_t n1x = uint32_t(__shfl_xor(int(u1.x), 1)), n1y = uint32_t(__shfl_xor(int(u1.y), 1));
+                const uint32_t w0[2] = {u0.x, u0.y}, w1[2] = {n0x, n0y}, w2[2] = {u1.x, u1.y}, w3[2] = {n1x, n1y};
+                float o[4];
+                for (int j = 0; j < 4; ++j) {
+                    const int sh = 16 * (j & 1);
+                    const float s = (bf2f(bf16_t(w0[j >> 1] >> sh)) + bf2f(bf16_t(w1[j >> 1] >> sh))) +
+                                    (bf2f(bf16_t(w2[j >> 1] >> sh)) + bf2f(bf16_t(w3[j >> 1] >> sh)));
+                    o[j] = s * 0.25f;
+                }
+                out.p[i][half] = make_uint2(f2bf2(o[0], o[1]), f2bf2(o[2], o[3]));
+            }
+    }
+}
+template <int CT, bool POOL>
+__device__ __forceinline__ void conv_tile_store(const ConvArgs& a, const TileOut<CT, POOL>& out, int img, int oy, int ox, int wv,
+                                                int g, int r, int co0) {
     for (int t = 0; t < 4; ++t) {
         const int y = oy + 2 * wv + (t >> 1), x = ox + 16 * (t & 1) + r;
         if (y >= a.h || x >= a.wd) continue;
         const size_t p = size_t(img * a.h + y) * a.wd + x;
-        for (int i = 0; i < CT; ++i) *reinterpret_cast<uint2*>(a.out + p * a.cout + co0 + i * 16 + 4 * g) = pack(i, t);
+        for (int i = 0; i < CT; ++i) *reinterpret_cast<uint2*>(a.out + p * a.cout + co0 + i * 16 + 4 * g) = out.o[i][t];
     }
-    if (!POOL) return;
-    const int ph = a.h >> 1, pw = a.wd >> 1;
-    const int y = oy + 2 * wv;
-    for (int half = 0; half < 2; ++half) {
-        const int x = ox + 16 * half + r;
-        for (int i = 0; i < CT; ++i) {
-            const uint2 u0 = pack(i, half), u1 = pack(i, half + 2);
-            const uint32_t n0x = uint32_t(__shfl_xor(int(u0.x), 1)), n0y = uint32_t(__shfl_xor(int(u0.y), 1));
-            const uint32_t n1x = uint32_t(__shfl_xor(int(u1.x), 1)), n1y = uint32_t(__shfl_xor(int(u1.y), 1));
+    if constexpr (POOL) {
+        const int ph = a.h >> 1, pw = a.wd >> 1;
+        const int y = oy + 2 * wv;
+        for (int half = 0; half < 2; ++half) {
+            const int x = ox + 16 * half + r;
             if ((r & 1) || y + 1 >= a.h || x + 1 >= a.wd) continue;
-            const uint32_t w0[2] = {u0.x, u0.y}, w1[2] = {n0x, n0y}, w2[2] = {u1.x, u1.y}, w3[2] = {n1x, n1y};
-            bf16_t o[4];
-            for (int j = 0; j < 4; ++j) {
-                const int sh = 16 * (j & 1);
-                const float s = (bf2f(bf16_t(w0[j >> 1] >> sh)) + bf2f(bf16_t(w1[j >> 1] >> sh))) +
-                                (bf2f(bf16_t(w2[j >> 1] >> sh)) + bf2f(bf16_t(w3[j >> 1] >> sh)));
-                o[j] = f2bf(s * 0.25f);
-            }
             const size_t q = size_t(img * ph + (y >> 1)) * pw + (x >> 1);
-            *reinterpret_cast<uint2*>(a.pool + q * a.cout + co0 + i * 16 + 4 * g) =
-                make_uint2(uint32_t(o[0]) | (uint32_t(o[1]) << 16), uint32_t(o[2]) | (uint32_t(o[3]) << 16));
+            for (int i = 0; i < CT; ++i) *reinterpret_cast<uint2*>(a.pool + q * a.cout + co0 + i * 16 + 4 * g) = out.p[i][half];
         }
     }
+}
+template <int CT, bool POOL>
+__device__ __forceinline__ void conv_tile_out(const ConvArgs& a, const f32x4 (&acc)[CT][4], int img, int oy, int ox, int wv,
+                                              int g, int r, int co0) {
+    float bias[CT][4];
+    conv_lane_bias<CT>(a, co0, g, bias);
+    TileOut<CT, POOL> out;
+    conv_tile_pack<CT, POOL>(a, acc, bias, out);
+    conv_tile_store<CT, POOL>(a, out, img, oy, ox, wv, g, r, co0);
 }
 constexpr int LH_PS = 40;            // bf16 a staged pixel: 32 channels + 8 pad
 constexpr int LW_RS = 9 * 32 + 8;    // bf16 a staged weight row: 9 taps x 32 channels + 8 pad
 template <int CT, bool POOL = false>
-__global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, int ntiles) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv_lds(ConvArgs a, int ntiles) {
     __shared__ __attribute__((aligned(16))) bf16_t hal[LH_H * LH_W * LH_PS];
     __shared__ __attribute__((aligned(16))) bf16_t wls[CT * 16 * LW_RS];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -362,6 +404,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         }
     };
     load(tile);
+    float bias[CT][4];
+    conv_lane_bias<CT>(a, a.co0, g, bias);
+    TileOut<CT, POOL> done;   // the previous tile's outputs, stored behind the next tile's loads
+    int dtile = -1;
     for (;;) {
         __syncthreads();   // every wave is done with the previous tile's halo
 #pragma unroll
@@ -372,6 +418,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         __syncthreads();
         const int ntile = tile + int(gridDim.x);
         if (ntile < ntiles) load(ntile);   // in flight during this tile's MFMAs
+        if (dtile >= 0) {
+            const int bx = dtile % tiles_x, q0 = dtile / tiles_x, by = q0 % tiles_y, img = q0 / tiles_y;
+            conv_tile_store<CT, POOL>(a, done, img, by * LT_H, bx * LT_W, wv, g, r, a.co0);
+        }
         f32x4 acc[CT][4];
         for (int i = 0; i < CT; ++i)
             for (int t = 0; t < 4; ++t) acc[i][t] = f32x4{0, 0, 0, 0};
@@ -388,11 +438,134 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                 for (int t = 0; t < 4; ++t) acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av.v, bv[t].v, acc[i][t], 0, 0, 0);
             }
         }
-        const int bx = tile % tiles_x, q0 = tile / tiles_x, by = q0 % tiles_y, img = q0 / tiles_y;
-        conv_tile_out<CT, POOL>(a, acc, img, by * LT_H, bx * LT_W, wv, g, r, a.co0);
+        conv_tile_pack<CT, POOL>(a, acc, bias, done);
+        dtile = tile;
         if (ntile >= ntiles) break;
         tile = ntile;
     }
+    const int bx = dtile % tiles_x, q0 = dtile / tiles_x, by = q0 % tiles_y, img = q0 / tiles_y;
+    conv_tile_store<CT, POOL>(a, done, img, by * LT_H, bx * LT_W, wv, g, r, a.co0);
+}
+
+// The persistent, double-buffered form of k_conv_lds for any layer of 32-channel chunks: a
+// block of WV waves takes work items (a 2 WV x 32 pixel tile, one group of CT*16 output
+// channels) b, b + gridDim.x, ...; the work item's chunks, then the next item's, form one
+// sequence of stages, and stage s + 1's input halo and weights are copied from global memory
+// straight into the other LDS buffer (global_load_lds_dwordx4: no registers, no LDS writes by
+// the waves) while stage s's MFMAs run. A copy instruction fills 64 consecutive 16-byte LDS
+// slots, one a lane, so the lanes choose the global pieces: the halo takes 5 slots a pixel
+// (its 4 pieces of 8 channels and a pad slot, the 80-byte pixel stride of k_conv_lds) and the
+// weights 37 a row (36 pieces and a pad), then the group's biases; pad slots and pixels outside
+// the image copy a zeroed global piece. Everything the MFMAs and the epilogue read comes from
+// LDS: the vector memory counter retires in order, so waiting on any global load during a stage
+// would also wait for the next stage's copies. The fragments, the chunk-major sum and the
+// epilogue are k_conv_lds's, so the outputs are the same bit for bit.
+__device__ __attribute__((aligned(16))) uint4 g_zero16[1];   // static device memory starts zeroed
+typedef __attribute__((address_space(3))) void lds_void;
+constexpr int DL_WAVES = 8;   // a block: 16 x 32 pixels (4 waves a block were slower: one block a CU either way)
+template <int CT, int WV, bool POOL = false>
+__global__ __launch_bounds__(64 * WV) void k_conv_dl(ConvArgs a, int ntiles, int items) {
+    constexpr int TH = 2 * WV, HH = TH + 2, HW = LT_W + 2;
+    constexpr int NH = HH * HW * 5, NW = CT * 16 * 37 + CT * 4;   // 16-byte slots: halo, weights + biases
+    __shared__ uint4 h0[NH], h1[NH], w0[NW], w1[NW];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int g = lane >> 4, r = lane & 15;
+    const int tiles_x = (a.wd + LT_W - 1) / LT_W, tiles_y = (a.h + TH - 1) / TH;
+    const int cin = a.c0 + a.c1, hh = a.h >> 1, hw = a.wd >> 1, nch = cin / 32;
+    const int step = int(gridDim.x);
+    int item = int(blockIdx.x);
+    if (item >= items) return;
+    struct Where { int img, oy, ox, co0; };
+    const auto where = [&](int it) __attribute__((always_inline)) {
+        const int tile = it % ntiles, q0 = (tile / tiles_x);
+        return Where{q0 / tiles_y, (q0 % tiles_y) * TH, (tile % tiles_x) * LT_W, a.co0 + (it / ntiles) * CT * 16};
+    };
+    const auto issue = [&](uint4* hb, uint4* wb, int it, int ch) __attribute__((always_inline)) {
+        const Where p = where(it);
+        const int c = 32 * ch;
+#pragma unroll
+        for (int k = 0; k < (NH + 64 * WV - 1) / (64 * WV); ++k) {
+            const int base = (k * WV + wv) * 64, s = base + lane;
+            if (base >= NH) break;
+            if (s < NH) {
+                const int hp = s / 5, q = s - 5 * hp;
+                const int gy = p.oy + hp / HW - 1, gx = p.ox + hp % HW - 1;
+                const void* src = g_zero16;
+                if (q < 4 && gy >= 0 && gy < a.h && gx >= 0 && gx < a.wd) {
+                    const int cc = c + 8 * q;
+                    src = cc < a.c0 ? a.in0 + (size_t(p.img * a.h + gy) * a.wd + gx) * a.c0 + cc
+                                    : a.in1 + (size_t(p.img * hh + (gy >> 1)) * hw + (gx >> 1)) * a.c1 + (cc - a.c0);
+                }
+                __builtin_amdgcn_global_load_lds(src, (lds_void*)(hb + base), 16, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < (NW + 64 * WV - 1) / (64 * WV); ++k) {
+            const int base = (k * WV + wv) * 64, s = base + lane;
+            if (base >= NW) break;
+            if (s < NW) {
+                const int row = s / 37, pc = s - row * 37;
+                const void* src = row >= CT * 16 ? (a.bias ? static_cast<const void*>(a.bias + p.co0 + 4 * (s - CT * 16 * 37))
+                                                           : static_cast<const void*>(g_zero16))
+                                  : pc < 36 ? a.w + size_t(p.co0 + row) * a.kpad + (pc >> 2) * cin + c + 8 * (pc & 3)
+                                            : static_cast<const void*>(g_zero16);
+                __builtin_amdgcn_global_load_lds(src, (lds_void*)(wb + base), 16, 0, 0);
+            }
+        }
+    };
+    f32x4 acc[CT][4];
+    TileOut<CT, POOL> done;   // a finished item's outputs, stored during the next stage
+    Where dw{};
+    bool pending = false;
+    int ch = 0;
+    // one stage: the current chunk from (hc, wc), the next stage's copies into (hn, wn);
+    // false once the block has no work left
+    const auto stage = [&](const uint4* hc, const uint4* wc, uint4* hn, uint4* wn) __attribute__((always_inline)) {
+        __builtin_amdgcn_s_waitcnt(0);   // this wave's copies into (hc, wc) have landed
+        __syncthreads();                 // everyone's have, and nobody still reads (hn, wn)
+        const bool last = ch + 1 == nch;
+        const int nit = last ? item + step : item, nc = last ? 0 : ch + 1;
+        if (nit < items) issue(hn, wn, nit, nc);
+        if (pending) {   // stored behind this stage's copies: the next stage's wait finds them done
+            conv_tile_store<CT, POOL>(a, done, dw.img, dw.oy, dw.ox, wv, g, r, dw.co0);
+            pending = false;
+        }
+        if (ch == 0)
+            for (int i = 0; i < CT; ++i)
+                for (int t = 0; t < 4; ++t) acc[i][t] = f32x4{0, 0, 0, 0};
+        V8 bv[2][4], av[2][CT];
+        const auto frag = [&](int tap, V8 (&b)[4], V8 (&w)[CT]) __attribute__((always_inline)) {
+            const int dy = tap / 3, dx = tap % 3;
+            for (int t = 0; t < 4; ++t) b[t].u = hc[((2 * wv + (t >> 1) + dy) * HW + 16 * (t & 1) + r + dx) * 5 + g];
+            for (int i = 0; i < CT; ++i) w[i].u = wc[(i * 16 + r) * 37 + tap * 4 + g];
+        };
+        frag(0, bv[0], av[0]);
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            if (tap < 8) frag(tap + 1, bv[(tap + 1) & 1], av[(tap + 1) & 1]);
+            for (int i = 0; i < CT; ++i)
+                for (int t = 0; t < 4; ++t)
+                    acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[tap & 1][i].v, bv[tap & 1][t].v, acc[i][t], 0, 0, 0);
+        }
+        if (last) {
+            const float* lb = reinterpret_cast<const float*>(wc + CT * 16 * 37);   // the group's biases, staged
+            float bias[CT][4];
+            for (int i = 0; i < CT; ++i)
+                for (int j = 0; j < 4; ++j) bias[i][j] = a.bias ? lb[i * 16 + 4 * g + j] : 0.0f;
+            conv_tile_pack<CT, POOL>(a, acc, bias, done);
+            dw = where(item);
+            pending = true;
+        }
+        item = nit;
+        ch = nc;
+        return nit < items;
+    };
+    issue(h0, w0, item, 0);
+    for (;;) {
+        if (!stage(h0, w0, h1, w1)) break;
+        if (!stage(h1, w1, h0, w0)) break;
+    }
+    if (pending) conv_tile_store<CT, POOL>(a, done, dw.img, dw.oy, dw.ox, wv, g, r, dw.co0);
 }
 
 // The first layer's convolution (8 input channels, K = 72 padded to 96): one 32-wide k step
@@ -813,7 +986,7 @@ bool conv3x3(Ctx& c, const bf16_t* in0, const bf16_t* in1, int c0, int c1, const
             k_pool_fwd<<<cdivu(size_t(n) * (h / 2) * (wd / 2) * cout, 256), 256, 0, s>>>(out, pool, n, h, wd, cout);
     };
     const bool chunked = (c0 + c1) % 32 == 0 && c0 % 32 == 0;
-    if (c0 == 8 && c1 == 0 && kpad == 96 && c.conv_kernel == 0 && (cout == 16 || cout == 32 || cout == 64)) {
+    if (c0 == 8 && c1 == 0 && kpad == 96 && (c.conv_kernel == 0 || c.conv_kernel == 4) && (cout == 16 || cout == 32 || cout == 64)) {
         if (c.cus == 0) {
             int cus = 0;
             c.cus = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) == hipSuccess && cus > 0 ? cus
@@ -830,7 +1003,39 @@ bool conv3x3(Ctx& c, const bf16_t* in0, const bf16_t* in1, int c0, int c1, const
         pool_after();
         return hip_ok(c, hipGetLastError(), "conv3x3");
     }
-    if (chunked && (c.conv_kernel == 0 || c.conv_kernel == 3) && (c1 == 0 || (h % 2 == 0 && wd % 2 == 0))) {
+    // the direct-copy persistent kernel: by default for a decoder's concatenated input with 32
+    // outputs (several chunks, one channel group: its copies overlap the MFMAs); option 4: always
+    const bool even = c1 == 0 || (h % 2 == 0 && wd % 2 == 0);
+    if (chunked && even && (c.conv_kernel == 4 || (c.conv_kernel == 0 && c1 > 0 && cout == 32))) {
+        if (c.cus == 0) {
+            int cus = 0;
+            c.cus = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) == hipSuccess && cus > 0 ? cus
+                                                                                                                     : 256;
+        }
+        a.pool = pool;
+        const int ntiles = int(size_t(n) * ((h + 2 * DL_WAVES - 1) / (2 * DL_WAVES)) * ((wd + LT_W - 1) / LT_W));
+        int ct = 3;   // two stages of 8 waves' halo and 64 channels' weights exceed the LDS
+        while (ct > 1 && (cout / (16 * ct)) * ntiles < 2 * c.cus) --ct;
+        for (int co0 = 0; co0 < cout;) {
+            const int w16 = std::min(ct, (cout - co0) / 16);
+            const int groups = (cout - co0) / (16 * w16);
+            const int items = ntiles * groups;
+            const unsigned blocks = unsigned(std::min(items, c.cus));   // one block a CU (its LDS)
+            a.co0 = co0;
+#define RAW_AO_DL(CT)                                                                                 \
+    (pool ? k_conv_dl<CT, DL_WAVES, true><<<blocks, 64 * DL_WAVES, 0, s>>>(a, ntiles, items)         \
+          : k_conv_dl<CT, DL_WAVES><<<blocks, 64 * DL_WAVES, 0, s>>>(a, ntiles, items))
+            switch (w16) {
+                case 1: RAW_AO_DL(1); break;
+                case 2: RAW_AO_DL(2); break;
+                default: RAW_AO_DL(3); break;
+            }
+#undef RAW_AO_DL
+            co0 += groups * 16 * w16;
+        }
+        return hip_ok(c, hipGetLastError(), "conv3x3");
+    }
+    if (chunked && (c.conv_kernel == 0 || c.conv_kernel == 3) && even) {
         if (c.cus == 0) {
             int cus = 0;
             c.cus = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) == hipSuccess && cus > 0 ? cus

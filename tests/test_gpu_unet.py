@@ -332,16 +332,20 @@ def test_weights_file_loads_through_the_abi(ctx, tmp_path):
 
 
 def test_persistent_conv_matches_lds_tiled(ctx):
-    """The persistent LDS-tiled form (layers of 32 inputs and 32 outputs) and the
-    one-tile-a-block form give the same bits: single layers at odd sizes, with several images
-    and more tiles than resident blocks, with and without bias and ReLU (and layers the
-    persistent form does not take, which both options route alike); and every activation of a
-    whole forward pass."""
+    """The persistent forms (the LDS-tiled one for 32 inputs and 32 outputs, the direct-copy one
+    for a concatenated input with 32 outputs by default and for every layer with option 4) and
+    the one-tile-a-block form give the same bits: single layers at odd sizes, with several images
+    and more tiles than resident blocks, with and without bias and ReLU, channel groups of 48
+    with a remainder of 16 or 32 (and layers the persistent forms do not take, which every option
+    routes alike); and every activation of a whole forward pass, the encoder's pooled outputs
+    included."""
     import raw_ao
     r = np.random.default_rng(93)
     try:
         for (n1, h1, w1, c0, c1, cout, relu) in ((2, 14, 38, 32, 0, 32, 1), (3, 9, 33, 32, 0, 32, 0), (1, 1, 1, 32, 0, 32, 1),
-                                                 (2, 600, 530, 32, 0, 32, 1), (1, 30, 70, 32, 64, 32, 1), (3, 9, 33, 64, 0, 64, 0)):
+                                                 (2, 600, 530, 32, 0, 32, 1), (1, 30, 70, 32, 64, 32, 1), (3, 9, 33, 64, 0, 64, 0),
+                                                 (2, 34, 66, 96, 128, 96, 1), (1, 17, 45, 128, 0, 128, 0), (2, 40, 72, 32, 64, 32, 0),
+                                                 (1, 2, 2, 64, 96, 64, 1)):
             print("persistent conv", n1, h1, w1, c0, c1, cout, flush=True)
             k = 9 * (c0 + c1)
             x0 = bf16_case(r, (n1, h1, w1, c0))
@@ -349,25 +353,27 @@ def test_persistent_conv_matches_lds_tiled(ctx):
             wt = bf16_case(r, (cout, k), np.sqrt(2.0 / k))
             bias = torch.from_numpy(r.normal(0, 0.1, cout).astype(np.float32)).cuda()
             outs = {}
-            for kern in (0, 3):
+            for kern in (0, 3, 4):
                 ctx.set_option(raw_ao.OPT_CONV_KERNEL, kern)
                 out = torch.zeros(n1 * h1 * w1 * cout, dtype=torch.int16, device="cuda")
                 kp = [dev(x0), dev(x1) if c1 else None, dev(wt)]
                 ctx.conv3x3_dev(kp[0], kp[1], c0, c1, kp[2], k, bias, cout, relu, n1, w1, h1, out)
                 outs[kern] = host16(out).copy()
             assert np.array_equal(outs[0], outs[3])
+            assert np.array_equal(outs[4], outs[3])
         n, h, w = 2, 40, 72
         ctx.net_set_params(random_params(94))
         xb, mask, _ = net_case(95, n, h, w)
         keep = [dev(xb), dev(mask)]
         ao = torch.zeros(n * h * w, dtype=torch.float32, device="cuda")
         acts = {}
-        for kern in (0, 3):
+        for kern in (0, 3, 4):
             ctx.set_option(raw_ao.OPT_CONV_KERNEL, kern)
             ctx.net_forward_dev(keep[0], keep[1], n, w, h, ao)
             acts[kern] = [ctx.net_read_act(i, n * (h >> lv) * (w >> lv) * co) for i, (_, _, co, lv) in enumerate(U.LAYERS)]
-        for a, b in zip(acts[0], acts[3]):
+        for a, b, c in zip(acts[0], acts[3], acts[4]):
             assert np.array_equal(a, b)
+            assert np.array_equal(c, b)
     finally:
         ctx.set_option(raw_ao.OPT_CONV_KERNEL, 0)
 
