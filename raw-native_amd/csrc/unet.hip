@@ -745,14 +745,26 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
     }
 }
 
-// grad[co][k] (bias grad at k == K) = sum over chunks of the partials, in chunk order.
-__global__ void k_wgrad_reduce(const float* __restrict__ part, int nchunks, int cout, int kpg, int K, float* __restrict__ gw,
+// The chunks' partials summed in a fixed order, two levels so every element's sum runs in
+// parallel: group gi of `per` consecutive chunks, in chunk order, to part2[gi][cout][kpg];
+// then the groups in group order to grad[co][k] (the bias grad at k == K).
+__global__ void k_wgrad_sum_groups(const float* __restrict__ part, int nchunks, int per, int cout, int kpg, int K,
+                                   float* __restrict__ part2) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x, gi = blockIdx.y;
+    if (e >= cout * (K + 1)) return;
+    const int co = e / (K + 1), k = e - co * (K + 1);
+    const int c0 = gi * per, c1 = min(nchunks, c0 + per);
+    float s = 0.0f;
+    for (int c = c0; c < c1; ++c) s += part[(size_t(c) * cout + co) * kpg + k];
+    part2[(size_t(gi) * cout + co) * kpg + k] = s;
+}
+__global__ void k_wgrad_reduce(const float* __restrict__ part2, int ngroups, int cout, int kpg, int K, float* __restrict__ gw,
                                float* __restrict__ gb) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= cout * (K + 1)) return;
     const int co = e / (K + 1), k = e - co * (K + 1);
     float s = 0.0f;
-    for (int c = 0; c < nchunks; ++c) s += part[(size_t(c) * cout + co) * kpg + k];
+    for (int gi = 0; gi < ngroups; ++gi) s += part2[(size_t(gi) * cout + co) * kpg + k];
     if (k < K) gw[size_t(co) * K + k] = s;
     else gb[co] = s;
 }
@@ -1119,7 +1131,9 @@ bool conv3x3_wgrad(Ctx& c, DevBuf& part, const bf16_t* dy, const bf16_t* in0, co
     int chunk = 128;
     while (size_t(ktiles) * ((px + chunk - 1) / chunk) > 2048 && chunk < (1 << 20)) chunk *= 2;
     const int nch = int((px + chunk - 1) / chunk);
-    if (!ensure(c, part, size_t(nch) * cout * kpg * 4)) return false;
+    const int per = (nch + 31) / 32, ngroups = (nch + per - 1) / per;   // at most 32 groups, none empty
+    const size_t psize = size_t(nch) * cout * kpg;
+    if (!ensure(c, part, (psize + size_t(ngroups) * cout * kpg) * 4)) return false;
     WgradArgs wa{dy, in0, in1, part.as<float>(), n, h, wd, c0, c1, cout, kpg, chunk};
     const dim3 grid{unsigned(ktiles), unsigned(nch), 1u};
     switch (cout / 16) {
@@ -1129,7 +1143,10 @@ bool conv3x3_wgrad(Ctx& c, DevBuf& part, const bf16_t* dy, const bf16_t* in0, co
         case 8: k_wgrad<8><<<grid, 256, 0, s>>>(wa); break;
         default: c.err = "conv3x3 weight grad: output channels must be 32, 64, 96 or 128"; return false;
     }
-    k_wgrad_reduce<<<cdivu(size_t(cout) * (K + 1), 256), 256, 0, s>>>(part.as<float>(), nch, cout, kpg, K, gw, gb);
+    float* part2 = part.as<float>() + psize;
+    const unsigned eb = cdivu(size_t(cout) * (K + 1), 256);
+    k_wgrad_sum_groups<<<dim3{eb, unsigned(ngroups), 1u}, 256, 0, s>>>(part.as<float>(), nch, per, cout, kpg, K, part2);
+    k_wgrad_reduce<<<eb, 256, 0, s>>>(part2, ngroups, cout, kpg, K, gw, gb);
     return hip_ok(c, hipGetLastError(), "conv3x3 weight grad");
 }
 

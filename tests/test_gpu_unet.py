@@ -114,6 +114,11 @@ def test_conv3x3_wgrad_matches_oracle(ctx, case):
     ab = np.abs(fdy.reshape(-1, cout)).sum(0)
     assert (np.abs(gw.cpu().numpy().reshape(cout, k) - rw) <= 1e-5 * aw + 1e-30).all()
     assert (np.abs(gb.cpu().numpy() - rb) <= 1e-5 * ab + 1e-30).all()
+    # the chunks' partials are summed in a fixed order: a second call gives the same bits
+    gw2, gb2 = torch.full_like(gw, 7.0), torch.full_like(gb, 7.0)
+    ctx.conv3x3_wgrad_dev(keep[0], keep[1], keep[2], c0, c1, cout, n, w, h, gw2, gb2)
+    torch.cuda.synchronize()
+    assert torch.equal(gw.view(torch.int32), gw2.view(torch.int32)) and torch.equal(gb.view(torch.int32), gb2.view(torch.int32))
 
 
 def random_params(seed):
