@@ -3,8 +3,9 @@
 //
 //   bounds   centroid box (two-level reduction) and the power-of-two scale
 //   morton   30-bit keys, one thread a triangle
-//   sort     stable LSD radix sort of (key, index), 8-bit digits, LDS-staged ranks from
-//            wave64 ballots; stability with ascending indices gives std::sort's order
+//   sort     stable LSD radix sort of (key, index), three passes of 10-bit digits over the
+//            30-bit keys, ranks from wave64 ballots; stability with ascending indices gives
+//            std::sort's order
 //            of (key, index) pairs (rt_bvh.cpp:73)
 //   leaves   one leaf a sorted position
 //   rounds   per round three launches: nearest neighbours within the radius from an
@@ -25,6 +26,8 @@ namespace {
 constexpr int BT = 256;          // threads a block
 constexpr int RS_ITEMS = 8;      // keys a thread in the radix passes
 constexpr int RS_TILE = BT * RS_ITEMS;
+constexpr int RS_BITS = 10, RS_DIG = 1 << RS_BITS;   // digit width and count
+constexpr int RS_PER = RS_DIG / BT;                  // digits a thread
 constexpr int MAX_R = 32;        // largest PLOC radius supported
 
 __device__ __forceinline__ unsigned long long lanemask_lt() {
@@ -121,16 +124,16 @@ __global__ __launch_bounds__(BT) void k_morton(const float* __restrict__ tris, u
 // ---- stable LSD radix sort ----------------------------------------------------------
 __global__ __launch_bounds__(BT) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n, int shift,
                                                    uint32_t* __restrict__ hist, uint32_t nblk) {
-    __shared__ uint32_t cnt[256];
-    cnt[threadIdx.x] = 0;
+    __shared__ uint32_t cnt[RS_DIG];
+    for (int k = 0; k < RS_PER; ++k) cnt[threadIdx.x + BT * k] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * RS_TILE;
     for (int it = 0; it < RS_ITEMS; ++it) {
         const uint32_t i = base + it * BT + threadIdx.x;
-        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
+        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & (RS_DIG - 1u)], 1u);
     }
     __syncthreads();
-    hist[threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
+    for (int k = 0; k < RS_PER; ++k) hist[(threadIdx.x + BT * k) * nblk + blockIdx.x] = cnt[threadIdx.x + BT * k];
 }
 
 // One block a digit: the exclusive scan of that digit's counts over the tiles (a contiguous
@@ -161,67 +164,86 @@ __global__ __launch_bounds__(BT) void k_radix_colscan(uint32_t* __restrict__ his
     if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
+// A tile's keys in 8 steps of 256; a step's keys of one digit keep their order (wave, then
+// lane). A wave's count of a digit in a step sits in LDS stamped with the step, so entries of
+// earlier steps read as zero without clearing 4 x 1,024 counters a step.
 __global__ __launch_bounds__(BT) void k_radix_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                       uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n,
                                                       int shift, const uint32_t* __restrict__ offs, uint32_t nblk,
                                                       const uint32_t* __restrict__ totals) {
-    __shared__ uint32_t run[256];
-    __shared__ uint32_t wcnt[BT / 64][256];
-    const int w = threadIdx.x >> 6;
-    {   // digit bases: exclusive scan of the 256 digit totals
-        const uint32_t v = totals[threadIdx.x];
-        uint32_t x = v;
-        const int lane = threadIdx.x & 63;
+    __shared__ uint32_t run[RS_DIG];
+    __shared__ uint32_t wcnt[BT / 64][RS_DIG];   // (step + 1) << 16 | count
+    __shared__ uint32_t part[BT];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    {   // digit bases: exclusive scan of the 1,024 digit totals (4 consecutive digits a thread)
+        uint32_t v[RS_PER], sum = 0;
+        for (int k = 0; k < RS_PER; ++k) {
+            v[k] = totals[threadIdx.x * RS_PER + k];
+            sum += v[k];
+        }
+        uint32_t x = sum;
         for (int off = 1; off < 64; off <<= 1) {
             const uint32_t y = __shfl_up(x, off);
             if (lane >= off) x += y;
         }
-        if (lane == 63) wcnt[0][w] = x;
+        if (lane == 63) part[w] = x;
+        for (int k = 0; k < RS_PER; ++k)
+            for (int q = 0; q < BT / 64; ++q) wcnt[q][threadIdx.x + BT * k] = 0;
         __syncthreads();
-        uint32_t before = 0;
-        for (int k = 0; k < w; ++k) before += wcnt[0][k];
-        run[threadIdx.x] = before + x - v + offs[threadIdx.x * nblk + blockIdx.x];
+        uint32_t before = x - sum;
+        for (int k = 0; k < w; ++k) before += part[k];
+        for (int k = 0; k < RS_PER; ++k) {
+            const uint32_t d = threadIdx.x * RS_PER + k;
+            run[d] = before + offs[d * nblk + blockIdx.x];
+            before += v[k];
+        }
         __syncthreads();
     }
     const unsigned long long lt = lanemask_lt();
     for (int it = 0; it < RS_ITEMS; ++it) {
-        for (int k = 0; k < BT / 64; ++k) wcnt[k][threadIdx.x] = 0;
-        __syncthreads();
+        const uint32_t stamp = uint32_t(it + 1) << 16;
         const uint32_t i = blockIdx.x * RS_TILE + it * BT + threadIdx.x;
         const bool valid = i < n;
         const uint32_t key = valid ? kin[i] : 0u;
-        const uint32_t d = (key >> shift) & 255u;
+        const uint32_t d = (key >> shift) & (RS_DIG - 1u);
         unsigned long long same = __ballot(valid);
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < RS_BITS; ++b) {
             const bool bit = (d >> b) & 1u;
             const unsigned long long bm = __ballot(bit);
             same &= bit ? bm : ~bm;
         }
-        const uint32_t rank = __popcll(same & lt);
-        if (valid && rank == 0) wcnt[w][d] = __popcll(same);
+        const uint32_t rank = __popcll(same & lt), cnt = __popcll(same);
+        if (valid && rank == 0) wcnt[w][d] = stamp | cnt;
         __syncthreads();
         if (valid) {
             uint32_t pos = run[d] + rank;
-            for (int k = 0; k < w; ++k) pos += wcnt[k][d];
+            for (int k = 0; k < w; ++k) {
+                const uint32_t e = wcnt[k][d];
+                pos += (e & 0xffff0000u) == stamp ? (e & 0xffffu) : 0u;
+            }
             kout[pos] = key;
             vout[pos] = vin[i];
         }
         __syncthreads();
-        uint32_t add = 0;
-        for (int k = 0; k < BT / 64; ++k) add += wcnt[k][threadIdx.x];
-        run[threadIdx.x] += add;
+        if (valid && rank == 0) atomicAdd(&run[d], cnt);   // integer sums: the order does not matter
         __syncthreads();
     }
 }
 
 // ---- leaves (rt_bvh.cpp:74-82) ----------------------------------------------------
+// Also the triangles in leaf order (the traversal's triangle records: 3 float4 a triangle,
+// the 9 coordinates and the triangle's index).
 __global__ __launch_bounds__(BT) void k_leaves(const float* __restrict__ tris, const uint32_t* __restrict__ order, uint32_t n,
                                                float* __restrict__ boxes, int32_t* __restrict__ links,
-                                               uint32_t* __restrict__ clus) {
+                                               uint32_t* __restrict__ clus, float4* __restrict__ tsorted) {
     const uint32_t p = blockIdx.x * BT + threadIdx.x;
     if (p >= n) return;
     const uint32_t t = order[p];
     const float* v = tris + 9 * size_t(t);
+    float4* o = tsorted + 3 * size_t(p);
+    o[0] = make_float4(v[0], v[1], v[2], v[3]);
+    o[1] = make_float4(v[4], v[5], v[6], v[7]);
+    o[2] = make_float4(v[8], __int_as_float(int32_t(t)), 0.0f, 0.0f);
     float* b = boxes + 6 * size_t(p);
     for (int a = 0; a < 3; ++a) {
         float l = v[a], h = v[a];
@@ -423,7 +445,7 @@ __global__ __launch_bounds__(BT) void k_ploc_merge(const PlocState* __restrict__
 // All remaining rounds once at most TAIL_M clusters are left, in one workgroup with the
 // clusters' ids and boxes in LDS: a thread owns TAIL_PER consecutive positions, so the
 // block-wide prefix sums run in position order. Same rounds, same order, same ids.
-constexpr int TAIL_T = 1024, TAIL_PER = 4, TAIL_M = TAIL_T * TAIL_PER;
+constexpr int TAIL_T = 1024, TAIL_PER = 4, TAIL_M = TAIL_T * TAIL_PER;   // 512 or 256 threads: 5% or 26% slower builds
 __global__ __launch_bounds__(TAIL_T) void k_ploc_tail(const PlocState* __restrict__ st_in, PlocState* __restrict__ st_out,
                                                       int radius, const uint32_t* __restrict__ cin, float* __restrict__ boxes,
                                                       int32_t* __restrict__ links) {
@@ -620,17 +642,18 @@ void enqueue_front(Ctx& c, const float* d_tris, uint32_t n, hipStream_t s) {
     const uint32_t nparts = std::min<uint32_t>(1024, nblk);
     k_bounds_part<<<nparts, BT, 0, s>>>(d_tris, n, c.bounds_part.as<float>());
     k_bounds_final<<<1, BT, 0, s>>>(c.bounds_part.as<float>(), int(nparts), c.g4.as<float>());
-    k_morton<<<nblk, BT, 0, s>>>(d_tris, n, c.g4.as<float>(), c.keys.as<uint32_t>(), c.vals.as<uint32_t>());
-    uint32_t *ki = c.keys.as<uint32_t>(), *vi = c.vals.as<uint32_t>(), *ko = c.keys2.as<uint32_t>(), *vo = c.vals2.as<uint32_t>();
-    for (int shift = 0; shift < 30; shift += 8) {
+    // keys into keys2/vals2: after the three passes the sorted pairs are in keys/vals
+    k_morton<<<nblk, BT, 0, s>>>(d_tris, n, c.g4.as<float>(), c.keys2.as<uint32_t>(), c.vals2.as<uint32_t>());
+    uint32_t *ki = c.keys2.as<uint32_t>(), *vi = c.vals2.as<uint32_t>(), *ko = c.keys.as<uint32_t>(), *vo = c.vals.as<uint32_t>();
+    for (int shift = 0; shift < 30; shift += RS_BITS) {
         k_radix_hist<<<nblk_rs, BT, 0, s>>>(ki, n, shift, c.hist.as<uint32_t>(), nblk_rs);
-        k_radix_colscan<<<256, BT, 0, s>>>(c.hist.as<uint32_t>(), nblk_rs, c.rs_tot.as<uint32_t>());
+        k_radix_colscan<<<RS_DIG, BT, 0, s>>>(c.hist.as<uint32_t>(), nblk_rs, c.rs_tot.as<uint32_t>());
         k_radix_scatter<<<nblk_rs, BT, 0, s>>>(ki, vi, ko, vo, n, shift, c.hist.as<uint32_t>(), nblk_rs, c.rs_tot.as<uint32_t>());
         std::swap(ki, ko);
         std::swap(vi, vo);
     }
-    // four passes: the sorted pairs are back in keys/vals
-    k_leaves<<<nblk, BT, 0, s>>>(d_tris, vi, n, c.boxes.as<float>(), c.links.as<int32_t>(), c.clus_a.as<uint32_t>());
+    k_leaves<<<nblk, BT, 0, s>>>(d_tris, vi, n, c.boxes.as<float>(), c.links.as<int32_t>(), c.clus_a.as<uint32_t>(),
+                                 c.tris_sorted.as<float4>());
     k_ploc_init<<<1, 1, 0, s>>>(c.state.as<PlocState>(), n);
     c.ploc_parity = 0;
     c.ploc_in_a = true;
@@ -649,11 +672,12 @@ int ploc_build(Ctx& c, const float* d_tris, uint32_t n, int radius, hipStream_t 
     const uint32_t nblk_rs = cdiv(n, RS_TILE);
     const uint32_t nblk = cdiv(n, BT);
     if (!ensure(c, c.keys, 4 * size_t(n)) || !ensure(c, c.vals, 4 * size_t(n)) || !ensure(c, c.keys2, 4 * size_t(n)) ||
-        !ensure(c, c.vals2, 4 * size_t(n)) || !ensure(c, c.hist, 4 * 256 * size_t(nblk_rs)) ||
+        !ensure(c, c.vals2, 4 * size_t(n)) || !ensure(c, c.hist, 4 * size_t(RS_DIG) * nblk_rs) ||
+        !ensure(c, c.tris_sorted, 48 * size_t(n)) ||
         !ensure(c, c.boxes, 24 * nn) || !ensure(c, c.links, 12 * nn) || !ensure(c, c.clus_a, 4 * size_t(n)) ||
         !ensure(c, c.clus_b, 4 * size_t(n)) || !ensure(c, c.nn, 4 * size_t(n)) || !ensure(c, c.flags, size_t(n)) ||
         !ensure(c, c.blk, 8 * size_t(nblk)) || !ensure(c, c.state, 3 * sizeof(PlocState)) || !ensure(c, c.g4, 16) ||
-        !ensure(c, c.bounds_part, 6 * 4 * 1024) || !ensure(c, c.rs_tot, 4 * 256))
+        !ensure(c, c.bounds_part, 6 * 4 * 1024) || !ensure(c, c.rs_tot, 4 * RS_DIG))
         return -2;
     if (!c.h_state && !hip_ok(c, hipHostMalloc(reinterpret_cast<void**>(&c.h_state), 256), "hipHostMalloc")) return -2;
     // Everything from the bounds to the root is enqueued at once: the round kernels read the

@@ -5,7 +5,8 @@
 //   node  64 B = 4 x float4: child 0 box, child 1 box (lo xyz hi xyz each), then the two
 //         child references (>= 0 an inner node, < 0 ~ a leaf slot). Both boxes arrive in
 //         one fetch, so a visit tests the two children together (Aila and Laine 2009).
-//   tri   48 B = 3 x float4: a xyz, b xyz, c xyz and the input index, in leaf order.
+//   tri   48 B = 3 x float4: a xyz, b xyz, c xyz and the input index, in leaf order
+//         (written with the PLOC leaves).
 // Boxes are padded outward, and the slab exit and the ray's tMax widened, so the box
 // test only ever adds visits: an any-hit answer is the triangle test's alone, i.e. the
 // scan of every triangle (accel.cpp:30-35) whatever the tree.
@@ -31,20 +32,11 @@ __device__ __forceinline__ float pad_of(float l, float h) {
     return (h - l) * 6.103515625e-05f + (fabsf(l) + fabsf(h)) * 9.5367431640625e-07f + 1e-30f;   // 2^-14, 2^-20
 }
 
-__global__ __launch_bounds__(BT) void k_layout(const float* __restrict__ boxes, const int32_t* __restrict__ links,
-                                               const float* __restrict__ tris, const uint32_t* __restrict__ order,
-                                               uint32_t n, int32_t root, Node* __restrict__ nodes, float4* __restrict__ tsorted) {
+__global__ __launch_bounds__(BT) void k_layout(const float* __restrict__ boxes, const int32_t* __restrict__ links, uint32_t n,
+                                               int32_t root, Node* __restrict__ nodes) {
     const uint32_t i = blockIdx.x * BT + threadIdx.x;
-    if (i < n) {
-        const uint32_t t = order[i];
-        const float* v = tris + 9 * size_t(t);
-        float4* o = tsorted + 3 * size_t(i);
-        o[0] = make_float4(v[0], v[1], v[2], v[3]);
-        o[1] = make_float4(v[4], v[5], v[6], v[7]);
-        o[2] = make_float4(v[8], __int_as_float(int32_t(t)), 0.0f, 0.0f);
-    }
     const uint32_t inner = n > 1 ? n - 1 : 1;
-    if (!nodes || i >= inner) return;
+    if (i >= inner) return;
     int32_t ch[2];
     if (n == 1) { ch[0] = 0; ch[1] = -1; }   // a single leaf under a one-record root
     else { ch[0] = links[3 * size_t(n + i)]; ch[1] = links[3 * size_t(n + i) + 1]; }
@@ -313,14 +305,13 @@ int bvh_layout(Ctx& c, hipStream_t s) {
     const uint32_t n = c.ntri;
     if (n == 0) return 0;
     const size_t inner = n > 1 ? n - 1 : 1;
-    // the binary records only for the binary traversal; the wide one reads the leaf-order triangles
+    // the binary records only for the binary traversal; the leaf-order triangles, which the wide
+    // one reads too, were written with the PLOC leaves
     const bool binary = c.trace_kernel == RAW_AO_TRACE_BINARY;
-    if ((binary && !ensure(c, c.nodes, sizeof(Node) * inner)) || !ensure(c, c.tris_sorted, 48 * size_t(n))) return -2;
-    k_layout<<<cdiv(binary ? std::max<size_t>(n, inner) : n, BT), BT, 0, s>>>(c.boxes.as<float>(), c.links.as<int32_t>(),
-                                                                              c.tris_in.as<float>(), c.vals.as<uint32_t>(), n, c.root,
-                                                                              binary ? c.nodes.as<Node>() : nullptr,
-                                                                              c.tris_sorted.as<float4>());
     c.trav_root = n > 1 ? uint32_t(c.root) - n : 0u;
+    if (!binary) return 0;
+    if (!ensure(c, c.nodes, sizeof(Node) * inner)) return -2;
+    k_layout<<<cdiv(inner, BT), BT, 0, s>>>(c.boxes.as<float>(), c.links.as<int32_t>(), n, c.root, c.nodes.as<Node>());
     return hip_ok(c, hipGetLastError(), "layout launch") ? 0 : -2;
 }
 
