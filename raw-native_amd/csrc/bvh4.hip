@@ -153,7 +153,7 @@ __global__ __launch_bounds__(BT) void k_wide_write(const WideState* __restrict__
                                                    const uint32_t* __restrict__ frontier, uint32_t* __restrict__ next,
                                                    const uint32_t* __restrict__ blk, const float* __restrict__ boxes,
                                                    const int32_t* __restrict__ links, uint32_t n, Node4* __restrict__ nodes,
-                                                   uint32_t defer_max) {
+                                                   uint32_t defer_max, uint32_t* __restrict__ hist) {
     __shared__ uint32_t ws[BT / 64], red[BT / 64];
     const WideState s = *st;
     if (s.count <= defer_max) {   // empty, or left to k_wide_small: the state and frontier move on unchanged
@@ -194,6 +194,7 @@ __global__ __launch_bounds__(BT) void k_wide_write(const WideState* __restrict__
             o.next_base = o.base + o.count;
             o.pad = s.pad + 1u;   // levels that had a frontier
             *st_next = o;
+            if (s.pad < uint32_t(kWideHist)) hist[s.pad] = s.count;   // this grid level's frontier, for the next build
         }
         if (p >= s.count) continue;
         uint32_t off = pre + x - c;
@@ -1912,6 +1913,7 @@ void wide_init(Ctx& c, uint32_t root, hipStream_t s) {
     k_wide_init<<<1, 1, 0, s>>>(c.wstate.as<WideState>(), c.wfront_a.as<uint32_t>(), root);
     c.wide_parity = 0;
     c.wide_in_a = true;
+    c.wide_level = 0;
 }
 
 uint32_t wide_grid(uint32_t n) { return std::min<uint32_t>(cdiv(n, BT), 2048u); }
@@ -1931,16 +1933,22 @@ uint32_t predicted_levels(const Ctx& c, uint32_t n) {
 // c.wide_in_a names). The state's pad counts the levels these grid kernels ran. With defer, a
 // frontier of at most SMALL_W entries is left to k_wide_small (enqueue_small).
 void enqueue_levels(Ctx& c, uint32_t levels, hipStream_t s, bool defer = true) {
-    const uint32_t n = c.ntri, G = wide_grid(n), defer_max = defer ? uint32_t(SMALL_W) : 0u;
+    const uint32_t n = c.ntri, defer_max = defer ? uint32_t(SMALL_W) : 0u;
     WideState* d_st = c.wstate.as<WideState>();
     for (uint32_t l = 0; l < levels; ++l) {
+        // a block a tile of the frontier the last build of this tree had at this grid level (the
+        // kernels stride over the tiles: any grid gives the same records), else the full grid
+        const uint32_t lv = c.wide_level++;
+        uint32_t G = wide_grid(n);
+        if (c.last_rounds_n == n && lv < c.level_m.size()) G = std::max(1u, std::min(G, cdiv(c.level_m[lv], BT)));
         uint32_t* fin = c.wide_in_a ? c.wfront_a.as<uint32_t>() : c.wfront_b.as<uint32_t>();
         uint32_t* fout = c.wide_in_a ? c.wfront_b.as<uint32_t>() : c.wfront_a.as<uint32_t>();
         const int parity = c.wide_parity;
         k_wide_count<<<G, BT, 0, s>>>(d_st + parity, fin, c.boxes.as<float>(), c.links.as<int32_t>(), n, c.wblk.as<uint32_t>(),
                                       defer_max);
         k_wide_write<<<G, BT, 0, s>>>(d_st + parity, d_st + (parity ^ 1), fin, fout, c.wblk.as<uint32_t>(), c.boxes.as<float>(),
-                                      c.links.as<int32_t>(), n, c.nodes4.as<Node4>(), defer_max);
+                                      c.links.as<int32_t>(), n, c.nodes4.as<Node4>(), defer_max,
+                                      c.ploc_hist.as<uint32_t>() + kPlocHist);
         c.wide_parity ^= 1;
         c.wide_in_a = !c.wide_in_a;
     }
@@ -2025,7 +2033,11 @@ int build_finish(Ctx& c, hipStream_t s) {
         return hip_ok(c, hipMemcpyAsync(h_wide, c.wstate.as<WideState>() + c.wide_parity, sizeof(WideState), hipMemcpyDeviceToHost, s),
                       "wide state");
     };
+    // and the grid rounds' cluster counts (the next build of these triangles sizes its rounds' grids by them)
+    auto* h_hist = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(c.h_state) + 1024);
+    const size_t nhist = std::min<size_t>(c.ploc_hist.cap / 4, kPlocHist + kWideHist);
     if (!hip_ok(c, hipMemcpyAsync(h_ploc, c.state.as<PlocState>() + 2, sizeof(PlocState), hipMemcpyDeviceToHost, s), "state read") ||
+        !hip_ok(c, hipMemcpyAsync(h_hist, c.ploc_hist.p, 4 * nhist, hipMemcpyDeviceToHost, s), "round counts") ||
         (wide && !read_wide()) || !hip_ok(c, hipStreamSynchronize(s), "build"))
         return -2;
     bool redo = false;
@@ -2044,6 +2056,12 @@ int build_finish(Ctx& c, hipStream_t s) {
     c.rounds = f.rounds;
     c.last_rounds_n = n;
     c.last_grid_rounds = f.grid_rounds;
+    if (!redo) {   // round r starts with the clusters round r - 1 left (round 0 with n)
+        c.round_m.assign(1, n);
+        for (uint32_t r = 1; r < f.grid_rounds && r <= uint32_t(kPlocHist); ++r) c.round_m.push_back(h_hist[r - 1]);
+    } else {
+        c.round_m.clear();
+    }
     if (redo) {
         int r = bvh_layout(c, s);
         if (!r && wide) r = wide_build(c, s);
@@ -2051,9 +2069,15 @@ int build_finish(Ctx& c, hipStream_t s) {
         if (wide && (!read_wide() || !hip_ok(c, hipStreamSynchronize(s), "build"))) return -2;
     }
     if (!wide) return 0;
-    if (h_wide->count != 0) return wide_continue(c, s);   // the enqueued levels fell short
+    if (h_wide->count != 0) {   // the enqueued levels fell short
+        c.level_m.clear();
+        return wide_continue(c, s);
+    }
     c.wide_nodes = h_wide->base;
     c.last_wide_levels = std::max(1u, h_wide->pad);
+    c.level_m.clear();   // grid level l starts with the frontier it recorded
+    if (!redo && nhist >= size_t(kPlocHist + kWideHist))
+        for (uint32_t l = 0; l < h_wide->pad && l < uint32_t(kWideHist); ++l) c.level_m.push_back(h_hist[kPlocHist + l]);
     return 0;
 }
 

@@ -104,7 +104,7 @@ void raw_ao_close(raw_ao_ctx* x) {
                                   &c.clus_a, &c.clus_b, &c.nn, &c.flags, &c.blk, &c.state, &c.g4, &c.bounds_part,
                                   &c.nodes, &c.tris_sorted, &c.nodes4, &c.wfront_a, &c.wfront_b, &c.wblk, &c.wstate, &c.vis_open, &c.st_pos, &c.st_nrm, &c.st_mask, &c.st_out, &c.st_out2,
                                   &c.st_cnt, &c.st_rays, &c.st_max, &c.counters, &c.r_keys, &c.r_tris, &c.r_counts,
-                                  &c.r_offs, &c.r_scan_tmp, &c.fr_mesh, &c.fr_scalars, &c.fr_planes};
+                                  &c.r_offs, &c.r_scan_tmp, &c.fr_mesh, &c.fr_scalars, &c.fr_planes, &c.ploc_hist};
         for (auto* b : bufs)
             if (b->p) (void)hipFree(b->p);
         if (c.d_trig) (void)hipFree(c.d_trig);

@@ -31,6 +31,9 @@ struct PlocState {
 
 struct UNet;
 
+constexpr int kPlocHist = 240;   // grid rounds whose cluster counts a build records (Ctx::ploc_hist)
+constexpr int kWideHist = 64;    // and collapse grid levels whose frontier sizes it records (after them)
+
 struct TraceCounters {
     unsigned long long visits, tests, rays, overflows;
 };
@@ -70,7 +73,8 @@ struct Ctx {
     DevBuf boxes, links;         // (2n-1) x 6 floats, (2n-1) x 3 ints: the reference's node order
     DevBuf clus_a, clus_b, nn, flags, blk;
     DevBuf state;                // PlocState[2]
-    PlocState* h_state{nullptr}; // pinned read-back of the build's final states (PLOC, then the collapse's)
+    PlocState* h_state{nullptr}; // pinned read-back of the build's final states (PLOC at 0, the collapse's at 128)
+                                 // and the grid rounds' cluster counts (at 1024)
     DevBuf g4;                   // centroid box lo xyz and scale
     DevBuf bounds_part;
     // traversal forms
@@ -101,6 +105,13 @@ struct Ctx {
     // grids of the last eager build, replayed by the capturable build (ploc_build_fixed,
     // wide_build_fixed)
     std::vector<uint32_t> plan_ploc_nb, plan_wide_nb;
+    DevBuf ploc_hist;                    // clusters left after each grid round of the build in flight, then
+                                         // the frontier size of each collapse grid level
+    std::vector<uint32_t> round_m;       // clusters at the start of each grid round of the last finished
+                                         // build of last_rounds_n triangles (sizes those rounds' grids)
+    std::vector<uint32_t> level_m;       // and the frontier at the start of each collapse grid level
+    uint32_t ploc_round{0};              // grid rounds enqueued so far in the build in flight
+    uint32_t wide_level{0};              // collapse grid levels enqueued so far
     uint32_t plan_n{0};
     int plan_radius{0};
     // the grid rounds and wide-collapse levels of the last finished build, and of how many

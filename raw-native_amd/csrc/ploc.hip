@@ -387,7 +387,8 @@ __global__ __launch_bounds__(BT) void k_ploc_merge(const PlocState* __restrict__
                                                    const uint32_t* __restrict__ nn, const uint8_t* __restrict__ flags,
                                                    const uint32_t* __restrict__ counts, uint32_t maxt,
                                                    const uint32_t* __restrict__ cin, uint32_t* __restrict__ cout,
-                                                   float* __restrict__ boxes, int32_t* __restrict__ links, uint32_t tail_at) {
+                                                   float* __restrict__ boxes, int32_t* __restrict__ links, uint32_t tail_at,
+                                                   uint32_t* __restrict__ hist) {
     __shared__ uint32_t wsum[BT / 64][2];
     __shared__ uint32_t red[BT / 64][2];
     const PlocState in = *st;
@@ -418,6 +419,7 @@ __global__ __launch_bounds__(BT) void k_ploc_merge(const PlocState* __restrict__
             o.rounds = in.rounds + (in.m > 1 ? 1u : 0u);
             o.grid_rounds = in.grid_rounds + 1;
             *st_next = o;
+            if (in.grid_rounds < uint32_t(kPlocHist)) hist[in.grid_rounds] = o.m;
         }
         if (survive) {
             uint32_t S = pm + __popcll(bm & lt), SV = ps + __popcll(bs & lt);
@@ -597,6 +599,15 @@ namespace {
 
 // The grid of the round kernels (they stride over the tiles of the current clusters).
 uint32_t round_grid(uint32_t n) { return std::min<uint32_t>(cdiv(n, BT), 2048u); }
+// Round r's grid: a block a tile of the clusters the last finished build of the same n had at
+// the start of round r (PLOC is deterministic: the same scene gives the same counts), else the
+// full grid. The kernels stride over the tiles, so any grid gives the same tree; one block a tile
+// dispatches no idle blocks, which matters when the build overlaps another frame's trace.
+uint32_t round_grid(const Ctx& c, uint32_t n, uint32_t r) {
+    const uint32_t G = round_grid(n);
+    if (c.last_rounds_n != n || r >= c.round_m.size()) return G;
+    return std::max(1u, std::min(G, cdiv(c.round_m[r], BT)));
+}
 uint32_t tail_threshold(const Ctx& c) { return uint32_t(std::max(1, std::min(c.ploc_tail, TAIL_M))); }
 
 // Grid rounds to enqueue for n triangles: as many as the last finished build of n triangles ran,
@@ -614,9 +625,10 @@ uint32_t predicted_rounds(const Ctx& c, uint32_t n) {
 // Enqueues `rounds` grid rounds from the state at c.ploc_parity (clusters in the buffer
 // c.ploc_in_a names), then the tail into state 2.
 void enqueue_rounds(Ctx& c, uint32_t rounds, int radius, hipStream_t s) {
-    const uint32_t n = c.ntri, maxt = cdiv(n, BT), G = round_grid(n), tail_at = tail_threshold(c);
+    const uint32_t n = c.ntri, maxt = cdiv(n, BT), tail_at = tail_threshold(c);
     PlocState* d_st = c.state.as<PlocState>();
     for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t G = round_grid(c, n, c.ploc_round++);
         uint32_t* cin = c.ploc_in_a ? c.clus_a.as<uint32_t>() : c.clus_b.as<uint32_t>();
         uint32_t* cout = c.ploc_in_a ? c.clus_b.as<uint32_t>() : c.clus_a.as<uint32_t>();
         const int parity = c.ploc_parity;
@@ -627,7 +639,8 @@ void enqueue_rounds(Ctx& c, uint32_t rounds, int radius, hipStream_t s) {
             k_ploc_nn<0><<<G, BT, 0, s>>>(d_st + parity, radius, c.boxes.as<float>(), cin, c.nn.as<uint32_t>(),
                                           c.flags.as<uint8_t>(), c.blk.as<uint32_t>(), maxt, tail_at);
         k_ploc_merge<<<G, BT, 0, s>>>(d_st + parity, d_st + (parity ^ 1), c.nn.as<uint32_t>(), c.flags.as<uint8_t>(),
-                                      c.blk.as<uint32_t>(), maxt, cin, cout, c.boxes.as<float>(), c.links.as<int32_t>(), tail_at);
+                                      c.blk.as<uint32_t>(), maxt, cin, cout, c.boxes.as<float>(), c.links.as<int32_t>(), tail_at,
+                                      c.ploc_hist.as<uint32_t>());
         c.ploc_parity ^= 1;
         c.ploc_in_a = !c.ploc_in_a;
     }
@@ -657,6 +670,7 @@ void enqueue_front(Ctx& c, const float* d_tris, uint32_t n, hipStream_t s) {
     k_ploc_init<<<1, 1, 0, s>>>(c.state.as<PlocState>(), n);
     c.ploc_parity = 0;
     c.ploc_in_a = true;
+    c.ploc_round = 0;
 }
 
 }  // namespace
@@ -677,9 +691,10 @@ int ploc_build(Ctx& c, const float* d_tris, uint32_t n, int radius, hipStream_t 
         !ensure(c, c.boxes, 24 * nn) || !ensure(c, c.links, 12 * nn) || !ensure(c, c.clus_a, 4 * size_t(n)) ||
         !ensure(c, c.clus_b, 4 * size_t(n)) || !ensure(c, c.nn, 4 * size_t(n)) || !ensure(c, c.flags, size_t(n)) ||
         !ensure(c, c.blk, 8 * size_t(nblk)) || !ensure(c, c.state, 3 * sizeof(PlocState)) || !ensure(c, c.g4, 16) ||
-        !ensure(c, c.bounds_part, 6 * 4 * 1024) || !ensure(c, c.rs_tot, 4 * RS_DIG))
+        !ensure(c, c.bounds_part, 6 * 4 * 1024) || !ensure(c, c.rs_tot, 4 * RS_DIG) || !ensure(c, c.ploc_hist, 4 * (kPlocHist + kWideHist)))
         return -2;
-    if (!c.h_state && !hip_ok(c, hipHostMalloc(reinterpret_cast<void**>(&c.h_state), 256), "hipHostMalloc")) return -2;
+    if (!c.h_state && !hip_ok(c, hipHostMalloc(reinterpret_cast<void**>(&c.h_state), 1024 + 4 * (kPlocHist + kWideHist)), "hipHostMalloc"))
+        return -2;
     // Everything from the bounds to the root is enqueued at once: the round kernels read the
     // cluster count on the device, the tail finishes below its threshold, and build_finish reads
     // the final state back once, at the end of the whole build.
