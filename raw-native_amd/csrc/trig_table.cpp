@@ -8,7 +8,8 @@
 // with 0 none, 1 +1 ulp of magnitude, 3 -1 ulp. A difference beyond one ulp is an
 // error (the table cannot express it), never a silent mismatch. Layout: a bitmap of the
 // indices with a nonzero nibble (2^21 bytes, which stays in L2), then the nibbles, so the
-// kernels touch the 8 MB nibble table only for the ~1.3% of angles that need a step.
+// kernels touch the 8 MB nibble table only for the angles that need a step (2.6% with
+// glibc 2.35).
 #include "ao_math.hpp"
 #include "ctx.hpp"
 #include <algorithm>
