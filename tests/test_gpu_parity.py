@@ -355,6 +355,40 @@ def test_enqueued_build_rounds_do_not_change_the_tree(ctx, rounds):
     assert (got_tree[2], got_tree[3]) == (oroot, orounds) and np.array_equal(got_tree[1], ol)
 
 
+def test_recorded_grids_with_another_scene_of_as_many_triangles(ctx):
+    """A build sizes each PLOC round's and collapse level's grid by the counts the last build of
+    as many triangles recorded. Another scene with the same triangle count (its rounds keep other
+    cluster counts, so some grids are too small, some too large) still gets the oracle's tree,
+    since the kernels stride over their tiles, and the same 4-wide tree as a fresh context's: AO
+    equal bit for bit on points of its own triangles."""
+    import raw_ao
+    n = 200000
+    scenes = [O.scene_tris("soup", n, 3), O.scene_tris("soup", n, 4)]
+    r = np.random.default_rng(5)
+    for k in (0, 1, 0, 1):
+        print("recorded grids, soup seed", 3 + k, flush=True)
+        tris = scenes[k]
+        ctx.build_ploc(tris, 16)
+        got = ctx.read_tree()
+        ob, ol, oroot, orounds = O.ploc(tris, 16)
+        assert (got[2], got[3]) == (oroot, orounds) and np.array_equal(got[1], ol)
+        # a 64 x 64 G-buffer of points on the scene's own triangles, their normals
+        pick = r.choice(n, 64 * 64, replace=False)
+        v = tris[pick].reshape(-1, 3, 3).astype(np.float64)
+        pos = v.mean(axis=1).astype(np.float32)
+        nrm = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+        nrm = (nrm / np.maximum(np.linalg.norm(nrm, axis=1, keepdims=True), 1e-30)).astype(np.float32)
+        mask = np.ones(64 * 64, np.uint8)
+        ao = ctx.compute_rtao(pos.reshape(64, 64, 3), nrm.reshape(64, 64, 3), mask.reshape(64, 64), 64, 0.05)
+        fresh = raw_ao.Context(0)
+        try:
+            fresh.build_ploc(tris, 16)
+            want = fresh.compute_rtao(pos.reshape(64, 64, 3), nrm.reshape(64, 64, 3), mask.reshape(64, 64), 64, 0.05)
+        finally:
+            fresh.close()
+        assert np.array_equal(bits(ao), bits(want))
+
+
 def test_ploc_tail_threshold_does_not_change_the_tree(ctx):
     """The single-workgroup tail and the grid-wide rounds run the same rounds: the tree is the
     same whichever takes over where."""
