@@ -148,7 +148,7 @@ BatchDev dev_of(Batch& b) {
     return BatchDev{b.d_views.as<BatchView>(), b.d_scenes.as<BatchScene>(), uint32_t(b.views.size()), b.total_pix,
                     b.cov.as<uint32_t>(), b.cnt.as<uint32_t>(), b.ncov.as<uint32_t>(), b.gbase.as<uint32_t>(),
                     b.head.as<uint32_t>(), c.host_trig ? c.d_trig : nullptr, b.tune.tri_q, b.tune.chunk,
-                    float(c.wave_cut)};
+                    float(c.wave_cut), uint32_t(std::max(0, c.refill))};
 }
 
 // Everything of one batched frame on stream s, with no host copies, waits or memset nodes.

@@ -42,6 +42,7 @@ struct BatchDev {
     uint32_t tri_q;         // triangle phase threshold (of 64)
     uint32_t chunk;         // groups a wave takes from the queue at a time
     float cut_k;            // > 0: a group's rays start from the wave's cut at cut_k x radius
+    uint32_t refill;        // > 0 (with cut_k > 0): the refill form, a wave's idle lanes refilled at this many
 };
 
 // The tunables of a batched launch.

@@ -1886,6 +1886,250 @@ __global__ __launch_bounds__(BT) void k_batch_trace(BatchDev d) {
         }
     }
 }
+// The refill form of the batched trace (the queue's unit a range of whole pixels of one view).
+// A range: at most BR_PIX consecutive covered pixels and at most 256 rays (spp < 256: 256 / spp
+// pixels, at most BR_PIX; from 256 spp one pixel). The wave walks the cut of the range's union
+// reach box, traces the range's rays one a lane from the cut and refills idle lanes with the
+// range's next rays (k_trace_ao4rc's loop, with the view's spp, radius and scene at run time); a
+// pixel's open rays are counted in LDS and stored once (every ray of a pixel is in its range).
+constexpr uint32_t BR_PIX = 16;
+__host__ __device__ inline uint32_t range_pixels(uint32_t spp) {
+    return spp >= 256u ? 1u : (256u / spp < BR_PIX ? 256u / spp : BR_PIX);
+}
+__global__ void k_batch_plan_rc(BatchDev d) {
+    if (threadIdx.x != 0) return;
+    uint64_t g = 0;
+    for (uint32_t v = 0; v < d.nviews; ++v) {
+        d.gbase[v] = uint32_t(g);
+        const uint32_t p = range_pixels(d.views[v].spp);
+        g += (uint64_t(d.ncov[v]) + p - 1u) / p;
+    }
+    d.gbase[d.nviews] = uint32_t(g);
+}
+__global__ __launch_bounds__(BT) void k_batch_trace_rc(BatchDev d) {
+    __shared__ uint32_t stk[STACK_LDS * BT];
+    __shared__ uint32_t cutmem[CUT_WAVE * (BT / 64)];
+    __shared__ float pixmem[PIX_WORDS * BR_PIX * (BT / 64)];
+    __shared__ uint32_t cntmem[BR_PIX * (BT / 64)];
+    const uint32_t lane = threadIdx.x & 63u, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t* const lds_col = stk + threadIdx.x;
+    float* const pm = pixmem + wave * (PIX_WORDS * BR_PIX);
+    uint32_t* const cl = cntmem + wave * BR_PIX;
+    CutView cv;
+    cv.id = cutmem + wave * CUT_WAVE;
+    cv.box = reinterpret_cast<float*>(cv.id + CUT_N);
+    cv.leaf = cv.id + 7 * CUT_N;
+    cv.lbox = reinterpret_cast<float*>(cv.id + 7 * CUT_N + CUT_LEAF);
+    const uint32_t nranges = d.gbase[d.nviews];
+    for (;;) {
+        uint32_t g0 = 0;
+        if (lane == 0) g0 = atomicAdd(d.head, d.chunk);
+        g0 = __builtin_amdgcn_readfirstlane(__shfl(g0, 0));
+        if (g0 >= nranges) break;
+        const uint32_t g1 = min(nranges, g0 + d.chunk);
+        for (uint32_t g = g0; g < g1; ++g) {
+            uint32_t vlo = 0, vhi = d.nviews;   // the view: the last v with gbase[v] <= g
+            while (vhi - vlo > 1) {
+                const uint32_t mid = (vlo + vhi) >> 1;
+                if (d.gbase[mid] <= g) vlo = mid;
+                else vhi = mid;
+            }
+            vlo = __builtin_amdgcn_readfirstlane(vlo);
+            const BatchView& bv = d.views[vlo];
+            const BatchScene& sc = d.scenes[bv.scene];
+            const uint32_t spp = bv.spp, P = range_pixels(spp);
+            const uint32_t ci0 = (g - d.gbase[vlo]) * P, np = min(P, d.ncov[vlo] - ci0);
+            const uint32_t total = np * spp;
+            const float radius = bv.radius;
+            // lane k < np: pixel k's set-up (as k_trace_ao4rc's) and its reach box
+            wave_sync();   // the previous range's LDS reads are done
+            V3 lo = v3(kFar, kFar, kFar), hi = v3(-kFar, -kFar, -kFar);
+            uint32_t mypix = 0;
+            if (lane < np) {
+                mypix = d.cov[bv.pix_base + ci0 + lane];
+                const V3 p = v3(bv.pos[3 * size_t(mypix)], bv.pos[3 * size_t(mypix) + 1], bv.pos[3 * size_t(mypix) + 2]);
+                const V3 n = v3(bv.nrm[3 * size_t(mypix)], bv.nrm[3 * size_t(mypix) + 1], bv.nrm[3 * size_t(mypix) + 2]);
+                const V3 o = add(p, scl(n, 1e-3f));
+                V3 t, b;
+                basis(n, t, b);
+                const uint32_t y = mypix / bv.w, x = mypix - y * bv.w;
+                float* q = pm + lane * PIX_WORDS;
+                q[0] = o.x; q[1] = o.y; q[2] = o.z;
+                q[3] = n.x; q[4] = n.y; q[5] = n.z;
+                q[6] = t.x; q[7] = t.y; q[8] = t.z;
+                q[9] = b.x; q[10] = b.y; q[11] = b.z;
+                q[12] = __uint_as_float(x);
+                q[13] = __uint_as_float(y);
+                const float big = fmaxf(fabsf(o.x), fmaxf(fabsf(o.y), fabsf(o.z)));
+                const float half = radius * 1.001f + (big + radius) * 1e-5f;
+                lo = v3(o.x - half, o.y - half, o.z - half);
+                hi = v3(o.x + half, o.y + half, o.z + half);
+                cl[lane] = sc.ntri ? 0u : spp;   // nothing to hit: every ray is open
+            }
+            lo = v3(wave_min(lo.x), wave_min(lo.y), wave_min(lo.z));
+            hi = v3(wave_max(hi.x), wave_max(hi.y), wave_max(hi.z));
+            wave_sync();
+            if (sc.ntri) {
+                // the angle steps, when the range's rays fit one a lane four times: nibble k of
+                // lane L = ray L + 64 k
+                const bool pre = d.trig && total <= 256u;
+                uint32_t fixes = 0;
+                if (pre) {
+                    uint32_t words[4], idx[4];
+                    for (int k = 0; k < 4; ++k) {
+                        const uint32_t ri = lane + 64u * uint32_t(k);
+                        idx[k] = 0;
+                        words[k] = 0;
+                        if (ri < total) {
+                            const uint32_t pk = ri / spp;
+                            const uint32_t x = __float_as_uint(pm[pk * PIX_WORDS + 12]), y = __float_as_uint(pm[pk * PIX_WORDS + 13]);
+                            idx[k] = hash24(x, y, 2 * (ri - pk * spp) + 1);
+                            words[k] = reinterpret_cast<const uint32_t*>(d.trig)[idx[k] >> 5];
+                        }
+                    }
+                    for (int k = 0; k < 4; ++k)
+                        if ((words[k] >> (idx[k] & 31u)) & 1u)
+                            fixes |= ((uint32_t(d.trig[kTrigBitmapBytes + (idx[k] >> 1)]) >> ((idx[k] & 1u) * 4)) & 15u) << (4 * k);
+                }
+                cv.n = -1;
+                cv.nleaf = 0;
+                const float span = 4.0f * radius;
+                if (d.cut_k > 0.0f && hi.x - lo.x <= span && hi.y - lo.y <= span && hi.z - lo.z <= span) {
+                    uint32_t steps = 0;
+                    wave_cut(static_cast<const Node4*>(sc.nodes), static_cast<const TopEntry*>(sc.top), *sc.ntop, lo, hi,
+                             d.cut_k * radius, stk + (threadIdx.x & ~63u), cv, steps);
+                }
+                const char* const base = reinterpret_cast<const char*>(sc.nodes);
+                const float4* const tris = sc.tris;
+                const uint32_t ntri = sc.ntri;
+                const float reach = radius * kReach;
+                const int ncut = cv.n;
+                Ray r;
+                uint32_t ox = 0, oy = 0, oz = 0;
+                uint32_t spill[STACK_SPILL];
+                int sp = 0, pk = 0, ci = 0;
+                int32_t node = -1;
+                uint32_t pend = 0, lp = 0, work = 0;
+                int4 leafref = make_int4(0, 0, 0, 0);
+                const auto next_cut = [&]() -> int32_t {
+                    while (ci < ncut) {
+                        const int k = ci++;
+                        if (CutView::enters(r, cv.box, CUT_N, k, reach, kExit)) return int32_t(cv.id[k]);
+                    }
+                    return -1;
+                };
+                const auto setup = [&](uint32_t ri, uint32_t nib) {
+                    pk = int(ri / spp);
+                    const uint32_t s = ri - uint32_t(pk) * spp;
+                    const float* q = pm + pk * PIX_WORDS;
+                    const V3 origin = v3(q[0], q[1], q[2]), n = v3(q[3], q[4], q[5]), t = v3(q[6], q[7], q[8]),
+                             b = v3(q[9], q[10], q[11]);
+                    const uint32_t x = __float_as_uint(q[12]), y = __float_as_uint(q[13]);
+                    const uint32_t k1 = hash24(x, y, 2 * s), k2 = hash24(x, y, 2 * s + 1);
+                    const float u1 = float(k1) / float(0x1000000), u2 = float(k2) / float(0x1000000);
+                    float sphi, cphi;
+                    sincos_base(6.2831853f * u2, sphi, cphi);
+                    if (pre) trig_adjust(nib, sphi, cphi);
+                    else if (d.trig) trig_fix(d.trig, k2, sphi, cphi);
+                    r = make_ray(origin, ao_dir(u1, sphi, cphi, n, t, b));
+                    ox = r.inv.x >= 0.0f ? 0u : 48u;
+                    oy = r.inv.y >= 0.0f ? 0u : 48u;
+                    oz = r.inv.z >= 0.0f ? 0u : 48u;
+                    sp = 0;
+                    pend = 0;
+                    work = 0;
+                    ci = 0;
+                    lp = 0;
+                    for (int i = 0; i < cv.nleaf; ++i)   // nleaf is 0 when the cut overflowed
+                        if (cv.leaf[i] < ntri && CutView::enters(r, cv.lbox, CUT_LEAF, i, reach, kExit)) lp |= 1u << i;
+                    node = ncut < 0 ? 0 : next_cut();
+                };
+                bool live = lane < total;
+                if (live) setup(lane, fixes & 15u);
+                uint32_t next = 64;
+                while (true) {
+                    if (live && pend == 0 && node >= 0) {
+                        const uint32_t off = uint32_t(node) * uint32_t(sizeof(Node4));
+                        const uint32_t hit = slab4s(r, base, off, ox, oy, oz, reach, kExit);
+                        const int4 ref = *reinterpret_cast<const int4*>(base + off + 96);
+                        const uint32_t masks = *reinterpret_cast<const uint32_t*>(base + off + 112);
+                        pend = hit & masks & 15u;
+                        leafref = ref;
+                        bool overflow = false;
+                        uint32_t ih = hit & (masks >> 4);
+                        if (ih) {
+                            node = ref_of(ref, __builtin_ctz(ih));
+                            ih &= ih - 1u;
+                            while (ih) {
+                                const uint32_t v = uint32_t(ref_of(ref, __builtin_ctz(ih)));
+                                ih &= ih - 1u;
+                                if (sp < STACK_LDS) lds_col[sp * BT] = v;
+                                else if (sp < STACK_LDS + STACK_SPILL) spill[sp - STACK_LDS] = v;
+                                else overflow = true;
+                                ++sp;
+                            }
+                        } else if (sp > 0) {
+                            --sp;
+                            node = int32_t(sp < STACK_LDS ? lds_col[sp * BT] : spill[sp - STACK_LDS]);
+                        } else {
+                            node = next_cut();
+                        }
+                        if (++work > ntri + 4) overflow = true;
+                        if (overflow) {   // answered by the scan of every triangle
+                            if (!scan_occluded(r, radius, tris, ntri)) atomicAdd(&cl[pk], 1u);
+                            live = false;
+                            pend = 0;
+                            lp = 0;
+                        }
+                    }
+                    const bool waiting = live && (pend != 0 || lp != 0);
+                    const bool can_step = live && pend == 0 && node >= 0;
+                    const unsigned long long wb = __ballot(waiting), sb = __ballot(can_step), lb = __ballot(live);
+                    if (lb != 0 && (sb == 0 || 64u * uint32_t(__popcll(wb)) >= d.tri_q * uint32_t(__popcll(lb)))) {
+                        if (waiting) {
+                            uint32_t slot;
+                            if (pend) {
+                                const int k = __builtin_ctz(pend);
+                                pend &= pend - 1u;
+                                slot = uint32_t(~ref_of(leafref, k));
+                            } else {
+                                slot = cv.leaf[__builtin_ctz(lp)];
+                                lp &= lp - 1u;
+                            }
+                            float t;
+                            if (slot < ntri && tri_hit(r, tris, slot, t) && t < radius) {
+                                live = false;   // occluded
+                                pend = 0;
+                                lp = 0;
+                            }
+                        }
+                    }
+                    if (live && pend == 0 && lp == 0 && node < 0) {   // finished: open
+                        atomicAdd(&cl[pk], 1u);
+                        live = false;
+                    }
+                    const unsigned long long idle = __ballot(!live);
+                    const uint32_t nidle = uint32_t(__popcll(idle));
+                    if (next < total && (nidle >= d.refill || nidle == 64u)) {
+                        const uint32_t ri = next + uint32_t(__popcll(idle & below));
+                        // every lane takes part in the shuffle (its source lane may be busy tracing)
+                        const uint32_t nib = (uint32_t(__shfl(int(fixes), int(ri & 63u))) >> (4u * ((ri >> 6) & 3u))) & 15u;
+                        if (!live && ri < total) {
+                            setup(ri, nib);
+                            live = true;
+                        }
+                        next += nidle;
+                    }
+                    if (__ballot(live) == 0 && next >= total) break;
+                }
+            }
+            wave_sync();
+            if (lane < np) d.cnt[bv.pix_base + mypix] = cl[lane];
+        }
+    }
+}
+
 // AO = open / spp where covered, 1 elsewhere (k_ao_finish's formula).
 __global__ __launch_bounds__(BT) void k_batch_finish(BatchDev d) {
     const BatchView& bv = d.views[blockIdx.y];
@@ -2187,9 +2431,14 @@ int closest_hits4(Ctx& c, const float* d_rays, uint32_t n, float* d_hits, hipStr
 int batch_trace_enqueue(const BatchDev& d, uint32_t persistent_blocks, hipStream_t s) {
     k_batch_clear<<<1024, BT, 0, s>>>(d);
     k_batch_compact<<<dim3(256, d.nviews, 1), BT, 0, s>>>(d);
-    k_batch_plan<<<1, 64, 0, s>>>(d);
-    if (d.cut_k > 0.0f) k_batch_trace<true><<<persistent_blocks, BT, 0, s>>>(d);
-    else k_batch_trace<false><<<persistent_blocks, BT, 0, s>>>(d);
+    if (d.refill > 0 && d.cut_k > 0.0f) {
+        k_batch_plan_rc<<<1, 64, 0, s>>>(d);
+        k_batch_trace_rc<<<persistent_blocks, BT, 0, s>>>(d);
+    } else {
+        k_batch_plan<<<1, 64, 0, s>>>(d);
+        if (d.cut_k > 0.0f) k_batch_trace<true><<<persistent_blocks, BT, 0, s>>>(d);
+        else k_batch_trace<false><<<persistent_blocks, BT, 0, s>>>(d);
+    }
     k_batch_finish<<<dim3(256, d.nviews, 1), BT, 0, s>>>(d);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
