@@ -111,7 +111,18 @@ enum {
      * calls after consecutive fall-backs, then try the network again; a frame whose probe passes
      * clears it. For serving loops where the network misses the bound for a while: the frames
      * that skip the network cost what plain ray tracing costs. */
-    RAW_AO_OPT_LEARNED_BACKOFF = 12
+    RAW_AO_OPT_LEARNED_BACKOFF = 12,
+    /* 0, 1 or 2 (default 1): which kernels read the wide tree as 64 B quantized records (per
+     * axis a base and a power-of-two step, one byte a slot plane, every box rounded outward)
+     * instead of the 128 B ones: 0 none; 1 the visibility pass's sky rays (64 spp, the split form
+     * of RAW_AO_OPT_SKY_REFILL), the records made by the first such pass after each build; 2 also
+     * the 64-spp AO rays of the refill kernel (RAW_AO_OPT_REFILL and RAW_AO_OPT_WAVE_CUT above
+     * 0), the records made after each wide build, or at the next trace call when the option is
+     * set after the build (the no-allocation trace uses them only once made). The boxes only
+     * grow, so every result is the same bit for bit; only the speed and the bytes read differ
+     * (1080p on the 1M-triangle scene: the latency-bound sky rays run faster, the AO rays, bound
+     * by VALU issue, slower). The wave cut and the other kernels read the 128 B records. */
+    RAW_AO_OPT_QUANTIZED_NODES = 13
 };
 enum {
     RAW_AO_TRACE_WIDE = 0,     /* 4-wide nodes, LDS stack, triangle tests batched per wave */
@@ -330,6 +341,14 @@ int raw_ao_last_ms(raw_ao_ctx* ctx, float* build_ms, float* trace_ms, float* ssa
 int raw_ao_angle_table(raw_ao_ctx* ctx, float* sin_out, float* cos_out);
 /* Bytes of the context's device-resident BVH (nodes + triangles) in its traversal form. */
 int raw_ao_bvh_bytes(raw_ao_ctx* ctx, uint64_t* node_bytes, uint64_t* tri_bytes, uint32_t* node_record_bytes);
+/* Verification entry point (no reference counterpart): the wide tree's records copied to the
+ * host, up to cap of each kind: records128 the 128 B records (6 x 4 slot planes lo x, y, z, hi
+ * x, y, z as floats, 4 int32 refs, the masks word, 12 B spare), records64 their 64 B quantized
+ * forms (RAW_AO_OPT_QUANTIZED_NODES: base x, y, z as floats, the exponents word, the lo x, y, z
+ * and hi x, y, z byte words, the masks word, 0, the 4 refs); either may be NULL. *count = the
+ * tree's record count. RAW_AO_ERR_ARG without a wide tree, or for records64 when the quantized
+ * records are not made. */
+int raw_ao_read_wide_records(raw_ao_ctx* ctx, void* records128, void* records64, uint32_t cap, uint32_t* count);
 
 /* ---- batched AO: many views of several scenes ---------------------------------------
  * No reference function (the reference renders one view of one scene per call,

@@ -280,6 +280,100 @@ __global__ void k_wide_single(const float* __restrict__ boxes, Node4* __restrict
     nodes[0] = r;
 }
 
+// ---- quantized records ---------------------------------------------------------------
+// Record (64 B, 4 x 16 B), the same node ids as the 128 B records: per axis a base b and a
+// power-of-two step s, each slot's lo and hi plane as a byte q (plane = b + q s); the masks
+// word; the slot refs. Words: {b x, b y, b z, exps} {q lo x, q lo y, q lo z, q hi x} {q hi y,
+// q hi z, masks, 0} {refs}; exps holds the biased exponents of the x, y and z steps in bytes
+// 0-2 and, in byte 3, 15 for a node whose planes could not be quantized (every slot entered).
+// A q word holds slot k's byte at bits 8k.
+//
+// The builder takes b a multiple of s and keeps |b / s| + 255 below 2^24, so b + q s is an
+// integer times s below 2^24 in magnitude: a float, and fmaf(q, s, b) yields it exactly. It
+// rounds each slot's lo plane down and hi plane up to that grid, so every quantized box
+// contains the slot's 128 B box. The slab test's plane distances fmaf(plane, 1/d, -o/d) are
+// monotone in the plane, so a containing box gives a near distance no larger and a far one no
+// smaller: every slot the 128 B record admits, the 64 B one admits too. Visits can only be
+// added, and the any-hit answer, which the triangle tests give, stays the same.
+struct alignas(16) Node4Q { float bx, by, bz; uint32_t exps; uint32_t qlx, qly, qlz, qhx, qhy, qhz, masks, zero; int4 ref; };
+static_assert(sizeof(Node4Q) == 64, "a quantized record is 64 bytes");
+
+// One axis of a node over its slots `valid`: the step's biased exponent e, the base, each slot's
+// bytes; false when no step works (a plane infinite or NaN).
+__device__ bool quantize_axis(const float (&lo)[4], const float (&hi)[4], uint32_t valid, float& base, uint32_t& e_out,
+                              uint32_t& qlo, uint32_t& qhi) {
+    float mn = INFINITY, mx = -INFINITY;
+    for (int k = 0; k < 4; ++k)
+        if ((valid >> k) & 1u) {
+            mn = fminf(mn, lo[k]);
+            mx = fmaxf(mx, hi[k]);
+        }
+    if (!(mn <= mx) || !isfinite(mn) || !isfinite(mx)) return false;
+    for (int k = 0; k < 4; ++k)
+        if (((valid >> k) & 1u) && !(isfinite(lo[k]) && isfinite(hi[k]) && lo[k] <= hi[k])) return false;
+    // the smallest step that spans the slots in 255 steps with the grid indices below 2^24 - 256
+    const float need = fmaxf((mx - mn) * (1.0f / 253.0f), fmaxf(fabsf(mn), fabsf(mx)) * 0x1p-23f);
+    int e = need > 0.0f ? ilogbf(need) + 127 : 1;
+    e = e < 1 ? 1 : (e > 254 ? 254 : e);
+    for (; e <= 254; ++e) {
+        const float st = __uint_as_float(uint32_t(e) << 23);
+        // floor and ceil of plane / step on the grid; ldexpf is exact but for a subnormal
+        // quotient, which the products below catch
+        float fl = floorf(ldexpf(mn, 127 - e));
+        if (fl * st > mn) fl -= 1.0f;
+        float ch = ceilf(ldexpf(mx, 127 - e));
+        if (ch * st < mx) ch += 1.0f;
+        if (!(fabsf(fl) <= 16776960.0f && fabsf(ch) <= 16776960.0f && ch - fl <= 255.0f && isfinite(fl * st) && isfinite(ch * st)))
+            continue;
+        qlo = qhi = 0;
+        for (int k = 0; k < 4; ++k) {
+            uint32_t a = 255u, b = 0u;   // an unused slot: lo above hi, never entered
+            if ((valid >> k) & 1u) {
+                float fa = floorf(ldexpf(lo[k], 127 - e));
+                if (fa * st > lo[k]) fa -= 1.0f;
+                float fb = ceilf(ldexpf(hi[k], 127 - e));
+                if (fb * st < hi[k]) fb += 1.0f;
+                a = uint32_t(fmaxf(fa, fl) - fl);
+                b = uint32_t(fminf(fb, ch) - fl);
+            }
+            qlo |= a << (8 * k);
+            qhi |= b << (8 * k);
+        }
+        base = fl * st;
+        e_out = uint32_t(e);
+        return true;
+    }
+    return false;
+}
+
+// The 64 B records of the first *count (or nmax, without count) 128 B records, one a thread.
+__global__ __launch_bounds__(BT) void k_quantize4(const Node4* __restrict__ nodes, const WideState* __restrict__ st, uint32_t nmax,
+                                                  Node4Q* __restrict__ out) {
+    const uint32_t n = st ? min(st->base, nmax) : nmax;
+    for (uint32_t i = blockIdx.x * BT + threadIdx.x; i < n; i += gridDim.x * BT) {
+        const Node4 nd = nodes[i];
+        const uint32_t masks = __float_as_uint(nd.spare.x);
+        const uint32_t valid = (masks | (masks >> 4)) & 15u;
+        const float lo[3][4] = {{nd.lx.x, nd.lx.y, nd.lx.z, nd.lx.w}, {nd.ly.x, nd.ly.y, nd.ly.z, nd.ly.w}, {nd.lz.x, nd.lz.y, nd.lz.z, nd.lz.w}};
+        const float hi[3][4] = {{nd.hx.x, nd.hx.y, nd.hx.z, nd.hx.w}, {nd.hy.x, nd.hy.y, nd.hy.z, nd.hy.w}, {nd.hz.x, nd.hz.y, nd.hz.z, nd.hz.w}};
+        Node4Q q;
+        float b[3];
+        uint32_t e[3], ql[3], qh[3];
+        bool ok = true;
+        for (int a = 0; a < 3; ++a) ok = ok && quantize_axis(lo[a], hi[a], valid, b[a], e[a], ql[a], qh[a]);
+        if (!ok)
+            for (int a = 0; a < 3; ++a) { b[a] = 0.0f; e[a] = 127u; ql[a] = 0u; qh[a] = 0u; }
+        q.bx = b[0]; q.by = b[1]; q.bz = b[2];
+        q.exps = e[0] | (e[1] << 8) | (e[2] << 16) | (ok ? 0u : 15u << 24);
+        q.qlx = ql[0]; q.qly = ql[1]; q.qlz = ql[2];
+        q.qhx = qh[0]; q.qhy = qh[1]; q.qhz = qh[2];
+        q.masks = masks;
+        q.zero = 0u;
+        q.ref = nd.ref;
+        out[i] = q;
+    }
+}
+
 // ---- traversal -------------------------------------------------------------------
 struct Ray {
     V3 o, d, inv, oinv;
@@ -365,6 +459,37 @@ __device__ __forceinline__ uint32_t slab4s_t(const Ray& r, const char* __restric
     };
     return (one(nx.x, ny.x, nz.x, fx.x, fy.x, fz.x, t0s[0]) ? 1u : 0u) | (one(nx.y, ny.y, nz.y, fx.y, fy.y, fz.y, t0s[1]) ? 2u : 0u) |
            (one(nx.z, ny.z, nz.z, fx.z, fy.z, fz.z, t0s[2]) ? 4u : 0u) | (one(nx.w, ny.w, nz.w, fx.w, fy.w, fz.w, t0s[3]) ? 8u : 0u);
+}
+// slab4s_t on a 64 B record at byte offset off of qbase: the same hit bits but for the slots the
+// quantized boxes add (see Node4Q), each slot's entry distance; also the refs and masks word.
+__device__ __forceinline__ uint32_t slab4q(const Ray& r, const char* __restrict__ qbase, uint32_t off, uint32_t ox, uint32_t oy,
+                                           uint32_t oz, float reach, float exit_scale, int4& ref, uint32_t& masks, float (&t0s)[4],
+                                           float tlo = 0.0f) {
+    const uint4 w0 = *reinterpret_cast<const uint4*>(qbase + off);
+    const uint4 w1 = *reinterpret_cast<const uint4*>(qbase + off + 16);
+    const uint4 w2 = *reinterpret_cast<const uint4*>(qbase + off + 32);
+    ref = *reinterpret_cast<const int4*>(qbase + off + 48);
+    masks = w2.z;
+    const float sx = __uint_as_float((w0.w & 255u) << 23), sy = __uint_as_float(((w0.w >> 8) & 255u) << 23),
+                sz = __uint_as_float(((w0.w >> 16) & 255u) << 23);
+    const float bx = __uint_as_float(w0.x), by = __uint_as_float(w0.y), bz = __uint_as_float(w0.z);
+    // near planes: lo where 1/d >= 0 (offset 0), else hi
+    const uint32_t nxw = ox ? w1.w : w1.x, fxw = ox ? w1.x : w1.w;
+    const uint32_t nyw = oy ? w2.x : w1.y, fyw = oy ? w1.y : w2.x;
+    const uint32_t nzw = oz ? w2.y : w1.z, fzw = oz ? w1.z : w2.y;
+    uint32_t m = w0.w >> 24;   // 15: every slot entered
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const auto pl = [&](uint32_t w, float st, float b) { return fmaf(float((w >> (8 * k)) & 255u), st, b); };
+        const float nx = fmaf(pl(nxw, sx, bx), r.inv.x, -r.oinv.x), fx = fmaf(pl(fxw, sx, bx), r.inv.x, -r.oinv.x);
+        const float ny = fmaf(pl(nyw, sy, by), r.inv.y, -r.oinv.y), fy = fmaf(pl(fyw, sy, by), r.inv.y, -r.oinv.y);
+        const float nz = fmaf(pl(nzw, sz, bz), r.inv.z, -r.oinv.z), fz = fmaf(pl(fzw, sz, bz), r.inv.z, -r.oinv.z);
+        const float t0 = fmaxf(fmaxf(nx, ny), fmaxf(nz, tlo));
+        const float t1 = fminf(fminf(fx, fy), fminf(fz, reach));
+        t0s[k] = t0;
+        if (t0 <= t1 * exit_scale) m |= 1u << k;
+    }
+    return m;
 }
 __device__ __forceinline__ int32_t ref_of(const int4& r, int k) {
     const int32_t lo = (k & 1) ? r.y : r.x, hi = (k & 1) ? r.w : r.z;
@@ -743,6 +868,7 @@ struct TraceArgs4 {
     float* sky;            // SKY: sky out (64 spp) or the unblocked counts (below)
     uint32_t* cnt_free;
     uint64_t* open_mask;   // k_trace_ao4rc<.., true> out, k_sky4r in: bit s of a pixel = its AO sample s is open
+    const Node4Q* nodesq;  // Q: the 64 B records (the wave cut still walks the 128 B ones)
 };
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -1222,7 +1348,7 @@ constexpr int SKY_WAVES = 2;    // k_sky4r: waves a block
 // PIX_WORDS x PW floats; om: the wave's 2 PW words (MASK, 64 spp); cl: the wave's PW open counts
 // (below 64 spp). COUNT: the wave's lanes add their node visits, triangle tests and rays to
 // visits, tests, rays.
-template <int SPP, bool COUNT, bool MASK, int SB>
+template <int SPP, bool COUNT, bool MASK, int SB, bool Q = false>
 __device__ __forceinline__ void trace_group_rc(const TraceArgs4& a, uint32_t refill_min, uint32_t pix0, uint32_t* stk,
                                                uint32_t* cutmem, float* pm, uint32_t* om, uint32_t* cl, uint32_t& visits,
                                                uint32_t& tests, uint32_t& rays) {
@@ -1381,11 +1507,19 @@ __device__ __forceinline__ void trace_group_rc(const TraceArgs4& a, uint32_t ref
     uint32_t next = 64;
     while (true) {
         if (live && pend == 0 && node >= 0) {
-            const uint32_t off = uint32_t(node) * uint32_t(sizeof(Node4));
             if (COUNT) ++visits;
-            const uint32_t hit = slab4s(r, base, off, ox, oy, oz, reach, kExit);
-            const int4 ref = *reinterpret_cast<const int4*>(base + off + 96);
-            const uint32_t masks = *reinterpret_cast<const uint32_t*>(base + off + 112);
+            uint32_t hit, masks;
+            int4 ref;
+            if constexpr (Q) {
+                float t0s[4];   // unused
+                hit = slab4q(r, reinterpret_cast<const char*>(a.nodesq), uint32_t(node) * uint32_t(sizeof(Node4Q)), ox, oy, oz, reach,
+                             kExit, ref, masks, t0s);
+            } else {
+                const uint32_t off = uint32_t(node) * uint32_t(sizeof(Node4));
+                hit = slab4s(r, base, off, ox, oy, oz, reach, kExit);
+                ref = *reinterpret_cast<const int4*>(base + off + 96);
+                masks = *reinterpret_cast<const uint32_t*>(base + off + 112);
+            }
             pend = hit & masks & 15u;
             leafref = ref;
             bool overflow = false;
@@ -1459,7 +1593,7 @@ __device__ __forceinline__ void trace_group_rc(const TraceArgs4& a, uint32_t ref
 // The kernel: WB waves a block, each tracing one group of PW consecutive pixels. One wave a block
 // frees a finished wave's slot at once for the next group (with more, it waits for the block's
 // slowest wave).
-template <int SPP, bool COUNT, bool MASK = false, int WB = rc_waves(SPP, MASK)>
+template <int SPP, bool COUNT, bool MASK = false, bool Q = false, int WB = rc_waves(SPP, MASK)>
 __global__ __launch_bounds__(64 * WB) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_trace_ao4rc(TraceArgs4 a,
                                                                                                      uint32_t refill_min) {
     constexpr int PW = 256 / SPP, SB = 64 * WB;
@@ -1473,7 +1607,7 @@ __global__ __launch_bounds__(64 * WB) __attribute__((amdgpu_waves_per_eu(6, 6)))
     const uint32_t pix0 = (blockIdx.x * WB + wave) * PW;
     if (pix0 >= a.npix) return;
     uint32_t visits = 0, tests = 0, rays = 0;
-    trace_group_rc<SPP, COUNT, MASK, SB>(a, refill_min, pix0, stk, cutmem + wave * CUT_WAVE, pixmem + wave * (PIX_WORDS * PW),
+    trace_group_rc<SPP, COUNT, MASK, SB, Q>(a, refill_min, pix0, stk, cutmem + wave * CUT_WAVE, pixmem + wave * (PIX_WORDS * PW),
                                          openmem + (MASK ? wave * 2 * PW : 0), cntmem + (PW > 4 ? wave * PW : 0), visits, tests,
                                          rays);
     if (COUNT) {
@@ -1494,7 +1628,7 @@ __global__ __launch_bounds__(64 * WB) __attribute__((amdgpu_waves_per_eu(6, 6)))
 // 8, 16 and 32 pixels a wave at 1080p on the 1M-triangle scene: 8 is the fastest). A pixel's sky is
 // its unblocked rays / 64 (counted in LDS), 1 off the mask: the same rays and the same any-hit
 // answers as k_vis4c, so the same values.
-template <int G, int SL, int WB = SKY_WAVES>
+template <int G, int SL, bool Q = false, int WB = SKY_WAVES>
 __global__ __launch_bounds__(64 * WB) void k_sky4r(TraceArgs4 a, uint32_t refill_min) {
     static_assert(G <= 16 && (G & (G - 1)) == 0, "one pixel a lane for the set-up, 4 bits a pixel in klass");
     constexpr int SB = 64 * WB;   // the block's threads: the stack's row stride
@@ -1624,13 +1758,25 @@ __global__ __launch_bounds__(64 * WB) void k_sky4r(TraceArgs4 a, uint32_t refill
         // with every lane idle the refill above takes the rest of the list, so none live = done
         if (__ballot(live) == 0ull) break;
         if (live && pend == 0 && node >= 0) {
-            const uint32_t off = uint32_t(node) * uint32_t(sizeof(Node4));
             float t0s[4];
-            const uint32_t hit = slab4s_t(r, base, off, ox, oy, oz, reach, kExit, t0s, tlo);
-            const int4 ref = *reinterpret_cast<const int4*>(base + off + 96);
-            // leaf slots have negative refs, inner ones non-negative, and an empty slot's box
-            // is never entered, so the refs give the slot kinds without the masks word's load
-            const uint32_t neg = (ref.x < 0 ? 1u : 0u) | (ref.y < 0 ? 2u : 0u) | (ref.z < 0 ? 4u : 0u) | (ref.w < 0 ? 8u : 0u);
+            uint32_t hit, neg;
+            int4 ref;
+            if constexpr (Q) {
+                // the 64 B record: slot kinds from its masks word (an unused slot's quantized box is
+                // never entered either)
+                uint32_t masks;
+                hit = slab4q(r, reinterpret_cast<const char*>(a.nodesq), uint32_t(node) * uint32_t(sizeof(Node4Q)), ox, oy, oz, reach,
+                             kExit, ref, masks, t0s, tlo);
+                hit &= (masks | (masks >> 4)) & 15u;
+                neg = masks & 15u;
+            } else {
+                const uint32_t off = uint32_t(node) * uint32_t(sizeof(Node4));
+                hit = slab4s_t(r, base, off, ox, oy, oz, reach, kExit, t0s, tlo);
+                ref = *reinterpret_cast<const int4*>(base + off + 96);
+                // leaf slots have negative refs, inner ones non-negative, and an empty slot's box
+                // is never entered, so the refs give the slot kinds without the masks word's load
+                neg = (ref.x < 0 ? 1u : 0u) | (ref.y < 0 ? 2u : 0u) | (ref.z < 0 ? 4u : 0u) | (ref.w < 0 ? 8u : 0u);
+            }
             pend = hit & neg;
             leafref = ref;
             bool overflow = false;
@@ -2156,10 +2302,27 @@ __global__ __launch_bounds__(BT) void k_batch_finish(BatchDev d) {
 size_t top_count_offset() { return TOP_MAX * (sizeof(TopEntry) / 4); }
 
 // The top table of the wide tree in c.nodes4 (TOP_MAX entries, then the count).
-static int top_table(Ctx& c, hipStream_t s) {
+// The 64 B records of the first `count` 128 B records (count on the device: st->base, at most
+// nmax; without st: nmax).
+static int quantize_records(Ctx& c, const WideState* st, uint32_t nmax, hipStream_t s) {
+    if (!ensure(c, c.nodes4q, sizeof(Node4Q) * std::max<size_t>(1, nmax))) return -2;
+    k_quantize4<<<std::max(1u, std::min<uint32_t>(cdiv(nmax, BT), 2048u)), BT, 0, s>>>(c.nodes4.as<Node4>(), st, nmax,
+                                                                                          c.nodes4q.as<Node4Q>());
+    if (!hip_ok(c, hipGetLastError(), "quantize launch")) return -2;
+    c.q_built = true;
+    return 0;
+}
+// The top table of the wide tree in c.nodes4 (TOP_MAX entries, then the count), and with
+// RAW_AO_OPT_QUANTIZED_NODES = 2 the 64 B records (of the records the state at c.wide_parity
+// counts, or the one record of a single-triangle tree).
+static int top_table(Ctx& c, hipStream_t s, bool single = false) {
     if (!ensure(c, c.top, sizeof(TopEntry) * TOP_MAX + 16)) return -2;
     k_top_table<<<1, 64, 0, s>>>(c.nodes4.as<Node4>(), c.top.as<TopEntry>(), c.top.as<uint32_t>() + TOP_MAX * (sizeof(TopEntry) / 4));
-    return hip_ok(c, hipGetLastError(), "wide build") ? 0 : -2;
+    if (!hip_ok(c, hipGetLastError(), "wide build")) return -2;
+    c.q_built = false;
+    if (c.node_q < 2) return 0;   // made by the next visibility pass that needs them
+    return single ? quantize_records(c, nullptr, 1, s)
+                  : quantize_records(c, c.wstate.as<WideState>() + c.wide_parity, uint32_t(std::max<size_t>(1, c.ntri - 1)), s);
 }
 
 namespace {
@@ -2231,7 +2394,7 @@ int wide_build(Ctx& c, hipStream_t s) {
         k_wide_single<<<1, 64, 0, s>>>(c.boxes.as<float>(), c.nodes4.as<Node4>());
         c.wide_nodes = 1;
         c.plan_wide_nb.clear();
-        return hip_ok(c, hipGetLastError(), "wide launch") ? top_table(c, s) : -2;
+        return hip_ok(c, hipGetLastError(), "wide launch") ? top_table(c, s, true) : -2;
     }
     // The binary root is the last merge's node (2n - 2); build_finish confirms it and that the
     // enqueued levels emptied the frontier.
@@ -2271,7 +2434,7 @@ int wide_build_fixed(Ctx& c, hipStream_t s) {
     if (n == 0) return 0;
     if (n == 1) {
         k_wide_single<<<1, 64, 0, s>>>(c.boxes.as<float>(), c.nodes4.as<Node4>());
-        return hip_ok(c, hipGetLastError(), "wide launch") ? top_table(c, s) : -2;
+        return hip_ok(c, hipGetLastError(), "wide launch") ? top_table(c, s, true) : -2;
     }
     if (c.plan_wide_nb.empty()) { c.err = "no recorded wide collapse plan"; return -1; }
     wide_init(c, uint32_t(c.root), s);
@@ -2352,6 +2515,8 @@ int trace_ao4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_m
     const uint32_t npix = uint32_t(w) * uint32_t(h);
     if (!ensure(c, c.counters, sizeof(TraceCounters))) return -2;
     if (spp != 64 && !ensure(c, c.st_cnt, 4 * size_t(npix))) return -2;
+    // RAW_AO_OPT_QUANTIZED_NODES = 2 set after the tree was built: its 64 B records now
+    if (c.node_q >= 2 && !c.q_built && c.ntri > 0 && quantize_records(c, nullptr, c.wide_nodes, s)) return -2;
     return trace_ao4_into(c, d_pos, d_nrm, d_mask, w, h, spp, radius, d_ao, c.counters.as<TraceCounters>(),
                           spp != 64 ? c.st_cnt.as<uint32_t>() : nullptr, true, s);
 }
@@ -2366,6 +2531,9 @@ int trace_ao4_into(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t
                  c.host_trig ? c.d_trig : nullptr, d_counters, uint32_t(w), npix, uint32_t(spp),
                  c.ntri, uint32_t(c.tri_q), radius, float(c.wave_cut) * radius, 4.0f * radius, c.top.as<TopEntry>(),
                  c.top.p ? c.top.as<uint32_t>() + TOP_MAX * (sizeof(TopEntry) / 4) : nullptr};
+    // the 64 B records (RAW_AO_OPT_QUANTIZED_NODES = 2) where they are made: the 64-spp refill kernel
+    const bool q = c.node_q >= 2 && c.q_built && c.ntri > 0;
+    if (q) a.nodesq = c.nodes4q.as<Node4Q>();
     // 8, 16, 24 and 32 spp: the refill kernel with 256 / spp pixels a wave, which writes the AO itself
     const bool rc_low = (spp == 8 || spp == 16 || spp == 24 || spp == 32) && c.refill > 0 && c.wave_cut;
     if (!spp64 && !rc_low) {
@@ -2381,7 +2549,10 @@ int trace_ao4_into(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t
         const uint32_t wb = uint32_t(rc_waves(spp_k, false));
         kernel<<<cdiv(npix, uint32_t(256 / spp_k) * wb), 64 * wb, 0, s>>>(a, uint32_t(c.refill));
     };
-    if (spp64 && c.refill > 0 && c.wave_cut) {
+    if (spp64 && c.refill > 0 && c.wave_cut && q) {
+        if (count) rc_launch(k_trace_ao4rc<64, true, false, true>, 64);
+        else rc_launch(k_trace_ao4rc<64, false, false, true>, 64);
+    } else if (spp64 && c.refill > 0 && c.wave_cut) {
         if (count) rc_launch(k_trace_ao4rc<64, true>, 64);
         else rc_launch(k_trace_ao4rc<64, false>, 64);
     } else if (rc_low) {
@@ -2488,13 +2659,21 @@ int trace_vis4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_
     // then the open ones' sky rays by k_sky4r
     const bool split = cut && spp64 && c.refill > 0 && c.sky_refill > 0;
     if (split && !ensure(c, c.vis_open, 8 * size_t(npix))) return -2;
+    // the 64 B records (RAW_AO_OPT_QUANTIZED_NODES) for the split form's sky rays (1, the default)
+    // and its AO rays (2)
+    const bool q = split && c.node_q >= 1 && c.ntri > 0, qa = q && c.node_q >= 2;
+    if (q && !c.q_built && quantize_records(c, nullptr, c.wide_nodes, s)) return -2;
+    if (q) a.nodesq = c.nodes4q.as<Node4Q>();
     (void)hipEventRecord(c.ev[2], s);
     constexpr int PW = 4, SKY_G = 8, SKY_STACK = STACK_LDS;
     if (split) {
         a.open_mask = c.vis_open.as<uint64_t>();
         constexpr int WBM = rc_waves(64, true);
-        k_trace_ao4rc<64, false, true><<<cdiv(npix, uint32_t(PW * WBM)), 64 * WBM, 0, s>>>(a, uint32_t(c.refill));
-        k_sky4r<SKY_G, SKY_STACK><<<cdiv(npix, uint32_t(SKY_G * SKY_WAVES)), 64 * SKY_WAVES, 0, s>>>(a, uint32_t(c.sky_refill));
+        const dim3 g_ao(cdiv(npix, uint32_t(PW * WBM))), g_sky(cdiv(npix, uint32_t(SKY_G * SKY_WAVES)));
+        if (qa) k_trace_ao4rc<64, false, true, true><<<g_ao, 64 * WBM, 0, s>>>(a, uint32_t(c.refill));
+        else k_trace_ao4rc<64, false, true><<<g_ao, 64 * WBM, 0, s>>>(a, uint32_t(c.refill));
+        if (q) k_sky4r<SKY_G, SKY_STACK, true><<<g_sky, 64 * SKY_WAVES, 0, s>>>(a, uint32_t(c.sky_refill));
+        else k_sky4r<SKY_G, SKY_STACK><<<g_sky, 64 * SKY_WAVES, 0, s>>>(a, uint32_t(c.sky_refill));
     } else if (cut && spp64) k_vis4c<true><<<blocks, BT, 0, s>>>(a);
     else if (cut) k_vis4c<false><<<blocks, BT, 0, s>>>(a);
     else if (spp64) k_trace_vis4<true><<<blocks, BT, 0, s>>>(a, d_sky, nullptr);

@@ -102,7 +102,7 @@ void raw_ao_close(raw_ao_ctx* x) {
         raw_ao::net_release(c);
         raw_ao::DevBuf* bufs[] = {&c.tris_in, &c.keys, &c.vals, &c.keys2, &c.vals2, &c.hist, &c.rs_tot, &c.boxes, &c.links,
                                   &c.clus_a, &c.clus_b, &c.nn, &c.flags, &c.blk, &c.state, &c.g4, &c.bounds_part,
-                                  &c.nodes, &c.tris_sorted, &c.nodes4, &c.wfront_a, &c.wfront_b, &c.wblk, &c.wstate, &c.vis_open, &c.st_pos, &c.st_nrm, &c.st_mask, &c.st_out, &c.st_out2,
+                                  &c.nodes, &c.tris_sorted, &c.nodes4, &c.nodes4q, &c.wfront_a, &c.wfront_b, &c.wblk, &c.wstate, &c.vis_open, &c.st_pos, &c.st_nrm, &c.st_mask, &c.st_out, &c.st_out2,
                                   &c.st_cnt, &c.st_rays, &c.st_max, &c.counters, &c.r_keys, &c.r_tris, &c.r_counts,
                                   &c.r_offs, &c.r_scan_tmp, &c.fr_mesh, &c.fr_scalars, &c.fr_planes, &c.ploc_hist};
         for (auto* b : bufs)
@@ -145,6 +145,11 @@ int raw_ao_set_option(raw_ao_ctx* x, int option, int value) {
         if (value < 0 || value > 1024) return fail(c, RAW_AO_ERR_ARG, "learned back-off must be in 0..1024");
         c.learned_backoff = value;
         c.lb_next = c.lb_left = 0;
+        return RAW_AO_OK;
+    }
+    if (option == RAW_AO_OPT_QUANTIZED_NODES) {
+        if (value < 0 || value > 2) return fail(c, RAW_AO_ERR_ARG, "quantized nodes must be 0, 1 or 2");
+        c.node_q = value;
         return RAW_AO_OK;
     }
     if (option == RAW_AO_OPT_SKY_REFILL) {
@@ -751,6 +756,23 @@ int raw_ao_bvh_bytes(raw_ao_ctx* x, uint64_t* node_bytes, uint64_t* tri_bytes, u
     if (node_bytes) *node_bytes = !n ? 0 : (wide ? 128ull * c.wide_nodes : 64ull * (n > 1 ? n - 1 : 1));
     if (tri_bytes) *tri_bytes = 48ull * n;
     if (node_record_bytes) *node_record_bytes = wide ? 128 : 64;
+    return RAW_AO_OK;
+}
+
+int raw_ao_read_wide_records(raw_ao_ctx* x, void* records128, void* records64, uint32_t cap, uint32_t* count) {
+    if (!x) return RAW_AO_ERR_ARG;
+    Ctx& c = x->c;
+    c.err.clear();
+    if (c.ntri == 0 || c.root < 0 || c.built_kernel != RAW_AO_TRACE_WIDE) return fail(c, RAW_AO_ERR_ARG, "no wide tree");
+    if (records64 && !c.q_built)
+        return fail(c, RAW_AO_ERR_ARG, "no quantized records: set RAW_AO_OPT_QUANTIZED_NODES and build or trace first");
+    if (!hip_ok(c, hipSetDevice(c.device), "hipSetDevice") || !hip_ok(c, hipStreamSynchronize(c.stream), "sync")) return RAW_AO_ERR_DEVICE;
+    const uint32_t n = std::min(cap, c.wide_nodes);
+    if (count) *count = c.wide_nodes;
+    if (records128 && n && !hip_ok(c, hipMemcpy(records128, c.nodes4.p, 128 * size_t(n), hipMemcpyDeviceToHost), "records"))
+        return RAW_AO_ERR_DEVICE;
+    if (records64 && n && !hip_ok(c, hipMemcpy(records64, c.nodes4q.p, 64 * size_t(n), hipMemcpyDeviceToHost), "records"))
+        return RAW_AO_ERR_DEVICE;
     return RAW_AO_OK;
 }
 

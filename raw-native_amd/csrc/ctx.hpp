@@ -82,6 +82,9 @@ struct Ctx {
     DevBuf tris_sorted;          // 48 B each, sorted (leaf) order
     uint32_t trav_root{0};
     DevBuf nodes4;               // 4-wide records, 128 B each, root at 0
+    DevBuf nodes4q;              // RAW_AO_OPT_QUANTIZED_NODES: the same records quantized, 64 B each
+    int node_q{1};               // RAW_AO_OPT_QUANTIZED_NODES: 0 none, 1 the sky rays, 2 also the 64-spp AO rays
+    bool q_built{false};         // nodes4q holds the current tree's records (or is enqueued after its build)
     DevBuf top;                  // the wide tree's top table (slots to depth 3) and its entry count
     DevBuf wfront_a, wfront_b, wblk, wstate;
     DevBuf vis_open;             // the visibility pass's per-pixel open AO samples (64 bits a pixel)

@@ -30,6 +30,7 @@ OPT_SKEW = 9
 OPT_BUILD_ROUNDS = 10
 OPT_SKY_REFILL = 11
 OPT_LEARNED_BACKOFF = 12
+OPT_QUANTIZED_NODES = 13
 NOALLOC_MAX_PIXELS = 1 << 22
 TRACE_WIDE, TRACE_BINARY = 0, 1
 
@@ -82,6 +83,7 @@ SIGNATURES = [
                                  C.POINTER(C.c_float)]),
     ("raw_ao_angle_table", C.c_int, [_vp, _f32p, _f32p]),
     ("raw_ao_bvh_bytes", C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    ("raw_ao_read_wide_records", C.c_int, [_vp, _vp, _vp, C.c_uint32, C.POINTER(C.c_uint32)]),
     ("raw_ao_batch_open", C.c_int, [C.c_int, C.POINTER(_vp)]),
     ("raw_ao_batch_close", None, [_vp]),
     ("raw_ao_batch_last_error", C.c_char_p, [_vp]),
@@ -476,6 +478,17 @@ class Context:
         nb, tb, rec = C.c_uint64(), C.c_uint64(), C.c_uint32()
         self._check(lib().raw_ao_bvh_bytes(self._h, C.byref(nb), C.byref(tb), C.byref(rec)))
         return nb.value, tb.value, rec.value
+
+    def read_wide_records(self, quantized=True):
+        """The wide tree's 128 B records as (n, 32) uint32 words and, with quantized, their 64 B
+        quantized forms as (n, 16) uint32 words (raw_ao_read_wide_records)."""
+        n = C.c_uint32()
+        self._check(lib().raw_ao_read_wide_records(self._h, None, None, 0, C.byref(n)))
+        r128 = np.zeros((max(1, n.value), 32), np.uint32)
+        r64 = np.zeros((max(1, n.value), 16), np.uint32) if quantized else None
+        self._check(lib().raw_ao_read_wide_records(self._h, r128.ctypes.data, r64.ctypes.data if quantized else None, n.value,
+                                                    C.byref(n)))
+        return (r128[:n.value], r64[:n.value] if quantized else None)
 
     # ---- learned AO surrogate ------------------------------------------------------
     def net_param_count(self):
