@@ -1629,7 +1629,8 @@ __global__ __launch_bounds__(64 * WB) __attribute__((amdgpu_waves_per_eu(6, 6)))
 // its unblocked rays / 64 (counted in LDS), 1 off the mask: the same rays and the same any-hit
 // answers as k_vis4c, so the same values.
 template <int G, int SL, bool Q = false, int WB = SKY_WAVES>
-__global__ __launch_bounds__(64 * WB) void k_sky4r(TraceArgs4 a, uint32_t refill_min) {
+__global__ __launch_bounds__(64 * WB) __attribute__((amdgpu_waves_per_eu(Q ? 8 : 6, Q ? 8 : 6))) void k_sky4r(TraceArgs4 a,
+                                                                                                        uint32_t refill_min) {
     static_assert(G <= 16 && (G & (G - 1)) == 0, "one pixel a lane for the set-up, 4 bits a pixel in klass");
     constexpr int SB = 64 * WB;   // the block's threads: the stack's row stride
     __shared__ uint32_t stk[SL * SB];
@@ -2665,14 +2666,17 @@ int trace_vis4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_
     if (q && !c.q_built && quantize_records(c, nullptr, c.wide_nodes, s)) return -2;
     if (q) a.nodesq = c.nodes4q.as<Node4Q>();
     (void)hipEventRecord(c.ev[2], s);
-    constexpr int PW = 4, SKY_G = 8, SKY_STACK = STACK_LDS;
+    // the sky rays on the 64 B records: 12 LDS stack rows (9.3 KB a block of 2 waves) and at most 64
+    // VGPRs, for 8 waves a SIMD (17.40 ms the pass at 1080p, against 17.82 with 16 rows at 7 waves
+    // and 18.35 with 16 rows at 6); on the 128 B records 16 rows at 6 waves (18.97 ms; 21.28 at 8)
+    constexpr int PW = 4, SKY_G = 8, SKY_STACK_Q = 12, SKY_STACK = STACK_LDS;
     if (split) {
         a.open_mask = c.vis_open.as<uint64_t>();
         constexpr int WBM = rc_waves(64, true);
         const dim3 g_ao(cdiv(npix, uint32_t(PW * WBM))), g_sky(cdiv(npix, uint32_t(SKY_G * SKY_WAVES)));
         if (qa) k_trace_ao4rc<64, false, true, true><<<g_ao, 64 * WBM, 0, s>>>(a, uint32_t(c.refill));
         else k_trace_ao4rc<64, false, true><<<g_ao, 64 * WBM, 0, s>>>(a, uint32_t(c.refill));
-        if (q) k_sky4r<SKY_G, SKY_STACK, true><<<g_sky, 64 * SKY_WAVES, 0, s>>>(a, uint32_t(c.sky_refill));
+        if (q) k_sky4r<SKY_G, SKY_STACK_Q, true><<<g_sky, 64 * SKY_WAVES, 0, s>>>(a, uint32_t(c.sky_refill));
         else k_sky4r<SKY_G, SKY_STACK><<<g_sky, 64 * SKY_WAVES, 0, s>>>(a, uint32_t(c.sky_refill));
     } else if (cut && spp64) k_vis4c<true><<<blocks, BT, 0, s>>>(a);
     else if (cut) k_vis4c<false><<<blocks, BT, 0, s>>>(a);
