@@ -1341,7 +1341,9 @@ constexpr int PIX_WORDS = 16;   // per pixel: origin, n, t, b (12 floats), x, y
 // that the next frame's build kernels on other streams fill (6.40 ms a frame in flight against
 // 6.80 with two); eight: 6.63 ms alone. Below 64 spp, 24 spp runs faster with four.
 constexpr int rc_waves(int spp, bool mask) { return (spp == 64 && !mask) || spp == 24 ? 4 : 2; }
-constexpr int SKY_WAVES = 2;    // k_sky4r: waves a block
+// k_sky4r: waves a block; on the 64 B records one (the pass at 1080p 17.21 ms against 17.44 with
+// two and 17.63 with four; 4 or 16 pixels a wave instead of 8: 18.5 and 18.3 ms)
+constexpr int sky_waves(bool q) { return q ? 1 : 2; }
 // One group of PW = 256 / SPP pixels (from pix0) for the calling wave: its PW x SPP (at most 256)
 // rays, ray ri = sample ri % SPP of pixel pix0 + ri / SPP. stk: the block's traversal stack (STACK_LDS rows of
 // SB words, SB the block's threads); cutmem: the wave's CUT_WAVE words; pm: the wave's
@@ -1628,7 +1630,7 @@ __global__ __launch_bounds__(64 * WB) __attribute__((amdgpu_waves_per_eu(6, 6)))
 // 8, 16 and 32 pixels a wave at 1080p on the 1M-triangle scene: 8 is the fastest). A pixel's sky is
 // its unblocked rays / 64 (counted in LDS), 1 off the mask: the same rays and the same any-hit
 // answers as k_vis4c, so the same values.
-template <int G, int SL, bool Q = false, int WB = SKY_WAVES>
+template <int G, int SL, bool Q = false, int WB = sky_waves(Q)>
 __global__ __launch_bounds__(64 * WB) __attribute__((amdgpu_waves_per_eu(Q ? 8 : 6, Q ? 8 : 6))) void k_sky4r(TraceArgs4 a,
                                                                                                         uint32_t refill_min) {
     static_assert(G <= 16 && (G & (G - 1)) == 0, "one pixel a lane for the set-up, 4 bits a pixel in klass");
@@ -2673,11 +2675,12 @@ int trace_vis4(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t* d_
     if (split) {
         a.open_mask = c.vis_open.as<uint64_t>();
         constexpr int WBM = rc_waves(64, true);
-        const dim3 g_ao(cdiv(npix, uint32_t(PW * WBM))), g_sky(cdiv(npix, uint32_t(SKY_G * SKY_WAVES)));
+        constexpr int SWQ = sky_waves(true), SW = sky_waves(false);
+        const dim3 g_ao(cdiv(npix, uint32_t(PW * WBM))), g_sky(cdiv(npix, uint32_t(SKY_G * (q ? SWQ : SW))));
         if (qa) k_trace_ao4rc<64, false, true, true><<<g_ao, 64 * WBM, 0, s>>>(a, uint32_t(c.refill));
         else k_trace_ao4rc<64, false, true><<<g_ao, 64 * WBM, 0, s>>>(a, uint32_t(c.refill));
-        if (q) k_sky4r<SKY_G, SKY_STACK_Q, true><<<g_sky, 64 * SKY_WAVES, 0, s>>>(a, uint32_t(c.sky_refill));
-        else k_sky4r<SKY_G, SKY_STACK><<<g_sky, 64 * SKY_WAVES, 0, s>>>(a, uint32_t(c.sky_refill));
+        if (q) k_sky4r<SKY_G, SKY_STACK_Q, true><<<g_sky, 64 * SWQ, 0, s>>>(a, uint32_t(c.sky_refill));
+        else k_sky4r<SKY_G, SKY_STACK><<<g_sky, 64 * SW, 0, s>>>(a, uint32_t(c.sky_refill));
     } else if (cut && spp64) k_vis4c<true><<<blocks, BT, 0, s>>>(a);
     else if (cut) k_vis4c<false><<<blocks, BT, 0, s>>>(a);
     else if (spp64) k_trace_vis4<true><<<blocks, BT, 0, s>>>(a, d_sky, nullptr);
