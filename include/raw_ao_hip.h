@@ -99,7 +99,7 @@ enum {
      * is the same for every value; only the speed differs. */
     RAW_AO_OPT_BUILD_ROUNDS = 10,
     /* 64-spp visibility (raw_ao_trace_visibility[_dev], the frame's lighting model 2), 0..64
-     * (default 32): with n > 0 (and RAW_AO_OPT_REFILL and RAW_AO_OPT_WAVE_CUT above 0) the AO rays
+     * (default 20): with n > 0 (and RAW_AO_OPT_REFILL and RAW_AO_OPT_WAVE_CUT above 0) the AO rays
      * are traced first, then the open ones' sky rays apart from them, a wave taking 8 pixels'
      * rays and a lane whose ray ends the next one once n lanes are idle; 0 = each sky ray right
      * after its AO ray in the same lane. Results are the same for every value; only the speed

@@ -54,7 +54,7 @@ struct Ctx {
     int lb_next{0};              // the skip length after the next fall-back (1, 2, 4, ..., learned_backoff)
     int lb_left{0};              // calls left to skip
     int lb_w{0}, lb_h{0};        // the image size the skipping applies to
-    int sky_refill{32};          // 64-spp visibility: sky rays apart from the AO rays, lanes refilled at this many idle (0: off)
+    int sky_refill{20};          // 64-spp visibility: sky rays apart from the AO rays, lanes refilled at this many idle (0: off)
     bool skew{false};            // the skew control: AO query distances x 0.99f
     int build_rounds{0};         // PLOC rounds and collapse levels a build enqueues (0: automatic)
     bool count_next{false};
