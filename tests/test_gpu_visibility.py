@@ -144,7 +144,7 @@ def test_visibility_sky_refill(ctx, refill):
         ctx.set_option(raw_ao.OPT_SKY_REFILL, refill)
         got = ctx.compute_rt_visibility(pos, nrm, mask, DEFAULT_TO_LIGHT, 64, 1.0)
     finally:
-        ctx.set_option(raw_ao.OPT_SKY_REFILL, 32)
+        ctx.set_option(raw_ao.OPT_SKY_REFILL, 20)   # the default
     for g, o in zip(got, want):
         assert np.array_equal(bits(g), bits(o))
     assert 0.0 < float(np.mean(got[1][mask != 0])) < 1.0
