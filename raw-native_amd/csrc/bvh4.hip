@@ -1822,7 +1822,9 @@ __global__ __launch_bounds__(64 * WB) __attribute__((amdgpu_waves_per_eu(Q ? 8 :
                 pend &= pend - 1u;
                 const uint32_t slot = uint32_t(~ref_of(leafref, k));
                 float t;
-                if (slot < a.ntri && tri_hit<false>(r, a.tris, slot, t) && t < kFarT) {
+                // the change of frame: per-component selects on the 64 B records' kernel (8 waves a
+                // SIMD: the pass 17.03 -> 16.90 ms), whole-vector rotations on the other
+                if (slot < a.ntri && tri_hit<Q>(r, a.tris, slot, t) && t < kFarT) {
                     live = false;   // blocked
                     pend = 0;
                 }
