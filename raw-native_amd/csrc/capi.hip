@@ -112,6 +112,12 @@ void raw_ao_close(raw_ao_ctx* x) {
         if (c.fr_host) (void)hipHostFree(c.fr_host);
         for (auto& e : c.ev)
             if (e) (void)hipEventDestroy(e);
+        if (c.aux_fork) (void)hipEventDestroy(c.aux_fork);
+        if (c.aux_join) (void)hipEventDestroy(c.aux_join);
+        if (c.aux) {
+            (void)hipStreamSynchronize(c.aux);
+            (void)hipStreamDestroy(c.aux);
+        }
         (void)hipStreamDestroy(c.stream);
     }
     delete x;

@@ -41,6 +41,8 @@ struct TraceCounters {
 struct Ctx {
     int device{0};
     hipStream_t stream{nullptr};
+    hipStream_t aux{nullptr};            // the whole frame's raster beside its build (made on first use, default priority)
+    hipEvent_t aux_fork{nullptr}, aux_join{nullptr};
     std::string err;
     bool host_trig{true};
     int trace_kernel{0};         // RAW_AO_TRACE_WIDE or RAW_AO_TRACE_BINARY
