@@ -58,6 +58,27 @@ def test_surrogate_beats_ssao_on_a_held_out_view(ctx):
     assert abs(rep["probe_rmse"] - r_net) < r_net / 3
 
 
+@pytest.mark.skipif(not os.path.exists(L.SURROGATE), reason="no trained surrogate weights")
+@pytest.mark.parametrize("w,h", [(1920, 1080), (200, 136)])
+def test_learned_frame_equals_its_parts(ctx, w, h):
+    """The learned frame (raw_ao_learned_ao_dev, no probe) equals bit for bit its parts called one
+    by one: SSAO -> features -> the network's forward pass; 200 x 136 leaves partial 8 x 32 tiles."""
+    print("learned frame parts", w, h, flush=True)
+    tris, eye, mask, P, N, M = frame(raw_ao.host.SCENE_1M, w, h)
+    ctx.net_load(L.SURROGATE)
+    lao = torch.empty((h, w), dtype=torch.float32, device="cuda")
+    ctx.learned_ao_dev(P, N, M, None, eye, w, h, lao, probe_stride=0)
+    ss = torch.empty((h, w), dtype=torch.float32, device="cuda")
+    ctx.compute_ssao_dev(P, N, M, w, h, 24, 1.0, ss)
+    x = torch.empty((h, w, 8), dtype=torch.int16, device="cuda")
+    ctx.net_features_dev(P, N, M, ss, None, eye, w, h, x)
+    ao = torch.empty(h * w, dtype=torch.float32, device="cuda")
+    ctx.net_forward_dev(x, M, 1, w, h, ao)
+    torch.cuda.synchronize()
+    assert torch.equal(lao.reshape(-1).view(torch.int32), ao.view(torch.int32))
+    assert float(lao[M != 0].std()) > 0.01
+
+
 @pytest.mark.skipif(not os.path.exists(L.DENOISER), reason="no trained denoiser weights")
 def test_denoiser_beats_low_spp_ray_tracing(ctx):
     w, h = 3840, 2160
