@@ -623,7 +623,8 @@ def main():
         learned_dn = {k: denoise_1080p(ctx, raw_ao, torch, d_pos, d_nrm, d_mask, d_ao, mask, eye, W, H, radius, a.steps,
                                        float(np.mean(ms["trace"])), frame_ms, stream, path)
                       for k, path in (("24spp", L.DENOISER_1080P_24), ("20spp", L.DENOISER_1080P_20),
-                                      ("16spp", L.DENOISER_1080P_16), ("8spp", L.DENOISER_1080P))}
+                                      ("18spp", L.DENOISER_1080P_18), ("16spp", L.DENOISER_1080P_16),
+                                      ("8spp", L.DENOISER_1080P))}
 
     rays = covered * spp
     value = rays * a.steps / wall

@@ -1354,7 +1354,8 @@ template <int SPP, bool COUNT, bool MASK, int SB, bool Q = false>
 __device__ __forceinline__ void trace_group_rc(const TraceArgs4& a, uint32_t refill_min, uint32_t pix0, uint32_t* stk,
                                                uint32_t* cutmem, float* pm, uint32_t* om, uint32_t* cl, uint32_t& visits,
                                                uint32_t& tests, uint32_t& rays) {
-    static_assert(SPP == 8 || SPP == 16 || SPP == 20 || SPP == 24 || SPP == 32 || SPP == 64, "8, 16, 20, 24, 32 or 64 rays a pixel");
+    static_assert(SPP == 8 || SPP == 16 || SPP == 18 || SPP == 20 || SPP == 24 || SPP == 32 || SPP == 64,
+                  "8, 16, 18, 20, 24, 32 or 64 rays a pixel");
     static_assert(!MASK || SPP == 64, "the open masks are 64 bits a pixel");
     constexpr int PW = 256 / SPP;
     // 64 spp: a lane's open counts in one register, a byte a pixel; below, LDS counters
@@ -1504,7 +1505,7 @@ __device__ __forceinline__ void trace_group_rc(const TraceArgs4& a, uint32_t ref
         else atomicAdd(&cl[pk], 1u);
         if (MASK) atomicOr(&om[2 * pk + (cur_s >> 5)], 1u << (cur_s & 31u));
     };
-    constexpr uint32_t total = PW * SPP;   // 256 (240 at 20 and 24 spp)
+    constexpr uint32_t total = PW * SPP;   // 256 (252 at 18 spp, 240 at 20 and 24)
     bool live = setup(lane, fixes & 15u);
     uint32_t next = 64;
     while (true) {
@@ -2539,9 +2540,9 @@ int trace_ao4_into(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t
     // the 64 B records (RAW_AO_OPT_QUANTIZED_NODES = 2) where they are made: the 64-spp refill kernel
     const bool q = c.node_q >= 2 && c.q_built && c.ntri > 0;
     if (q) a.nodesq = c.nodes4q.as<Node4Q>();
-    // 8, 16, 20, 24 and 32 spp: the refill kernel with 256 / spp pixels a wave (at most 256 rays),
+    // 8, 16, 18, 20, 24 and 32 spp: the refill kernel with 256 / spp pixels a wave (at most 256 rays),
     // which writes the AO itself
-    const bool rc_low = (spp == 8 || spp == 16 || spp == 20 || spp == 24 || spp == 32) && c.refill > 0 && c.wave_cut;
+    const bool rc_low = (spp == 8 || spp == 16 || spp == 18 || spp == 20 || spp == 24 || spp == 32) && c.refill > 0 && c.wave_cut;
     if (!spp64 && !rc_low) {
         a.cnt = d_cnt;
         k_zero_u32<<<std::min<uint32_t>(2048, cdiv(npix, BT)), BT, 0, s>>>(a.cnt, npix);
@@ -2568,6 +2569,9 @@ int trace_ao4_into(Ctx& c, const float* d_pos, const float* d_nrm, const uint8_t
         } else if (spp == 16) {
             if (count) rc_launch(k_trace_ao4rc<16, true>, 16);
             else rc_launch(k_trace_ao4rc<16, false>, 16);
+        } else if (spp == 18) {
+            if (count) rc_launch(k_trace_ao4rc<18, true>, 18);
+            else rc_launch(k_trace_ao4rc<18, false>, 18);
         } else if (spp == 20) {
             if (count) rc_launch(k_trace_ao4rc<20, true>, 20);
             else rc_launch(k_trace_ao4rc<20, false>, 20);

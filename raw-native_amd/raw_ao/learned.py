@@ -28,9 +28,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS_DIR = os.path.join(os.path.dirname(HERE), "weights")
 SURROGATE = os.path.join(WEIGHTS_DIR, "ao_surrogate.raow")
 DENOISER = os.path.join(WEIGHTS_DIR, "ao_denoiser.raow")
-# trained at 1920 x 1080 on 8-spp (16-, 20-, 24-spp) inputs to give the 64-spp frame they are the first samples of
+# trained at 1920 x 1080 on 8-spp (16-, 18-, 20-, 24-spp) inputs to give the 64-spp frame they are the first samples of
 DENOISER_1080P = os.path.join(WEIGHTS_DIR, "ao_denoiser_1080p.raow")
 DENOISER_1080P_16 = os.path.join(WEIGHTS_DIR, "ao_denoiser_1080p_16spp.raow")
+DENOISER_1080P_18 = os.path.join(WEIGHTS_DIR, "ao_denoiser_1080p_18spp.raow")
 DENOISER_1080P_20 = os.path.join(WEIGHTS_DIR, "ao_denoiser_1080p_20spp.raow")
 DENOISER_1080P_24 = os.path.join(WEIGHTS_DIR, "ao_denoiser_1080p_24spp.raow")
 FEATURES = 8

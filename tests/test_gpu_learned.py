@@ -118,11 +118,11 @@ def test_1080p_denoiser_from_8_spp(ctx):
     assert r_dn256 < r_64_256
 
 
-@pytest.mark.parametrize("k", [16, 20, 24])
+@pytest.mark.parametrize("k", [16, 18, 20, 24])
 def test_1080p_denoiser_from_more_samples(ctx, k):
-    """The denoisers trained on 16-, 20- and 24-spp inputs: from 20 and 24 spp the frame keeps the
+    """The denoisers trained on 16-, 18-, 20- and 24-spp inputs: from 18 spp on the frame keeps the
     0.02 RMSE bound against the 64-spp frame (from 16 spp it comes within a few 1e-4 of it)."""
-    path = {16: L.DENOISER_1080P_16, 20: L.DENOISER_1080P_20, 24: L.DENOISER_1080P_24}[k]
+    path = {16: L.DENOISER_1080P_16, 18: L.DENOISER_1080P_18, 20: L.DENOISER_1080P_20, 24: L.DENOISER_1080P_24}[k]
     if not os.path.exists(path):
         pytest.skip("no trained %d-spp 1080p denoiser weights" % k)
     w, h = 1920, 1080
@@ -140,7 +140,7 @@ def test_1080p_denoiser_from_more_samples(ctx, k):
     r_dn256, r_64_256 = rmse(ctx, dn, ref, M, mask, w, h), rmse(ctx, rt64, ref, M, mask, w, h)
     print("1080p denoised from %d spp: RMSE %.4f vs 64 spp (input %.4f); vs 256 spp %.4f (64 spp %.4f)"
           % (k, r_dn, r_k, r_dn256, r_64_256))
-    if k >= 20:
+    if k >= 18:
         assert r_dn <= 0.02   # the learned frame's bound against 64-spp ray tracing
     assert r_dn < 0.7 * r_k
     assert r_dn256 < r_64_256
