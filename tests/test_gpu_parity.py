@@ -1,6 +1,6 @@
 """The HIP path against the oracle and the reference's golden vectors, through the C ABI.
 Integer and byte results (Morton keys, sort order, trees, AO counts) must be bit-exact;
-AO images are compared bit for bit, with the north-star bound (within 1/64 on >= 99.9%
+AO images are compared bit for bit, with the accuracy bound of the AO path (within 1/64 on >= 99.9%
 of pixels and RMSE <= 1e-3) asserted as well where a different tree could matter."""
 import numpy as np
 import pytest
@@ -127,7 +127,7 @@ def big_frame(w, h, seed=1):
 def test_1m_scene_ao_matches_oracle(ctx):
     """The ~1M-triangle scene at its full 1920 x 1080 frame, radius 1.0, 64 spp (101 M rays):
     every pixel against the CPU reference restatement (binned-SAH BVH), bit for bit. The
-    north-star bound (within 1/64 on >= 99.9% of pixels, RMSE <= 1e-3) is asserted too."""
+    AO accuracy bound (within 1/64 on >= 99.9% of pixels, RMSE <= 1e-3) is asserted too."""
     print("1M-triangle scene, 1920x1080, against the oracle", flush=True)
     tris, pos, nrm, mask = big_frame(1920, 1080)
     ctx.build_ploc(tris, 16)
